@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the FP64 SU(3) leapfrog (Dynamics::SUNc::Update) on one MI355X.
+
+Workload: the large single-event configuration, 256 x 256 x 1024 sites (67 108 864), a = 1, dt = a/4,
+open z boundaries.  A "step" is one leapfrog step of the whole lattice (one launch of the fused step
+kernel).  Inputs are resident in HBM before timing starts.  The fields are an MV collision initial state
+built on the device when the initial-condition path is available (--init mv), otherwise synthetic
+random SU(3) links and random E / J (--init random); the per-site work of the step does not depend on
+the field values.
+
+Prints ONE JSON line:  metric, value (site-updates/s), roofline (HBM, 1760 algorithmic bytes per
+site-update / measured kernel time), cpu_baseline (the oracle timed on this host on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_SITE_UPDATE = 1760  # U 432 + E 192 + J+,J-,J+stat,J-stat 4*64, each read once and written once
+HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "leapfrog_step_256x256x1024_traffic.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per step-kernel launch from the committed rocprofv3 --pmc summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(sample_n, steps):
+    """The oracle (CPU restatement, OpenMP) on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    from soerenpragya3dglasma_amd import Glasma
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.lib().ref_set_threads(threads)
+    with Glasma(sample_n, (1.0, 1.0, 1.0), 0.25) as g:
+        g.fill_random(seed=7, link_scale=0.5, e_scale=0.1, j_scale=0.05)
+        d = g.get_fields()
+    f = O.Fields(sample_n)
+    f.U, f.E, f.Jp, f.Jm, f.JpS, f.JmS = d["U"], d["E"], d["Jp"], d["Jm"], d["JpS"], d["JmS"]
+    O.update(f, (1.0, 1.0, 1.0), 0.25, 1)  # warm-up step
+    t0 = time.perf_counter()
+    O.update(f, (1.0, 1.0, 1.0), 0.25, steps)
+    dt = time.perf_counter() - t0
+    sites = sample_n[0] * sample_n[1] * sample_n[2]
+    return {
+        "value": sites * steps / dt,
+        "unit": "site-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (C++/OpenMP restatement of Dynamics::SUNc::Update) on a {sample_n[0]}x{sample_n[1]}x"
+                  f"{sample_n[2]} slab, {steps} timed steps after 1 warm-up ({dt:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, nargs=3, default=[256, 256, 1024])
+    ap.add_argument("--init", choices=["mv", "random"], default="mv")
+    ap.add_argument("--cpu-sample", type=int, nargs=3, default=[256, 256, 16])
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU (z-slab) decomposition is a design, "
+              "not an implementation. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from soerenpragya3dglasma_amd import Glasma
+    N = tuple(args.n)
+    S = N[0] * N[1] * N[2]
+    a = (1.0, 1.0, 1.0)
+    dt = 0.25
+    log(f"[bench] lattice {N} ({S} sites), init={args.init}")
+    g = Glasma(N, a, dt)
+    init = args.init
+    if init == "mv":
+        try:
+            xl, xr = Glasma.draw_charges(1, N[0], N[1], 0.5, 0.5)
+            mv = Glasma.mv_params(lam=1.65, meff=0.1, sigma_l=8.0, sigma_r=8.0,
+                                  center_l=int(0.85 * N[2] / 2 - 1), center_r=int(1.15 * N[2] / 2 - 1))
+            t0 = time.perf_counter()
+            g.mv_initial_state(0, mv, xl, xr)
+            log(f"[bench] MV collision initial state built on the device in {time.perf_counter() - t0:.2f} s")
+        except Exception as e:  # the initial-condition path is optional for this benchmark
+            log(f"[bench] MV initial state unavailable ({e}); using random fields")
+            init = "random"
+    if init == "random":
+        g.fill_random(seed=1, link_scale=0.5, e_scale=0.1, j_scale=0.05)
+    log(f"[bench] warmup {args.warmup} steps")
+    g.update_async(args.warmup)
+    g.synchronize()
+    log(f"[bench] timing {args.steps} steps")
+    g.synchronize()
+    t0 = time.perf_counter()
+    g.update_async(args.steps)
+    g.synchronize()
+    t1 = time.perf_counter()
+    kern_ms = g.last_step_kernel_ms()
+    wall = t1 - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = S * args.steps / wall
+    achieved = BYTES_PER_SITE_UPDATE * S / (kern_ms * 1e-3) / 1e9
+    umax, _ = g.unitarity()
+    log(f"[bench] {ms_per_step:.3f} ms/step wall, {kern_ms:.3f} ms/kernel, {achieved:.0f} GB/s algorithmic, "
+        f"unitarity max {umax:.2e}")
+    g.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (oracle on the host)")
+        try:
+            cpu = cpu_baseline(tuple(args.cpu_sample), args.cpu_steps)
+        except Exception as e:
+            log(f"[bench] cpu baseline failed: {e}")
+
+    out = {
+        "metric": "FP64 site-updates/s @256^2x1024 (SU(3) leapfrog)",
+        "value": value,
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (" + ("MV collision initial state, seed 1" if init == "mv" else "random SU(3) fields") + ")",
+        "config": {"workload": "large single-event FP64 collision, 256x256x1024, a=1, dt=a/4, open z",
+                   "lattice": list(N), "sites": S, "init": init, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "k_step", "kernel_ms": kern_ms, "bytes_per_site_update": BYTES_PER_SITE_UPDATE},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * glasma.h — C ABI of the MI355X-native 3+1D glasma engine (libglasma.so).
+ *
+ * The reference (pragyasingh1729/SoerenPragya3DGlasma) has no plugin or FFI layer: its "API" is a set
+ * of C++ namespace functions acting on global field objects.  Every entry point below replaces one of
+ * those functions; the citation gives the reference file:line (paths relative to
+ * Pragya_workingdirectory_worksonm1/Workingdirectory/SourceCode/).
+ *
+ * Conventions
+ *   - Plain C types only: pointers, sizes, doubles.  No torch types.
+ *   - Return value: 0 on success, negative glasma_status on failure; glasma_last_error() gives text.
+ *     The library never calls exit() (the reference exits with status 0 on every error).
+ *   - Host field arrays use the reference array-of-structs layout INCLUDING the two z ghost layers:
+ *       U, UOld : complex double (re,im interleaved) at 9*(mu + 3*Index3D) + i + 3*j
+ *       E       : double at a + 8*(mu + 3*Index3D)
+ *       J       : double at a + 8*Index3D
+ *     with Index3D = x + Nx*(y + Ny*(z+1)), z in [-1, Nz]   (LATTICE/GaugeLinks.cpp:27-36).
+ *     Inputs: ghost layers are ignored (the device keeps no ghost layers; open z boundaries read the
+ *     edge layer, which is what SynchronizeGhostCells guarantees before every read in the reference).
+ *     Outputs: ghost layers are filled as SynchronizeGhostCells would (LATTICE/GaugeLinks.cpp:59-73).
+ *   - A context owns all device memory; no caller pointer is retained past a call.  Calls block until
+ *     the device work is done.  A context is not thread-safe; separate contexts are independent.
+ *   - Only SU(3) (nc = 3) runs on the device.
+ */
+#ifndef GLASMA_H_
+#define GLASMA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glasma_ctx glasma_ctx;
+
+typedef enum glasma_status {
+    GLASMA_OK = 0,
+    GLASMA_ERR_ARG = -1,     /* invalid argument / unsupported configuration */
+    GLASMA_ERR_HIP = -2,     /* HIP runtime error (text in glasma_last_error) */
+    GLASMA_ERR_NOMEM = -3,   /* device or host allocation failed */
+    GLASMA_ERR_STATE = -4,   /* call out of order (e.g. observable before fields are set) */
+    GLASMA_ERR_NUMERIC = -5  /* non-finite value or non-unitary update detected */
+} glasma_status;
+
+/* Lattice and dynamics parameters: Lattice::N/a (LATTICE/3DGrid.cpp:14-34), Dynamics::dTStep
+ * (DYNAMICS/SUNcDynamics.cpp:30), BOUNDARY_CONDITIONS_FLAG (LATTICE/3DGrid.cpp:10). */
+typedef struct glasma_params {
+    int nc;       /* gauge group SU(nc); must be 3 */
+    int n[3];     /* Nx, Ny, Nz */
+    double a[3];  /* lattice spacings ax, ay, az */
+    double dt;    /* time step */
+    int bc_z;     /* 0 = open (edge-copy ghost layers, the reference default), 1 = periodic */
+} glasma_params;
+
+/* MV-model parameters: LATTICE/Parameters.cpp:4-50, COLLIDINGOBJECTS/CollidingObjects.cpp:30-102 */
+typedef struct glasma_mv_params {
+    double lambda;      /* UV regulator Lambda (Parameters::Lambda) */
+    double m_eff;       /* IR regulator m (Parameters::MEff) */
+    double sigma_l;     /* longitudinal width in index units (Parameters::SigmaLOverAzL) */
+    double sigma_r;     /* (Parameters::SigmaLOverAzR) */
+    double nucleus_l;   /* multiplier of the left charge (Parameters::LeftNucleus) */
+    double nucleus_r;   /* (Parameters::RightNucleus) */
+    int center_l;       /* z centre of the left nucleus (Simulation.cpp:344: 0.85*Nz/2-1) */
+    int center_r;       /* (Simulation.cpp:348: 1.15*Nz/2-1) */
+} glasma_mv_params;
+
+/* ---- context ---------------------------------------------------------------------------------- */
+/* Replaces Lattice::Init + Simulation::Init (LATTICE/SUNcVariables.cpp:71-90, Simulation.cpp:115-130):
+ * allocates every field on the current HIP device; U = UOld = 1, E = J = 0, t = 0. */
+int glasma_create(const glasma_params* p, glasma_ctx** out);
+int glasma_destroy(glasma_ctx* ctx);
+const char* glasma_last_error(const glasma_ctx* ctx);   /* ctx may be NULL (creation errors) */
+const char* glasma_version(void);
+/* device bytes held by the context */
+int glasma_memory_bytes(const glasma_ctx* ctx, size_t* bytes);
+
+/* ---- fields (LATTICE/GaugeLinks.cpp, ElectricFields.cpp, ChargeDensity.cpp) ------------------ */
+/* Any pointer may be NULL (that field is left unchanged / not returned). */
+int glasma_set_fields(glasma_ctx* ctx, const double* U, const double* E, const double* Jp, const double* Jm,
+                      const double* JpStat, const double* JmStat);
+int glasma_get_fields(glasma_ctx* ctx, double* U, double* E, double* Jp, double* Jm, double* JpStat,
+                      double* JmStat);
+/* UOld = links at t - dt/2 of the last step (Dynamics::SUNc::SaveOldLink, DYNAMICS/SUNcDynamics.cpp:108-128) */
+int glasma_set_uold(glasma_ctx* ctx, const double* UOld);
+int glasma_get_uold(glasma_ctx* ctx, double* UOld);
+/* Dynamics::t / tSteps (DYNAMICS/SUNcDynamics.cpp:22-50) */
+int glasma_get_time(const glasma_ctx* ctx, double* t, long* tsteps);
+int glasma_set_time(glasma_ctx* ctx, double t, long tsteps);
+
+/* ---- leapfrog (DYNAMICS/SUNcDynamics.cpp:353-378) ---------------------------------------------- */
+/* Dynamics::SUNc::Update applied nsteps times: E update (with the J_z source), J-stat advection,
+ * half-step links, UOld save, J advection, half-step links.  One fused HIP kernel per step.
+ * UOld is written on the last step only (earlier values are overwritten by the reference anyway). */
+int glasma_update(glasma_ctx* ctx, int nsteps);
+/* Same, asynchronous on the context stream (no host synchronisation); for timing loops. */
+int glasma_update_async(glasma_ctx* ctx, int nsteps);
+int glasma_synchronize(glasma_ctx* ctx);
+/* Average duration of the step kernel over the last glasma_update_async/glasma_update call, measured
+ * with HIP events on the context stream (ms per launch). */
+int glasma_last_step_kernel_ms(const glasma_ctx* ctx, double* ms);
+
+/* ---- observables ------------------------------------------------------------------------------ */
+/* Observables::Bulk::Update (OBSERVABLES/BulkObservables.cpp:172-187): out9 = ESqr[3], BSqr[3],
+ * EDotJ[3] (raw sums; T00..SOURCE follow from BulkObservables.cpp:22-62). */
+int glasma_bulk(glasma_ctx* ctx, double* out9);
+/* GaussLaw::UpdateViolation (OBSERVABLES/GaussLawViolation.cpp:16-124): per_z[Nz] (may be NULL),
+ * global max |G^a| and RMS. */
+int glasma_gauss(glasma_ctx* ctx, double* per_z, double* gmax, double* grms);
+/* Unitarity::UpdateViolation (OBSERVABLES/UnitarityViolation.cpp:13-51) */
+int glasma_unitarity(glasma_ctx* ctx, double* umax, double* urms);
+/* EnergyMomentumTensor::CreateMap (OBSERVABLES/EnergyMomentumTensor.cpp:92-170): out[10*z + k] =
+ * T00 TXX TYY TZZ T0X T0Y T0Z TXY TYZ TZX transverse averages per z. */
+int glasma_emt(glasma_ctx* ctx, double* out);
+
+/* ---- MV initial conditions ---------------------------------------------------------------------- */
+/* Charges of both nuclei from one MT19937 stream seeded as gsl_rng_set (seed 0 -> 4357), Gaussian
+ * by gsl_ran_gaussian's polar method; left first, then right, each in y,x,a order scaled by qs
+ * (MISC/RNG/GSLRandomNumberGenerator.cpp:12-33, COLLIDINGOBJECTS/CollidingObjects.cpp:335-342,
+ * Simulation.cpp:344-348).  xi arrays hold Nx*Ny*8 doubles, index a + 8*(x + Nx*y).  Host only. */
+int glasma_draw_charges(unsigned long seed, int nx, int ny, double qs_l, double qs_r, double* xi_l,
+                        double* xi_r);
+/* Simulation::Run's initial state (Simulation.cpp:327-378): SetWilsonLines for both nuclei
+ * (CollidingObjects.cpp:318-371) then SingleSheet (INITIALCONDITIONS/SUNCCURRENTS/RevSingleNucleus.cpp:7-210)
+ * for driver 0 = collision (Simulation.cpp:359-361), 1 = left-moving (LeftNucleusSimulation.cpp:357),
+ * 2 = right-moving (RightNucleusSimulation.cpp:357).  Resets fields and time first. */
+int glasma_mv_initial_state(glasma_ctx* ctx, int driver, const glasma_mv_params* mv, const double* xi_l,
+                            const double* xi_r);
+/* CollidingObjects::SetWilsonLines for one nucleus ('L' or 'R'): V (complex, AoS with ghosts,
+ * 9*Index3D + i + 3j) and the smoothed charge rho (a + 8*Index3D, with ghosts).  Either may be NULL. */
+int glasma_set_wilson_lines(glasma_ctx* ctx, char dir, const glasma_mv_params* mv, const double* xi,
+                            double* V_out, double* rho_out);
+
+/* ---- Coulomb gauge fixing (GAUGETRANSFORMATION/CoulombGaugeFixing.cpp:32-88 with
+ *      ALGORITHMS/FourierAcceleration.cpp:15-148) ------------------------------------------------- */
+/* Fourier-accelerated steepest descent on the gauge transformation G (kept in the context across
+ * calls, identity at creation), until max |Delta| <= tol or maxit iterations (reference: 1e-15, 1000,
+ * alpha = 2).  The gauge-fixed copies U^G, E^G are kept in the context; the dynamical fields are not
+ * modified.  stats7 = iterations, max dev, avg dev, max step, avg step, min functional, mean functional. */
+int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, double* stats7);
+/* GaugeTransformation::Reset / SetIdentity (CoulombGaugeFixing.cpp:26-30) */
+int glasma_gauge_reset(glasma_ctx* ctx);
+/* Gauge-fixed copies (GaugeFixedVariables::U/E, GAUGETRANSFORMATION/GaugeTransformation.cpp:8-21) and G
+ * (complex, 9*Index3D + i + 3j with ghosts).  Any pointer may be NULL. */
+int glasma_get_gauge_fixed(glasma_ctx* ctx, double* UG, double* EG, double* G);
+int glasma_set_gauge_transformation(glasma_ctx* ctx, const double* G);
+
+/* ---- device FFT (replaces FFT/FFT1D.cpp, FFT2D.cpp, FFT3D.cpp; FFTW conventions: forward e^{-2 pi i},
+ *      backward e^{+2 pi i}, unnormalised) -------------------------------------------------------- */
+/* Multi-dimensional complex transform of a host array: dims[0] fastest, `batch` interleaved
+ * components fastest of all (XIndex = a + batch*(x + Nx*(y + Ny*z)), FFT/FFT3D.cpp:41-47);
+ * sign -1 forward, +1 backward.  Runs on the current device; for tests and tools. */
+int glasma_fft_host(double* data, int nd, const int* dims, long batch, int sign);
+
+/* ---- synthetic input (no counterpart in the reference; for benchmarks and tests) ---------------- */
+/* Fills the current links with exp(i link_scale alpha^a t^a), alpha_a uniform in [-1,1), and E, J+,
+ * J-, J+stat, J-stat with uniform values in [-scale, scale), from a counter-based hash of (seed, site). */
+int glasma_fill_random(glasma_ctx* ctx, unsigned long seed, double link_scale, double e_scale, double j_scale);
+
+/* ---- per-site SU(3) algebra on the device, batched (tests and tools) ------------------------- */
+/* op: 0 exp(i c alpha t) [in: n*8 alpha, out: n*18], 1 log [in: n*18 U, out: n*8],
+ *     2 V alpha V^dag [in: n*18 V, in2: n*8, out: n*8], 3 V^dag alpha V [same],
+ *     4 Re tr(i t^a V) [in: n*18, out: n*8]. Complex matrices interleaved, column-major. */
+int glasma_su3_batch(int op, long n, double c, const double* in, const double* in2, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLASMA_H_ */

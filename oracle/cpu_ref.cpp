@@ -1,0 +1,1391 @@
+// cpu_ref — the ORACLE: a plain FP64 C++/OpenMP CPU restatement of the reference glasma solver.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load this library, and only as the checker (or the timed CPU baseline).  The product library
+// (soerenpragya3dglasma_amd/csrc) never includes, links or calls anything in this directory.
+//
+// Layout and loop order follow the reference exactly: array-of-structs with one ghost layer at each
+// z end, Index3D = x + Nx*(y + Ny*(z+1)) with x,y periodic (SC/LATTICE/GaugeLinks.cpp:27-36), SU(3)
+// links as std::complex<double>[9] column-major i+3j (SC/SUNc/SU3/GroupOperations.cpp:7-15), algebra
+// vectors double[8] in Gell-Mann order with t^a = lambda^a/2 (SC/SUNc/SU3/AlgebraOperations.cpp:6-45).
+// SU(2) uses the reference's quaternion form U = u3*1 + i u_k sigma_{k+1} (SC/SUNc/SU2/GroupOperations.cpp).
+// Paths cited as SC/... are under Pragya_workingdirectory_worksonm1/Workingdirectory/SourceCode/.
+//
+// Pinning (what checks this file, see tests/test_oracle_*.py):
+//   * SU(3) products / traces / projections        -> numpy brute force with explicit Gell-Mann matrices
+//   * MatrixIExp (Jacobi eigen-decomposition)       -> scipy.linalg.expm, unitarity, det = 1
+//   * MatrixILog (two-pass Jacobi, principal)       -> exp(log U) = U round trip, scipy.linalg.logm
+//   * (Inverse)AdjointMultiplication, GetAdjoint     -> numpy V A V^dagger, Adj_ab = 2 tr(t^a V t^b V^dagger)
+//   * SU(2) ops                                      -> the SU(2) <-> SU(3) embedding test of a full step
+//   * leapfrog                                       -> gauge covariance, Gauss-law conservation, energy
+//                                                      conservation ~dt^2, time reversal, free Maxwell
+//                                                      dispersion, exact J shift at dt = a_z
+//   * observables                                    -> numpy brute force on tiny lattices
+//   * FFT                                            -> numpy.fft
+//   * MT19937                                        -> 10000th output KAT 4123659995, numpy RandomState
+//   * Gaussian draw (GSL gsl_ran_gaussian)           -> parity unpinned (GSL absent; published polar method)
+//   * Poisson / smoothing / z-evolution              -> single-mode closed forms
+//   * Coulomb gauge fixing                           -> gauge orbit recovery, residual <= tol, invariants
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <omp.h>
+
+typedef std::complex<double> cd;
+static const double SQRT2 = 1.41421356237309504880168872420969808;
+static const double SQRT3 = 1.73205080756887729352744634150587237;
+static const double PI = 3.14159265358979323846264338327950288;
+
+static inline int MODI(long x, long y) { return (int)(x >= 0 ? (x % y) : ((x + y) % y)); }
+template <class T> static inline T SQR(T x) { return x * x; }
+
+// ============================================================================================
+//  Hermitian 3x3 Jacobi eigen-decomposition (oracle-only helper), templated on the real type so that
+//  the exponential and the logarithm can be evaluated in 80-bit extended precision and rounded once to
+//  FP64: the oracle then returns the mathematical function correctly rounded, with no systematic bias
+//  (a bias of a few 1e-16 per exponential accumulates visibly in plaquette sums over many steps).
+//  Cyclic complex Jacobi: each rotation R = D(phase) * Givens zeroes one off-diagonal pair; V
+//  accumulates the rotations so that A = V diag(ev) V^dagger.
+// ============================================================================================
+template <class R>
+static void herm_jacobi3(std::complex<R> A[3][3], std::complex<R> V[3][3], R ev[3]) {
+    typedef std::complex<R> C;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) V[i][j] = (i == j) ? R(1) : R(0);
+    for (int sweep = 0; sweep < 60; sweep++) {
+        R off = std::norm(A[0][1]) + std::norm(A[0][2]) + std::norm(A[1][2]);
+        R dia = std::norm(A[0][0]) + std::norm(A[1][1]) + std::norm(A[2][2]);
+        if (off == R(0) || off <= R(1e-40L) * dia) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                R r = std::abs(A[p][q]);
+                if (r == R(0)) continue;
+                C ph = A[p][q] / r;  // e^{i phi}
+                R app = A[p][p].real(), aqq = A[q][q].real();
+                R th = (aqq - app) / (R(2) * r);
+                R t = (th >= 0 ? R(1) : R(-1)) / (std::fabs(th) + std::sqrt(th * th + R(1)));
+                R c = R(1) / std::sqrt(t * t + R(1)), s = t * c;
+                // R acts on columns p,q so that (R^dag A R)_pq = 0
+                C Rm[3][3];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) Rm[i][j] = (i == j) ? R(1) : R(0);
+                Rm[p][p] = c;
+                Rm[q][q] = c;
+                Rm[p][q] = s * ph;
+                Rm[q][p] = -s * std::conj(ph);
+                C T[3][3], B[3][3];
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        C acc = 0;
+                        for (int k = 0; k < 3; k++) acc += A[i][k] * Rm[k][j];
+                        T[i][j] = acc;
+                    }
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        C acc = 0;
+                        for (int k = 0; k < 3; k++) acc += std::conj(Rm[k][i]) * T[k][j];
+                        B[i][j] = acc;
+                    }
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) A[i][j] = B[i][j];
+                A[p][q] = 0;
+                A[q][p] = 0;
+                for (int i = 0; i < 3; i++) A[i][i] = A[i][i].real();
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) {
+                        C acc = 0;
+                        for (int k = 0; k < 3; k++) acc += V[i][k] * Rm[k][j];
+                        T[i][j] = acc;
+                    }
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) V[i][j] = T[i][j];
+            }
+    }
+    for (int i = 0; i < 3; i++) ev[i] = A[i][i].real();
+}
+
+// ============================================================================================
+//  SU(3) group and algebra operations
+// ============================================================================================
+struct SU3 {
+    typedef cd elem;
+    static const int NC = 3, NG = 8, MS = 9;
+    static inline int MI(int i, int j) { return i + 3 * j; }  // SC/SUNc/SU3/GroupOperations.cpp:13-15
+
+    static void Unit(elem* V) {
+        for (int k = 0; k < 9; k++) V[k] = 0.0;
+        V[0] = V[4] = V[8] = 1.0;
+    }
+    // SC/SUNc/SU3/GroupOperations.cpp:133-196
+    static void UU(const elem* A, const elem* B, elem* C) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cd acc = 0.0;
+                for (int k = 0; k < 3; k++) acc += A[MI(i, k)] * B[MI(k, j)];
+                C[MI(i, j)] = acc;
+            }
+    }
+    static void UD(const elem* A, const elem* B, elem* C) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cd acc = 0.0;
+                for (int k = 0; k < 3; k++) acc += A[MI(i, k)] * std::conj(B[MI(j, k)]);
+                C[MI(i, j)] = acc;
+            }
+    }
+    static void DU(const elem* A, const elem* B, elem* C) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cd acc = 0.0;
+                for (int k = 0; k < 3; k++) acc += std::conj(A[MI(k, i)]) * B[MI(k, j)];
+                C[MI(i, j)] = acc;
+            }
+    }
+    static void DD(const elem* A, const elem* B, elem* C) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cd acc = 0.0;
+                for (int k = 0; k < 3; k++) acc += std::conj(A[MI(k, i)]) * std::conj(B[MI(j, k)]);
+                C[MI(i, j)] = acc;
+            }
+    }
+    static void Dag(const elem* A, elem* C) {
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) C[MI(i, j)] = std::conj(A[MI(j, i)]);
+    }
+    // Re tr(1 - V)  SC/SUNc/SU3/GroupOperations.cpp:71-73
+    static double ReTrIDMinusU(const elem* V) { return 3.0 - V[0].real() - V[4].real() - V[8].real(); }
+    static double ReTr(const elem* V) { return V[0].real() + V[4].real() + V[8].real(); }
+    // Re tr(i t^a V)  SC/SUNc/SU3/GroupOperations.cpp:93-103
+    static void ReTrIGenU(const elem* V, double* r) {
+        r[0] = -0.5 * (V[MI(0, 1)].imag() + V[MI(1, 0)].imag());
+        r[1] = -0.5 * (V[MI(0, 1)].real() - V[MI(1, 0)].real());
+        r[2] = -0.5 * (V[MI(0, 0)].imag() - V[MI(1, 1)].imag());
+        r[3] = -0.5 * (V[MI(0, 2)].imag() + V[MI(2, 0)].imag());
+        r[4] = -0.5 * (V[MI(0, 2)].real() - V[MI(2, 0)].real());
+        r[5] = -0.5 * (V[MI(1, 2)].imag() + V[MI(2, 1)].imag());
+        r[6] = -0.5 * (V[MI(1, 2)].real() - V[MI(2, 1)].real());
+        r[7] = -0.5 * (V[MI(0, 0)].imag() + V[MI(1, 1)].imag() - 2.0 * V[MI(2, 2)].imag()) / SQRT3;
+    }
+    // ||V V^dag - 1||_F  SC/SUNc/SU3/GroupOperations.cpp:337-349
+    static double UnitarityNorm(const elem* V) {
+        elem C[9];
+        UD(V, V, C);
+        double n = 0.0;
+        for (int k = 0; k < 9; k++) n += std::norm(C[k] - ((k % 4 == 0) ? cd(1.0) : cd(0.0)));
+        return std::sqrt(n);
+    }
+    // t^a alpha^a as a 3x3 matrix A[i][j]  SC/SUNc/SU3/AlgebraOperations.cpp:31-45
+    static void MatrixForm(const double* al, cd A[3][3]) {
+        A[0][0] = 0.5 * (al[2] + al[7] / SQRT3);
+        A[1][1] = -0.5 * (al[2] - al[7] / SQRT3);
+        A[2][2] = -al[7] / SQRT3;
+        A[1][0] = 0.5 * cd(al[0], al[1]);
+        A[2][0] = 0.5 * cd(al[3], al[4]);
+        A[2][1] = 0.5 * cd(al[5], al[6]);
+        A[0][1] = std::conj(A[1][0]);
+        A[0][2] = std::conj(A[2][0]);
+        A[1][2] = std::conj(A[2][1]);
+    }
+    // algebra components of a Hermitian matrix H: alpha^a = 2 tr(t^a H)
+    static void AlgebraComponents(const cd H[3][3], double* al) {
+        al[0] = H[1][0].real() + H[0][1].real();
+        al[1] = H[1][0].imag() - H[0][1].imag();
+        al[2] = H[0][0].real() - H[1][1].real();
+        al[3] = H[2][0].real() + H[0][2].real();
+        al[4] = H[2][0].imag() - H[0][2].imag();
+        al[5] = H[2][1].real() + H[1][2].real();
+        al[6] = H[2][1].imag() - H[1][2].imag();
+        al[7] = (H[0][0].real() + H[1][1].real() - 2.0 * H[2][2].real()) / SQRT3;
+    }
+    // exp(i c t^a alpha^a)  (SC/SUNc/SU3/AlgebraOperations.cpp:48-89, which diagonalises the matrix).
+    // Here: the Cayley-Hamilton closed form exp(iQ) = f0 + f1 Q + f2 Q^2 with f_j from the eigenvalues
+    // of Q (Morningstar & Peardon, Phys. Rev. D 69, 054501 (2004), eqs. 23-34), evaluated in long double
+    // and rounded once, so the result is the exponential correctly rounded to FP64.
+    static void MatrixIExp(double c, const double* al, elem* X) {
+        std::complex<long double> XL[3][3];
+        ExpILD(c, al, XL);
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) X[MI(i, k)] = cd((double)XL[i][k].real(), (double)XL[i][k].imag());
+    }
+    static void ExpILD(double c, const double* al, std::complex<long double> Xo[3][3]) {
+        typedef long double R;
+        typedef std::complex<R> C;
+        const R s3 = std::sqrt(R(3));
+        C Q[3][3], Q2[3][3];
+        Q[0][0] = R(c) * R(0.5) * (R(al[2]) + R(al[7]) / s3);
+        Q[1][1] = -R(c) * R(0.5) * (R(al[2]) - R(al[7]) / s3);
+        Q[2][2] = -R(c) * R(al[7]) / s3;
+        Q[1][0] = R(c) * R(0.5) * C(al[0], al[1]);
+        Q[2][0] = R(c) * R(0.5) * C(al[3], al[4]);
+        Q[2][1] = R(c) * R(0.5) * C(al[5], al[6]);
+        Q[0][1] = std::conj(Q[1][0]);
+        Q[0][2] = std::conj(Q[2][0]);
+        Q[1][2] = std::conj(Q[2][1]);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                C acc = 0;
+                for (int k = 0; k < 3; k++) acc += Q[i][k] * Q[k][j];
+                Q2[i][j] = acc;
+            }
+        R c1 = R(0.5) * (Q2[0][0].real() + Q2[1][1].real() + Q2[2][2].real());
+        if (!(c1 > 0)) {
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) Xo[i][k] = (i == k) ? R(1) : R(0);
+            return;
+        }
+        C det = Q[0][0] * (Q[1][1] * Q[2][2] - Q[1][2] * Q[2][1]) - Q[0][1] * (Q[1][0] * Q[2][2] - Q[1][2] * Q[2][0]) +
+                Q[0][2] * (Q[1][0] * Q[2][1] - Q[1][1] * Q[2][0]);
+        R c0 = det.real();
+        bool neg = c0 < 0;
+        if (neg) c0 = -c0;
+        R c0max = R(2) * std::pow(c1 / R(3), R(1.5));
+        R ratio = c0max > 0 ? c0 / c0max : R(1);
+        if (ratio > 1) ratio = 1;
+        R th = std::acos(ratio);
+        R u = std::sqrt(c1 / R(3)) * std::cos(th / R(3));
+        R w = std::sqrt(c1) * std::sin(th / R(3));
+        R xi0 = (std::fabs(w) < R(1e-4)) ? R(1) - w * w / R(6) * (R(1) - w * w / R(20)) : std::sin(w) / w;
+        C em = std::exp(C(0, -u)), e2 = std::exp(C(0, 2 * u));
+        C h0 = (u * u - w * w) * e2 + em * C(R(8) * u * u * std::cos(w), R(2) * u * (R(3) * u * u + w * w) * xi0);
+        C h1 = R(2) * u * e2 - em * C(R(2) * u * std::cos(w), -(R(3) * u * u - w * w) * xi0);
+        C h2 = e2 - em * C(std::cos(w), R(3) * u * xi0);
+        R den = R(9) * u * u - w * w;
+        C f0 = h0 / den, f1 = h1 / den, f2 = h2 / den;
+        if (neg) { f0 = std::conj(f0); f1 = -std::conj(f1); f2 = std::conj(f2); }
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) {
+                Xo[i][k] = f1 * Q[i][k] + f2 * Q2[i][k] + (i == k ? f0 : C(0));
+            }
+    }
+    // U <- exp(i c alpha^a t^a) U with the exponential and the product both in long double and the
+    // result rounded once (the link update of SC/DYNAMICS/SUNcDynamics.cpp:86-96, correctly rounded).
+    static void ExpIMul(double c, const double* al, elem* U) {
+        typedef long double R;
+        typedef std::complex<R> C;
+        C X[3][3];
+        ExpILD(c, al, X);
+        C T[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                C acc = 0;
+                for (int k = 0; k < 3; k++) acc += X[i][k] * C(U[MI(k, j)].real(), U[MI(k, j)].imag());
+                T[i][j] = acc;
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) U[MI(i, j)] = cd((double)T[i][j].real(), (double)T[i][j].imag());
+    }
+    // Solves exp(i C A^a t^a) = U on the principal traceless branch.  Absent from the reference for SU(3);
+    // its meaning is fixed by the SU(2) version SC/SUNc/SU2/AlgebraOperations.cpp:92-107.
+    // Eigenvectors: Jacobi on K = (U - U^dag)/2i, then Jacobi on the Hermitian part in that basis, which
+    // separates eigenvalues that share sin(phi).  Phases in (-pi, pi]; if they sum to +-2pi the largest
+    // (smallest) is shifted by -2pi (+2pi) so that the logarithm is traceless.
+    static void MatrixILog(double Cc, const elem* U, double* out) {
+        typedef long double R;
+        typedef std::complex<R> C;
+        const R PIL = 3.141592653589793238462643383279502884L;
+        C Ul[3][3], K[3][3], W[3][3], W2[3][3], Hc[3][3], T[3][3];
+        R ev[3];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) Ul[i][j] = C(U[MI(i, j)].real(), U[MI(i, j)].imag());
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) K[i][j] = (Ul[i][j] - std::conj(Ul[j][i])) / C(0, 2);
+        herm_jacobi3<R>(K, W, ev);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                C acc = 0;
+                for (int k = 0; k < 3; k++)
+                    for (int l = 0; l < 3; l++) acc += std::conj(W[k][i]) * R(0.5) * (Ul[k][l] + std::conj(Ul[l][k])) * W[l][j];
+                Hc[i][j] = acc;
+            }
+        herm_jacobi3<R>(Hc, W2, ev);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                C acc = 0;
+                for (int k = 0; k < 3; k++) acc += W[i][k] * W2[k][j];
+                T[i][j] = acc;
+            }
+        R ph[3];
+        for (int j = 0; j < 3; j++) {
+            C d = 0;
+            for (int k = 0; k < 3; k++)
+                for (int l = 0; l < 3; l++) d += std::conj(T[k][j]) * Ul[k][l] * T[l][j];
+            ph[j] = std::atan2(d.imag(), d.real());
+        }
+        R sm = ph[0] + ph[1] + ph[2];
+        if (sm > PIL) {
+            int m = (ph[0] >= ph[1] && ph[0] >= ph[2]) ? 0 : (ph[1] >= ph[2] ? 1 : 2);
+            ph[m] -= 2 * PIL;
+        } else if (sm < -PIL) {
+            int m = (ph[0] <= ph[1] && ph[0] <= ph[2]) ? 0 : (ph[1] <= ph[2] ? 1 : 2);
+            ph[m] += 2 * PIL;
+        }
+        C H[3][3];
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) {
+                C acc = 0;
+                for (int j = 0; j < 3; j++) acc += T[i][j] * ph[j] * std::conj(T[k][j]);
+                H[i][k] = acc;
+            }
+        const R s3 = std::sqrt(R(3));
+        R al[8];
+        al[0] = H[1][0].real() + H[0][1].real();
+        al[1] = H[1][0].imag() - H[0][1].imag();
+        al[2] = H[0][0].real() - H[1][1].real();
+        al[3] = H[2][0].real() + H[0][2].real();
+        al[4] = H[2][0].imag() - H[0][2].imag();
+        al[5] = H[2][1].real() + H[1][2].real();
+        al[6] = H[2][1].imag() - H[1][2].imag();
+        al[7] = (H[0][0].real() + H[1][1].real() - 2 * H[2][2].real()) / s3;
+        for (int a = 0; a < 8; a++) out[a] = (double)(al[a] / R(Cc));
+    }
+    // V (alpha^b t^b) V^dag, components.  Missing from the reference for SU(3); meaning fixed by
+    // SC/SUNc/SU2/AlgebraOperations.cpp:110-116 (V.E.VDagger).
+    static void AdjointMultiplication(const elem* V, const double* al, double* out) {
+        cd A[3][3], T[3][3], H[3][3];
+        MatrixForm(al, A);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cd acc = 0;
+                for (int k = 0; k < 3; k++) acc += V[MI(i, k)] * A[k][j];
+                T[i][j] = acc;
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                cd acc = 0;
+                for (int k = 0; k < 3; k++) acc += T[i][k] * std::conj(V[MI(j, k)]);
+                H[i][j] = acc;
+            }
+        AlgebraComponents(H, out);
+    }
+    // V^dag (alpha^b t^b) V; SU(2) meaning at SC/SUNc/SU2/AlgebraOperations.cpp:119-124
+    static void InverseAdjointMultiplication(const elem* V, const double* al, double* out) {
+        elem Vd[9];
+        Dag(V, Vd);
+        AdjointMultiplication(Vd, al, out);
+    }
+    // Adj_ab = 2 tr(t^a V t^b V^dag)  SC/SUNc/SU3/GroupOperations.cpp:199-331 (computed here by its definition)
+    static void GetAdjoint(const elem* V, double* adj) {
+        for (int b = 0; b < 8; b++) {
+            double e[8] = {0, 0, 0, 0, 0, 0, 0, 0}, r[8];
+            e[b] = 1.0;
+            AdjointMultiplication(V, e, r);
+            for (int a = 0; a < 8; a++) adj[a + 8 * b] = r[a];
+        }
+    }
+};
+
+// ============================================================================================
+//  SU(2) group and algebra operations (the reference's shipped gauge group), U = u3 + i u_k sigma_{k+1}
+//  Every formula restates SC/SUNc/SU2/GroupOperations.cpp / AlgebraOperations.cpp.
+// ============================================================================================
+struct SU2 {
+    typedef double elem;
+    static const int NC = 2, NG = 3, MS = 4;
+    static void Unit(elem* V) { V[0] = V[1] = V[2] = 0.0; V[3] = 1.0; }
+    static void UU(const elem* A, const elem* B, elem* C) {  // :103-110
+        C[0] = A[3] * B[0] + A[0] * B[3] - (A[1] * B[2] - A[2] * B[1]);
+        C[1] = A[3] * B[1] + A[1] * B[3] + (A[0] * B[2] - A[2] * B[0]);
+        C[2] = A[3] * B[2] + A[2] * B[3] - (A[0] * B[1] - A[1] * B[0]);
+        C[3] = A[3] * B[3] - A[0] * B[0] - A[1] * B[1] - A[2] * B[2];
+    }
+    static void UD(const elem* A, const elem* B, elem* C) {  // :113-119
+        C[0] = -A[3] * B[0] + A[0] * B[3] + (A[1] * B[2] - A[2] * B[1]);
+        C[1] = -A[3] * B[1] + A[1] * B[3] - (A[0] * B[2] - A[2] * B[0]);
+        C[2] = -A[3] * B[2] + A[2] * B[3] + (A[0] * B[1] - A[1] * B[0]);
+        C[3] = A[3] * B[3] + A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+    }
+    static void DU(const elem* A, const elem* B, elem* C) {  // :122-128
+        C[0] = A[3] * B[0] - A[0] * B[3] + (A[1] * B[2] - A[2] * B[1]);
+        C[1] = A[3] * B[1] - A[1] * B[3] - (A[0] * B[2] - A[2] * B[0]);
+        C[2] = A[3] * B[2] - A[2] * B[3] + (A[0] * B[1] - A[1] * B[0]);
+        C[3] = A[3] * B[3] + A[0] * B[0] + A[1] * B[1] + A[2] * B[2];
+    }
+    static void DD(const elem* A, const elem* B, elem* C) {  // :131-137
+        C[0] = -A[3] * B[0] - A[0] * B[3] - (A[1] * B[2] - A[2] * B[1]);
+        C[1] = -A[3] * B[1] - A[1] * B[3] + (A[0] * B[2] - A[2] * B[0]);
+        C[2] = -A[3] * B[2] - A[2] * B[3] - (A[0] * B[1] - A[1] * B[0]);
+        C[3] = A[3] * B[3] - A[0] * B[0] - A[1] * B[1] - A[2] * B[2];
+    }
+    static double ReTrIDMinusU(const elem* V) { return 2.0 * (1.0 - V[3]); }  // :70-72
+    static double ReTr(const elem* V) { return 2.0 * V[3]; }
+    static void ReTrIGenU(const elem* V, double* r) { r[0] = -V[0]; r[1] = -V[1]; r[2] = -V[2]; }  // :78-84
+    static double UnitarityNorm(const elem* V) {  // :160-172
+        elem C[4];
+        UD(V, V, C);
+        double n = SQR(C[0]) + SQR(C[1]) + SQR(C[2]) + SQR(C[3] - 1.0);
+        return std::sqrt(n);
+    }
+    static void MatrixIExp(double c, const double* al, elem* X) {  // SC/SUNc/SU2/AlgebraOperations.cpp:39-57
+        double d = std::sqrt(SQR(c * al[0]) + SQR(c * al[1]) + SQR(c * al[2]));
+        double arg = 0.5 * d;
+        if (d != 0) {
+            double M = c * std::sin(arg) / d;
+            X[0] = M * al[0]; X[1] = M * al[1]; X[2] = M * al[2]; X[3] = std::cos(arg);
+        } else { X[0] = X[1] = X[2] = 0.0; X[3] = 1.0; }
+    }
+    static void ExpIMul(double c, const double* al, elem* U) {  // SC/DYNAMICS/SUNcDynamics.cpp:86-96
+        elem X[4], Old[4];
+        MatrixIExp(c, al, X);
+        std::memcpy(Old, U, sizeof(Old));
+        UU(X, Old, U);
+    }
+    static void MatrixILog(double C, const elem* U, double* L) {  // :92-107
+        double d = std::sqrt(SQR(U[0]) + SQR(U[1]) + SQR(U[2]));
+        if (d != 0) {
+            double arg = std::atan2(d, U[3]);
+            double M = 2.0 * arg / (C * d);
+            L[0] = M * U[0]; L[1] = M * U[1]; L[2] = M * U[2];
+        } else { L[0] = L[1] = L[2] = 0.0; }
+    }
+    static void AdjointMultiplication(const elem* V, const double* E, double* N) {  // :110-116
+        N[0] = E[0] * (V[0] * V[0] - V[1] * V[1] - V[2] * V[2] + V[3] * V[3]) + 2.0 * (E[2] * (V[0] * V[2] - V[1] * V[3]) + E[1] * (V[0] * V[1] + V[2] * V[3]));
+        N[1] = E[1] * (V[1] * V[1] - V[2] * V[2] - V[0] * V[0] + V[3] * V[3]) + 2.0 * (E[2] * (V[1] * V[2] + V[0] * V[3]) + E[0] * (V[0] * V[1] - V[2] * V[3]));
+        N[2] = E[2] * (V[2] * V[2] - V[0] * V[0] - V[1] * V[1] + V[3] * V[3]) + 2.0 * (E[1] * (V[1] * V[2] - V[0] * V[3]) + E[0] * (V[0] * V[2] + V[1] * V[3]));
+    }
+    static void InverseAdjointMultiplication(const elem* V, const double* E, double* N) {  // :119-124
+        N[0] = E[0] * (V[0] * V[0] - V[1] * V[1] - V[2] * V[2] + V[3] * V[3]) + 2.0 * (E[2] * (V[0] * V[2] + V[1] * V[3]) + E[1] * (V[0] * V[1] - V[2] * V[3]));
+        N[1] = E[1] * (V[1] * V[1] - V[2] * V[2] - V[0] * V[0] + V[3] * V[3]) + 2.0 * (E[2] * (V[1] * V[2] - V[0] * V[3]) + E[0] * (V[0] * V[1] + V[2] * V[3]));
+        N[2] = E[2] * (V[2] * V[2] - V[0] * V[0] - V[1] * V[1] + V[3] * V[3]) + 2.0 * (E[1] * (V[1] * V[2] + V[0] * V[3]) + E[0] * (V[0] * V[2] - V[1] * V[3]));
+    }
+};
+
+// four-link products  SC/SUNc/AdvancedOperations.cpp:175-228
+template <class G> static void UUDD(const typename G::elem* a, const typename G::elem* b, const typename G::elem* c, const typename G::elem* d, typename G::elem* o) {
+    typename G::elem t1[G::MS], t2[G::MS];
+    G::UU(a, b, t1); G::DD(c, d, t2); G::UU(t1, t2, o);
+}
+template <class G> static void DDUU(const typename G::elem* a, const typename G::elem* b, const typename G::elem* c, const typename G::elem* d, typename G::elem* o) {
+    typename G::elem t1[G::MS], t2[G::MS];
+    G::DD(a, b, t1); G::UU(c, d, t2); G::UU(t1, t2, o);
+}
+template <class G> static void UDDU(const typename G::elem* a, const typename G::elem* b, const typename G::elem* c, const typename G::elem* d, typename G::elem* o) {
+    typename G::elem t1[G::MS], t2[G::MS];
+    G::UD(a, b, t1); G::DU(c, d, t2); G::UU(t1, t2, o);
+}
+template <class G> static void DUUD(const typename G::elem* a, const typename G::elem* b, const typename G::elem* c, const typename G::elem* d, typename G::elem* o) {
+    typename G::elem t1[G::MS], t2[G::MS];
+    G::DU(a, b, t1); G::UD(c, d, t2); G::UU(t1, t2, o);
+}
+
+// ============================================================================================
+//  Lattice with z ghost layers (SC/LATTICE/*.cpp).  bc: 0 open (edge copy), 1 periodic.
+// ============================================================================================
+template <class G> struct Lat {
+    typedef typename G::elem elem;
+    int N[3];
+    double a[3], aCube;
+    int bc;
+    long plane() const { return (long)N[0] * N[1]; }
+    long idx3(int x, int y, int z) const {
+        if (z < -1 || z > N[2]) { fprintf(stderr, "oracle: z %d out of range\n", z); abort(); }
+        return MODI(x, N[0]) + (long)N[0] * (MODI(y, N[1]) + (long)N[1] * (z + 1));
+    }
+    elem* Ul(elem* U, int x, int y, int z, int mu) const { return U + G::MS * (mu + 3 * idx3(x, y, z)); }
+    double* El(double* E, int x, int y, int z, int mu) const { return E + G::NG * (mu + 3 * idx3(x, y, z)); }
+    double* Jl(double* J, int x, int y, int z) const { return J + G::NG * idx3(x, y, z); }
+    // SynchronizeGhostCells for an array with `rec` scalars per site (SC/LATTICE/GaugeLinks.cpp:59-73)
+    template <class T> void sync(T* A, long rec) const {
+        long P = plane() * rec;
+        if (bc == 0) {
+            std::memcpy(A, A + P, P * sizeof(T));
+            std::memcpy(A + (long)(N[2] + 1) * P, A + (long)N[2] * P, P * sizeof(T));
+        } else {
+            std::memcpy(A, A + (long)N[2] * P, P * sizeof(T));
+            std::memcpy(A + (long)(N[2] + 1) * P, A + P, P * sizeof(T));
+        }
+    }
+    void syncU(elem* U) const { sync(U, 3L * G::MS); }
+    void syncE(double* E) const { sync(E, 3L * G::NG); }
+    void syncJ(double* J) const { sync(J, (long)G::NG); }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Leapfrog: SC/DYNAMICS/SUNcDynamics.cpp
+// ---------------------------------------------------------------------------------------------
+template <class G> static void UpdateElectricFields(const Lat<G>& L, typename G::elem* U, double* E, double* Jp, double* Jm, double dt) {
+    typedef typename G::elem elem;
+    const int NG = G::NG;
+    double cE[3], gamma = -2.0 * dt * L.aCube;  // :144-149
+    for (int mu = 0; mu < 3; mu++) cE[mu] = 1.0 / SQR(L.a[mu]);
+    L.syncU(U);  // :151
+#pragma omp parallel for
+    for (int z = 0; z < L.N[2]; z++)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                elem Uxy[G::MS], Uyz[G::MS], Uzx[G::MS], UMxy[G::MS], UMyz[G::MS], UMzx[G::MS], UxMy[G::MS], UyMz[G::MS], UzMx[G::MS];
+                // SC/CALC/Plaquettes.cpp:83-97
+                UUDD<G>(L.Ul(U, x, y, z, 0), L.Ul(U, x + 1, y, z, 1), L.Ul(U, x, y + 1, z, 0), L.Ul(U, x, y, z, 1), Uxy);
+                UUDD<G>(L.Ul(U, x, y, z, 1), L.Ul(U, x, y + 1, z, 2), L.Ul(U, x, y, z + 1, 1), L.Ul(U, x, y, z, 2), Uyz);
+                UUDD<G>(L.Ul(U, x, y, z, 2), L.Ul(U, x, y, z + 1, 0), L.Ul(U, x + 1, y, z, 2), L.Ul(U, x, y, z, 0), Uzx);
+                UDDU<G>(L.Ul(U, x, y, z, 1), L.Ul(U, x - 1, y + 1, z, 0), L.Ul(U, x - 1, y, z, 1), L.Ul(U, x - 1, y, z, 0), UMxy);
+                UDDU<G>(L.Ul(U, x, y, z, 2), L.Ul(U, x, y - 1, z + 1, 1), L.Ul(U, x, y - 1, z, 2), L.Ul(U, x, y - 1, z, 1), UMyz);
+                UDDU<G>(L.Ul(U, x, y, z, 0), L.Ul(U, x + 1, y, z - 1, 2), L.Ul(U, x, y, z - 1, 0), L.Ul(U, x, y, z - 1, 2), UMzx);
+                DUUD<G>(L.Ul(U, x, y - 1, z, 1), L.Ul(U, x, y - 1, z, 0), L.Ul(U, x + 1, y - 1, z, 1), L.Ul(U, x, y, z, 0), UxMy);
+                DUUD<G>(L.Ul(U, x, y, z - 1, 2), L.Ul(U, x, y, z - 1, 1), L.Ul(U, x, y + 1, z - 1, 2), L.Ul(U, x, y, z, 1), UyMz);
+                DUUD<G>(L.Ul(U, x - 1, y, z, 0), L.Ul(U, x - 1, y, z, 2), L.Ul(U, x - 1, y, z + 1, 0), L.Ul(U, x, y, z, 2), UzMx);
+                double tXY[8], tYZ[8], tZX[8], tMXY[8], tMYZ[8], tMZX[8], tXMY[8], tYMZ[8], tZMX[8];
+                G::ReTrIGenU(Uxy, tXY); G::ReTrIGenU(Uyz, tYZ); G::ReTrIGenU(Uzx, tZX);
+                G::ReTrIGenU(UMxy, tMXY); G::ReTrIGenU(UMyz, tMYZ); G::ReTrIGenU(UMzx, tMZX);
+                G::ReTrIGenU(UxMy, tXMY); G::ReTrIGenU(UyMz, tYMZ); G::ReTrIGenU(UzMx, tZMX);
+                for (int a = 0; a < NG; a++) {  // :179-185
+                    L.El(E, x, y, z, 0)[a] += gamma * cE[0] * (cE[1] * (tXY[a] - tXMY[a]) - cE[2] * (tZX[a] - tMZX[a]));
+                    L.El(E, x, y, z, 1)[a] += gamma * cE[1] * (cE[2] * (tYZ[a] - tYMZ[a]) - cE[0] * (tXY[a] - tMXY[a]));
+                    L.El(E, x, y, z, 2)[a] += gamma * cE[2] * (cE[0] * (tZX[a] - tZMX[a]) - cE[1] * (tYZ[a] - tMYZ[a]));
+                }
+                for (int a = 0; a < NG; a++) {  // :188-193
+                    double Jz = (L.Jl(Jp, x, y, z)[a] - L.Jl(Jm, x, y, z)[a]) / SQRT2;
+                    L.El(E, x, y, z, 2)[a] += -(dt / L.a[2]) * Jz;
+                }
+            }
+    L.syncE(E);
+}
+
+template <class G> static void UpdateJStat(const Lat<G>& L, typename G::elem* U, double* JpNew, double* JmNew, double* JpOld, double* JmOld, double dt) {  // :205-236
+    const int NG = G::NG;
+#pragma omp parallel for
+    for (int z = 0; z < L.N[2]; z++)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                double JpDown[8], JmDown[8];
+                G::InverseAdjointMultiplication(L.Ul(U, x, y, z - 1, 2), L.Jl(JpOld, x, y, z - 1), JpDown);
+                G::InverseAdjointMultiplication(L.Ul(U, x, y, z - 1, 2), L.Jl(JmOld, x, y, z - 1), JmDown);
+                for (int a = 0; a < NG; a++) {
+                    L.Jl(JpNew, x, y, z)[a] += -(dt / L.a[2]) * (L.Jl(JpOld, x, y, z)[a] - JpDown[a]);
+                    L.Jl(JmNew, x, y, z)[a] += (dt / L.a[2]) * (L.Jl(JmOld, x, y, z)[a] - JmDown[a]);
+                }
+            }
+    L.syncJ(JmNew);
+    L.syncJ(JpNew);
+}
+
+template <class G> static void UpdateGaugeLinks(const Lat<G>& L, typename G::elem* U, double* E, double dt) {  // :62-106
+    double cU[3];
+    for (int mu = 0; mu < 3; mu++) cU[mu] = -dt * SQR(L.a[mu]) / L.aCube;
+#pragma omp parallel for
+    for (int z = 0; z < L.N[2]; z++)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++)
+                for (int mu = 0; mu < 3; mu++) G::ExpIMul(cU[mu], L.El(E, x, y, z, mu), L.Ul(U, x, y, z, mu));
+    L.syncU(U);
+}
+
+template <class G> static void SaveOldLink(const Lat<G>& L, typename G::elem* U, typename G::elem* UOld) {  // :108-128
+    L.syncU(U);
+    long n = (long)(L.N[2] + 2) * L.plane() * 3 * G::MS;
+    std::memcpy(UOld, U, n * sizeof(typename G::elem));
+    L.syncU(UOld);
+}
+
+template <class G> static void UpdateJ(const Lat<G>& L, typename G::elem* U, double* JpNew, double* JmNew, double* JpOld, double* JmOld, double dt) {  // :238-269
+    const int NG = G::NG;
+#pragma omp parallel for
+    for (int z = 0; z < L.N[2]; z++)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                double JpUp[8], JmUp[8];
+                G::AdjointMultiplication(L.Ul(U, x, y, z, 2), L.Jl(JpOld, x, y, z + 1), JpUp);
+                G::AdjointMultiplication(L.Ul(U, x, y, z, 2), L.Jl(JmOld, x, y, z + 1), JmUp);
+                for (int a = 0; a < NG; a++) {
+                    L.Jl(JpNew, x, y, z)[a] += -(dt / L.a[2]) * (JpUp[a] - L.Jl(JpOld, x, y, z)[a]);
+                    L.Jl(JmNew, x, y, z)[a] += (dt / L.a[2]) * (JmUp[a] - L.Jl(JmOld, x, y, z)[a]);
+                }
+            }
+    L.syncJ(JmNew);
+    L.syncJ(JpNew);
+}
+
+// Dynamics::SUNc::Update  SC/DYNAMICS/SUNcDynamics.cpp:353-378
+template <class G> static void Update(const Lat<G>& L, typename G::elem* U, typename G::elem* UOld, double* E, double* Jp, double* Jm, double* JpS, double* JmS, double dt) {
+    UpdateElectricFields<G>(L, U, E, Jp, Jm, dt);
+    UpdateJStat<G>(L, U, JpS, JmS, Jp, Jm, dt);
+    UpdateGaugeLinks<G>(L, U, E, 0.5 * dt);
+    SaveOldLink<G>(L, U, UOld);
+    UpdateJ<G>(L, U, Jp, Jm, JpS, JmS, dt);
+    UpdateGaugeLinks<G>(L, U, E, 0.5 * dt);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Observables
+// ---------------------------------------------------------------------------------------------
+// Observables::Bulk::Update  SC/OBSERVABLES/BulkObservables.cpp:65-187.  out = ESqr[3], BSqr[3], EDotJ[3]
+template <class G> static void Bulk(const Lat<G>& L, typename G::elem* U, double* E, double* Jp, double* Jm, double* out) {
+    typedef typename G::elem elem;
+    double B0 = 0, B1 = 0, B2 = 0, E0 = 0, E1 = 0, E2 = 0, S2 = 0;
+#pragma omp parallel for reduction(+ : B0, B1, B2, E0, E1, E2, S2)
+    for (int z = 0; z < L.N[2]; z++)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                elem Uxy[G::MS], Uyz[G::MS], Uzx[G::MS];
+                UUDD<G>(L.Ul(U, x, y, z, 0), L.Ul(U, x + 1, y, z, 1), L.Ul(U, x, y + 1, z, 0), L.Ul(U, x, y, z, 1), Uxy);
+                UUDD<G>(L.Ul(U, x, y, z, 1), L.Ul(U, x, y + 1, z, 2), L.Ul(U, x, y, z + 1, 1), L.Ul(U, x, y, z, 2), Uyz);
+                UUDD<G>(L.Ul(U, x, y, z, 2), L.Ul(U, x, y, z + 1, 0), L.Ul(U, x + 1, y, z, 2), L.Ul(U, x, y, z, 0), Uzx);
+                B0 += 4.0 * G::ReTrIDMinusU(Uyz);
+                B1 += 4.0 * G::ReTrIDMinusU(Uzx);
+                B2 += 4.0 * G::ReTrIDMinusU(Uxy);
+                for (int a = 0; a < G::NG; a++) {
+                    E0 += SQR(L.El(E, x, y, z, 0)[a]);
+                    E1 += SQR(L.El(E, x, y, z, 1)[a]);
+                    E2 += SQR(L.El(E, x, y, z, 2)[a]);
+                    S2 += L.El(E, x, y, z, 2)[a] * (L.Jl(Jp, x, y, z)[a] - L.Jl(Jm, x, y, z)[a]) / SQRT2;
+                }
+            }
+    out[0] = E0; out[1] = E1; out[2] = E2;
+    out[3] = B0; out[4] = B1; out[5] = B2;
+    out[6] = 0.0; out[7] = 0.0; out[8] = S2;
+}
+
+// COMPUTE_GAUSS_VIOLATION SC/CALC/GaussViolation.cpp:13-23 ; GaussLaw::UpdateViolation
+// SC/OBSERVABLES/GaussLawViolation.cpp:16-124.  per_z[z] = (1/N_perp^2) sum_perp sum_a G^2.
+template <class G> static void Gauss(const Lat<G>& L, typename G::elem* U, double* E, double* JpS, double* JmS, double* per_z, double* gmax, double* grms) {
+    L.syncJ(JpS); L.syncJ(JmS); L.syncU(U); L.syncE(E);  // :34-38
+    double norm = 1.0 / (L.a[0] * L.a[1] * L.a[2]);
+    double mx = 0.0, sq = 0.0;
+    std::vector<double> pz(L.N[2], 0.0);
+#pragma omp parallel for reduction(+ : sq) reduction(max : mx)
+    for (int z = 0; z < L.N[2]; z++) {
+        double acc = 0.0;
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                double Xd[8], Yd[8], Zd[8], loc[8];
+                G::InverseAdjointMultiplication(L.Ul(U, x - 1, y, z, 0), L.El(E, x - 1, y, z, 0), Xd);
+                G::InverseAdjointMultiplication(L.Ul(U, x, y - 1, z, 1), L.El(E, x, y - 1, z, 1), Yd);
+                G::InverseAdjointMultiplication(L.Ul(U, x, y, z - 1, 2), L.El(E, x, y, z - 1, 2), Zd);
+                double v = 0.0;
+                for (int a = 0; a < G::NG; a++) {
+                    loc[a] = norm * ((L.El(E, x, y, z, 0)[a] - Xd[a]) + (L.El(E, x, y, z, 1)[a] - Yd[a]) + (L.El(E, x, y, z, 2)[a] - Zd[a]) - (L.Jl(JpS, x, y, z)[a] + L.Jl(JmS, x, y, z)[a]) / SQRT2);
+                    v += SQR(loc[a]);
+                    mx = std::max(mx, std::fabs(loc[a]));
+                    sq += SQR(loc[a]);
+                }
+                acc += v;
+            }
+        pz[z] = acc / ((double)L.N[0] * L.N[1]);
+    }
+    if (per_z) for (int z = 0; z < L.N[2]; z++) per_z[z] = pz[z];
+    *gmax = mx;
+    *grms = std::sqrt(sq / (G::NG * (double)L.N[0] * L.N[1] * L.N[2]));
+}
+
+// Observables::Unitarity  SC/OBSERVABLES/UnitarityViolation.cpp:13-51
+template <class G> static void Unitarity(const Lat<G>& L, typename G::elem* U, double* umax, double* urms) {
+    double mx = 0.0, sq = 0.0;
+#pragma omp parallel for reduction(+ : sq) reduction(max : mx)
+    for (int z = 0; z < L.N[2]; z++)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++)
+                for (int mu = 0; mu < 3; mu++) {
+                    double v = G::UnitarityNorm(L.Ul(U, x, y, z, mu));
+                    mx = std::max(mx, v);
+                    sq += v * v;
+                }
+    *umax = mx;
+    *urms = std::sqrt(sq / (3.0 * L.N[0] * L.N[1] * L.N[2]));
+}
+
+// EnergyMomentumTensor::CreateMap  SC/OBSERVABLES/EnergyMomentumTensor.cpp:92-170 with
+// COMPUTE_AVG_FIELD_STRENGTH SC/CALC/AvgFieldStrength.cpp:68-145.  B from the clover on UOld, E averaged
+// over x and x-mu (parallel transported by the current U for the colour vectors).
+// out[10*z + k]: T00 TXX TYY TZZ T0X T0Y T0Z TXY TYZ TZX
+template <class G> static void EMT(const Lat<G>& L, typename G::elem* U, typename G::elem* UO, double* E, double* out) {
+    typedef typename G::elem elem;
+    const int NG = G::NG;
+    L.syncU(UO); L.syncU(U);
+    double q[3];
+    for (int mu = 0; mu < 3; mu++) q[mu] = L.a[mu] / L.aCube;
+    double inv = 1.0 / ((double)L.N[0] * L.N[1]);
+    for (int z = 0; z < L.N[2]; z++) {
+        double T00 = 0, TXX = 0, TYY = 0, TZZ = 0, T0X = 0, T0Y = 0, T0Z = 0, TXY = 0, TYZ = 0, TZX = 0;
+#pragma omp parallel for reduction(+ : T00, TXX, TYY, TZZ, T0X, T0Y, T0Z, TXY, TYZ, TZX)
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                elem Uxy[G::MS], Uyz[G::MS], Uzx[G::MS], UMxy[G::MS], UMyz[G::MS], UMzx[G::MS], UxMy[G::MS], UyMz[G::MS], UzMx[G::MS], UMxMy[G::MS], UMyMz[G::MS], UMzMx[G::MS];
+                elem* W = UO;
+                UUDD<G>(L.Ul(W, x, y, z, 0), L.Ul(W, x + 1, y, z, 1), L.Ul(W, x, y + 1, z, 0), L.Ul(W, x, y, z, 1), Uxy);
+                UUDD<G>(L.Ul(W, x, y, z, 1), L.Ul(W, x, y + 1, z, 2), L.Ul(W, x, y, z + 1, 1), L.Ul(W, x, y, z, 2), Uyz);
+                UUDD<G>(L.Ul(W, x, y, z, 2), L.Ul(W, x, y, z + 1, 0), L.Ul(W, x + 1, y, z, 2), L.Ul(W, x, y, z, 0), Uzx);
+                UDDU<G>(L.Ul(W, x, y, z, 1), L.Ul(W, x - 1, y + 1, z, 0), L.Ul(W, x - 1, y, z, 1), L.Ul(W, x - 1, y, z, 0), UMxy);
+                UDDU<G>(L.Ul(W, x, y, z, 2), L.Ul(W, x, y - 1, z + 1, 1), L.Ul(W, x, y - 1, z, 2), L.Ul(W, x, y - 1, z, 1), UMyz);
+                UDDU<G>(L.Ul(W, x, y, z, 0), L.Ul(W, x + 1, y, z - 1, 2), L.Ul(W, x, y, z - 1, 0), L.Ul(W, x, y, z - 1, 2), UMzx);
+                DUUD<G>(L.Ul(W, x, y - 1, z, 1), L.Ul(W, x, y - 1, z, 0), L.Ul(W, x + 1, y - 1, z, 1), L.Ul(W, x, y, z, 0), UxMy);
+                DUUD<G>(L.Ul(W, x, y, z - 1, 2), L.Ul(W, x, y, z - 1, 1), L.Ul(W, x, y + 1, z - 1, 2), L.Ul(W, x, y, z, 1), UyMz);
+                DUUD<G>(L.Ul(W, x - 1, y, z, 0), L.Ul(W, x - 1, y, z, 2), L.Ul(W, x - 1, y, z + 1, 0), L.Ul(W, x, y, z, 2), UzMx);
+                // SC/CALC/Plaquettes.cpp:99-103
+                DDUU<G>(L.Ul(W, x - 1, y, z, 0), L.Ul(W, x - 1, y - 1, z, 1), L.Ul(W, x - 1, y - 1, z, 0), L.Ul(W, x, y - 1, z, 1), UMxMy);
+                DDUU<G>(L.Ul(W, x, y - 1, z, 1), L.Ul(W, x, y - 1, z - 1, 2), L.Ul(W, x, y - 1, z - 1, 1), L.Ul(W, x, y, z - 1, 2), UMyMz);
+                DDUU<G>(L.Ul(W, x, y, z - 1, 2), L.Ul(W, x - 1, y, z - 1, 0), L.Ul(W, x - 1, y, z - 1, 2), L.Ul(W, x - 1, y, z, 0), UMzMx);
+                elem Azx[G::MS], Axy[G::MS], Ayz[G::MS];
+                for (int s = 0; s < G::MS; s++) {  // AvgMatrix SC/SUNc/AdvancedOperations.cpp:35-40
+                    Azx[s] = 0.25 * (Uzx[s] + UMzx[s] + UzMx[s] + UMzMx[s]);
+                    Axy[s] = 0.25 * (Uxy[s] + UMxy[s] + UxMy[s] + UMxMy[s]);
+                    Ayz[s] = 0.25 * (Uyz[s] + UMyz[s] + UyMz[s] + UMyMz[s]);
+                }
+                double B0[8], B1[8], B2[8];
+                G::ReTrIGenU(Ayz, B0); G::ReTrIGenU(Azx, B1); G::ReTrIGenU(Axy, B2);
+                for (int a = 0; a < NG; a++) { B0[a] *= 2.0; B1[a] *= 2.0; B2[a] *= 2.0; }
+                double B0S = 4.0 * G::ReTrIDMinusU(Ayz), B1S = 4.0 * G::ReTrIDMinusU(Azx), B2S = 4.0 * G::ReTrIDMinusU(Axy);
+                double E0d[8], E1d[8], E2d[8], E0[8], E1[8], E2[8];
+                G::InverseAdjointMultiplication(L.Ul(U, x - 1, y, z, 0), L.El(E, x - 1, y, z, 0), E0d);
+                G::InverseAdjointMultiplication(L.Ul(U, x, y - 1, z, 1), L.El(E, x, y - 1, z, 1), E1d);
+                G::InverseAdjointMultiplication(L.Ul(U, x, y, z - 1, 2), L.El(E, x, y, z - 1, 2), E2d);
+                double E0S = 0, E1S = 0, E2S = 0;
+                for (int a = 0; a < NG; a++) {
+                    E0[a] = 0.5 * (L.El(E, x, y, z, 0)[a] + E0d[a]);
+                    E1[a] = 0.5 * (L.El(E, x, y, z, 1)[a] + E1d[a]);
+                    E2[a] = 0.5 * (L.El(E, x, y, z, 2)[a] + E2d[a]);
+                }
+                for (int a = 0; a < NG; a++) {
+                    E0S += 0.5 * (SQR(L.El(E, x, y, z, 0)[a]) + SQR(L.El(E, x - 1, y, z, 0)[a]));
+                    E1S += 0.5 * (SQR(L.El(E, x, y, z, 1)[a]) + SQR(L.El(E, x, y - 1, z, 1)[a]));
+                    E2S += 0.5 * (SQR(L.El(E, x, y, z, 2)[a]) + SQR(L.El(E, x, y, z - 1, 2)[a]));
+                }
+                auto dot = [&](const double* p, const double* r) { double s = 0; for (int a = 0; a < NG; a++) s += p[a] * r[a]; return s; };
+                T00 += (0.5 * (SQR(q[0]) * E0S + SQR(q[1]) * E1S + SQR(q[2]) * E2S + SQR(q[0]) * B0S + SQR(q[1]) * B1S + SQR(q[2]) * B2S)) * inv;
+                TXX += (0.5 * ((SQR(q[1]) * E1S + SQR(q[2]) * E2S + SQR(q[1]) * B1S + SQR(q[2]) * B2S) - (SQR(q[0]) * E0S + SQR(q[0]) * B0S))) * inv;
+                TYY += (0.5 * ((SQR(q[0]) * E0S + SQR(q[2]) * E2S + SQR(q[0]) * B0S + SQR(q[2]) * B2S) - (SQR(q[1]) * E1S + SQR(q[1]) * B1S))) * inv;
+                TZZ += (0.5 * ((SQR(q[0]) * E0S + SQR(q[1]) * E1S + SQR(q[0]) * B0S + SQR(q[1]) * B1S) - (SQR(q[2]) * E2S + SQR(q[2]) * B2S))) * inv;
+                T0X += (q[1] * q[2] * dot(E1, B2) - q[2] * q[1] * dot(E2, B1)) * inv;
+                T0Y += (q[2] * q[0] * dot(E2, B0) - q[0] * q[2] * dot(E0, B2)) * inv;
+                T0Z += (q[0] * q[1] * dot(E0, B1) - q[1] * q[0] * dot(E1, B0)) * inv;
+                TXY += (q[0] * q[1] * dot(E0, E1) + q[0] * q[1] * dot(B0, B1)) * inv;
+                TYZ += (q[1] * q[2] * dot(E1, E2) + q[1] * q[2] * dot(B1, B2)) * inv;
+                TZX += (q[2] * q[0] * dot(E2, E0) + q[2] * q[0] * dot(B2, B0)) * inv;
+            }
+        double* o = out + 10L * z;
+        o[0] = T00; o[1] = TXX; o[2] = TYY; o[3] = TZZ; o[4] = T0X; o[5] = T0Y; o[6] = T0Z; o[7] = TXY; o[8] = TYZ; o[9] = TZX;
+    }
+}
+
+// ============================================================================================
+//  FFT (oracle's own mixed-radix recursion; FFTW conventions of SC/FFT/FFT*D.cpp: forward e^{-2 pi i},
+//  backward e^{+2 pi i}, both unnormalised).
+// ============================================================================================
+static void dft_rec(const cd* in, long istride, cd* out, long n, int sign) {
+    if (n == 1) { out[0] = in[0]; return; }
+    long p = 2;
+    while (n % p) p++;
+    long m = n / p;
+    std::vector<cd> sub(n);
+    for (long q = 0; q < p; q++) dft_rec(in + q * istride, istride * p, sub.data() + q * m, m, sign);
+    for (long k = 0; k < n; k++) {
+        cd acc = 0.0;
+        for (long q = 0; q < p; q++) {
+            double ang = sign * 2.0 * PI * (double)((q * k) % n) / (double)n;
+            acc += sub[q * m + (k % m)] * cd(std::cos(ang), std::sin(ang));
+        }
+        out[k] = acc;
+    }
+}
+// one 1D transform of length n over elements data[base + j*stride]
+static void fft_line(cd* data, long base, long stride, long n, int sign) {
+    std::vector<cd> tmp(n), res(n);
+    for (long j = 0; j < n; j++) tmp[j] = data[base + j * stride];
+    dft_rec(tmp.data(), 1, res.data(), n, sign);
+    for (long j = 0; j < n; j++) data[base + j * stride] = res[j];
+}
+// Multi-dimensional transform of an array with dims[0] fastest ... dims[nd-1] slowest, `batch` interleaved
+// components fastest of all (the reference's XIndex = a + Ng*(x + Nx*(y + Ny*z)), SC/FFT/FFT3D.cpp:41-47).
+static void fft_nd(cd* data, int nd, const int* dims, long batch, int sign) {
+    long total = batch;
+    for (int d = 0; d < nd; d++) total *= dims[d];
+    long stride = batch;
+    for (int d = 0; d < nd; d++) {
+        long n = dims[d];
+        long lines = total / n;
+        long st = stride;
+#pragma omp parallel for
+        for (long l = 0; l < lines; l++) {
+            long lo = l % st, hi = l / st;
+            fft_line(data, lo + hi * st * n, st, n, sign);
+        }
+        stride *= n;
+    }
+}
+
+// ============================================================================================
+//  RNG: MT19937 with GSL's seeding (seed 0 -> 4357) and gsl_ran_gaussian's polar method.
+//  MT19937: Matsumoto & Nishimura, ACM TOMACS 8 (1998) 3; init_genrand of the 2002 reference code.
+//  Gaussian: Knuth TAOCP vol. 2, 3.4.1 algorithm P (polar method), returning sigma*y*sqrt(-2 ln r2/r2)
+//  with uniforms from gsl_rng_uniform_pos.  Draw order SC/MISC/RNG/GSLRandomNumberGenerator.cpp:12-33.
+// ============================================================================================
+struct MT {
+    uint32_t mt[624];
+    int mti;
+    void seed(unsigned long s) {
+        if (s == 0) s = 4357;
+        mt[0] = (uint32_t)(s & 0xffffffffUL);
+        for (int i = 1; i < 624; i++) mt[i] = (uint32_t)(1812433253UL * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i);
+        mti = 624;
+    }
+    void seed_std(uint32_t s) {  // std::mt19937 seeding (identical recursion, no 0 -> 4357 mapping)
+        mt[0] = s;
+        for (int i = 1; i < 624; i++) mt[i] = (uint32_t)(1812433253UL * (mt[i - 1] ^ (mt[i - 1] >> 30)) + i);
+        mti = 624;
+    }
+    uint32_t next() {
+        static const uint32_t mag01[2] = {0x0u, 0x9908b0dfu};
+        if (mti >= 624) {
+            int kk;
+            for (kk = 0; kk < 624 - 397; kk++) {
+                uint32_t y = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+                mt[kk] = mt[kk + 397] ^ (y >> 1) ^ mag01[y & 1u];
+            }
+            for (; kk < 623; kk++) {
+                uint32_t y = (mt[kk] & 0x80000000u) | (mt[kk + 1] & 0x7fffffffu);
+                mt[kk] = mt[kk + (397 - 624)] ^ (y >> 1) ^ mag01[y & 1u];
+            }
+            uint32_t y = (mt[623] & 0x80000000u) | (mt[0] & 0x7fffffffu);
+            mt[623] = mt[396] ^ (y >> 1) ^ mag01[y & 1u];
+            mti = 0;
+        }
+        uint32_t y = mt[mti++];
+        y ^= (y >> 11);
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= (y >> 18);
+        return y;
+    }
+    double uniform() { return next() / 4294967296.0; }
+    double uniform_pos() { double x; do { x = uniform(); } while (x == 0); return x; }
+    double gauss() {
+        double x, y, r2;
+        do {
+            x = -1.0 + 2.0 * uniform_pos();
+            y = -1.0 + 2.0 * uniform_pos();
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0);
+        return y * std::sqrt(-2.0 * std::log(r2) / r2);
+    }
+};
+
+// ============================================================================================
+//  MV initial conditions: SC/COLLIDINGOBJECTS/CollidingObjects.cpp and
+//  SC/INITIALCONDITIONS/SUNCCURRENTS/RevSingleNucleus.cpp, SU(3) only.
+// ============================================================================================
+struct MVParams {
+    int N[3];
+    double a[3];
+    double dt;
+    int bc;
+    double lambda, meff;
+    double sigma_l, sigma_r;
+    double nucleus_l, nucleus_r;
+};
+
+static double LongitudinalProfile(int z, int center, double w) {  // :30-32
+    return std::exp(-0.5 * SQR((z - center) / w)) / std::sqrt(2 * PI * w * w);
+}
+
+// 2D (x,y) transform of the 8 interleaved colours of one z slice: F[a + 8*(x + Nx*y)]
+static void fft2d(std::vector<cd>& F, int Nx, int Ny, int sign) {
+    int dims[2] = {Nx, Ny};
+    fft_nd(F.data(), 2, dims, 8, sign);
+}
+
+// Transverse momentum squared  :68 / :142
+static double psqr2(int px, int py, const int* N, const double* a) {
+    return (2.0 - 2.0 * std::cos(2.0 * PI * px / (double)N[0])) / SQR(a[0]) + (2.0 - 2.0 * std::cos(2.0 * PI * py / (double)N[1])) / SQR(a[1]);
+}
+
+// SolveLaplaceEquation  :131-176  (F holds rho in X space on entry, A on exit)
+static void SolveLaplace(std::vector<cd>& F, const MVParams& P) {
+    int Nx = P.N[0], Ny = P.N[1];
+    fft2d(F, Nx, Ny, -1);
+    double nf = 1.0 / ((double)Nx * Ny);
+    for (int py = 0; py < Ny; py++)
+        for (int px = 0; px < Nx; px++) {
+            double p2 = psqr2(px, py, P.N, P.a);
+            for (int a = 0; a < 8; a++) {
+                long i = a + 8L * (px + (long)Nx * py);
+                if (px == 0 && py == 0) F[i] = 0.0;
+                else F[i] = nf * F[i] / p2;
+            }
+        }
+    fft2d(F, Nx, Ny, +1);
+}
+
+// SetNewWilsonLines :179-226 ; Wil (N_perp^2 matrices) is the running product, V gets a copy per z
+static void SetNewWilsonLines(std::vector<cd>& F, const MVParams& P, int z, char D, cd* Wil, cd* V) {
+    Lat<SU3> L{{P.N[0], P.N[1], P.N[2]}, {P.a[0], P.a[1], P.a[2]}, P.a[0] * P.a[1] * P.a[2], P.bc};
+#pragma omp parallel for
+    for (int y = 0; y < P.N[1]; y++)
+        for (int x = 0; x < P.N[0]; x++) {
+            double col[8];
+            long i2 = x + (long)P.N[0] * y;
+            for (int a = 0; a < 8; a++) col[a] = F[a + 8 * i2].real();
+            cd X[9], Old[9];
+            SU3::MatrixIExp((D == 'R' ? 1.0 : -1.0) / SQRT2, col, X);
+            std::memcpy(Old, Wil + 9 * i2, sizeof(Old));
+            SU3::UU(X, Old, Wil + 9 * i2);
+            std::memcpy(V + 9 * L.idx3(x, y, z), Wil + 9 * i2, 9 * sizeof(cd));
+        }
+}
+
+// CollidingObjects::SetWilsonLines(rhoOld, rho, V, D, Center, QsA)  :318-371 with the charges xi already
+// drawn (xi[a + 8*(x + Nx*y)] = QsA*Gauss(), drawn in y,x,a order).  rhoOld/rho/V carry z ghost layers.
+static void SetWilsonLines(const MVParams& P, const double* xi, char D, int center, double* rhoOld, double* rho, cd* V) {
+    int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    long np = (long)Nx * Ny;
+    std::vector<cd> Wil(9 * np);
+    for (long i = 0; i < np; i++) SU3::Unit(&Wil[9 * i]);
+    std::vector<cd> F(8 * np);
+    double norm = std::sqrt(P.a[0] * P.a[1]);
+    double nf = 1.0 / ((double)Nx * Ny);
+    auto one_z = [&](int z) {
+        // SetChargeDensities :34-102
+        double prof = LongitudinalProfile(z, center, D == 'L' ? P.sigma_l : P.sigma_r);
+        double mult = (D == 'L') ? P.nucleus_l : P.nucleus_r;
+        for (long i = 0; i < np; i++)
+            for (int a = 0; a < 8; a++) {
+                double v = mult * norm * prof * xi[a + 8 * i];
+                rhoOld[a + 8 * (i + np * (z + 1))] = v;
+                F[a + 8 * i] = v;
+            }
+        fft2d(F, Nx, Ny, -1);
+        for (int py = 0; py < Ny; py++)
+            for (int px = 0; px < Nx; px++) {
+                double p2 = psqr2(px, py, P.N, P.a);
+                for (int a = 0; a < 8; a++) {
+                    long i = a + 8L * (px + (long)Nx * py);
+                    F[i] = nf * F[i] * std::exp(-p2 / (2 * SQR(P.lambda))) * p2 / (p2 + SQR(P.meff));
+                }
+            }
+        fft2d(F, Nx, Ny, +1);
+        for (long i = 0; i < np; i++)
+            for (int a = 0; a < 8; a++) rho[a + 8 * (i + np * (z + 1))] = F[a + 8 * i].real();
+        for (long i = 0; i < np; i++)
+            for (int a = 0; a < 8; a++) F[a + 8 * i] = rho[a + 8 * (i + np * (z + 1))];
+        SolveLaplace(F, P);
+        SetNewWilsonLines(F, P, z, D, Wil.data(), V);
+    };
+    if (D == 'R') for (int z = 0; z < Nz; z++) one_z(z);
+    else for (int z = Nz - 1; z >= 0; z--) one_z(z);
+}
+
+// SetWilsonLinesForFirstStep :377-411 (Poisson solve of the evolved charge, no smoothing)
+static void SetWilsonLinesForFirstStep(const MVParams& P, cd* V, const double* J, char D) {
+    int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    long np = (long)Nx * Ny;
+    std::vector<cd> Wil(9 * np);
+    for (long i = 0; i < np; i++) SU3::Unit(&Wil[9 * i]);
+    std::vector<cd> F(8 * np);
+    auto one_z = [&](int z) {
+        for (long i = 0; i < np; i++)
+            for (int a = 0; a < 8; a++) F[a + 8 * i] = J[a + 8 * (i + np * (z + 1))];
+        SolveLaplace(F, P);
+        SetNewWilsonLines(F, P, z, D, Wil.data(), V);
+    };
+    if (D == 'R') for (int z = 0; z < Nz; z++) one_z(z);
+    else for (int z = Nz - 1; z >= 0; z--) one_z(z);
+}
+
+// Dynamics::SUNc::EvolveRhoViaFourier  SC/DYNAMICS/SUNcDynamics.cpp:275-350 (all z; ghost sync after)
+static void EvolveRhoViaFourier(const MVParams& P, double* JNew, const double* JOld, double dt, char D) {
+    int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    long np = (long)Nx * Ny, B = 8 * np;
+    std::vector<cd> F(B * Nz);
+    for (int z = 0; z < Nz; z++)
+        for (long i = 0; i < B; i++) F[i + B * z] = JOld[i + B * (z + 1)];
+    int dims[1] = {Nz};
+    fft_nd(F.data(), 1, dims, B, -1);
+    double nf = 1.0 / Nz;
+    for (int pz = 0; pz < Nz; pz++) {
+        double om = std::sqrt((2.0 - 2.0 * std::cos(2.0 * PI * (double)pz / (double)Nz)) / SQR(P.a[2]));
+        double sgn;
+        if (D == 'L') sgn = (pz < Nz / 2) ? -1.0 : 1.0;
+        else sgn = (pz < Nz / 2) ? 1.0 : -1.0;
+        cd ph = std::exp(cd(0.0, sgn * om * dt));
+        for (long i = 0; i < B; i++) F[i + B * pz] = ph * (nf * F[i + B * pz]);
+    }
+    fft_nd(F.data(), 1, dims, B, +1);
+    for (int z = 0; z < Nz; z++)
+        for (long i = 0; i < B; i++) JNew[i + B * (z + 1)] = F[i + B * z].real();
+}
+
+// InitialConditions::SingleSheet  SC/INITIALCONDITIONS/SUNCCURRENTS/RevSingleNucleus.cpp:7-210
+static void SingleSheet(const MVParams& P, double* rho, int zLow, int zHigh, double* RhoEvolved, cd* V, char D, double* JStat, double* J, cd* U, cd* UOld, double* E) {
+    Lat<SU3> L{{P.N[0], P.N[1], P.N[2]}, {P.a[0], P.a[1], P.a[2]}, P.a[0] * P.a[1] * P.a[2], P.bc};
+    int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    long np = (long)Nx * Ny;
+    auto syncD_U = [&](cd* A) {  // GaugeLinks::SynchronizeGhostCells(char)  SC/LATTICE/GaugeLinks.cpp:75-87
+        long Pl = np * 27;
+        if (D == 'L') std::memcpy(A, A + Pl, Pl * sizeof(cd));
+        if (D == 'R') std::memcpy(A + (long)(Nz + 1) * Pl, A + (long)Nz * Pl, Pl * sizeof(cd));
+        else L.syncU(A);
+    };
+    auto syncD_E = [&](double* A) {  // SC/LATTICE/ElectricFields.cpp:63-76
+        long Pl = np * 24;
+        if (D == 'L') std::memcpy(A, A + Pl, Pl * sizeof(double));
+        if (D == 'R') std::memcpy(A + (long)(Nz + 1) * Pl, A + (long)Nz * Pl, Pl * sizeof(double));
+        else L.syncE(A);
+    };
+    L.syncJ(rho);
+    L.sync(V, 9L);
+    for (int z = zLow; z <= zHigh; z++)  // :14-25
+        for (int y = 0; y < Ny; y++)
+            for (int x = 0; x < Nx; x++) {
+                SU3::DU(V + 9 * L.idx3(x, y, z), V + 9 * L.idx3(x + 1, y, z), L.Ul(UOld, x, y, z, 0));
+                SU3::DU(V + 9 * L.idx3(x, y, z), V + 9 * L.idx3(x, y + 1, z), L.Ul(UOld, x, y, z, 1));
+                SU3::Unit(L.Ul(UOld, x, y, z, 2));
+            }
+    syncD_U(UOld);
+    EvolveRhoViaFourier(P, RhoEvolved, rho, P.dt, D);  // :33
+    L.syncJ(RhoEvolved);
+    SetWilsonLinesForFirstStep(P, V, RhoEvolved, D);
+    L.sync(V, 9L);
+    for (int z = zLow; z <= zHigh; z++)  // :47-58
+        for (int y = 0; y < Ny; y++)
+            for (int x = 0; x < Nx; x++) {
+                SU3::DU(V + 9 * L.idx3(x, y, z), V + 9 * L.idx3(x + 1, y, z), L.Ul(U, x, y, z, 0));
+                SU3::DU(V + 9 * L.idx3(x, y, z), V + 9 * L.idx3(x, y + 1, z), L.Ul(U, x, y, z, 1));
+                SU3::Unit(L.Ul(U, x, y, z, 2));
+            }
+    syncD_U(U);
+    double coff[2];  // :66-70
+    for (int mu = 0; mu < 2; mu++) coff[mu] = -P.dt * SQR(P.a[mu]) / L.aCube;
+#pragma omp parallel for
+    for (int z = zLow; z <= zHigh; z++)
+        for (int y = 0; y < Ny; y++)
+            for (int x = 0; x < Nx; x++) {
+                cd XP[9], YP[9];
+                SU3::UD(L.Ul(U, x, y, z, 0), L.Ul(UOld, x, y, z, 0), XP);
+                SU3::UD(L.Ul(U, x, y, z, 1), L.Ul(UOld, x, y, z, 1), YP);
+                SU3::MatrixILog(coff[0], XP, L.El(E, x, y, z, 0));
+                SU3::MatrixILog(coff[1], YP, L.El(E, x, y, z, 1));
+                for (int a = 0; a < 8; a++) L.El(E, x, y, z, 2)[a] = 0.0;
+            }
+    syncD_E(E);
+#pragma omp parallel for
+    for (int z = 0; z < Nz; z++)  // :101-123
+        for (int y = 0; y < Ny; y++)
+            for (int x = 0; x < Nx; x++) {
+                if (z >= zLow && z <= zHigh) {
+                    double Xd[8], Yd[8];
+                    SU3::InverseAdjointMultiplication(L.Ul(U, x - 1, y, z, 0), L.El(E, x - 1, y, z, 0), Xd);
+                    SU3::InverseAdjointMultiplication(L.Ul(U, x, y - 1, z, 1), L.El(E, x, y - 1, z, 1), Yd);
+                    for (int a = 0; a < 8; a++)
+                        L.Jl(JStat, x, y, z)[a] = SQRT2 * ((L.El(E, x, y, z, 0)[a] - Xd[a]) + (L.El(E, x, y, z, 1)[a] - Yd[a]));
+                } else
+                    for (int a = 0; a < 8; a++) L.Jl(JStat, x, y, z)[a] = 0.0;
+            }
+    L.syncJ(JStat);
+    // J from J_stat through the z-FFT continuity inversion  :131-209
+    long B = 8 * np;
+    std::vector<cd> F(B * Nz);
+    for (int z = 0; z < Nz; z++)
+        for (long i = 0; i < B; i++) F[i + B * z] = JStat[i + B * (z + 1)];
+    int dims[1] = {Nz};
+    fft_nd(F.data(), 1, dims, B, -1);
+    double nf = 1.0 / Nz;
+    for (int pz = 0; pz < Nz; pz++) {
+        double om = std::sqrt((2.0 - 2.0 * std::cos(2.0 * PI * (double)pz / (double)Nz)) / SQR(P.a[2]));
+        cd e = std::exp(cd(0.0, -2.0 * PI * (double)pz / (double)Nz));
+        cd pZD, pNpos, pNneg;
+        if (D == 'R') {
+            pZD = (1.0 - e) / P.a[2];
+            pNpos = std::exp(cd(0.0, om * P.dt)) - 1.0;
+            pNneg = std::exp(cd(0.0, -om * P.dt)) - 1.0;
+        } else {
+            pZD = -(1.0 - e) / P.a[2];
+            pNpos = std::exp(cd(0.0, -om * P.dt)) - 1.0;
+            pNneg = std::exp(cd(0.0, om * P.dt)) - 1.0;
+        }
+        for (long i = 0; i < B; i++) {
+            cd js = nf * F[i + B * pz], jt;
+            if (pz == 0) jt = js;
+            else if (pz < Nz / 2) jt = (js * pNpos) / (P.dt * pZD);
+            else if (pz == Nz / 2) jt = 0.0;
+            else jt = (js * pNneg) / (P.dt * pZD);
+            F[i + B * pz] = jt;
+        }
+    }
+    fft_nd(F.data(), 1, dims, B, +1);
+    for (int z = 0; z < Nz; z++)
+        for (long i = 0; i < B; i++) J[i + B * (z + 1)] = F[i + B * z].real();
+    L.syncJ(J);
+}
+
+// ============================================================================================
+//  Coulomb gauge fixing, Fourier-accelerated steepest descent:
+//  SC/GAUGETRANSFORMATION/CoulombGaugeFixing.cpp:32-88, ALGORITHMS/FourierAcceleration.cpp:15-148,
+//  CoulombGaugeDeviation.cpp:23-73, GaugeTransformation.cpp:93-164, SC/CALC/GaugeTransformation.cpp:5-17.
+//  Reproduced as the reference computes it: the gauge-fixed copy's ghost layers stay the identity
+//  (only the interior is ever copied or transformed), G's ghost layers are synchronised after each
+//  link transformation (so the transformation in an iteration sees the previous iteration's top ghost),
+//  periodic momenta in all three directions, and the zero mode passed through unnormalised.
+//  stats: [0] iterations, [1] max dev, [2] avg dev, [3] max step, [4] avg step, [5] min functional, [6] sum functional
+// ============================================================================================
+static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, double* EG, double tol, int maxit, double alpha, double* stats) {
+    int Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
+    long np = L.plane(), V = np * Nz;
+    auto syncG = [&]() { L.sync(Gt, 9L); };
+    auto transform_links = [&]() {  // GaugeTransformLinks(UOld=U, UNew=UG, G)
+#pragma omp parallel for
+        for (int z = 0; z < Nz; z++)
+            for (int y = 0; y < Ny; y++)
+                for (int x = 0; x < Nx; x++) {
+                    cd t[9];
+                    const cd* g = Gt + 9 * L.idx3(x, y, z);
+                    SU3::UU(g, L.Ul(U, x, y, z, 0), t); SU3::UD(t, Gt + 9 * L.idx3(x + 1, y, z), L.Ul(UG, x, y, z, 0));
+                    SU3::UU(g, L.Ul(U, x, y, z, 1), t); SU3::UD(t, Gt + 9 * L.idx3(x, y + 1, z), L.Ul(UG, x, y, z, 1));
+                    SU3::UU(g, L.Ul(U, x, y, z, 2), t); SU3::UD(t, Gt + 9 * L.idx3(x, y, z + 1), L.Ul(UG, x, y, z, 2));
+                }
+        syncG();
+    };
+    // CopyFields: only the interior (the ghosts of UG keep their identity / previous contents)
+    std::memcpy(L.Ul(UG, 0, 0, 0, 0), L.Ul(U, 0, 0, 0, 0), 27 * V * sizeof(cd));
+    std::memcpy(L.El(EG, 0, 0, 0, 0), L.El(E, 0, 0, 0, 0), 24 * V * sizeof(double));
+    transform_links();
+    int dims[3] = {Nx, Ny, Nz};
+    std::vector<cd> F(8 * V);
+    double gmax = 1.0, gavg = 0, smax = 0, savg = 0, fmin = 0, fsum = 0;
+    int n = 0;
+    while (n < maxit && gmax > tol) {
+        double mx = 0.0, sq = 0.0, fs = 0.0;
+        double fm = 6.0 * (1.0 / SQR(L.a[0]) + 1.0 / SQR(L.a[1]) + 1.0 / SQR(L.a[2]));
+#pragma omp parallel for reduction(+ : sq, fs) reduction(max : mx) reduction(min : fm)
+        for (int z = 0; z < Nz; z++)
+            for (int y = 0; y < Ny; y++)
+                for (int x = 0; x < Nx; x++) {
+                    cd d0[9], d1[9], d2[9], w[9];
+                    SU3::Dag(L.Ul(UG, x - 1, y, z, 0), d0);
+                    SU3::Dag(L.Ul(UG, x, y - 1, z, 1), d1);
+                    SU3::Dag(L.Ul(UG, x, y, z - 1, 2), d2);
+                    const cd* u0 = L.Ul(UG, x, y, z, 0);
+                    const cd* u1 = L.Ul(UG, x, y, z, 1);
+                    const cd* u2 = L.Ul(UG, x, y, z, 2);
+                    double zf = (Nz > 1) ? 1.0 : 0.0;
+                    for (int k = 0; k < 9; k++)
+                        w[k] = SQR(1.0 / L.a[0]) * (u0[k] + d0[k]) + SQR(1.0 / L.a[1]) * (u1[k] + d1[k]) + zf * SQR(1.0 / L.a[2]) * (u2[k] + d2[k]);
+                    double dv[8];
+                    SU3::ReTrIGenU(w, dv);
+                    for (int a = 0; a < 8; a++) {
+                        mx = std::max(mx, std::fabs(dv[a]));
+                        sq += SQR(dv[a]);
+                    }
+                    double lf = SU3::ReTr(w);
+                    fm = std::min(fm, lf);
+                    fs += lf;
+                    long s = x + (long)Nx * (y + (long)Ny * z);
+                    for (int a = 0; a < 8; a++) F[a + 8 * s] = dv[a];
+                }
+        fft_nd(F.data(), 3, dims, 8, -1);
+        double nf = 1.0 / ((double)V);
+        for (int pz = 0; pz < Nz; pz++)
+            for (int py = 0; py < Ny; py++)
+                for (int px = 0; px < Nx; px++) {
+                    double p2 = (2.0 - 2.0 * std::cos(2.0 * PI * px / (double)Nx)) / SQR(L.a[0]) + (2.0 - 2.0 * std::cos(2.0 * PI * py / (double)Ny)) / SQR(L.a[1]) + (2.0 - 2.0 * std::cos(2.0 * PI * pz / (double)Nz)) / SQR(L.a[2]);
+                    if (p2 > 0.0) {
+                        long s = px + (long)Nx * (py + (long)Ny * pz);
+                        for (int a = 0; a < 8; a++) F[a + 8 * s] = nf * F[a + 8 * s] / p2;
+                    }
+                }
+        fft_nd(F.data(), 3, dims, 8, +1);
+        double sm = 0.0, ssq = 0.0;
+#pragma omp parallel for reduction(+ : ssq) reduction(max : sm)
+        for (int z = 0; z < Nz; z++)
+            for (int y = 0; y < Ny; y++)
+                for (int x = 0; x < Nx; x++) {
+                    long s = x + (long)Nx * (y + (long)Ny * z);
+                    double ub[8];
+                    for (int a = 0; a < 8; a++) ub[a] = F[a + 8 * s].real();
+                    cd wt[9], gn[9];
+                    SU3::MatrixIExp(-alpha, ub, wt);
+                    double st = std::abs(3.0 - (wt[0] + wt[4] + wt[8]));
+                    sm = std::max(sm, st);
+                    ssq += st * st;
+                    cd* g = Gt + 9 * L.idx3(x, y, z);
+                    SU3::DU(wt, g, gn);
+                    std::memcpy(g, gn, sizeof(gn));
+                }
+        transform_links();
+        gmax = mx; gavg = std::sqrt(sq / (V * 8.0));
+        smax = sm; savg = std::sqrt(ssq / (V * 8.0));
+        fmin = fm; fsum = fs / V;
+        n++;
+    }
+    transform_links();
+#pragma omp parallel for
+    for (int z = 0; z < Nz; z++)
+        for (int y = 0; y < Ny; y++)
+            for (int x = 0; x < Nx; x++)
+                for (int mu = 0; mu < 3; mu++)
+                    SU3::AdjointMultiplication(Gt + 9 * L.idx3(x, y, z), L.El(E, x, y, z, mu), L.El(EG, x, y, z, mu));
+    syncG();
+    stats[0] = n; stats[1] = gmax; stats[2] = gavg; stats[3] = smax; stats[4] = savg; stats[5] = fmin; stats[6] = fsum;
+}
+
+// ============================================================================================
+//  extern "C" surface used by tests/ and bench.py (ctypes).  All field arrays are host arrays in the
+//  reference AoS layout including the two z ghost layers; complex numbers interleaved (re, im).
+// ============================================================================================
+static Lat<SU3> mk3(const int* N, const double* a, int bc) { return Lat<SU3>{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, a[0] * a[1] * a[2], bc}; }
+static Lat<SU2> mk2(const int* N, const double* a, int bc) { return Lat<SU2>{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, a[0] * a[1] * a[2], bc}; }
+
+extern "C" {
+
+int ref_num_threads() { return omp_get_max_threads(); }
+void ref_set_threads(int n) { omp_set_num_threads(n); }
+
+// --- SU(3) per-site algebra (for unit tests) ---
+void ref_su3_mul(const double* A, const double* B, double* C, int mode) {
+    const cd* a = (const cd*)A; const cd* b = (const cd*)B; cd* c = (cd*)C;
+    if (mode == 0) SU3::UU(a, b, c); else if (mode == 1) SU3::UD(a, b, c); else if (mode == 2) SU3::DU(a, b, c); else SU3::DD(a, b, c);
+}
+void ref_su3_retrigenu(const double* V, double* out) { SU3::ReTrIGenU((const cd*)V, out); }
+double ref_su3_retridminusu(const double* V) { return SU3::ReTrIDMinusU((const cd*)V); }
+double ref_su3_unitarity_norm(const double* V) { return SU3::UnitarityNorm((const cd*)V); }
+void ref_su3_expi(double c, const double* al, double* X) { SU3::MatrixIExp(c, al, (cd*)X); }
+void ref_su3_ilog(double c, const double* U, double* out) { SU3::MatrixILog(c, (const cd*)U, out); }
+void ref_su3_adj_mul(const double* V, const double* al, double* out) { SU3::AdjointMultiplication((const cd*)V, al, out); }
+void ref_su3_inv_adj_mul(const double* V, const double* al, double* out) { SU3::InverseAdjointMultiplication((const cd*)V, al, out); }
+void ref_su3_get_adjoint(const double* V, double* adj) { SU3::GetAdjoint((const cd*)V, adj); }
+void ref_su2_expi(double c, const double* al, double* X) { SU2::MatrixIExp(c, al, X); }
+void ref_su2_ilog(double c, const double* U, double* out) { SU2::MatrixILog(c, U, out); }
+
+// --- leapfrog ---
+void ref_update(int nc, const int* N, const double* a, int bc, double dt, int nsteps,
+                double* U, double* UOld, double* E, double* Jp, double* Jm, double* JpS, double* JmS) {
+    if (nc == 3) {
+        Lat<SU3> L = mk3(N, a, bc);
+        for (int s = 0; s < nsteps; s++) Update<SU3>(L, (cd*)U, (cd*)UOld, E, Jp, Jm, JpS, JmS, dt);
+    } else {
+        Lat<SU2> L = mk2(N, a, bc);
+        for (int s = 0; s < nsteps; s++) Update<SU2>(L, U, UOld, E, Jp, Jm, JpS, JmS, dt);
+    }
+}
+void ref_sync_ghosts(int nc, const int* N, int bc, double* A, long rec_doubles) {
+    static const double one[3] = {1.0, 1.0, 1.0};
+    (void)nc;
+    Lat<SU3> L = mk3(N, one, bc);
+    L.sync(A, rec_doubles);
+}
+// --- observables ---
+void ref_bulk(int nc, const int* N, const double* a, int bc, double* U, double* E, double* Jp, double* Jm, double* out9) {
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); Bulk<SU3>(L, (cd*)U, E, Jp, Jm, out9); }
+    else { Lat<SU2> L = mk2(N, a, bc); Bulk<SU2>(L, U, E, Jp, Jm, out9); }
+}
+void ref_gauss(int nc, const int* N, const double* a, int bc, double* U, double* E, double* JpS, double* JmS, double* per_z, double* gmax, double* grms) {
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); Gauss<SU3>(L, (cd*)U, E, JpS, JmS, per_z, gmax, grms); }
+    else { Lat<SU2> L = mk2(N, a, bc); Gauss<SU2>(L, U, E, JpS, JmS, per_z, gmax, grms); }
+}
+void ref_unitarity(int nc, const int* N, const double* a, int bc, double* U, double* umax, double* urms) {
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); Unitarity<SU3>(L, (cd*)U, umax, urms); }
+    else { Lat<SU2> L = mk2(N, a, bc); Unitarity<SU2>(L, U, umax, urms); }
+}
+void ref_emt(int nc, const int* N, const double* a, int bc, double* U, double* UOld, double* E, double* out) {
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); EMT<SU3>(L, (cd*)U, (cd*)UOld, E, out); }
+    else { Lat<SU2> L = mk2(N, a, bc); EMT<SU2>(L, U, UOld, E, out); }
+}
+// --- FFT / RNG ---
+void ref_fft_nd(double* data, int nd, const int* dims, long batch, int sign) { fft_nd((cd*)data, nd, dims, batch, sign); }
+void ref_mt_raw(unsigned long seed, int std_seeding, long n, uint32_t* out) {
+    MT m;
+    if (std_seeding) m.seed_std((uint32_t)seed); else m.seed(seed);
+    for (long i = 0; i < n; i++) out[i] = m.next();
+}
+void ref_gauss_draws(unsigned long seed, long n, double* out) {
+    MT m; m.seed(seed);
+    for (long i = 0; i < n; i++) out[i] = m.gauss();
+}
+// The charges of both nuclei, drawn from one stream (L first, then R), each scaled by qs:
+// SC/COLLIDINGOBJECTS/CollidingObjects.cpp:335-342 called at SC/Simulation.cpp:344 and :348.
+void ref_draw_charges(unsigned long seed, int Nx, int Ny, double qs_l, double qs_r, double* xi_l, double* xi_r) {
+    MT m; m.seed(seed);
+    long n = 8L * Nx * Ny;
+    for (long i = 0; i < n; i++) xi_l[i] = qs_l * m.gauss();
+    for (long i = 0; i < n; i++) xi_r[i] = qs_r * m.gauss();
+}
+// --- MV initial conditions ---
+// p: N[3], a[3] as doubles? no: explicit arguments.
+void ref_set_wilson_lines(const int* N, const double* a, int bc, double lambda, double meff, double sigma, double mult,
+                          const double* xi, char D, int center, double* rhoOld, double* rho, double* V) {
+    MVParams P{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, 0.0, bc, lambda, meff, sigma, sigma, mult, mult};
+    SetWilsonLines(P, xi, D, center, rhoOld, rho, (cd*)V);
+}
+void ref_single_sheet(const int* N, const double* a, int bc, double dt, double* rho, int zLow, int zHigh, double* RhoEvolved,
+                      double* V, char D, double* JStat, double* J, double* U, double* UOld, double* E) {
+    MVParams P{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, dt, bc, 0, 0, 0, 0, 1, 1};
+    SingleSheet(P, rho, zLow, zHigh, RhoEvolved, (cd*)V, D, JStat, J, (cd*)U, (cd*)UOld, E);
+}
+void ref_evolve_rho(const int* N, const double* a, double dt, char D, const double* JOld, double* JNew) {
+    MVParams P{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, dt, 0, 0, 0, 0, 0, 1, 1};
+    EvolveRhoViaFourier(P, JNew, JOld, dt, D);
+}
+// Poisson solve of one slice in place: F[a + 8*(x + Nx*y)] real parts in/out
+void ref_solve_laplace(const int* N, const double* a, double* rho_in_A_out) {
+    MVParams P{{N[0], N[1], 1}, {a[0], a[1], 1.0}, 0, 0, 0, 0, 0, 0, 1, 1};
+    long n = 8L * N[0] * N[1];
+    std::vector<cd> F(n);
+    for (long i = 0; i < n; i++) F[i] = rho_in_A_out[i];
+    SolveLaplace(F, P);
+    for (long i = 0; i < n; i++) rho_in_A_out[i] = F[i].real();
+}
+// Full initial state of Simulation::Run (SC/Simulation.cpp:327-378) given both nuclei's charges.
+// driver: 0 collision, 1 left-moving (SC/LeftNucleusSimulation.cpp:357), 2 right-moving
+// (SC/RightNucleusSimulation.cpp:357).  Fields must be pre-initialised (U, UOld identity; E, J zero).
+void ref_mv_initial_state(int driver, const int* N, const double* a, int bc, double dt, double lambda, double meff,
+                          double sigma_l, double sigma_r, double mult_l, double mult_r, int center_l, int center_r,
+                          const double* xi_l, const double* xi_r,
+                          double* U, double* UOld, double* E, double* Jp, double* Jm, double* JpS, double* JmS) {
+    Lat<SU3> L = mk3(N, a, bc);
+    MVParams P{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, dt, bc, lambda, meff, sigma_l, sigma_r, mult_l, mult_r};
+    long np = L.plane(), nzg = N[2] + 2;
+    std::vector<double> rhoOldL(8 * np * nzg, 0.0), rhoL(8 * np * nzg, 0.0), rhoOldR(8 * np * nzg, 0.0), rhoR(8 * np * nzg, 0.0);
+    std::vector<double> RhoLT1(8 * np * nzg, 0.0), RhoRT1(8 * np * nzg, 0.0);
+    std::vector<cd> VL(9 * np * nzg), VR(9 * np * nzg);
+    for (long i = 0; i < np * nzg; i++) { SU3::Unit(&VL[9 * i]); SU3::Unit(&VR[9 * i]); }
+    SetWilsonLines(P, xi_l, 'L', center_l, rhoOldL.data(), rhoL.data(), VL.data());
+    SetWilsonLines(P, xi_r, 'R', center_r, rhoOldR.data(), rhoR.data(), VR.data());
+    if (driver == 0) {
+        SingleSheet(P, rhoL.data(), 0, (N[2] - 2) / 2, RhoLT1.data(), VL.data(), 'L', JpS, Jp, (cd*)U, (cd*)UOld, E);
+        SingleSheet(P, rhoR.data(), N[2] / 2, N[2] - 1, RhoRT1.data(), VR.data(), 'R', JmS, Jm, (cd*)U, (cd*)UOld, E);
+    } else if (driver == 1) {
+        SingleSheet(P, rhoL.data(), 0, N[2] - 1, RhoLT1.data(), VL.data(), 'L', JpS, Jp, (cd*)U, (cd*)UOld, E);
+    } else {
+        SingleSheet(P, rhoR.data(), 0, N[2] - 1, RhoRT1.data(), VR.data(), 'R', JmS, Jm, (cd*)U, (cd*)UOld, E);
+    }
+    L.syncJ(JpS); L.syncJ(Jp); L.syncJ(JmS); L.syncJ(Jm);
+    L.syncU((cd*)U); L.syncU((cd*)UOld); L.syncE(E);
+}
+// --- gauge fixing ---
+void ref_coulomb_gauge(const int* N, const double* a, double* U, double* E, double* G, double* UG, double* EG,
+                       double tol, int maxit, double alpha, double* stats) {
+    Lat<SU3> L = mk3(N, a, 0);
+    CoulombGauge(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, tol, maxit, alpha, stats);
+}
+// Gauge transformation of every field with a site-dependent g (with ghosts): U -> g U g^dag(x+mu), E, J -> g E g^dag.
+void ref_gauge_transform(const int* N, const double* a, int bc, const double* G, double* U, double* E, double* Jp, double* Jm, double* JpS, double* JmS) {
+    Lat<SU3> L = mk3(N, a, bc);
+    const cd* g = (const cd*)G;
+    long n = 27L * L.plane() * (N[2] + 2);
+    std::vector<cd> Uo((cd*)U, (cd*)U + n);
+    for (int z = 0; z < N[2]; z++)
+        for (int y = 0; y < N[1]; y++)
+            for (int x = 0; x < N[0]; x++) {
+                int nb[3][3] = {{x + 1, y, z}, {x, y + 1, z}, {x, y, z + 1}};
+                for (int mu = 0; mu < 3; mu++) {
+                    cd t[9];
+                    SU3::UU(g + 9 * L.idx3(x, y, z), L.Ul(Uo.data(), x, y, z, mu), t);
+                    SU3::UD(t, g + 9 * L.idx3(nb[mu][0], nb[mu][1], nb[mu][2]), L.Ul((cd*)U, x, y, z, mu));
+                    double o[8];
+                    SU3::AdjointMultiplication(g + 9 * L.idx3(x, y, z), L.El(E, x, y, z, mu), o);
+                    std::memcpy(L.El(E, x, y, z, mu), o, sizeof(o));
+                }
+                double* Js[4] = {Jp, Jm, JpS, JmS};
+                for (int k = 0; k < 4; k++) {
+                    double o[8];
+                    SU3::AdjointMultiplication(g + 9 * L.idx3(x, y, z), L.Jl(Js[k], x, y, z), o);
+                    std::memcpy(L.Jl(Js[k], x, y, z), o, sizeof(o));
+                }
+            }
+    L.syncU((cd*)U); L.syncE(E);
+    L.syncJ(Jp); L.syncJ(Jm); L.syncJ(JpS); L.syncJ(JmS);
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Internal definitions shared by the HIP translation units of libglasma.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/glasma.h"
+#include "site.h"
+#include "su3.h"
+
+namespace glasma {
+
+// ------------------------------------------------------------------------------------------------
+// deterministic block reductions (256 threads = 4 wavefronts of 64)
+// ------------------------------------------------------------------------------------------------
+constexpr int kRedBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+}  // namespace glasma
+
+// ------------------------------------------------------------------------------------------------
+// the context
+// ------------------------------------------------------------------------------------------------
+struct glasma_ctx {
+    glasma_params p;
+    glasma::Geom g;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* U[2] = {nullptr, nullptr};
+    double* J[2] = {nullptr, nullptr};
+    double* E = nullptr;
+    double* UOld = nullptr;
+    int cur = 0;  // which of U[.] / J[.] holds the current state
+    double t = 0.0;
+    long tsteps = 0;
+    // reduction scratch
+    double* red = nullptr;
+    size_t red_cap = 0;  // doubles
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_step_ms = 0.0;
+    // gauge fixing state (allocated on first use)
+    double* G = nullptr;       // gauge transformation, 18 planes
+    double* Gtop = nullptr;    // the top ghost layer of G as the reference sees it (18 planes of P)
+    double* UG = nullptr;      // gauge-fixed links
+    double* EG = nullptr;      // gauge-fixed electric field
+    double* fftbuf = nullptr;  // complex work array (2 doubles per entry)
+    size_t fftbuf_cap = 0;     // doubles
+    std::string err;
+};
+
+namespace glasma {
+
+int set_error(glasma_ctx* ctx, int code, const std::string& msg);
+int hip_fail(glasma_ctx* ctx, hipError_t e, const char* what);
+int ensure_red(glasma_ctx* ctx, size_t doubles);
+int ensure_fftbuf(glasma_ctx* ctx, size_t doubles);
+
+// FFT over complex arrays in device memory: lines of length n with element stride `es`; line l
+// starts at (l % b1) * s1 + (l / b1) * s2 (units of complex entries).  sign -1 forward, +1 backward.
+int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign);
+// complex array c[(comp*nz + z)*P + y*nx + x] of `ncomp` components: 1D along z, 2D in (x,y), 3D.
+int fft_z(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);
+int fft_xy(hipStream_t st, double* c, const Geom& g, int ncomp, int nzl, int sign);
+int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);
+
+}  // namespace glasma
+
+#define GL_HIP(ctx, expr)                                         \
+    do {                                                          \
+        hipError_t _e = (expr);                                   \
+        if (_e != hipSuccess) return glasma::hip_fail(ctx, _e, #expr); \
+    } while (0)

@@ -1,0 +1,265 @@
+"""Python host mirror of the reference's field and observable API, over the libglasma C ABI.
+
+The reference keeps its state in global objects (SC/LATTICE/SUNcVariables.cpp:4-62) and acts on them with
+namespace functions.  ``Glasma`` owns one device context and exposes the same operations under the
+reference's names:
+
+===============================================  =================================================
+reference                                        here
+===============================================  =================================================
+Lattice::Init (LATTICE/SUNcVariables.cpp:71)     ``Glasma(N, a, dt)``
+Dynamics::SUNc::Update (SUNcDynamics.cpp:353)    ``Glasma.Update(nsteps)``
+Observables::Bulk::Update (BulkObservables:172)  ``Glasma.bulk()``
+GaussLaw::UpdateViolation (GaussLawViolation:16) ``Glasma.gauss()``
+Unitarity::UpdateViolation (UnitarityViol.:13)   ``Glasma.unitarity()``
+EnergyMomentumTensor::CreateMap (EMT.cpp:92)     ``Glasma.emt_profile()``
+CollidingObjects::SetWilsonLines + SingleSheet   ``Glasma.mv_initial_state(...)``
+CoulombGaugeFixing::SetCoulombGauge (CGF.cpp:32) ``Glasma.coulomb_gauge(...)``
+===============================================  =================================================
+
+Host arrays follow the reference layout with z ghost layers (see ``layout.py``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import GlasmaError, MVParams, Params, check
+
+OBS_NAMES = ("T00", "TXX", "TYY", "TZZ", "T0X", "T0Y", "T0Z", "TXY", "TYZ", "TZX")
+
+
+def _p(a: np.ndarray):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    if a.dtype == np.complex128:
+        a = a.view(np.float64)
+    assert a.dtype == np.float64
+    return a.ctypes.data_as(_lib.dp)
+
+
+def bulk_from_raw(raw, a) -> dict:
+    """T00, TXX, TYY, TZZ, ELECTRIC, MAGNETIC, SOURCE from E^2, B^2, E.J sums (BulkObservables.cpp:22-62)."""
+    a = np.asarray(a, float)
+    acube = a.prod()
+    c = (a / acube) ** 2
+    cS = a / acube ** 2
+    raw = np.asarray(raw)
+    ES, BS, EJ = raw[0:3], raw[3:6], raw[6:9]
+    t = c * ES + c * BS
+    return {
+        "T00": 0.5 * t.sum(), "TXX": 0.5 * (t[1] + t[2] - t[0]), "TYY": 0.5 * (t[0] + t[2] - t[1]),
+        "TZZ": 0.5 * (t[0] + t[1] - t[2]), "ELECTRIC": 0.5 * (c * ES).sum(), "MAGNETIC": 0.5 * (c * BS).sum(),
+        "SOURCE": (cS * EJ).sum(),
+    }
+
+
+class Glasma:
+    """One SU(3) lattice on the current GPU."""
+
+    def __init__(self, N, a=(1.0, 1.0, 1.0), dt: float = 0.25, bc: int = 0):
+        self.N = tuple(int(n) for n in N)
+        self.a = tuple(float(x) for x in a)
+        self.dt = float(dt)
+        self.bc = int(bc)
+        p = Params()
+        p.nc = 3
+        for i in range(3):
+            p.n[i] = self.N[i]
+            p.a[i] = self.a[i]
+        p.dt = self.dt
+        p.bc_z = self.bc
+        h = ctypes.c_void_p()
+        rc = _lib.lib().glasma_create(ctypes.byref(p), ctypes.byref(h))
+        if rc != 0:
+            raise GlasmaError(f"glasma_create failed ({rc}): {_lib.lib().glasma_last_error(None)}")
+        self._h = h
+
+    # -- lifetime -----------------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().glasma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc, what):
+        check(rc, self._h, what)
+
+    # -- fields -------------------------------------------------------------------------------------
+    def shapes(self):
+        Nx, Ny, Nz = self.N
+        return {
+            "U": ((Nz + 2, Ny, Nx, 3, 3, 3), np.complex128),
+            "E": ((Nz + 2, Ny, Nx, 3, 8), np.float64),
+            "J": ((Nz + 2, Ny, Nx, 8), np.float64),
+        }
+
+    def set_fields(self, U=None, E=None, Jp=None, Jm=None, JpS=None, JmS=None):
+        arrs = [U, E, Jp, Jm, JpS, JmS]
+        arrs = [None if x is None else np.ascontiguousarray(x) for x in arrs]
+        self._chk(_lib.lib().glasma_set_fields(self._h, *[_p(x) for x in arrs]), "glasma_set_fields")
+
+    def get_fields(self) -> dict:
+        sh = self.shapes()
+        out = {"U": np.empty(*sh["U"]), "E": np.empty(*sh["E"])}
+        for k in ("Jp", "Jm", "JpS", "JmS"):
+            out[k] = np.empty(*sh["J"])
+        self._chk(_lib.lib().glasma_get_fields(self._h, _p(out["U"]), _p(out["E"]), _p(out["Jp"]), _p(out["Jm"]),
+                                               _p(out["JpS"]), _p(out["JmS"])), "glasma_get_fields")
+        return out
+
+    def get_uold(self) -> np.ndarray:
+        U = np.empty(*self.shapes()["U"])
+        self._chk(_lib.lib().glasma_get_uold(self._h, _p(U)), "glasma_get_uold")
+        return U
+
+    def set_uold(self, U):
+        U = np.ascontiguousarray(U, np.complex128)
+        self._chk(_lib.lib().glasma_set_uold(self._h, _p(U)), "glasma_set_uold")
+
+    def time(self):
+        t = ctypes.c_double()
+        n = ctypes.c_long()
+        self._chk(_lib.lib().glasma_get_time(self._h, ctypes.byref(t), ctypes.byref(n)), "glasma_get_time")
+        return t.value, n.value
+
+    def set_time(self, t: float, tsteps: int):
+        self._chk(_lib.lib().glasma_set_time(self._h, float(t), int(tsteps)), "glasma_set_time")
+
+    def fill_random(self, seed: int = 1, link_scale: float = 0.5, e_scale: float = 0.1, j_scale: float = 0.05):
+        """Synthetic fields generated on the device (no counterpart in the reference)."""
+        self._chk(_lib.lib().glasma_fill_random(self._h, int(seed), link_scale, e_scale, j_scale), "glasma_fill_random")
+
+    # -- dynamics -----------------------------------------------------------------------------------
+    def Update(self, nsteps: int = 1):
+        """Dynamics::SUNc::Update, nsteps times (DYNAMICS/SUNcDynamics.cpp:353-378)."""
+        self._chk(_lib.lib().glasma_update(self._h, int(nsteps)), "glasma_update")
+
+    update = Update
+
+    def update_async(self, nsteps: int):
+        self._chk(_lib.lib().glasma_update_async(self._h, int(nsteps)), "glasma_update_async")
+
+    def synchronize(self):
+        self._chk(_lib.lib().glasma_synchronize(self._h), "glasma_synchronize")
+
+    def last_step_kernel_ms(self) -> float:
+        ms = ctypes.c_double()
+        self._chk(_lib.lib().glasma_last_step_kernel_ms(self._h, ctypes.byref(ms)), "glasma_last_step_kernel_ms")
+        return ms.value
+
+    # -- observables --------------------------------------------------------------------------------
+    def bulk_raw(self) -> np.ndarray:
+        out = np.zeros(9)
+        self._chk(_lib.lib().glasma_bulk(self._h, _p(out)), "glasma_bulk")
+        return out
+
+    def bulk(self) -> dict:
+        return bulk_from_raw(self.bulk_raw(), self.a)
+
+    def gauss(self):
+        pz = np.zeros(self.N[2])
+        mx = ctypes.c_double()
+        rms = ctypes.c_double()
+        self._chk(_lib.lib().glasma_gauss(self._h, _p(pz), ctypes.byref(mx), ctypes.byref(rms)), "glasma_gauss")
+        return pz, mx.value, rms.value
+
+    def unitarity(self):
+        mx = ctypes.c_double()
+        rms = ctypes.c_double()
+        self._chk(_lib.lib().glasma_unitarity(self._h, ctypes.byref(mx), ctypes.byref(rms)), "glasma_unitarity")
+        return mx.value, rms.value
+
+    def emt_profile(self) -> np.ndarray:
+        out = np.zeros((self.N[2], 10))
+        self._chk(_lib.lib().glasma_emt(self._h, _p(out)), "glasma_emt")
+        return out
+
+    # -- initial conditions -------------------------------------------------------------------------
+    @staticmethod
+    def draw_charges(seed: int, Nx: int, Ny: int, qs_l: float, qs_r: float):
+        xl = np.zeros((Ny, Nx, 8))
+        xr = np.zeros((Ny, Nx, 8))
+        check(_lib.lib().glasma_draw_charges(int(seed), Nx, Ny, qs_l, qs_r, _p(xl), _p(xr)), None, "glasma_draw_charges")
+        return xl, xr
+
+    @staticmethod
+    def mv_params(lam, meff, sigma_l, sigma_r, center_l, center_r, mult_l=1.0, mult_r=1.0) -> MVParams:
+        m = MVParams()
+        m.lambda_, m.m_eff, m.sigma_l, m.sigma_r = lam, meff, sigma_l, sigma_r
+        m.nucleus_l, m.nucleus_r = mult_l, mult_r
+        m.center_l, m.center_r = int(center_l), int(center_r)
+        return m
+
+    def mv_initial_state(self, driver: int, mv: MVParams, xi_l, xi_r):
+        """Simulation::Run's initial state (SC/Simulation.cpp:327-378); driver 0 collision, 1 left, 2 right."""
+        xl = np.ascontiguousarray(xi_l, np.float64)
+        xr = np.ascontiguousarray(xi_r, np.float64)
+        self._chk(_lib.lib().glasma_mv_initial_state(self._h, int(driver), ctypes.byref(mv), _p(xl), _p(xr)),
+                  "glasma_mv_initial_state")
+
+    def set_wilson_lines(self, direction: str, mv: MVParams, xi):
+        Nx, Ny, Nz = self.N
+        V = np.empty((Nz + 2, Ny, Nx, 3, 3), np.complex128)
+        rho = np.empty((Nz + 2, Ny, Nx, 8))
+        x = np.ascontiguousarray(xi, np.float64)
+        self._chk(_lib.lib().glasma_set_wilson_lines(self._h, direction.encode()[0:1], ctypes.byref(mv), _p(x),
+                                                     _p(V), _p(rho)), "glasma_set_wilson_lines")
+        return V, rho
+
+    # -- gauge fixing -------------------------------------------------------------------------------
+    def coulomb_gauge(self, tol: float = 1e-15, maxit: int = 1000, alpha: float = 2.0) -> np.ndarray:
+        stats = np.zeros(7)
+        self._chk(_lib.lib().glasma_coulomb_gauge(self._h, tol, int(maxit), alpha, _p(stats)), "glasma_coulomb_gauge")
+        return stats
+
+    def gauge_reset(self):
+        self._chk(_lib.lib().glasma_gauge_reset(self._h), "glasma_gauge_reset")
+
+    def get_gauge_fixed(self):
+        sh = self.shapes()
+        Nx, Ny, Nz = self.N
+        UG = np.empty(*sh["U"])
+        EG = np.empty(*sh["E"])
+        G = np.empty((Nz + 2, Ny, Nx, 3, 3), np.complex128)
+        self._chk(_lib.lib().glasma_get_gauge_fixed(self._h, _p(UG), _p(EG), _p(G)), "glasma_get_gauge_fixed")
+        return UG, EG, G
+
+    def set_gauge_transformation(self, G):
+        G = np.ascontiguousarray(G, np.complex128)
+        self._chk(_lib.lib().glasma_set_gauge_transformation(self._h, _p(G)), "glasma_set_gauge_transformation")
+
+
+def fft_host(data: np.ndarray, dims, batch: int, sign: int) -> np.ndarray:
+    """Device FFT of a host complex array (FFTW conventions, unnormalised)."""
+    d = np.ascontiguousarray(data, np.complex128).copy()
+    dv = (ctypes.c_int * len(dims))(*[int(x) for x in dims])
+    check(_lib.lib().glasma_fft_host(_p(d), len(dims), dv, int(batch), int(sign)), None, "glasma_fft_host")
+    return d
+
+
+def su3_batch(op: int, c: float, inp: np.ndarray, in2: np.ndarray | None = None) -> np.ndarray:
+    """Batched per-site SU(3) algebra on the device (see glasma_su3_batch in include/glasma.h)."""
+    inp = np.ascontiguousarray(inp)
+    n = inp.shape[0]
+    out_w = {0: 18, 1: 8, 2: 8, 3: 8, 4: 8}[op]
+    out = np.zeros((n, out_w))
+    i2 = None if in2 is None else np.ascontiguousarray(in2, np.float64)
+    a = inp.view(np.float64) if inp.dtype == np.complex128 else inp
+    check(_lib.lib().glasma_su3_batch(int(op), n, float(c), _p(np.ascontiguousarray(a)), _p(i2), _p(out)), None,
+          "glasma_su3_batch")
+    return out

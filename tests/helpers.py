@@ -1,0 +1,69 @@
+"""Shared test helpers: seeded SU(3) configurations in the reference layout and comparison utilities."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import oracle as O
+from soerenpragya3dglasma_amd import layout as L
+
+# Gell-Mann matrices / 2 (the reference's t^a, SC/SUNc/SU3/AlgebraOperations.cpp:6-45)
+_lam = np.zeros((8, 3, 3), complex)
+_lam[0][0, 1] = _lam[0][1, 0] = 1
+_lam[1][0, 1] = -1j
+_lam[1][1, 0] = 1j
+_lam[2][0, 0] = 1
+_lam[2][1, 1] = -1
+_lam[3][0, 2] = _lam[3][2, 0] = 1
+_lam[4][0, 2] = -1j
+_lam[4][2, 0] = 1j
+_lam[5][1, 2] = _lam[5][2, 1] = 1
+_lam[6][1, 2] = -1j
+_lam[6][2, 1] = 1j
+_lam[7] = np.diag([1, 1, -2]) / np.sqrt(3)
+T = _lam / 2
+
+
+def herm(alpha):
+    return np.einsum("...a,aij->...ij", alpha, T)
+
+
+def random_su3(rng, shape, scale):
+    """exp(i alpha^a t^a) with Gaussian alpha of width `scale` (via eigendecomposition in numpy)."""
+    al = rng.normal(size=tuple(shape) + (8,)) * scale
+    H = herm(al)
+    w, v = np.linalg.eigh(H)
+    return np.einsum("...ij,...j,...kj->...ik", v, np.exp(1j * w), v.conj())
+
+
+def random_fields(N, seed=0, link_scale=0.6, e_scale=0.3, j_scale=0.2, bc=0) -> O.Fields:
+    rng = np.random.default_rng(seed)
+    Nx, Ny, Nz = N
+    f = O.Fields(N)
+    M = random_su3(rng, (Nz, Ny, Nx, 3), link_scale)  # [..., i, j]
+    f.U[1:-1] = np.swapaxes(M, -1, -2)  # storage [..., j, i]
+    f.E[1:-1] = rng.normal(size=f.E[1:-1].shape) * e_scale
+    for J in (f.Jp, f.Jm, f.JpS, f.JmS):
+        J[1:-1] = rng.normal(size=J[1:-1].shape) * j_scale
+    f.sync(bc)
+    return f
+
+
+def upload(g, f: O.Fields):
+    g.set_fields(f.U, f.E, f.Jp, f.Jm, f.JpS, f.JmS)
+
+
+def relerr(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def fields_maxdiff(d: dict, f: O.Fields) -> dict:
+    return {
+        "U": float(np.abs(d["U"] - f.U).max()),
+        "E": float(np.abs(d["E"] - f.E).max()),
+        "Jp": float(np.abs(d["Jp"] - f.Jp).max()),
+        "Jm": float(np.abs(d["Jm"] - f.Jm).max()),
+        "JpS": float(np.abs(d["JpS"] - f.JpS).max()),
+        "JmS": float(np.abs(d["JmS"] - f.JmS).max()),
+    }
