@@ -7,7 +7,8 @@ import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "_lib")
-LIB_PATH = os.path.join(LIB_DIR, "libglasma.so")
+# GLASMA_LIB selects another build of the same library (used for A/B measurements of build variants)
+LIB_PATH = os.environ.get("GLASMA_LIB") or os.path.join(LIB_DIR, "libglasma.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "glasma.h")
 
 _lib = None

@@ -31,6 +31,8 @@ HIP_FLAGS = [
 HOST_CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 
 LIB_NAME = "libglasma.so"
+# extra compiler flags for experiments (e.g. -DGLASMA_NT_STORES) and an alternative output directory
+EXTRA = os.environ.get("GLASMA_CXXFLAGS", "").split()
 
 
 def _run(cmd):
@@ -46,19 +48,22 @@ def _sources():
     return hip, cpp
 
 
-def build(verbose: bool = False, jobs: int = 8) -> str:
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose: bool = False, jobs: int = 8, out_dir: str | None = None) -> str:
+    out = os.path.abspath(out_dir) if out_dir else OUT
+    obj = os.path.join(out, "obj")
+    os.makedirs(obj, exist_ok=True)
+    flags = HIP_FLAGS + EXTRA
     hip, cpp = _sources()
     cmds = []
     objs = []
     for f in hip:
-        o = os.path.join(OBJ, f + ".o")
+        o = os.path.join(obj, f + ".o")
         objs.append(o)
-        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, f), "-o", o])
+        cmds.append([HIPCC, *flags, "-c", os.path.join(CSRC, f), "-o", o])
     for f in cpp:
-        o = os.path.join(OBJ, f + ".o")
+        o = os.path.join(obj, f + ".o")
         objs.append(o)
-        cmds.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, f), "-o", o])
+        cmds.append([HIPCC, *flags, "-x", "hip", "-c", os.path.join(CSRC, f), "-o", o])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         outs = list(ex.map(_run, cmds))
     if verbose:
@@ -66,20 +71,23 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
             print(" ".join(c))
             if o.strip():
                 print(o)
-    lib_tmp = os.path.join(OUT, LIB_NAME + ".tmp")
+    lib_tmp = os.path.join(out, LIB_NAME + ".tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib_tmp, *objs])
-    lib = os.path.join(OUT, LIB_NAME)
+    lib = os.path.join(out, LIB_NAME)
     os.replace(lib_tmp, lib)
     main = os.path.join(CSRC, "glasma_solver_main.cpp")
     if os.path.exists(main):
-        _run([HIPCC, *HIP_FLAGS, "-x", "hip", main, "-o", os.path.join(OUT, "glasma_solver"),
-              "-L", OUT, "-lglasma", f"-Wl,-rpath,{OUT}", "-Wl,-rpath,$ORIGIN"])
+        _run([HIPCC, *flags, "-x", "hip", main, "-o", os.path.join(out, "glasma_solver"),
+              "-L", out, "-lglasma", "-Wl,-rpath,$ORIGIN"])
     hc = os.path.join(CSRC, "hostcheck.cpp")
     if os.path.exists(hc) and shutil.which("g++"):
-        _run(["g++", *HOST_CXX_FLAGS, "-shared", "-o", os.path.join(OUT, "libglasma_hostcheck.so"), hc])
+        _run(["g++", *HOST_CXX_FLAGS, "-shared", "-o", os.path.join(out, "libglasma_hostcheck.so"), hc])
     return lib
 
 
 if __name__ == "__main__":
-    path = build(verbose="-v" in sys.argv)
+    od = None
+    if "--out" in sys.argv:
+        od = sys.argv[sys.argv.index("--out") + 1]
+    path = build(verbose="-v" in sys.argv, out_dir=od)
     print(path)

@@ -40,22 +40,24 @@ int ensure_fftbuf(glasma_ctx* ctx, size_t doubles) {
 // ------------------------------------------------------------------------------------------------
 // AoS <-> SoA transposition (record of R doubles per site)
 // ------------------------------------------------------------------------------------------------
-__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long S, int R) {
+// record of R doubles per site (AoS) <-> planes p0 .. p0+R-1 of a tiled field with NP planes
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long S, int R, int NP, int p0) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S * R) return;
     long s = i / R;
     int j = (int)(i % R);
-    soa[(long)j * S + s] = aos[i];
+    soa[lidx<long>(NP, p0 + j, s)] = aos[i];
 }
-__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long S, int R) {
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long S, int R, int NP, int p0) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= S * R) return;
     long s = i / R;
     int j = (int)(i % R);
-    aos[i] = soa[(long)j * S + s];
+    aos[i] = soa[lidx<long>(NP, p0 + j, s)];
 }
 
-static int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* dev_soa, int R) {
+static int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* dev_soa, int R, int NP = -1, int p0 = 0) {
+    if (NP < 0) NP = R;
     const long S = ctx->g.S, P = ctx->g.P;
     double* tmp = nullptr;
     GL_HIP(ctx, hipMalloc(&tmp, (size_t)S * R * sizeof(double)));
@@ -63,7 +65,8 @@ static int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* d
                                   hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
         long n = S * R;
-        hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, tmp, dev_soa, S, R);
+        hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, tmp, dev_soa, S, R,
+                           NP, p0);
         e = hipStreamSynchronize(ctx->stream);
     }
     hipFree(tmp);
@@ -84,12 +87,15 @@ static void host_sync_ghosts(const glasma_ctx* ctx, double* h, int R) {
     }
 }
 
-static int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_with_ghosts, int R) {
+static int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_with_ghosts, int R, int NP = -1,
+                        int p0 = 0) {
+    if (NP < 0) NP = R;
     const long S = ctx->g.S, P = ctx->g.P;
     double* tmp = nullptr;
     GL_HIP(ctx, hipMalloc(&tmp, (size_t)S * R * sizeof(double)));
     long n = S * R;
-    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, dev_soa, tmp, S, R);
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, dev_soa, tmp, S, R, NP,
+                       p0);
     hipError_t e = hipMemcpyAsync(host_with_ghosts + (size_t)P * R, tmp, (size_t)S * R * sizeof(double),
                                   hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -99,12 +105,19 @@ static int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_wit
     return 0;
 }
 
+static int upload_j(glasma_ctx* ctx, const double* h, double* J, int w) { return upload_aos(ctx, h, J, 8, kJPlanes, 8 * w); }
+static int download_j(glasma_ctx* ctx, const double* J, double* h, int w) {
+    return download_aos(ctx, J, h, 8, kJPlanes, 8 * w);
+}
+
 // identity links / zero fields
+// identity matrices in all nmat matrix slots of a field with 18*nmat planes
 __global__ void k_set_identity(double* __restrict__ U, long S, int nmat) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     for (int m = 0; m < nmat; m++)
-        for (int e = 0; e < 18; e++) U[((long)m * 18 + e) * S + s] = (e == 0 || e == 8 || e == 16) ? 1.0 : 0.0;
+        for (int e = 0; e < 18; e++)
+            U[lidx<long>(18 * nmat, 18 * m + e, s)] = (e == 0 || e == 8 || e == 16) ? 1.0 : 0.0;
 }
 
 // counter-based uniform in (0,1): splitmix64 finaliser of (seed, index)
@@ -126,11 +139,13 @@ __global__ void k_fill_random(double* __restrict__ U, double* __restrict__ E, do
         for (int a = 0; a < 8; a++) al[a] = 2.0 * hash_uniform(seed, base + mu * 8 + a) - 1.0;
         M3 X;
         expi(su, al, X);
-        stm(U + mu * 18 * S, S, s, X);
-        for (int a = 0; a < 8; a++) E[(mu * 8 + a) * S + s] = se * (2.0 * hash_uniform(seed ^ 0x5555ull, base + mu * 8 + a) - 1.0);
+        stm<long>(U, kUPlanes, 18 * mu, s, X);
+        for (int a = 0; a < 8; a++)
+            E[lidx<long>(kEPlanes, mu * 8 + a, s)] = se * (2.0 * hash_uniform(seed ^ 0x5555ull, base + mu * 8 + a) - 1.0);
     }
     for (int w = 0; w < 4; w++)
-        for (int a = 0; a < 8; a++) J[(w * 8 + a) * S + s] = sj * (2.0 * hash_uniform(seed ^ 0xAAAAull, base + w * 8 + a) - 1.0);
+        for (int a = 0; a < 8; a++)
+            J[lidx<long>(kJPlanes, w * 8 + a, s)] = sj * (2.0 * hash_uniform(seed ^ 0xAAAAull, base + w * 8 + a) - 1.0);
 }
 
 int reset_fields(glasma_ctx* ctx) {
@@ -139,9 +154,10 @@ int reset_fields(glasma_ctx* ctx) {
     hipLaunchKernelGGL(k_set_identity, grid, dim3(256), 0, ctx->stream, ctx->U[0], S, 3);
     hipLaunchKernelGGL(k_set_identity, grid, dim3(256), 0, ctx->stream, ctx->U[1], S, 3);
     hipLaunchKernelGGL(k_set_identity, grid, dim3(256), 0, ctx->stream, ctx->UOld, S, 3);
-    GL_HIP(ctx, hipMemsetAsync(ctx->E, 0, (size_t)kEPlanes * S * sizeof(double), ctx->stream));
-    GL_HIP(ctx, hipMemsetAsync(ctx->J[0], 0, (size_t)kJPlanes * S * sizeof(double), ctx->stream));
-    GL_HIP(ctx, hipMemsetAsync(ctx->J[1], 0, (size_t)kJPlanes * S * sizeof(double), ctx->stream));
+    const long Sp = pad_sites(S);
+    GL_HIP(ctx, hipMemsetAsync(ctx->E, 0, (size_t)kEPlanes * Sp * sizeof(double), ctx->stream));
+    GL_HIP(ctx, hipMemsetAsync(ctx->J[0], 0, (size_t)kJPlanes * Sp * sizeof(double), ctx->stream));
+    GL_HIP(ctx, hipMemsetAsync(ctx->J[1], 0, (size_t)kJPlanes * Sp * sizeof(double), ctx->stream));
     GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->cur = 0;
     ctx->t = 0.0;
@@ -209,19 +225,14 @@ __device__ __forceinline__ bool site_of(const ObsArgs& A, int& x, int& y, int& z
 
 __device__ __forceinline__ void plaq_full(const double* __restrict__ U, long S, int kind, int m1, long s1, int m2,
                                           long s2, int m3, long s3, int m4, long s4, M3& Pq) {
-    M3 A, B, L1, L2;
-    ldm(U + m1 * 18 * S, S, s1, A);
-    ldm(U + m2 * 18 * S, S, s2, B);
-    if (kind == 0) mul_nn(A, B, L1);
-    else if (kind == 1) mul_nd(A, B, L1);
-    else if (kind == 2) mul_dn(A, B, L1);
-    else mul_dd(A, B, L1);
-    ldm(U + m3 * 18 * S, S, s3, A);
-    ldm(U + m4 * 18 * S, S, s4, B);
-    if (kind == 0) { mul_nn(B, A, L2); mul_nd(L1, L2, Pq); }
-    else if (kind == 1) { mul_dn(A, B, L2); mul_nn(L1, L2, Pq); }
-    else if (kind == 2) { mul_nd(A, B, L2); mul_nn(L1, L2, Pq); }
-    else { mul_nn(A, B, L2); mul_nn(L1, L2, Pq); }  // DDUU = (v1^dag v2^dag)(v3 v4)
+    (void)S;
+    plaq_mat<long>(U, kind, m1, s1, m2, s2, m3, s3, m4, s4, Pq);
+}
+__device__ __forceinline__ double Ev(const double* __restrict__ E, int mu, int a, long s) {
+    return E[lidx<long>(kEPlanes, mu * 8 + a, s)];
+}
+__device__ __forceinline__ double Jv(const double* __restrict__ J, int w, int a, long s) {
+    return J[lidx<long>(kJPlanes, w * 8 + a, s)];
 }
 
 // Observables::Bulk (SC/OBSERVABLES/BulkObservables.cpp:65-169): K = 9 (E^2[3], B^2[3], E.J[3])
@@ -241,11 +252,11 @@ __global__ void __launch_bounds__(256) k_bulk(ObsArgs A) {
         plaq_full(A.U, S, 0, 2, s, 0, szp, 2, sxp, 0, s, Pq);
         v[4] = 4.0 * retr_one_minus(Pq);  // B1 from Uzx
         for (int a = 0; a < 8; a++) {
-            const double e0 = A.E[(0 * 8 + a) * S + s], e1 = A.E[(1 * 8 + a) * S + s], e2 = A.E[(2 * 8 + a) * S + s];
+            const double e0 = Ev(A.E, 0, a, s), e1 = Ev(A.E, 1, a, s), e2 = Ev(A.E, 2, a, s);
             v[0] += e0 * e0;
             v[1] += e1 * e1;
             v[2] += e2 * e2;
-            v[8] += e2 * (A.J[(JP * 8 + a) * S + s] - A.J[(JM * 8 + a) * S + s]) / kSqrt2;
+            v[8] += e2 * (Jv(A.J, JP, a, s) - Jv(A.J, JM, a, s)) / kSqrt2;
         }
     }
     block_store<9>(v, A.red + ((long)blockIdx.y * A.bpp + blockIdx.x) * 9);
@@ -264,19 +275,18 @@ __global__ void __launch_bounds__(256) k_gauss(ObsArgs A) {
         const double norm = 1.0 / (A.a[0] * A.a[1] * A.a[2]);
         M3 V;
         double e[8], xd[8], yd[8], zd[8];
-        ldm(A.U + 0 * 18 * S, S, sxm, V);
-        ldv(A.E + 0 * 8 * S, S, sxm, e);
+        ldm<long>(A.U, kUPlanes, 0, sxm, V);
+        ldv<long>(A.E, kEPlanes, 0, sxm, e);
         inv_adj_mul(V, e, xd);
-        ldm(A.U + 1 * 18 * S, S, sym, V);
-        ldv(A.E + 1 * 8 * S, S, sym, e);
+        ldm<long>(A.U, kUPlanes, 18, sym, V);
+        ldv<long>(A.E, kEPlanes, 8, sym, e);
         inv_adj_mul(V, e, yd);
-        ldm(A.U + 2 * 18 * S, S, szm, V);
-        ldv(A.E + 2 * 8 * S, S, szm, e);
+        ldm<long>(A.U, kUPlanes, 36, szm, V);
+        ldv<long>(A.E, kEPlanes, 16, szm, e);
         inv_adj_mul(V, e, zd);
         for (int a = 0; a < 8; a++) {
-            const double G = norm * ((A.E[(0 * 8 + a) * S + s] - xd[a]) + (A.E[(1 * 8 + a) * S + s] - yd[a]) +
-                                     (A.E[(2 * 8 + a) * S + s] - zd[a]) -
-                                     (A.J[(JPS * 8 + a) * S + s] + A.J[(JMS * 8 + a) * S + s]) / kSqrt2);
+            const double G = norm * ((Ev(A.E, 0, a, s) - xd[a]) + (Ev(A.E, 1, a, s) - yd[a]) + (Ev(A.E, 2, a, s) - zd[a]) -
+                                     (Jv(A.J, JPS, a, s) + Jv(A.J, JMS, a, s)) / kSqrt2);
             v[0] += G * G;
             m[0] = fmax(m[0], fabs(G));
         }
@@ -294,7 +304,7 @@ __global__ void __launch_bounds__(256) k_unitarity(ObsArgs A) {
     if (site_of(A, x, y, z, s)) {
         for (int mu = 0; mu < 3; mu++) {
             M3 V;
-            ldm(A.U + mu * 18 * A.g.S, A.g.S, s, V);
+            ldm<long>(A.U, kUPlanes, 18 * mu, s, V);
             const double n = unitarity_norm(V);
             v[0] += n * n;
             m[0] = fmax(m[0], n);
@@ -363,16 +373,16 @@ __global__ void __launch_bounds__(256) k_emt(ObsArgs A) {
         const long sm[3] = {sxm, sym, szm};
         for (int mu = 0; mu < 3; mu++) {
             double* Em = mu == 0 ? E0 : (mu == 1 ? E1 : E2);
-            ldm(A.U + mu * 18 * S, S, sm[mu], V);
-            ldv(A.E + mu * 8 * S, S, sm[mu], e);
+            ldm<long>(A.U, kUPlanes, 18 * mu, sm[mu], V);
+            ldv<long>(A.E, kEPlanes, 8 * mu, sm[mu], e);
             inv_adj_mul(V, e, ed);
             double es = 0.0;
             for (int a = 0; a < 8; a++) {
-                const double eh = A.E[(mu * 8 + a) * S + s0];
+                const double eh = Ev(A.E, mu, a, s0);
                 Em[a] = 0.5 * (eh + ed[a]);
             }
             for (int a = 0; a < 8; a++) {
-                const double eh = A.E[(mu * 8 + a) * S + s0];
+                const double eh = Ev(A.E, mu, a, s0);
                 es += 0.5 * (eh * eh + e[a] * e[a]);
             }
             if (mu == 0) E0S = es; else if (mu == 1) E1S = es; else E2S = es;
@@ -475,7 +485,7 @@ int glasma_create(const glasma_params* p, glasma_ctx** out) {
     ctx->g.bc = p->bc_z;
     ctx->g.P = (long)p->n[0] * p->n[1];
     ctx->g.S = ctx->g.P * p->n[2];
-    const long S = ctx->g.S;
+    const long S = pad_sites(ctx->g.S);
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess) e = hipStreamCreate(&ctx->stream);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
@@ -535,7 +545,7 @@ int glasma_set_fields(glasma_ctx* ctx, const double* U, const double* E, const d
     if (E && (rc = upload_aos(ctx, E, ctx->E, kEPlanes))) return rc;
     const double* Js[4] = {Jp, Jm, JpStat, JmStat};
     for (int w = 0; w < 4; w++)
-        if (Js[w] && (rc = upload_aos(ctx, Js[w], ctx->J[ctx->cur] + (size_t)w * 8 * S, 8))) return rc;
+        if (Js[w] && (rc = upload_j(ctx, Js[w], ctx->J[ctx->cur], w))) return rc;
     return 0;
 }
 
@@ -547,7 +557,7 @@ int glasma_get_fields(glasma_ctx* ctx, double* U, double* E, double* Jp, double*
     if (E && (rc = download_aos(ctx, ctx->E, E, kEPlanes))) return rc;
     double* Js[4] = {Jp, Jm, JpStat, JmStat};
     for (int w = 0; w < 4; w++)
-        if (Js[w] && (rc = download_aos(ctx, ctx->J[ctx->cur] + (size_t)w * 8 * S, Js[w], 8))) return rc;
+        if (Js[w] && (rc = download_j(ctx, ctx->J[ctx->cur], Js[w], w))) return rc;
     return 0;
 }
 

@@ -1,7 +1,7 @@
-// Per-site kernels bodies of the glasma engine, written once for device and host.
+// Device field layout and the per-site bodies of the glasma engine, written once for device and host.
 //
 // The HIP kernels call these functions for their site; the host build (hostcheck.cpp, compiled with
-// g++) calls the very same code so that the arithmetic of the product can also be exercised on a CPU.
+// g++) runs the very same code so that the product's arithmetic can also be exercised on a CPU.
 #pragma once
 
 #include "su3.h"
@@ -9,15 +9,26 @@
 namespace glasma {
 
 // ------------------------------------------------------------------------------------------------
-// Device layout (structure of arrays, no ghost layers).  Site index s = x + Nx*(y + Ny*z), S sites.
-//   links    U[(mu*18 + 2k + c) * S + s]    k = i + 3j, c = 0 real / 1 imaginary   (54 planes)
-//   E field  E[(mu*8 + a) * S + s]                                                  (24 planes)
-//   currents J[(w*8 + a) * S + s]  w = 0 J+, 1 J-, 2 J+stat, 3 J-stat              (32 planes)
-// Every plane is contiguous in x, so a 64-lane wavefront reading one plane entry for 64 consecutive
-// sites issues one fully coalesced 512-byte access.
+// Device layout: tiles of 64 consecutive sites (s = x + Nx*(y + Ny*z)), structure of arrays inside a
+// tile.  A field with NP scalar planes per site stores plane p of site s at
+//        ((s / 64) * NP + p) * 64 + s % 64.
+// A wavefront reading one plane for its 64 sites issues one aligned 512-byte access, and all planes
+// of a tile are contiguous (a site's 3 links: 27.6 KB per tile), so a wavefront touches a few pages
+// instead of one page per plane.  The site count is padded to a multiple of 64.
+//   links    NP = 54: plane mu*18 + 2k + c, k = i + 3j (column-major), c = 0 real / 1 imaginary
+//   E field  NP = 24: plane mu*8 + a
+//   currents NP = 32: plane w*8 + a, w = 0 J+, 1 J-, 2 J+stat, 3 J-stat
+//   scalar   NP = 18 (one matrix per site: Wilson lines, gauge transformations)
 // ------------------------------------------------------------------------------------------------
-constexpr int kUPlanes = 54, kEPlanes = 24, kJPlanes = 32;
+constexpr int kTile = 64;
+constexpr int kUPlanes = 54, kEPlanes = 24, kJPlanes = 32, kGPlanes = 18;
 enum { JP = 0, JM = 1, JPS = 2, JMS = 3 };
+
+GHD long pad_sites(long S) { return (S + kTile - 1) / kTile * kTile; }
+template <class I>
+GHD long lidx(int NP, int p, I s) {
+    return ((long)(s / (I)kTile) * NP + p) * kTile + (long)(s % (I)kTile);
+}
 
 struct Geom {
     int nx, ny, nz;
@@ -35,46 +46,139 @@ GHD long sidx(const Geom& g, int x, int y, int z) {
     return (long)xwrap(x, g.nx) + (long)g.nx * ((long)xwrap(y, g.ny) + (long)g.ny * z);
 }
 
-GHD void ldm(const double* __restrict__ base, long S, long s, M3& A) {
+// Address of plane p0 of site s.  On the device the address is passed through an empty asm: each
+// access then starts from an opaque base, so identical loads in different stages of a kernel are not
+// merged (merging keeps whole matrices live across stages and exhausts the register file); the
+// plane offsets k*512 bytes are folded into the instructions' immediate offsets.
+template <class I>
+GHD const double* site_ptr(const double* base, int NP, int p0, I s) {
+    const double* q = base + lidx<I>(NP, p0, s);
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(q));
+#endif
+    return q;
+}
+template <class I>
+GHD double* site_ptr(double* base, int NP, int p0, I s) {
+    double* q = base + lidx<I>(NP, p0, s);
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(q));
+#endif
+    return q;
+}
+// streamed outputs are written non-temporally so that they do not evict neighbour data from L2 / MALL
+GHD void st_nt(double* p, double v) {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GLASMA_NO_NT_STORES)
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+template <class I>
+GHD void ldm(const double* __restrict__ base, int NP, int p0, I s, M3& A) {
+    const double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        A.re[k] = base[(2 * k) * S + s];
-        A.im[k] = base[(2 * k + 1) * S + s];
+        A.re[k] = q[(2 * k) * kTile];
+        A.im[k] = q[(2 * k + 1) * kTile];
     }
 }
-GHD void stm(double* __restrict__ base, long S, long s, const M3& A) {
+template <class I>
+GHD void stm(double* __restrict__ base, int NP, int p0, I s, const M3& A) {
+    double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        base[(2 * k) * S + s] = A.re[k];
-        base[(2 * k + 1) * S + s] = A.im[k];
+        st_nt(q + (2 * k) * kTile, A.re[k]);
+        st_nt(q + (2 * k + 1) * kTile, A.im[k]);
     }
 }
-GHD void ldv(const double* __restrict__ base, long S, long s, double* v) {
+template <class I>
+GHD void ldv(const double* __restrict__ base, int NP, int p0, I s, double* v) {
+    const double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
-    for (int a = 0; a < 8; a++) v[a] = base[a * S + s];
+    for (int a = 0; a < 8; a++) v[a] = q[a * kTile];
 }
-GHD void stv(double* __restrict__ base, long S, long s, const double* v) {
+template <class I>
+GHD void stv(double* __restrict__ base, int NP, int p0, I s, const double* v) {
+    double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
-    for (int a = 0; a < 8; a++) base[a * S + s] = v[a];
+    for (int a = 0; a < 8; a++) st_nt(q + a * kTile, v[a]);
+}
+
+template <class I>
+GHD void site_coords(const Geom& g, I s, int& x, int& y, int& z) {
+    x = (int)(s % (I)g.nx);
+    const I r_ = s / (I)g.nx;
+    y = (int)(r_ % (I)g.ny);
+    z = (int)(r_ / (I)g.ny);
+}
+
+// site index of (x,y,z) moved by s1 along d1 and s2 along d2 (0 = no move); x, y periodic, z with the
+// reference's ghost semantics
+template <class I>
+GHD I nbr(const Geom& g, int x, int y, int z, int d1, int s1, int d2, int s2) {
+    int xx = x + (d1 == 0 ? s1 : 0) + (d2 == 0 ? s2 : 0);
+    int yy = y + (d1 == 1 ? s1 : 0) + (d2 == 1 ? s2 : 0);
+    const int dz = (d1 == 2 ? s1 : 0) + (d2 == 2 ? s2 : 0);
+    xx = xwrap(xx, g.nx);
+    yy = xwrap(yy, g.ny);
+    const int zz = dz > 0 ? zup(g, z) : (dz < 0 ? zdn(g, z) : z);
+    return (I)xx + (I)g.nx * ((I)yy + (I)g.ny * (I)zz);
+}
+
+// four-link products of SC/CALC/Plaquettes.cpp:83-103 (SC/SUNc/AdvancedOperations.cpp:175-228):
+// KIND 0: UUDD(v1,v2,v3,v4) = (v1 v2)(v4 v3)^dag ; 1: UDDU = (v1 v2^dag)(v3^dag v4);
+// 2: DUUD = (v1^dag v2)(v3 v4^dag) ; 3: DDUU = (v1^dag v2^dag)(v3 v4).  U has 54 planes, link mu at
+// plane 18*mu.
+template <class I>
+GHD void plaq_mat(const double* __restrict__ U, int kind, int m1, I s1, int m2, I s2, int m3, I s3, int m4, I s4,
+                  M3& Pq) {
+    M3 A, B, L1, L2;
+    ldm<I>(U, kUPlanes, 18 * m1, s1, A);
+    ldm<I>(U, kUPlanes, 18 * m2, s2, B);
+    if (kind == 0) mul_nn(A, B, L1);
+    else if (kind == 1) mul_nd(A, B, L1);
+    else if (kind == 2) mul_dn(A, B, L1);
+    else mul_dd(A, B, L1);
+    ldm<I>(U, kUPlanes, 18 * m3, s3, A);
+    ldm<I>(U, kUPlanes, 18 * m4, s4, B);
+    if (kind == 0) {
+        mul_nn(B, A, L2);
+        mul_nd(L1, L2, Pq);
+    } else if (kind == 1) {
+        mul_dn(A, B, L2);
+        mul_nn(L1, L2, Pq);
+    } else if (kind == 2) {
+        mul_nd(A, B, L2);
+        mul_nn(L1, L2, Pq);
+    } else {
+        mul_nn(A, B, L2);
+        mul_nn(L1, L2, Pq);
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
-// The fused leapfrog step of one site: Dynamics::SUNc::Update (SC/DYNAMICS/SUNcDynamics.cpp:353-378).
+// The leapfrog step: Dynamics::SUNc::Update (SC/DYNAMICS/SUNcDynamics.cpp:353-378) fused per site.
 //
-//   E'(x)       = E(x) + gamma c_mu sum_nu c_nu [Re tr(i t P_mu,nu) - Re tr(i t P_mu,-nu)]  (:134-203)
-//                  - (dt/a_z) (J+(x) - J-(x)) / sqrt2  on E_z
-//   J+-stat'(x) = J+-stat(x) -+ (dt/a_z) [J+-(x) - U_z^dag(x-z) J+-(x-z) U_z(x-z)]          (:205-236)
-//   X_mu        = exp(i c_mu E'_mu(x)),  c_mu = -(dt/2) a_mu^2 / a^3                         (:62-106)
-//   U_h         = X U,  UOld = U_h                                                             (:108-128)
-//   J+-'(x)     = J+-(x) -+ (dt/a_z) [U_h,z(x) J+-stat'(x+z) U_h,z^dag(x) - J+-stat'(x)]       (:238-269)
+//   E'_mu(x)    = E_mu(x) + gamma c_mu Re tr(i t^a U_mu(x) Sigma_mu(x))                       (:134-203)
+//                 (- (dt/a_z)(J+(x) - J-(x))/sqrt2 on E_z), with the staple sum
+//   Sigma_mu    = sum_{nu != mu} c_nu [ U_nu(x+mu) U_mu^dag(x+nu) U_nu^dag(x)
+//                                      + U_nu^dag(x+mu-nu) U_mu^dag(x-nu) U_nu(x-nu) ]
+//   J+-stat'(x) = J+-stat(x) -+ (dt/a_z) [J+-(x) - U_z^dag(x-z) J+-(x-z) U_z(x-z)]            (:205-236)
+//   X_mu        = exp(i c_mu E'_mu(x)),  c_mu = -(dt/2) a_mu^2 / a^3                           (:62-106)
+//   U_h         = X U,  UOld = U_h                                                               (:108-128)
+//   J+-'(x)     = J+-(x) -+ (dt/a_z) [U_h,z(x) J+-stat'(x+z) U_h,z^dag(x) - J+-stat'(x)]         (:238-269)
 //   U'          = X U_h
 //
-// Every stage reads only time-t data in a small neighbourhood, so a site advances itself from
-// time-t buffers into time-(t+dt) buffers: J+-stat'(x+z) is recomputed at x from time-t data (it
-// needs U_z(x), J(x), J(x+z), J-stat(x+z)), and no grid-wide barrier is needed.  U and J are
-// ping-pong buffers; E is updated in place (only site x reads E(x)).  X is computed once and
-// applied twice; the reference recomputes it from identical inputs.  It is applied as U + (X - 1) U
-// with X - 1 formed directly (expi_m1), which keeps rounding from biasing U away from SU(3).
+// The staple form is the reference's nine plaquettes regrouped: since Re tr(i t^a X^dag) =
+// -Re tr(i t^a X), the six plaquettes of :181-183 that contain U_mu(x) combine into U_mu(x) Sigma_mu
+// (the same 27 matrix products per site).  Every stage reads only time-t data in a small
+// neighbourhood, so a site advances itself from time-t buffers into time-(t+dt) buffers (U and J are
+// ping-pong buffers; E is updated in place, only site x reads E(x)); J+-stat'(x+z) is recomputed at x
+// from time-t data, so no grid-wide barrier is needed.  X is computed once and applied twice (the
+// reference recomputes it from identical inputs), as U + (X - 1) U with X - 1 formed directly, which
+// keeps rounding from biasing U away from SU(3).
 // ------------------------------------------------------------------------------------------------
 struct StepArgs {
     const double* __restrict__ U;  // time t links
@@ -101,173 +205,123 @@ GHD void fill_step_coefficients(StepArgs& A, const double* a, double dt) {
     A.dtz = dt / a[2];
 }
 
-// r = Re tr(i t^a P) for the four-link products of SC/CALC/Plaquettes.cpp:83-103
-// (SC/SUNc/AdvancedOperations.cpp:175-228):
-// KIND 0: UUDD(v1,v2,v3,v4) = (v1 v2)(v4 v3)^dag ; 1: UDDU = (v1 v2^dag)(v3^dag v4);
-// 2: DUUD = (v1^dag v2)(v3 v4^dag) ; 3: DDUU = (v1^dag v2^dag)(v3 v4)
-template <int KIND>
-GHD void plaq_mat(const double* __restrict__ U, long S, int m1, long s1, int m2, long s2, int m3, long s3, int m4,
-                  long s4, M3& Pq) {
-    M3 A, B, L1, L2;
-    ldm(U + m1 * 18 * S, S, s1, A);
-    ldm(U + m2 * 18 * S, S, s2, B);
-    if (KIND == 0) mul_nn(A, B, L1);
-    else if (KIND == 1) mul_nd(A, B, L1);
-    else if (KIND == 2) mul_dn(A, B, L1);
-    else mul_dd(A, B, L1);
-    ldm(U + m3 * 18 * S, S, s3, A);
-    ldm(U + m4 * 18 * S, S, s4, B);
-    if (KIND == 0) {
-        mul_nn(B, A, L2);
-        mul_nd(L1, L2, Pq);
-    } else if (KIND == 1) {
-        mul_dn(A, B, L2);
-        mul_nn(L1, L2, Pq);
-    } else if (KIND == 2) {
-        mul_nd(A, B, L2);
-        mul_nn(L1, L2, Pq);
-    } else {
-        mul_nn(A, B, L2);
-        mul_nn(L1, L2, Pq);
-    }
-}
-template <int KIND>
-GHD void plaq_tr(const double* __restrict__ U, long S, int m1, long s1, int m2, long s2, int m3, long s3, int m4,
-                 long s4, double* r) {
-    M3 Pq;
-    plaq_mat<KIND>(U, S, m1, s1, m2, s2, m3, s3, m4, s4, Pq);
-    retr_igen(Pq, r);
+// J+-stat' of site xs from time-t data, xd = xs - z (SC/DYNAMICS/SUNcDynamics.cpp:205-236)
+template <class I>
+GHD void jstat_new(const StepArgs& A, I xs, I xd, double* jps, double* jms) {
+    M3 Ud;
+    double jv[8], jdn[8], down[8];
+    ldm<I>(A.U, kUPlanes, 36, xd, Ud);
+    ldv<I>(A.J, kJPlanes, JPS * 8, xs, jps);
+    ldv<I>(A.J, kJPlanes, JP * 8, xd, jdn);
+    inv_adj_mul(Ud, jdn, down);
+    ldv<I>(A.J, kJPlanes, JP * 8, xs, jv);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jps[a] += -A.dtz * (jv[a] - down[a]);
+    ldv<I>(A.J, kJPlanes, JMS * 8, xs, jms);
+    ldv<I>(A.J, kJPlanes, JM * 8, xd, jdn);
+    inv_adj_mul(Ud, jdn, down);
+    ldv<I>(A.J, kJPlanes, JM * 8, xs, jv);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jms[a] += A.dtz * (jv[a] - down[a]);
 }
 
-GHD void step_site(const StepArgs& A, long s) {
-    const Geom g = A.g;
-    const long S = g.S;
-    const int x = (int)(s % g.nx);
-    const long r_ = s / g.nx;
-    const int y = (int)(r_ % g.ny);
-    const int z = (int)(r_ / g.ny);
-    const int xp = (x + 1 == g.nx) ? 0 : x + 1, xm = (x == 0) ? g.nx - 1 : x - 1;
-    const int yp = (y + 1 == g.ny) ? 0 : y + 1, ym = (y == 0) ? g.ny - 1 : y - 1;
-    const int zp = zup(g, z), zm = zdn(g, z);
-    const long nx = g.nx, P = g.P;
-    const long s0 = s;
-    const long sxp = xp + nx * y + P * z, sxm = xm + nx * y + P * z;
-    const long syp = x + nx * yp + P * z, sym = x + nx * ym + P * z;
-    const long szp = x + nx * y + P * zp, szm = x + nx * y + P * zm;
-    const long s_mxpy = xm + nx * yp + P * z, s_xpmy = xp + nx * ym + P * z;
-    const long s_ympz = x + nx * ym + P * zp, s_ypmz = x + nx * yp + P * zm;
-    const long s_xpmz = xp + nx * y + P * zm, s_mxpz = xm + nx * y + P * zp;
-    const double* __restrict__ U = A.U;
-
-    // ---------------- electric field (SC/DYNAMICS/SUNcDynamics.cpp:164-193) ----------------
-    double p0[8], p1[8], p2[8];
-    {
-        double tA[8], tB[8];
-        plaq_tr<0>(U, S, 0, s0, 1, sxp, 0, syp, 1, s0, tA);       // Uxy
-        plaq_tr<2>(U, S, 1, sym, 0, sym, 1, s_xpmy, 0, s0, tB);   // UxMy
-#pragma unroll
-        for (int a = 0; a < 8; a++) p0[a] = A.cE[1] * (tA[a] - tB[a]);
-        plaq_tr<1>(U, S, 1, s0, 0, s_mxpy, 1, sxm, 0, sxm, tB);   // UMxy
-#pragma unroll
-        for (int a = 0; a < 8; a++) p1[a] = -A.cE[0] * (tA[a] - tB[a]);
-        plaq_tr<0>(U, S, 1, s0, 2, syp, 1, szp, 2, s0, tA);       // Uyz
-        plaq_tr<2>(U, S, 2, szm, 1, szm, 2, s_ypmz, 1, s0, tB);   // UyMz
-#pragma unroll
-        for (int a = 0; a < 8; a++) p1[a] = A.cE[2] * (tA[a] - tB[a]) + p1[a];
-        plaq_tr<1>(U, S, 2, s0, 1, s_ympz, 2, sym, 1, sym, tB);   // UMyz
-#pragma unroll
-        for (int a = 0; a < 8; a++) p2[a] = -A.cE[1] * (tA[a] - tB[a]);
-        plaq_tr<0>(U, S, 2, s0, 0, szp, 2, sxp, 0, s0, tA);       // Uzx
-        plaq_tr<1>(U, S, 0, s0, 2, s_xpmz, 0, szm, 2, szm, tB);   // UMzx
-#pragma unroll
-        for (int a = 0; a < 8; a++) p0[a] = p0[a] - A.cE[2] * (tA[a] - tB[a]);
-        plaq_tr<2>(U, S, 0, sxm, 2, sxm, 0, s_mxpz, 2, s0, tB);   // UzMx
-#pragma unroll
-        for (int a = 0; a < 8; a++) p2[a] = A.cE[0] * (tA[a] - tB[a]) + p2[a];
-    }
-    double Ex[8], Ey[8], Ez[8], jp[8], jm[8];
-    ldv(A.E + 0 * 8 * S, S, s0, Ex);
-    ldv(A.E + 1 * 8 * S, S, s0, Ey);
-    ldv(A.E + 2 * 8 * S, S, s0, Ez);
-    ldv(A.J + JP * 8 * S, S, s0, jp);
-    ldv(A.J + JM * 8 * S, S, s0, jm);
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        Ex[a] += A.gc[0] * p0[a];
-        Ey[a] += A.gc[1] * p1[a];
-        Ez[a] += A.gc[2] * p2[a];
-    }
-#pragma unroll
-    for (int a = 0; a < 8; a++) Ez[a] += -A.dtz * ((jp[a] - jm[a]) / kSqrt2);
-    stv(A.E + 0 * 8 * S, S, s0, Ex);
-    stv(A.E + 1 * 8 * S, S, s0, Ey);
-    stv(A.E + 2 * 8 * S, S, s0, Ez);
-
-    // ---------------- J-stat advection at x and at x+z (SC/DYNAMICS/SUNcDynamics.cpp:205-236) ----------------
-    double jps[8], jms[8];
-    {
-        M3 Uzm;
-        double jpd[8], jmd[8], down[8];
-        ldm(U + 2 * 18 * S, S, szm, Uzm);
-        ldv(A.J + JP * 8 * S, S, szm, jpd);
-        ldv(A.J + JM * 8 * S, S, szm, jmd);
-        ldv(A.J + JPS * 8 * S, S, s0, jps);
-        ldv(A.J + JMS * 8 * S, S, s0, jms);
-        inv_adj_mul(Uzm, jpd, down);
-#pragma unroll
-        for (int a = 0; a < 8; a++) jps[a] += -A.dtz * (jp[a] - down[a]);
-        inv_adj_mul(Uzm, jmd, down);
-#pragma unroll
-        for (int a = 0; a < 8; a++) jms[a] += A.dtz * (jm[a] - down[a]);
-    }
-    stv(A.Jn + JPS * 8 * S, S, s0, jps);
-    stv(A.Jn + JMS * 8 * S, S, s0, jms);
-    double jpsU[8], jmsU[8];  // J+-stat'(x+z)
+// J+-stat' at the site above (x+z), or at x itself on the open top layer (the reference's ghost copy)
+template <class I>
+GHD void jstat_new_up(const StepArgs& A, I s, int z, double* jpsU, double* jmsU) {
+    const int zp = zup(A.g, z);
     if (zp != z) {
-        M3 Uz0;
-        double jpu[8], jmu[8], down[8];
-        ldm(U + 2 * 18 * S, S, s0, Uz0);
-        ldv(A.J + JP * 8 * S, S, szp, jpu);
-        ldv(A.J + JM * 8 * S, S, szp, jmu);
-        ldv(A.J + JPS * 8 * S, S, szp, jpsU);
-        ldv(A.J + JMS * 8 * S, S, szp, jmsU);
-        inv_adj_mul(Uz0, jp, down);
-#pragma unroll
-        for (int a = 0; a < 8; a++) jpsU[a] += -A.dtz * (jpu[a] - down[a]);
-        inv_adj_mul(Uz0, jm, down);
-#pragma unroll
-        for (int a = 0; a < 8; a++) jmsU[a] += A.dtz * (jmu[a] - down[a]);
+        jstat_new<I>(A, (I)((long)s + A.g.P * (long)(zp - z)), s, jpsU, jmsU);
     } else {
-#pragma unroll
-        for (int a = 0; a < 8; a++) {
-            jpsU[a] = jps[a];
-            jmsU[a] = jms[a];
-        }
+        const int zm = zdn(A.g, z);
+        jstat_new<I>(A, s, (I)((long)s + A.g.P * (long)(zm - z)), jpsU, jmsU);
     }
+}
 
-    // ---------------- links: two half steps with one exponential (:62-106), UOld (:108-128) ----------------
+// Link direction MU of site s: E'_MU, the exponential, U_h (and UOld), U'_MU.  Returns U_h in Uh.
+template <class I, int MU>
+GHD void link_update(const StepArgs& A, I s, int x, int y, int z, M3& Uh) {
+    const Geom& g = A.g;
+    M3 Sig;
 #pragma unroll
-    for (int mu = 0; mu < 3; mu++) {
-        M3 D, Uo, Uh;
-        expi_m1(A.cU[mu], mu == 0 ? Ex : (mu == 1 ? Ey : Ez), D);  // X - 1
-        ldm(U + mu * 18 * S, S, s0, Uo);
-        add_mul(D, Uo, Uh);  // X U
-        if (A.UOld) stm(A.UOld + mu * 18 * S, S, s0, Uh);
-        if (mu == 2) {
-            // ---------------- J advection with U_z(t + dt/2) (:238-269) ----------------
-            double up[8];
-            adj_mul(Uh, jpsU, up);
+    for (int k = 0; k < 9; k++) { Sig.re[k] = 0.0; Sig.im[k] = 0.0; }
 #pragma unroll
-            for (int a = 0; a < 8; a++) jp[a] += -A.dtz * (up[a] - jps[a]);
-            adj_mul(Uh, jmsU, up);
-#pragma unroll
-            for (int a = 0; a < 8; a++) jm[a] += A.dtz * (up[a] - jms[a]);
-            stv(A.Jn + JP * 8 * S, S, s0, jp);
-            stv(A.Jn + JM * 8 * S, S, s0, jm);
-        }
-        add_mul(D, Uh, Uo);  // X U_h
-        stm(A.Un + mu * 18 * S, S, s0, Uo);
+    for (int q = 1; q <= 2; q++) {
+        const int nu = (MU + q) % 3;
+        const double cn = A.cE[nu];
+        M3 P1, P2, T;
+        // forward staple U_nu(x+mu) U_mu^dag(x+nu) U_nu^dag(x)
+        ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, MU, 1, nu, 0), P1);
+        ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(g, x, y, z, nu, 1, MU, 0), P2);
+        mul_nd(P1, P2, T);
+        ldm<I>(A.U, kUPlanes, 18 * nu, s, P1);
+        acc_mul_nd(cn, T, P1, Sig);
+        // backward staple U_nu^dag(x+mu-nu) U_mu^dag(x-nu) U_nu(x-nu)
+        const I sm = nbr<I>(g, x, y, z, nu, -1, MU, 0);
+        ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, MU, 1, nu, -1), P1);
+        ldm<I>(A.U, kUPlanes, 18 * MU, sm, P2);
+        mul_dd(P1, P2, T);
+        ldm<I>(A.U, kUPlanes, 18 * nu, sm, P1);
+        acc_mul_nn(cn, T, P1, Sig);
     }
+    M3 Uo;
+    ldm<I>(A.U, kUPlanes, 18 * MU, s, Uo);
+    double Em[8];
+    {
+        double F[8];
+        retr_igen_prod(Uo, Sig, F);
+        ldv<I>(A.E, kEPlanes, 8 * MU, s, Em);
+#pragma unroll
+        for (int a = 0; a < 8; a++) Em[a] += A.gc[MU] * F[a];
+    }
+    if (MU == 2) {
+        double jp[8], jm[8];
+        ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
+        ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
+#pragma unroll
+        for (int a = 0; a < 8; a++) Em[a] += -A.dtz * ((jp[a] - jm[a]) / kSqrt2);
+    }
+    stv<I>(A.E, kEPlanes, 8 * MU, s, Em);
+    M3 D;
+    expi_m1(A.cU[MU], Em, D);
+    add_mul(D, Uo, Uh);
+    if (A.UOld) stm<I>(A.UOld, kUPlanes, 18 * MU, s, Uh);
+    add_mul(D, Uh, Uo);
+    stm<I>(A.Un, kUPlanes, 18 * MU, s, Uo);
+}
+
+// J+-'(x) = J+-(x) -+ (dt/a_z) [U_h,z(x) J+-stat'(x+z) U_h,z^dag(x) - J+-stat'(x)]  (:238-269)
+template <class I>
+GHD void j_advect(const StepArgs& A, I s, const M3& Uhz, const double* jps, const double* jms, const double* jpsU,
+                  const double* jmsU) {
+    double up[8], jv[8];
+    adj_mul(Uhz, jpsU, up);
+    ldv<I>(A.J, kJPlanes, JP * 8, s, jv);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jv[a] += -A.dtz * (up[a] - jps[a]);
+    stv<I>(A.Jn, kJPlanes, JP * 8, s, jv);
+    adj_mul(Uhz, jmsU, up);
+    ldv<I>(A.J, kJPlanes, JM * 8, s, jv);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jv[a] += A.dtz * (up[a] - jms[a]);
+    stv<I>(A.Jn, kJPlanes, JM * 8, s, jv);
+}
+
+// Whole step of one site, in the order the direction-split kernel performs it (host equivalent).
+template <class I>
+GHD void step_site(const StepArgs& A, I s) {
+    int x, y, z;
+    site_coords<I>(A.g, s, x, y, z);
+    double jps[8], jms[8], jpsU[8], jmsU[8];
+    const int zm = zdn(A.g, z);
+    jstat_new<I>(A, s, (I)((long)s + A.g.P * (long)(zm - z)), jps, jms);
+    stv<I>(A.Jn, kJPlanes, JPS * 8, s, jps);
+    stv<I>(A.Jn, kJPlanes, JMS * 8, s, jms);
+    jstat_new_up<I>(A, s, z, jpsU, jmsU);
+    M3 Uh;
+    link_update<I, 0>(A, s, x, y, z, Uh);
+    link_update<I, 1>(A, s, x, y, z, Uh);
+    link_update<I, 2>(A, s, x, y, z, Uh);
+    j_advect<I>(A, s, Uh, jps, jms, jpsU, jmsU);
 }
 
 }  // namespace glasma

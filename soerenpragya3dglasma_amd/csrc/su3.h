@@ -302,6 +302,63 @@ GHD void add_mul(const M3& D, const M3& B, M3& C) {
     }
 }
 
+// Sig += c * T C^dag
+GHD void acc_mul_nd(double c, const M3& T, const M3& C, M3& Sig) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double r = 0.0, m = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                r = fma(T.re[mi(i, k)], C.re[mi(j, k)], r);
+                r = fma(T.im[mi(i, k)], C.im[mi(j, k)], r);
+                m = fma(T.im[mi(i, k)], C.re[mi(j, k)], m);
+                m = fma(-T.re[mi(i, k)], C.im[mi(j, k)], m);
+            }
+            Sig.re[mi(i, j)] = fma(c, r, Sig.re[mi(i, j)]);
+            Sig.im[mi(i, j)] = fma(c, m, Sig.im[mi(i, j)]);
+        }
+}
+// Sig += c * T C
+GHD void acc_mul_nn(double c, const M3& T, const M3& C, M3& Sig) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double r = 0.0, m = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                r = fma(T.re[mi(i, k)], C.re[mi(k, j)], r);
+                r = fma(-T.im[mi(i, k)], C.im[mi(k, j)], r);
+                m = fma(T.re[mi(i, k)], C.im[mi(k, j)], m);
+                m = fma(T.im[mi(i, k)], C.re[mi(k, j)], m);
+            }
+            Sig.re[mi(i, j)] = fma(c, r, Sig.re[mi(i, j)]);
+            Sig.im[mi(i, j)] = fma(c, m, Sig.im[mi(i, j)]);
+        }
+}
+// r = Re tr(i t^a A B): only the entries that the projection reads
+GHD void retr_igen_prod(const M3& A, const M3& B, double* r) {
+    M3 P;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double re = 0.0, im = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                re = fma(A.re[mi(i, k)], B.re[mi(k, j)], re);
+                re = fma(-A.im[mi(i, k)], B.im[mi(k, j)], re);
+                im = fma(A.re[mi(i, k)], B.im[mi(k, j)], im);
+                im = fma(A.im[mi(i, k)], B.re[mi(k, j)], im);
+            }
+            P.re[mi(i, j)] = re;
+            P.im[mi(i, j)] = im;
+        }
+    retr_igen(P, r);
+}
+
 // components of V (alpha^b t^b) V^dagger  (meaning of SC/SUNc/SU2/AlgebraOperations.cpp:110-116)
 GHD void adj_mul(const M3& V, const double* al, double* out) {
     M3 A, T, H;

@@ -6,10 +6,14 @@ two z ghost layers:
 * U: complex128 (Nz+2, Ny, Nx, 3, 3, 3) indexed [z+1, y, x, mu, j, i] (flat = 9*(mu + 3*Index3D) + i + 3*j)
 * E: float64 (Nz+2, Ny, Nx, 3, 8);  J: float64 (Nz+2, Ny, Nx, 8)
 
-Device layout (structure of arrays, no ghosts, s = x + Nx*(y + Ny*z)):
+Device layout (no ghosts, s = x + Nx*(y + Ny*z)): tiles of 64 sites, structure of arrays inside a tile;
+plane p of site s at ((s // 64) * NP + p) * 64 + s % 64, S padded to a multiple of 64.
 
-* U: float64 (54, S): plane (mu*18 + 2k + c), k = i + 3j, c = 0 real / 1 imaginary
-* E: float64 (24, S): plane (mu*8 + a);  J: float64 (32, S): plane (w*8 + a), w = J+, J-, J+stat, J-stat
+* U: NP = 54: plane (mu*18 + 2k + c), k = i + 3j, c = 0 real / 1 imaginary
+* E: NP = 24: plane (mu*8 + a);  J: NP = 32: plane (w*8 + a), w = J+, J-, J+stat, J-stat
+
+The ``*_soa`` functions below produce plane-major arrays (NP, S); ``tile``/``untile`` convert those to and
+from the tiled device layout.
 """
 from __future__ import annotations
 
@@ -69,3 +73,20 @@ def su3_flat(M: np.ndarray) -> np.ndarray:
 
 def su3_unflat(v: np.ndarray) -> np.ndarray:
     return np.asarray(v).reshape(3, 3).T
+
+
+TILE = 64
+
+
+def tile(soa: np.ndarray) -> np.ndarray:
+    """(NP, S) plane-major -> flat tiled device layout (S padded to a multiple of 64)."""
+    NP, S = soa.shape
+    Sp = (S + TILE - 1) // TILE * TILE
+    pad = np.zeros((NP, Sp))
+    pad[:, :S] = soa
+    return np.ascontiguousarray(pad.reshape(NP, Sp // TILE, TILE).transpose(1, 0, 2)).reshape(-1)
+
+
+def untile(flat: np.ndarray, NP: int, S: int) -> np.ndarray:
+    Sp = (S + TILE - 1) // TILE * TILE
+    return np.ascontiguousarray(flat.reshape(Sp // TILE, NP, TILE).transpose(1, 0, 2).reshape(NP, Sp)[:, :S])
