@@ -81,21 +81,71 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))
     }
 }
 
+// Four-wavefront variant: waves 0-2 own the link directions, wave 3 forms J+-stat'(x) and J+-stat'(x+z)
+// and, after the barrier, applies the J advection with U_h,z handed over through LDS.  Two such blocks
+// fill the 8 wavefront slots of a CU at two waves per SIMD.
+template <class I, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) k_step4(StepArgs A, int bpb) {
+    __shared__ double shU[18][kTile];  // U_h,z
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long sl = tile_first_site(A.g, bpb) + lane;
+    const bool valid = sl < A.g.S;
+    const I s = (I)(valid ? sl : 0);
+    int x, y, z;
+    site_coords<I>(A.g, s, x, y, z);
+    double jps[8], jms[8], jpsU[8], jmsU[8];
+    M3 Uh;
+    if (wave == 0) {
+        if (valid) link_update<I, 0>(A, s, x, y, z, Uh);
+    } else if (wave == 1) {
+        if (valid) link_update<I, 1>(A, s, x, y, z, Uh);
+    } else if (wave == 2) {
+        if (valid) {
+            link_update<I, 2>(A, s, x, y, z, Uh);
+            for (int k = 0; k < 9; k++) {
+                shU[2 * k][lane] = Uh.re[k];
+                shU[2 * k + 1][lane] = Uh.im[k];
+            }
+        }
+    } else {
+        if (valid) {
+            const int zm = zdn(A.g, z);
+            jstat_new<I>(A, s, (I)(sl + A.g.P * (long)(zm - z)), jps, jms);
+            stv<I>(A.Jn, kJPlanes, JPS * 8, s, jps);
+            stv<I>(A.Jn, kJPlanes, JMS * 8, s, jms);
+            jstat_new_up<I>(A, s, z, jpsU, jmsU);
+        }
+    }
+    __syncthreads();
+    if (wave == 3 && valid) {
+        for (int k = 0; k < 9; k++) {
+            Uh.re[k] = shU[2 * k][lane];
+            Uh.im[k] = shU[2 * k + 1][lane];
+        }
+        j_advect<I>(A, s, Uh, jps, jms, jpsU, jmsU);
+    }
+}
+
 static int env_int(const char* name, int dflt) {
     const char* v = std::getenv(name);
     return v ? std::atoi(v) : dflt;
 }
 
 template <class I>
-static void launch(int waves, int bpb, const StepArgs& A, hipStream_t st) {
-    dim3 grid((unsigned)(pad_sites(A.g.S) / kTile)), blk(192);
-    if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, blk, 0, st, A, bpb);
-    else hipLaunchKernelGGL((k_step<I, 2>), grid, blk, 0, st, A, bpb);
+static void launch(int waves, int four, int bpb, const StepArgs& A, hipStream_t st) {
+    dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
+    if (four) {
+        hipLaunchKernelGGL((k_step4<I, 2>), grid, dim3(256), 0, st, A, bpb);
+        return;
+    }
+    if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, dim3(192), 0, st, A, bpb);
+    else hipLaunchKernelGGL((k_step<I, 2>), grid, dim3(192), 0, st, A, bpb);
 }
 
 int launch_step(glasma_ctx* ctx, bool write_old) {
     static const int waves = env_int("GLASMA_STEP_WAVES", 2);
     static const int xcd = env_int("GLASMA_STEP_XCD", 1);
+    static const int four = env_int("GLASMA_STEP_FOUR", 1);
     StepArgs A;
     A.U = ctx->U[ctx->cur];
     A.Un = ctx->U[ctx->cur ^ 1];
@@ -107,8 +157,8 @@ int launch_step(glasma_ctx* ctx, bool write_old) {
     fill_step_coefficients(A, ctx->p.a, ctx->p.dt);
     // the band mapping needs whole tiles per band
     const int bpb = (xcd && ctx->g.P % (kXcds * kTile) == 0) ? (int)(ctx->g.P / (kXcds * kTile)) : 0;
-    if (A.g.S < (1L << 32)) launch<unsigned int>(waves, bpb, A, ctx->stream);
-    else launch<unsigned long>(waves, bpb, A, ctx->stream);
+    if (A.g.S < (1L << 32)) launch<unsigned int>(waves, four, bpb, A, ctx->stream);
+    else launch<unsigned long>(waves, four, bpb, A, ctx->stream);
     ctx->cur ^= 1;
     return 0;
 }

@@ -9,16 +9,19 @@
 namespace glasma {
 
 // ------------------------------------------------------------------------------------------------
-// Device layout: tiles of 64 consecutive sites (s = x + Nx*(y + Ny*z)), structure of arrays inside a
-// tile.  A field with NP scalar planes per site stores plane p of site s at
-//        ((s / 64) * NP + p) * 64 + s % 64.
-// A wavefront reading one plane for its 64 sites issues one aligned 512-byte access, and all planes
-// of a tile are contiguous (a site's 3 links: 27.6 KB per tile), so a wavefront touches a few pages
-// instead of one page per plane.  The site count is padded to a multiple of 64.
+// Device layout: tiles of 64 consecutive sites (s = x + Nx*(y + Ny*z)).  A field with NP scalar
+// planes per site (NP even) groups the planes in pairs (2q, 2q+1); inside a tile, pair q of the 64 sites
+// is stored contiguously with the two members of a pair adjacent:
+//        plane p of site s at  ((s / 64) * NP/2 + p/2) * 128 + (s % 64) * 2 + p % 2.
+// A lane then loads a pair (the real and imaginary part of a matrix entry, or two colour components)
+// with ONE 16-byte access, a wavefront issues one aligned 1 KiB access per pair, and all planes of a
+// tile are contiguous (a site's 3 links: 27.6 KB per tile), so a wavefront touches a few pages.  The
+// number of vector-memory instructions, which bounds this kernel, is half that of one plane per
+// access.  The site count is padded to a multiple of 64.
 //   links    NP = 54: plane mu*18 + 2k + c, k = i + 3j (column-major), c = 0 real / 1 imaginary
 //   E field  NP = 24: plane mu*8 + a
 //   currents NP = 32: plane w*8 + a, w = 0 J+, 1 J-, 2 J+stat, 3 J-stat
-//   scalar   NP = 18 (one matrix per site: Wilson lines, gauge transformations)
+//   one matrix per site (Wilson lines, gauge transformations): NP = 18
 // ------------------------------------------------------------------------------------------------
 constexpr int kTile = 64;
 constexpr int kUPlanes = 54, kEPlanes = 24, kJPlanes = 32, kGPlanes = 18;
@@ -27,8 +30,16 @@ enum { JP = 0, JM = 1, JPS = 2, JMS = 3 };
 GHD long pad_sites(long S) { return (S + kTile - 1) / kTile * kTile; }
 template <class I>
 GHD long lidx(int NP, int p, I s) {
-    return ((long)(s / (I)kTile) * NP + p) * kTile + (long)(s % (I)kTile);
+    return ((long)(s / (I)kTile) * (NP >> 1) + (p >> 1)) * (2 * kTile) + 2 * (long)(s % (I)kTile) + (p & 1);
 }
+
+#if defined(__clang__)
+typedef double d2_t __attribute__((ext_vector_type(2)));
+#else
+struct d2_t {
+    double x, y;
+};
+#endif
 
 struct Geom {
     int nx, ny, nz;
@@ -46,10 +57,10 @@ GHD long sidx(const Geom& g, int x, int y, int z) {
     return (long)xwrap(x, g.nx) + (long)g.nx * ((long)xwrap(y, g.ny) + (long)g.ny * z);
 }
 
-// Address of plane p0 of site s.  On the device the address is passed through an empty asm: each
-// access then starts from an opaque base, so identical loads in different stages of a kernel are not
-// merged (merging keeps whole matrices live across stages and exhausts the register file); the
-// plane offsets k*512 bytes are folded into the instructions' immediate offsets.
+// Address of (even) plane p0 of site s.  On the device the address is passed through an empty asm:
+// each access then starts from an opaque base, so identical loads in different stages of a kernel are
+// not merged (merging keeps whole matrices live across stages and exhausts the register file); the
+// pair offsets k*1024 bytes are folded into the instructions' immediate offsets.
 template <class I>
 GHD const double* site_ptr(const double* base, int NP, int p0, I s) {
     const double* q = base + lidx<I>(NP, p0, s);
@@ -67,11 +78,25 @@ GHD double* site_ptr(double* base, int NP, int p0, I s) {
     return q;
 }
 // streamed outputs are written non-temporally so that they do not evict neighbour data from L2 / MALL
-GHD void st_nt(double* p, double v) {
+GHD void st_nt(double* p, double v0, double v1) {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GLASMA_NO_NT_STORES)
-    __builtin_nontemporal_store(v, p);
+    d2_t v;
+    v.x = v0;
+    v.y = v1;
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) d2_t*)p);
 #else
-    *p = v;
+    p[0] = v0;
+    p[1] = v1;
+#endif
+}
+GHD d2_t ld2(const double* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) d2_t*)p;
+#else
+    d2_t v;
+    v.x = p[0];
+    v.y = p[1];
+    return v;
 #endif
 }
 
@@ -80,30 +105,32 @@ GHD void ldm(const double* __restrict__ base, int NP, int p0, I s, M3& A) {
     const double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        A.re[k] = q[(2 * k) * kTile];
-        A.im[k] = q[(2 * k + 1) * kTile];
+        const d2_t v = ld2(q + k * 2 * kTile);
+        A.re[k] = v.x;
+        A.im[k] = v.y;
     }
 }
 template <class I>
 GHD void stm(double* __restrict__ base, int NP, int p0, I s, const M3& A) {
     double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        st_nt(q + (2 * k) * kTile, A.re[k]);
-        st_nt(q + (2 * k + 1) * kTile, A.im[k]);
-    }
+    for (int k = 0; k < 9; k++) st_nt(q + k * 2 * kTile, A.re[k], A.im[k]);
 }
 template <class I>
 GHD void ldv(const double* __restrict__ base, int NP, int p0, I s, double* v) {
     const double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
-    for (int a = 0; a < 8; a++) v[a] = q[a * kTile];
+    for (int a = 0; a < 8; a += 2) {
+        const d2_t w = ld2(q + (a >> 1) * 2 * kTile);
+        v[a] = w.x;
+        v[a + 1] = w.y;
+    }
 }
 template <class I>
 GHD void stv(double* __restrict__ base, int NP, int p0, I s, const double* v) {
     double* q = site_ptr<I>(base, NP, p0, s);
 #pragma unroll
-    for (int a = 0; a < 8; a++) st_nt(q + a * kTile, v[a]);
+    for (int a = 0; a < 8; a += 2) st_nt(q + (a >> 1) * 2 * kTile, v[a], v[a + 1]);
 }
 
 template <class I>

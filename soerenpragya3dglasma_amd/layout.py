@@ -6,8 +6,9 @@ two z ghost layers:
 * U: complex128 (Nz+2, Ny, Nx, 3, 3, 3) indexed [z+1, y, x, mu, j, i] (flat = 9*(mu + 3*Index3D) + i + 3*j)
 * E: float64 (Nz+2, Ny, Nx, 3, 8);  J: float64 (Nz+2, Ny, Nx, 8)
 
-Device layout (no ghosts, s = x + Nx*(y + Ny*z)): tiles of 64 sites, structure of arrays inside a tile;
-plane p of site s at ((s // 64) * NP + p) * 64 + s % 64, S padded to a multiple of 64.
+Device layout (no ghosts, s = x + Nx*(y + Ny*z)): tiles of 64 sites; planes grouped in pairs, pair q of a
+tile stored contiguously with the pair members adjacent: plane p of site s at
+((s // 64) * NP/2 + p // 2) * 128 + (s % 64) * 2 + p % 2, S padded to a multiple of 64.
 
 * U: NP = 54: plane (mu*18 + 2k + c), k = i + 3j, c = 0 real / 1 imaginary
 * E: NP = 24: plane (mu*8 + a);  J: NP = 32: plane (w*8 + a), w = J+, J-, J+stat, J-stat
@@ -84,9 +85,12 @@ def tile(soa: np.ndarray) -> np.ndarray:
     Sp = (S + TILE - 1) // TILE * TILE
     pad = np.zeros((NP, Sp))
     pad[:, :S] = soa
-    return np.ascontiguousarray(pad.reshape(NP, Sp // TILE, TILE).transpose(1, 0, 2)).reshape(-1)
+    # (pair, member, tile, lane) -> (tile, pair, lane, member)
+    t = pad.reshape(NP // 2, 2, Sp // TILE, TILE).transpose(2, 0, 3, 1)
+    return np.ascontiguousarray(t).reshape(-1)
 
 
 def untile(flat: np.ndarray, NP: int, S: int) -> np.ndarray:
     Sp = (S + TILE - 1) // TILE * TILE
-    return np.ascontiguousarray(flat.reshape(Sp // TILE, NP, TILE).transpose(1, 0, 2).reshape(NP, Sp)[:, :S])
+    t = flat.reshape(Sp // TILE, NP // 2, TILE, 2).transpose(1, 3, 0, 2).reshape(NP, Sp)
+    return np.ascontiguousarray(t[:, :S])
