@@ -71,8 +71,27 @@ def cpu_baseline(sample_n, steps):
     }
 
 
+def calibrate_copy(gib: float) -> None:
+    """A plain device-to-device copy kernel of known size, for scaling the HBM byte counters of rocprofv3.
+
+    The copy reads and writes exactly ``gib`` GiB per launch; the FETCH_SIZE / WRITE_SIZE that the
+    profiler reports for it, divided by that size, is the counter's scale factor on this device.
+    """
+    import torch
+    n = int(gib * 2 ** 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                      "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", type=float, default=None, metavar="GIB",
+                    help="run only a plain copy kernel of GIB GiB (profiler counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
@@ -83,6 +102,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
+    if args.calibrate_copy is not None:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (z-slab) decomposition is a design, "
               "not an implementation. Run with --gpus 1.", file=sys.stderr)

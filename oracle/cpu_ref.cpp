@@ -14,7 +14,7 @@
 // Pinning (what checks this file, see tests/test_oracle_*.py):
 //   * SU(3) products / traces / projections        -> numpy brute force with explicit Gell-Mann matrices
 //   * MatrixIExp (Jacobi eigen-decomposition)       -> scipy.linalg.expm, unitarity, det = 1
-//   * MatrixILog (two-pass Jacobi, principal)       -> exp(log U) = U round trip, scipy.linalg.logm
+//   * MatrixILog (Cayley transform, principal)      -> exp(log U) = U round trip, scipy.linalg.logm
 //   * (Inverse)AdjointMultiplication, GetAdjoint     -> numpy V A V^dagger, Adj_ab = 2 tr(t^a V t^b V^dagger)
 //   * SU(2) ops                                      -> the SU(2) <-> SU(3) embedding test of a full step
 //   * leapfrog                                       -> gauge covariance, Gauss-law conservation, energy
@@ -285,37 +285,69 @@ struct SU3 {
     // Eigenvectors: Jacobi on K = (U - U^dag)/2i, then Jacobi on the Hermitian part in that basis, which
     // separates eigenvalues that share sin(phi).  Phases in (-pi, pi]; if they sum to +-2pi the largest
     // (smallest) is shifted by -2pi (+2pi) so that the logarithm is traceless.
+    // Principal logarithm through the Cayley transform: Y = i^k U with k in {0,1,2,3} chosen to maximise
+    // |det(1 + Y)| (so that no eigenvalue of Y is near -1), C = i (1 - Y)(1 + Y)^-1 is Hermitian with
+    // eigenvalues tan(psi_j / 2), a monotonic function of Y's eigenphases psi_j, so distinct phases are
+    // never degenerate in C (a Hermitian combination such as (Y + Y^dag)/2 is degenerate for psi and -psi,
+    // which makes its eigenvectors ill-determined near the identity).  Phases of U: psi_j - k pi/2 wrapped
+    // to (-pi, pi], with one moved by 2 pi when they sum to +-2 pi (det U = 1).
     static void MatrixILog(double Cc, const elem* U, double* out) {
         typedef long double R;
         typedef std::complex<R> C;
         const R PIL = 3.141592653589793238462643383279502884L;
-        C Ul[3][3], K[3][3], W[3][3], W2[3][3], Hc[3][3], T[3][3];
+        C Ul[3][3], Y[3][3], Cm[3][3], T[3][3];
         R ev[3];
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) Ul[i][j] = C(U[MI(i, j)].real(), U[MI(i, j)].imag());
+        auto det3 = [](C M[3][3]) {
+            return M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                   M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+        };
+        const C ik[4] = {C(1, 0), C(0, 1), C(-1, 0), C(0, -1)};
+        int kbest = 0;
+        R dbest = -1;
+        for (int k = 0; k < 4; k++) {
+            C A[3][3];
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) A[i][j] = ik[k] * Ul[i][j] + (i == j ? R(1) : R(0));
+            R d = std::abs(det3(A));
+            if (d > dbest) { dbest = d; kbest = k; }
+        }
+        C A[3][3], B[3][3], Ai[3][3];
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) K[i][j] = (Ul[i][j] - std::conj(Ul[j][i])) / C(0, 2);
-        herm_jacobi3<R>(K, W, ev);
+            for (int j = 0; j < 3; j++) {
+                Y[i][j] = ik[kbest] * Ul[i][j];
+                A[i][j] = Y[i][j] + (i == j ? R(1) : R(0));
+                B[i][j] = (i == j ? R(1) : R(0)) - Y[i][j];
+            }
+        const C dA = det3(A);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {  // adjugate: Ai[i][j] = cofactor(j, i) / det
+                int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+                Ai[i][j] = (A[r0][c0] * A[r1][c1] - A[r0][c1] * A[r1][c0]) / dA;
+            }
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
                 C acc = 0;
-                for (int k = 0; k < 3; k++)
-                    for (int l = 0; l < 3; l++) acc += std::conj(W[k][i]) * R(0.5) * (Ul[k][l] + std::conj(Ul[l][k])) * W[l][j];
-                Hc[i][j] = acc;
+                for (int k = 0; k < 3; k++) acc += B[i][k] * Ai[k][j];
+                Cm[i][j] = C(0, 1) * acc;
             }
-        herm_jacobi3<R>(Hc, W2, ev);
         for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                C acc = 0;
-                for (int k = 0; k < 3; k++) acc += W[i][k] * W2[k][j];
-                T[i][j] = acc;
+            for (int j = i; j < 3; j++) {
+                C h = R(0.5) * (Cm[i][j] + std::conj(Cm[j][i]));
+                Cm[i][j] = h;
+                Cm[j][i] = std::conj(h);
             }
+        herm_jacobi3<R>(Cm, T, ev);
         R ph[3];
         for (int j = 0; j < 3; j++) {
             C d = 0;
             for (int k = 0; k < 3; k++)
-                for (int l = 0; l < 3; l++) d += std::conj(T[k][j]) * Ul[k][l] * T[l][j];
-            ph[j] = std::atan2(d.imag(), d.real());
+                for (int l = 0; l < 3; l++) d += std::conj(T[k][j]) * Y[k][l] * T[l][j];
+            R p = std::atan2(d.imag(), d.real()) - R(kbest) * (PIL / 2);
+            while (p > PIL) p -= 2 * PIL;
+            while (p <= -PIL) p += 2 * PIL;
+            ph[j] = p;
         }
         R sm = ph[0] + ph[1] + ph[2];
         if (sm > PIL) {

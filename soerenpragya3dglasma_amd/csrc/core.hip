@@ -56,7 +56,7 @@ __global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict_
     aos[i] = soa[lidx<long>(NP, p0 + j, s)];
 }
 
-static int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* dev_soa, int R, int NP = -1, int p0 = 0) {
+int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* dev_soa, int R, int NP, int p0) {
     if (NP < 0) NP = R;
     const long S = ctx->g.S, P = ctx->g.P;
     double* tmp = nullptr;
@@ -75,7 +75,7 @@ static int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* d
 }
 
 // fill the two ghost layers of a host AoS array as SynchronizeGhostCells does
-static void host_sync_ghosts(const glasma_ctx* ctx, double* h, int R) {
+void host_sync_ghosts(const glasma_ctx* ctx, double* h, int R) {
     const long P = ctx->g.P * R;
     const int nz = ctx->g.nz;
     if (ctx->p.bc_z == 0) {
@@ -87,8 +87,7 @@ static void host_sync_ghosts(const glasma_ctx* ctx, double* h, int R) {
     }
 }
 
-static int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_with_ghosts, int R, int NP = -1,
-                        int p0 = 0) {
+int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_with_ghosts, int R, int NP, int p0) {
     if (NP < 0) NP = R;
     const long S = ctx->g.S, P = ctx->g.P;
     double* tmp = nullptr;

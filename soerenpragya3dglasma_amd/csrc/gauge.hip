@@ -1,0 +1,339 @@
+// Coulomb gauge fixing on the device: Fourier-accelerated steepest descent.
+//   CoulombGaugeFixing::SetCoulombGauge              SC/GAUGETRANSFORMATION/CoulombGaugeFixing.cpp:32-88
+//   FourierAccelerationAlgorithm::UpdateGaugeTransformation
+//                                                    SC/GAUGETRANSFORMATION/ALGORITHMS/FourierAcceleration.cpp:15-148
+//   GET_LOCAL_GAUGE_VIOLATION                        SC/GAUGETRANSFORMATION/CoulombGaugeDeviation.cpp:23-41
+//   GaugeTransformLinks / electric fields            SC/GAUGETRANSFORMATION/GaugeTransformation.cpp:93-164,
+//                                                    SC/CALC/GaugeTransformation.cpp:5-17
+//
+// One iteration: Delta^a(x) = Re tr(i t^a w(x)), w = sum_mu a_mu^-2 (U^G_mu(x) + U^G_mu^dag(x - mu)) (the z
+// term only for Nz > 1); Delta -> 3D FFT -> Delta~/(V p^2) for p^2 > 0 (periodic momenta in all three
+// directions, the zero mode passed through unscaled) -> inverse FFT -> g <- exp(-i alpha Delta t)^dag g,
+// U^G_mu(x) = g(x) U_mu(x) g^dag(x + mu).  Monitors (max / RMS of Delta and of the step, min / mean of the
+// functional Re tr w) come from deterministic two-level reductions.
+//
+// Ghost-layer behaviour of the reference, reproduced: the gauge-fixed links' z ghost layers are never
+// written, so U^G(z = -1) is the identity; the gauge transformation's top ghost layer is synchronised
+// after each link transformation, so a transformation sees the top layer of the previous iteration.  Here
+// that layer is the separate array Gtop (18 planes over one transverse plane).
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+namespace glasma {
+namespace {
+
+constexpr int kCgBlock = 256;
+
+inline dim3 grid1(long n) { return dim3((unsigned)((n + kCgBlock - 1) / kCgBlock)); }
+
+// U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu); the z neighbour of the top layer is Gtop
+__global__ void k_cg_transform(const double* __restrict__ U, const double* __restrict__ G,
+                               const double* __restrict__ Gtop, double* __restrict__ UG, Geom g) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    M3 Gx, Um, Gn, T, R;
+    ldm<long>(G, kGPlanes, 0, s, Gx);
+    for (int mu = 0; mu < 3; mu++) {
+        ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+        if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
+        else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
+        else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
+        else ldm<long>(Gtop, kGPlanes, 0, s2, Gn);
+        mul_nn(Gx, Um, T);
+        mul_nd(T, Gn, R);
+        stm<long>(UG, kUPlanes, 18 * mu, s, R);
+    }
+}
+
+// ghost synchronisation of G's top layer: open -> copy of layer Nz-1, periodic -> copy of layer 0
+__global__ void k_cg_sync_top(const double* __restrict__ G, double* __restrict__ Gtop, Geom g) {
+    long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s2 >= g.P) return;
+    M3 A;
+    ldm<long>(G, kGPlanes, 0, s2 + (g.bc ? 0 : (long)(g.nz - 1) * g.P), A);
+    stm<long>(Gtop, kGPlanes, 0, s2, A);
+}
+
+__global__ void k_cg_identity(double* __restrict__ G, long n) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    M3 A;
+    set_unit(A);
+    stm<long>(G, kGPlanes, 0, s, A);
+}
+
+// per-block partials: [0] sum Delta^2, [1] sum Re tr w, [2] max |Delta^a|, [3] min Re tr w
+__device__ void block_partials(double s0, double s1, double m0, double m1, double* out) {
+    __shared__ double sh[4][kCgBlock / 64];
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    m0 = wave_max(m0);
+    m1 = -wave_max(-m1);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[0][w] = s0;
+        sh[1][w] = s1;
+        sh[2][w] = m0;
+        sh[3][w] = m1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, b = 0, c = 0, d = sh[3][0];
+        for (int k = 0; k < kCgBlock / 64; k++) {
+            a += sh[0][k];
+            b += sh[1][k];
+            c = fmax(c, sh[2][k]);
+            d = fmin(d, sh[3][k]);
+        }
+        out[0] = a;
+        out[1] = b;
+        out[2] = c;
+        out[3] = d;
+    }
+}
+
+// Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52)
+__global__ __launch_bounds__(kCgBlock) void k_cg_dev(const double* __restrict__ UG, double* __restrict__ F,
+                                                      double* __restrict__ part, Geom g, double cx, double cy,
+                                                      double cz, double fm0) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0;
+    if (s < g.S) {
+        const int z = (int)(s / g.P);
+        const long s2 = s - (long)z * g.P;
+        const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+        M3 A, B, W;
+        const double zf = g.nz > 1 ? 1.0 : 0.0;
+        for (int k = 0; k < 9; k++) { W.re[k] = 0.0; W.im[k] = 0.0; }
+        for (int mu = 0; mu < 3; mu++) {
+            const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
+            ldm<long>(UG, kUPlanes, 18 * mu, s, A);
+            if (mu == 0) ldm<long>(UG, kUPlanes, 0, sidx(g, x - 1, y, z), B);
+            else if (mu == 1) ldm<long>(UG, kUPlanes, 18, sidx(g, x, y - 1, z), B);
+            else if (z > 0) ldm<long>(UG, kUPlanes, 36, s - g.P, B);
+            else set_unit(B);
+            // w += c (A + B^dag), grouped as the reference adds the three directions
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    W.re[mi(i, j)] += c * (A.re[mi(i, j)] + B.re[mi(j, i)]);
+                    W.im[mi(i, j)] += c * (A.im[mi(i, j)] - B.im[mi(j, i)]);
+                }
+        }
+        double dv[8];
+        retr_igen(W, dv);
+        for (int a = 0; a < 8; a++) {
+            mx = fmax(mx, fabs(dv[a]));
+            sq += dv[a] * dv[a];
+            F[2 * ((long)a * g.S + s)] = dv[a];
+            F[2 * ((long)a * g.S + s) + 1] = 0.0;
+        }
+        const double lf = W.re[0] + W.re[4] + W.re[8];
+        fm = fmin(fm, lf);
+        fs = lf;
+    }
+    block_partials(sq, fs, mx, fm, part + 4 * (long)blockIdx.x);
+}
+
+// Fourier acceleration: F *= 1/(V p^2) for p^2 > 0 (FourierAcceleration.cpp:55-75)
+__global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay, double az) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int pz = (int)(s / g.P);
+    const long s2 = s - (long)pz * g.P;
+    const int px = (int)(s2 % g.nx), py = (int)(s2 / g.nx);
+    const double p2 = (2.0 - 2.0 * cos(2.0 * kPi * px / (double)g.nx)) / (ax * ax) +
+                      (2.0 - 2.0 * cos(2.0 * kPi * py / (double)g.ny)) / (ay * ay) +
+                      (2.0 - 2.0 * cos(2.0 * kPi * pz / (double)g.nz)) / (az * az);
+    if (!(p2 > 0.0)) return;
+    const double nf = 1.0 / (double)g.S;
+    for (int a = 0; a < 8; a++) {
+        const long i = (long)a * g.S + s;
+        F[2 * i] = nf * F[2 * i] / p2;
+        F[2 * i + 1] = nf * F[2 * i + 1] / p2;
+    }
+}
+
+// g <- exp(-i alpha Delta t)^dag g and the step size |3 - tr exp(...)| (FourierAcceleration.cpp:80-140)
+__global__ __launch_bounds__(kCgBlock) void k_cg_update(const double* __restrict__ F, double* __restrict__ G,
+                                                         double* __restrict__ part, Geom g, double alpha) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double ssq = 0.0, sm = 0.0;
+    if (s < g.S) {
+        double ub[8];
+        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)a * g.S + s)];
+        M3 Wt, Gx, Gn;
+        expi(-alpha, ub, Wt);
+        const double tr = 3.0 - (Wt.re[0] + Wt.re[4] + Wt.re[8]);
+        const double ti = -(Wt.im[0] + Wt.im[4] + Wt.im[8]);
+        const double st = hypot(tr, ti);
+        sm = st;
+        ssq = st * st;
+        ldm<long>(G, kGPlanes, 0, s, Gx);
+        mul_dn(Wt, Gx, Gn);
+        stm<long>(G, kGPlanes, 0, s, Gn);
+    }
+    block_partials(ssq, 0.0, sm, 0.0, part + 4 * (long)blockIdx.x);
+}
+
+// fixed-order second level: one block of 256 threads, strided per-thread sums then the block tree
+__global__ __launch_bounds__(kCgBlock) void k_cg_reduce(const double* __restrict__ part, long nblk,
+                                                         double* __restrict__ out, double fm0) {
+    double s0 = 0, s1 = 0, m0 = 0, m1 = fm0;
+    for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
+        s0 += part[4 * b];
+        s1 += part[4 * b + 1];
+        m0 = fmax(m0, part[4 * b + 2]);
+        m1 = fmin(m1, part[4 * b + 3]);
+    }
+    block_partials(s0, s1, m0, m1, out);
+}
+
+// E^G_mu(x) = G(x) E_mu(x) G^dag(x)
+__global__ void k_cg_efield(const double* __restrict__ G, const double* __restrict__ E, double* __restrict__ EG,
+                            Geom g) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    M3 Gx;
+    ldm<long>(G, kGPlanes, 0, s, Gx);
+    for (int mu = 0; mu < 3; mu++) {
+        double e[8], o[8];
+        ldv<long>(E, kEPlanes, 8 * mu, s, e);
+        adj_mul(Gx, e, o);
+        stv<long>(EG, kEPlanes, 8 * mu, s, o);
+    }
+}
+
+int ensure_gauge(glasma_ctx* ctx) {
+    if (ctx->G) return 0;
+    const long Sp = pad_sites(ctx->g.S), Pp = pad_sites(ctx->g.P);
+    GL_HIP(ctx, hipMalloc(&ctx->G, (size_t)kGPlanes * Sp * sizeof(double)));
+    GL_HIP(ctx, hipMalloc(&ctx->Gtop, (size_t)kGPlanes * Pp * sizeof(double)));
+    hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->g.S);
+    hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.P), dim3(kCgBlock), 0, ctx->stream, ctx->Gtop, ctx->g.P);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+int ensure_gauge_fixed(glasma_ctx* ctx) {
+    if (!ctx->UG) GL_HIP(ctx, hipMalloc(&ctx->UG, (size_t)kUPlanes * pad_sites(ctx->g.S) * sizeof(double)));
+    if (!ctx->EG) GL_HIP(ctx, hipMalloc(&ctx->EG, (size_t)kEPlanes * pad_sites(ctx->g.S) * sizeof(double)));
+    return 0;
+}
+
+int transform_links(glasma_ctx* ctx) {
+    const Geom& g = ctx->g;
+    hipLaunchKernelGGL(k_cg_transform, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], ctx->G, ctx->Gtop,
+                       ctx->UG, g);
+    hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+}  // namespace glasma
+
+using namespace glasma;
+
+extern "C" int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, double* stats7) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    if (maxit < 0 || !(tol >= 0.0)) return set_error(ctx, GLASMA_ERR_ARG, "invalid tolerance or iteration cap");
+    int rc = ensure_gauge(ctx);
+    if (!rc) rc = ensure_gauge_fixed(ctx);
+    if (rc) return rc;
+    const Geom& g = ctx->g;
+    const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
+    rc = ensure_red(ctx, 4 * (size_t)nblk + 8);
+    if (!rc) rc = ensure_fftbuf(ctx, 16 * (size_t)g.S);
+    if (rc) return rc;
+    const double* a = ctx->p.a;
+    // weights (1/a_mu)^2 of w and the functional's starting minimum 6 sum_mu 1/a_mu^2, rounded as the
+    // reference writes them
+    const double cx = (1.0 / a[0]) * (1.0 / a[0]), cy = (1.0 / a[1]) * (1.0 / a[1]), cz = (1.0 / a[2]) * (1.0 / a[2]);
+    const double fm0 = 6.0 * (1.0 / (a[0] * a[0]) + 1.0 / (a[1] * a[1]) + 1.0 / (a[2] * a[2]));
+    double* part = ctx->red;
+    double* fin = ctx->red + 4 * nblk;
+    double* F = ctx->fftbuf;
+    if ((rc = transform_links(ctx))) return rc;
+    double gmax = 1.0, gavg = 0.0, smax = 0.0, savg = 0.0, fmin = 0.0, fsum = 0.0;
+    int n = 0;
+    const double V = (double)g.S;
+    while (n < maxit && gmax > tol) {
+        double h[8];
+        hipLaunchKernelGGL(k_cg_dev, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->UG, F, part, g, cx, cy, cz,
+                           fm0);
+        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
+        if ((rc = fft_3d(ctx->stream, F, g, 8, -1))) return set_error(ctx, rc, "gauge fixing FFT failed");
+        hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1], a[2]);
+        if ((rc = fft_3d(ctx->stream, F, g, 8, +1))) return set_error(ctx, rc, "gauge fixing FFT failed");
+        hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g, alpha);
+        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
+        if ((rc = transform_links(ctx))) return rc;
+        GL_HIP(ctx, hipMemcpyAsync(h, fin, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        gmax = h[2];
+        gavg = std::sqrt(h[0] / (V * 8.0));
+        fmin = h[3];
+        fsum = h[1] / V;
+        smax = h[6];
+        savg = std::sqrt(h[4] / (V * 8.0));
+        n++;
+    }
+    if ((rc = transform_links(ctx))) return rc;
+    hipLaunchKernelGGL(k_cg_efield, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->E, ctx->EG, g);
+    GL_HIP(ctx, hipGetLastError());
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (stats7) {
+        stats7[0] = n;
+        stats7[1] = gmax;
+        stats7[2] = gavg;
+        stats7[3] = smax;
+        stats7[4] = savg;
+        stats7[5] = fmin;
+        stats7[6] = fsum;
+    }
+    return 0;
+}
+
+extern "C" int glasma_gauge_reset(glasma_ctx* ctx) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    int rc = ensure_gauge(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->g.S);
+    hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.P), dim3(kCgBlock), 0, ctx->stream, ctx->Gtop, ctx->g.P);
+    GL_HIP(ctx, hipGetLastError());
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int glasma_get_gauge_fixed(glasma_ctx* ctx, double* UG, double* EG, double* G) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    if ((UG || EG) && !ctx->UG) return set_error(ctx, GLASMA_ERR_STATE, "no gauge-fixed fields yet");
+    int rc = 0;
+    if (UG && (rc = download_aos(ctx, ctx->UG, UG, kUPlanes))) return rc;
+    if (EG && (rc = download_aos(ctx, ctx->EG, EG, kEPlanes))) return rc;
+    if (G) {
+        if ((rc = ensure_gauge(ctx))) return rc;
+        if ((rc = download_aos(ctx, ctx->G, G, kGPlanes))) return rc;
+    }
+    return 0;
+}
+
+extern "C" int glasma_set_gauge_transformation(glasma_ctx* ctx, const double* G) {
+    if (!ctx || !G) return GLASMA_ERR_ARG;
+    int rc = ensure_gauge(ctx);
+    if (rc) return rc;
+    if ((rc = upload_aos(ctx, G, ctx->G, kGPlanes))) return rc;
+    // the top ghost layer (z = Nz) as given
+    const Geom& g = ctx->g;
+    std::vector<double> top((size_t)pad_sites(g.P) * kGPlanes);
+    const double* src = G + (size_t)(g.nz + 1) * g.P * 18;
+    for (long s2 = 0; s2 < g.P; s2++)
+        for (int p = 0; p < 18; p++) top[lidx<long>(kGPlanes, p, s2)] = src[18 * s2 + p];
+    GL_HIP(ctx, hipMemcpy(ctx->Gtop, top.data(), top.size() * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}

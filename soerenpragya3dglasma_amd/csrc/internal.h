@@ -67,6 +67,13 @@ int set_error(glasma_ctx* ctx, int code, const std::string& msg);
 int hip_fail(glasma_ctx* ctx, hipError_t e, const char* what);
 int ensure_red(glasma_ctx* ctx, size_t doubles);
 int ensure_fftbuf(glasma_ctx* ctx, size_t doubles);
+int reset_fields(glasma_ctx* ctx);
+// host array in the reference AoS layout with z ghosts (R doubles per site) <-> planes p0..p0+R-1 of a
+// tiled device field with NP planes (NP < 0: NP = R)
+int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* dev_soa, int R, int NP = -1, int p0 = 0);
+int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_with_ghosts, int R, int NP = -1, int p0 = 0);
+void host_sync_ghosts(const glasma_ctx* ctx, double* h, int R);
+__global__ void k_set_identity(double* __restrict__ U, long S, int nmat);
 
 // FFT over complex arrays in device memory: lines of length n with element stride `es`; line l
 // starts at (l % b1) * s1 + (l / b1) * s2 (units of complex entries).  sign -1 forward, +1 backward.

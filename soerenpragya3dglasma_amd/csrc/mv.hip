@@ -1,0 +1,453 @@
+// MV-model initial conditions on the device, SU(3):
+//   CollidingObjects::SetWilsonLines  SC/COLLIDINGOBJECTS/CollidingObjects.cpp:318-371
+//     (SetChargeDensities :34-102, SolveLaplaceEquation :131-176, SetNewWilsonLines :179-226,
+//      LongitudinalProfile :30-32)
+//   InitialConditions::SingleSheet    SC/INITIALCONDITIONS/SUNCCURRENTS/RevSingleNucleus.cpp:7-210
+//     with Dynamics::SUNc::EvolveRhoViaFourier SC/DYNAMICS/SUNcDynamics.cpp:275-350 and
+//     CollidingObjects::SetWilsonLinesForFirstStep SC/COLLIDINGOBJECTS/CollidingObjects.cpp:377-411
+//   the initial state of Simulation::Run  SC/Simulation.cpp:327-378 (collision) and of the left / right
+//     single-nucleus drivers (SC/LeftNucleusSimulation.cpp:343-357, SC/RightNucleusSimulation.cpp:343-357).
+//
+// The reference walks z sequentially and solves one transverse slice at a time.  Here every transverse
+// operation (charge, smoothing, Poisson solve) runs on all z slices at once as batched 2D FFTs; only the
+// path-ordered Wilson-line product is sequential in z, one thread per transverse site.
+//
+// Work arrays (allocated for the call, freed at the end):
+//   F     complex, F[(a*Nz + z)*P + y*Nx + x], a = colour 0..7 (the FFT layout)
+//   rho   real,    rho[a*S + s]  (s = x + Nx*(y + Ny*z))
+//   V     Wilson lines in the tiled matrix layout (18 planes)
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+namespace glasma {
+
+namespace {
+
+struct Wk {
+    double* F = nullptr;
+    double* rho = nullptr;
+    double* V = nullptr;
+    double* xi = nullptr;
+};
+
+__device__ __forceinline__ long fidx(const Geom& g, int a, long s) {
+    // complex entry (a, s) of F; s = s2 + P*z so (a*Nz + z)*P + s2 = a*S + s
+    return (long)a * g.S + s;
+}
+
+__device__ __forceinline__ double psqr2(const Geom& g, int px, int py, double ax, double ay) {
+    return (2.0 - 2.0 * cos(2.0 * kPi * px / (double)g.nx)) / (ax * ax) +
+           (2.0 - 2.0 * cos(2.0 * kPi * py / (double)g.ny)) / (ay * ay);
+}
+
+// SetChargeDensities: rho(z) = mult * sqrt(ax ay) * T(z) * xi with T the unit-area Gaussian in index units
+__global__ void k_charge(double* __restrict__ F, const double* __restrict__ xi, Geom g, double mult_norm,
+                         int center, double width) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const double u = (z - center) / width;
+    const double prof = exp(-0.5 * u * u) / sqrt(2.0 * kPi * width * width);
+    const double c = mult_norm * prof;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        F[2 * i] = c * xi[a + 8 * s2];
+        F[2 * i + 1] = 0.0;
+    }
+}
+
+// momentum-space factors of one z slice batch.  mode 0: smoothing nf * exp(-p^2/2 Lambda^2) p^2/(p^2+m^2);
+// mode 1: Poisson nf / p^2 with the zero mode set to 0.
+__global__ void k_transverse_filter(double* __restrict__ F, Geom g, double ax, double ay, double lam, double meff,
+                                   int mode) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const long s2 = s % g.P;
+    const int px = (int)(s2 % g.nx), py = (int)(s2 / g.nx);
+    const double p2 = psqr2(g, px, py, ax, ay);
+    const double nf = 1.0 / ((double)g.nx * g.ny);
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        double re = F[2 * i], im = F[2 * i + 1];
+        if (mode == 0) {
+            const double f = exp(-p2 / (2.0 * lam * lam));
+            re = nf * re * f * p2 / (p2 + meff * meff);
+            im = nf * im * f * p2 / (p2 + meff * meff);
+        } else if (px == 0 && py == 0) {
+            re = 0.0;
+            im = 0.0;
+        } else {
+            re = nf * re / p2;
+            im = nf * im / p2;
+        }
+        F[2 * i] = re;
+        F[2 * i + 1] = im;
+    }
+}
+
+// rho = Re F (optional), then F = (Re F, 0)
+__global__ void k_take_real(double* __restrict__ F, double* __restrict__ rho, Geom g) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        if (rho) rho[(long)a * g.S + s] = F[2 * i];
+        F[2 * i + 1] = 0.0;
+    }
+}
+
+// F = (rho, 0)
+__global__ void k_load_real(double* __restrict__ F, const double* __restrict__ rho, Geom g) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        F[2 * i] = rho[(long)a * g.S + s];
+        F[2 * i + 1] = 0.0;
+    }
+}
+
+// SetNewWilsonLines for all z: one thread per transverse site walks z in the reference's order
+// (L: Nz-1 down to 0 with exp(-i A t / sqrt 2); R: 0 up to Nz-1 with exp(+i A t / sqrt 2)),
+// W <- exp(+-i A(z) t / sqrt 2) W, V(z) = W.
+__global__ void k_wilson_scan(const double* __restrict__ F, double* __restrict__ V, Geom g, int right) {
+    long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s2 >= g.P) return;
+    M3 W, X, T;
+    set_unit(W);
+    const double c = (right ? 1.0 : -1.0) / kSqrt2;
+    for (int k = 0; k < g.nz; k++) {
+        const int z = right ? k : g.nz - 1 - k;
+        const long s = s2 + g.P * z;
+        double col[8];
+        for (int a = 0; a < 8; a++) col[a] = F[2 * fidx(g, a, s)];
+        expi(c, col, X);
+        mul_nn(X, W, T);
+        W = T;
+        stm<long>(V, kGPlanes, 0, s, W);
+    }
+}
+
+// transverse links V^dag(x) V(x + mu) for mu = x, y and the unit z link, for z in [zlo, zhi]
+__global__ void k_sheet_links(const double* __restrict__ V, double* __restrict__ U, Geom g, int zlo, int zhi) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)(zhi - zlo + 1) * g.P;
+    if (i >= n) return;
+    const long s = i + (long)zlo * g.P;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    M3 A, B, C;
+    ldm<long>(V, kGPlanes, 0, s, A);
+    ldm<long>(V, kGPlanes, 0, sidx(g, x + 1, y, z), B);
+    mul_dn(A, B, C);
+    stm<long>(U, kUPlanes, 0, s, C);
+    ldm<long>(V, kGPlanes, 0, sidx(g, x, y + 1, z), B);
+    mul_dn(A, B, C);
+    stm<long>(U, kUPlanes, 18, s, C);
+    set_unit(C);
+    stm<long>(U, kUPlanes, 36, s, C);
+}
+
+// EvolveRhoViaFourier phase: F(pz) *= nf * exp(i sgn omega(pz) dt), sgn by half-spectrum and direction
+__global__ void k_evolve_phase(double* __restrict__ F, Geom g, double az, double dt, int right) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int pz = (int)(s / g.P);
+    const double om = sqrt((2.0 - 2.0 * cos(2.0 * kPi * (double)pz / (double)g.nz)) / (az * az));
+    double sgn;
+    if (!right) sgn = (pz < g.nz / 2) ? -1.0 : 1.0;
+    else sgn = (pz < g.nz / 2) ? 1.0 : -1.0;
+    double sn, cs;
+    sincos(sgn * om * dt, &sn, &cs);
+    const double nf = 1.0 / g.nz;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        const double re = nf * F[2 * i], im = nf * F[2 * i + 1];
+        F[2 * i] = cs * re - sn * im;
+        F[2 * i + 1] = sn * re + cs * im;
+    }
+}
+
+// E_perp(t + dt/2) = MatrixILog(coff_mu, U_mu UOld_mu^dag), E_z = 0 for z in [zlo, zhi]  (:66-92)
+__global__ void k_sheet_efield(const double* __restrict__ U, const double* __restrict__ UO, double* __restrict__ E,
+                               Geom g, int zlo, int zhi, double coff0, double coff1) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)(zhi - zlo + 1) * g.P;
+    if (i >= n) return;
+    const long s = i + (long)zlo * g.P;
+    for (int mu = 0; mu < 2; mu++) {
+        M3 A, B, C;
+        ldm<long>(U, kUPlanes, 18 * mu, s, A);
+        ldm<long>(UO, kUPlanes, 18 * mu, s, B);
+        mul_nd(A, B, C);
+        double e[8];
+        ilog(mu == 0 ? coff0 : coff1, C, e);
+        stv<long>(E, kEPlanes, 8 * mu, s, e);
+    }
+    double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    stv<long>(E, kEPlanes, 16, s, zero);
+}
+
+// J_stat = sqrt2 * sum_{i=x,y} (E_i(x) - U_i^dag(x-i) E_i(x-i) U_i(x-i)) inside [zlo, zhi], 0 outside
+// (:98-125); also F = (J_stat, 0) for the continuity inversion.
+__global__ void k_sheet_jstat(const double* __restrict__ U, const double* __restrict__ E, double* __restrict__ J,
+                              double* __restrict__ F, Geom g, int zlo, int zhi, int wstat) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    double js[8];
+    if (z >= zlo && z <= zhi) {
+        const long sx = sidx(g, x - 1, y, z);
+        const long sy = sidx(g, x, y - 1, z);
+        M3 Ux, Uy;
+        double ex[8], ey[8], exm[8], eym[8], xd[8], yd[8];
+        ldm<long>(U, kUPlanes, 0, sx, Ux);
+        ldm<long>(U, kUPlanes, 18, sy, Uy);
+        ldv<long>(E, kEPlanes, 0, s, ex);
+        ldv<long>(E, kEPlanes, 8, s, ey);
+        ldv<long>(E, kEPlanes, 0, sx, exm);
+        ldv<long>(E, kEPlanes, 8, sy, eym);
+        inv_adj_mul(Ux, exm, xd);
+        inv_adj_mul(Uy, eym, yd);
+        for (int a = 0; a < 8; a++) js[a] = kSqrt2 * ((ex[a] - xd[a]) + (ey[a] - yd[a]));
+    } else {
+        for (int a = 0; a < 8; a++) js[a] = 0.0;
+    }
+    stv<long>(J, kJPlanes, 8 * wstat, s, js);
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        F[2 * i] = js[a];
+        F[2 * i + 1] = 0.0;
+    }
+}
+
+// continuity inversion in z momentum space (:131-209):
+//   J~(0) = J~stat(0); J~(N/2) = 0; otherwise J~ = J~stat (e^{-+ i omega dt} - 1) / (dt pZD)
+__global__ void k_jinv(double* __restrict__ F, Geom g, double az, double dt, int right) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int pz = (int)(s / g.P);
+    const int nz = g.nz;
+    const double om = sqrt((2.0 - 2.0 * cos(2.0 * kPi * (double)pz / (double)nz)) / (az * az));
+    double es, ec;
+    sincos(-2.0 * kPi * (double)pz / (double)nz, &es, &ec);
+    // pZD = +-(1 - e)/az
+    double zr = (1.0 - ec) / az, zi = -es / az;
+    double sp, cp, sm, cm;
+    sincos(om * dt, &sp, &cp);
+    sincos(-om * dt, &sm, &cm);
+    double pr, pi_, nr, ni;  // pNpos, pNneg
+    if (right) {
+        pr = cp - 1.0; pi_ = sp; nr = cm - 1.0; ni = sm;
+    } else {
+        zr = -zr; zi = -zi;
+        pr = cm - 1.0; pi_ = sm; nr = cp - 1.0; ni = sp;
+    }
+    const double dr = dt * zr, di = dt * zi;  // dt * pZD
+    const double den = dr * dr + di * di;
+    const double nf = 1.0 / nz;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        const double jr = nf * F[2 * i], ji = nf * F[2 * i + 1];
+        double outr, outi;
+        if (pz == 0) {
+            outr = jr; outi = ji;
+        } else if (pz == nz / 2) {
+            outr = 0.0; outi = 0.0;
+        } else {
+            const double mr = (pz < nz / 2) ? pr : nr, mi = (pz < nz / 2) ? pi_ : ni;
+            // (j * m) / d
+            const double qr = jr * mr - ji * mi, qi = jr * mi + ji * mr;
+            outr = (qr * dr + qi * di) / den;
+            outi = (qi * dr - qr * di) / den;
+        }
+        F[2 * i] = outr;
+        F[2 * i + 1] = outi;
+    }
+}
+
+// J = Re F into planes 8*w of the current-density field
+__global__ void k_store_j(const double* __restrict__ F, double* __restrict__ J, Geom g, int w) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    double v[8];
+    for (int a = 0; a < 8; a++) v[a] = F[2 * fidx(g, a, s)];
+    stv<long>(J, kJPlanes, 8 * w, s, v);
+}
+
+// rho (real, a*S + s) -> host AoS with ghosts (a + 8*Index3D)
+__global__ void k_rho_to_aos(const double* __restrict__ rho, double* __restrict__ out, Geom g) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 8 * g.S) return;
+    const long s = i / 8;
+    const int a = (int)(i % 8);
+    out[i] = rho[(long)a * g.S + s];
+}
+
+inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int alloc_work(glasma_ctx* ctx, Wk& w) {
+    const long S = ctx->g.S, Sp = pad_sites(S);
+    int rc = ensure_fftbuf(ctx, 16 * (size_t)S);
+    if (rc) return rc;
+    w.F = ctx->fftbuf;
+    GL_HIP(ctx, hipMalloc(&w.rho, 8 * (size_t)S * sizeof(double)));
+    GL_HIP(ctx, hipMalloc(&w.V, (size_t)kGPlanes * Sp * sizeof(double)));
+    GL_HIP(ctx, hipMalloc(&w.xi, 8 * (size_t)ctx->g.P * sizeof(double)));
+    return 0;
+}
+void free_work(Wk& w) {
+    if (w.rho) hipFree(w.rho);
+    if (w.V) hipFree(w.V);
+    if (w.xi) hipFree(w.xi);
+    w = Wk();
+}
+
+#define GL_RC(x)                  \
+    do {                          \
+        int _rc = (x);            \
+        if (_rc) return _rc;      \
+    } while (0)
+#define GL_FFT(ctx, x)                                                                       \
+    do {                                                                                     \
+        int _rc = (x);                                                                       \
+        if (_rc) return set_error(ctx, _rc, "FFT failed (lengths must be 2^a 3^b p <= 2048, " \
+                                            "prime factors <= 13)");                         \
+    } while (0)
+
+// Poisson solve of every transverse slice: F (real parts) -> A, imaginary parts of the result dropped
+int poisson_all(glasma_ctx* ctx, double* F) {
+    const Geom& g = ctx->g;
+    GL_FFT(ctx, fft_xy(ctx->stream, F, g, 8, g.nz, -1));
+    hipLaunchKernelGGL(k_transverse_filter, grid1(g.S), dim3(256), 0, ctx->stream, F, g, ctx->p.a[0], ctx->p.a[1], 0.0,
+                       0.0, 1);
+    GL_FFT(ctx, fft_xy(ctx->stream, F, g, 8, g.nz, +1));
+    return 0;
+}
+
+// CollidingObjects::SetWilsonLines for one nucleus: w.rho = smoothed charge, w.V = Wilson lines
+int set_wilson_lines(glasma_ctx* ctx, Wk& w, const glasma_mv_params* mv, const double* xi_host, bool right) {
+    const Geom& g = ctx->g;
+    GL_HIP(ctx, hipMemcpyAsync(w.xi, xi_host, 8 * (size_t)g.P * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const double norm = std::sqrt(ctx->p.a[0] * ctx->p.a[1]);
+    const double mult = right ? mv->nucleus_r : mv->nucleus_l;
+    hipLaunchKernelGGL(k_charge, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.xi, g, mult * norm,
+                       right ? mv->center_r : mv->center_l, right ? mv->sigma_r : mv->sigma_l);
+    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, -1));
+    hipLaunchKernelGGL(k_transverse_filter, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[0], ctx->p.a[1],
+                       mv->lambda, mv->m_eff, 0);
+    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, +1));
+    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.rho, g);
+    GL_RC(poisson_all(ctx, w.F));
+    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.V, g, right ? 1 : 0);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+// InitialConditions::SingleSheet for one nucleus over [zlo, zhi]; J-stat into plane group wstat, J into w
+int single_sheet(glasma_ctx* ctx, Wk& w, bool right, int zlo, int zhi, int wstat, int wj) {
+    const Geom& g = ctx->g;
+    const double dt = ctx->p.dt;
+    double* U = ctx->U[ctx->cur];
+    double* J = ctx->J[ctx->cur];
+    const long nsl = (long)(zhi - zlo + 1) * g.P;
+    // UOld = links of the sheet at t = 0 (:14-27)
+    hipLaunchKernelGGL(k_sheet_links, grid1(nsl), dim3(256), 0, ctx->stream, w.V, ctx->UOld, g, zlo, zhi);
+    // rho evolved by dt through the z FFT (:33, SUNcDynamics.cpp:275-350)
+    hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.rho, g);
+    GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, -1));
+    hipLaunchKernelGGL(k_evolve_phase, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[2], dt, right ? 1 : 0);
+    GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, +1));
+    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, ctx->stream, w.F, (double*)nullptr, g);
+    // Wilson lines of the evolved charge (SetWilsonLinesForFirstStep, no smoothing) and U(t + dt) (:40-60)
+    GL_RC(poisson_all(ctx, w.F));
+    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.V, g, right ? 1 : 0);
+    hipLaunchKernelGGL(k_sheet_links, grid1(nsl), dim3(256), 0, ctx->stream, w.V, U, g, zlo, zhi);
+    // E(t + dt/2) from the logarithm of U UOld^dag (:66-92)
+    const double acube = ctx->p.a[0] * ctx->p.a[1] * ctx->p.a[2];
+    const double coff0 = -dt * ctx->p.a[0] * ctx->p.a[0] / acube;
+    const double coff1 = -dt * ctx->p.a[1] * ctx->p.a[1] / acube;
+    hipLaunchKernelGGL(k_sheet_efield, grid1(nsl), dim3(256), 0, ctx->stream, U, ctx->UOld, ctx->E, g, zlo, zhi, coff0,
+                       coff1);
+    // J-stat from the transverse Gauss law and J from the z continuity inversion (:98-209)
+    hipLaunchKernelGGL(k_sheet_jstat, grid1(g.S), dim3(256), 0, ctx->stream, U, ctx->E, J, w.F, g, zlo, zhi, wstat);
+    GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, -1));
+    hipLaunchKernelGGL(k_jinv, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[2], dt, right ? 1 : 0);
+    GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, +1));
+    hipLaunchKernelGGL(k_store_j, grid1(g.S), dim3(256), 0, ctx->stream, w.F, J, g, wj);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+bool mv_valid(const glasma_mv_params* mv) {
+    return mv && mv->sigma_l > 0.0 && mv->sigma_r > 0.0 && mv->lambda > 0.0 && mv->m_eff >= 0.0;
+}
+
+}  // namespace
+
+}  // namespace glasma
+
+using namespace glasma;
+
+extern "C" int glasma_mv_initial_state(glasma_ctx* ctx, int driver, const glasma_mv_params* mv, const double* xi_l,
+                                       const double* xi_r) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    if (!mv_valid(mv)) return set_error(ctx, GLASMA_ERR_ARG, "invalid MV parameters");
+    if (driver < 0 || driver > 2) return set_error(ctx, GLASMA_ERR_ARG, "driver must be 0, 1 or 2");
+    if ((driver != 2 && !xi_l) || (driver != 1 && !xi_r)) return set_error(ctx, GLASMA_ERR_ARG, "missing charges");
+    int rc = reset_fields(ctx);
+    if (rc) return rc;
+    const int nz = ctx->g.nz;
+    Wk w;
+    rc = alloc_work(ctx, w);
+    if (rc == 0 && driver != 2) {
+        // collision: left sheet on [0, (Nz-2)/2] (SC/Simulation.cpp:359); single nucleus: all z
+        const int zhi = driver == 0 ? (nz - 2) / 2 : nz - 1;
+        rc = set_wilson_lines(ctx, w, mv, xi_l, false);
+        if (rc == 0) rc = single_sheet(ctx, w, false, 0, zhi, JPS, JP);
+    }
+    if (rc == 0 && driver != 1) {
+        const int zlo = driver == 0 ? nz / 2 : 0;
+        rc = set_wilson_lines(ctx, w, mv, xi_r, true);
+        if (rc == 0) rc = single_sheet(ctx, w, true, zlo, nz - 1, JMS, JM);
+    }
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "mv_initial_state");
+    }
+    free_work(w);
+    return rc;
+}
+
+extern "C" int glasma_set_wilson_lines(glasma_ctx* ctx, char dir, const glasma_mv_params* mv, const double* xi,
+                                       double* V_out, double* rho_out) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    if (!mv_valid(mv) || !xi || (dir != 'L' && dir != 'R'))
+        return set_error(ctx, GLASMA_ERR_ARG, "invalid arguments to glasma_set_wilson_lines");
+    Wk w;
+    int rc = alloc_work(ctx, w);
+    if (rc == 0) rc = set_wilson_lines(ctx, w, mv, xi, dir == 'R');
+    if (rc == 0 && V_out) rc = download_aos(ctx, w.V, V_out, kGPlanes);
+    if (rc == 0 && rho_out) {
+        const Geom& g = ctx->g;
+        double* tmp = nullptr;
+        hipError_t e = hipMalloc(&tmp, 8 * (size_t)g.S * sizeof(double));
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_rho_to_aos, grid1(8 * g.S), dim3(256), 0, ctx->stream, w.rho, tmp, g);
+            e = hipMemcpyAsync(rho_out + 8 * g.P, tmp, 8 * (size_t)g.S * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            hipFree(tmp);
+        }
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "set_wilson_lines rho");
+        else host_sync_ghosts(ctx, rho_out, 8);
+    }
+    free_work(w);
+    return rc;
+}

@@ -482,37 +482,113 @@ GHD void jacobi_herm(M3& A, M3& W) {
     }
 }
 
-// Principal traceless logarithm: returns A^a with exp(i C A^a t^a) = U.  Eigenvectors from the
-// Hermitian K = (U - U^dag)/2i, refined by the Hermitian part (U + U^dag)/2 in K's eigenbasis (this
-// separates eigenvalue pairs with equal sine); phases in (-pi, pi], shifted by 2 pi on the largest
-// (smallest) one when they sum to +2pi (-2pi).  Same meaning as SC/SUNc/SU2/AlgebraOperations.cpp:92-107.
-GHD void ilog(double C, const M3& U, double* out) {
-    M3 K, W, Hc, W2, T;
+// complex determinant and inverse (adjugate / det) of a 3x3 matrix
+GHD void det3(const M3& A, double& dr, double& di) {
+    dr = 0.0;
+    di = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        // A_0j * (A_1j1 A_2j2 - A_1j2 A_2j1)
+        const double mr = A.re[mi(1, j1)] * A.re[mi(2, j2)] - A.im[mi(1, j1)] * A.im[mi(2, j2)] -
+                          (A.re[mi(1, j2)] * A.re[mi(2, j1)] - A.im[mi(1, j2)] * A.im[mi(2, j1)]);
+        const double mim = A.re[mi(1, j1)] * A.im[mi(2, j2)] + A.im[mi(1, j1)] * A.re[mi(2, j2)] -
+                           (A.re[mi(1, j2)] * A.im[mi(2, j1)] + A.im[mi(1, j2)] * A.re[mi(2, j1)]);
+        dr += A.re[mi(0, j)] * mr - A.im[mi(0, j)] * mim;
+        di += A.re[mi(0, j)] * mim + A.im[mi(0, j)] * mr;
+    }
+}
+GHD void inv3(const M3& A, M3& B) {
+    double dr, di;
+    det3(A, dr, di);
+    const double n = dr * dr + di * di;
+    const double ir = dr / n, ii = -di / n;  // 1/det
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            // (U - U^dag)/(2i): (a - conj(b))/(2i) with a = U_ij, b = U_ji
-            const double dr = U.re[mi(i, j)] - U.re[mi(j, i)];
-            const double di = U.im[mi(i, j)] + U.im[mi(j, i)];
-            K.re[mi(i, j)] = 0.5 * di;
-            K.im[mi(i, j)] = -0.5 * dr;
-            Hc.re[mi(i, j)] = 0.5 * (U.re[mi(i, j)] + U.re[mi(j, i)]);
-            Hc.im[mi(i, j)] = 0.5 * (U.im[mi(i, j)] - U.im[mi(j, i)]);
+            // B_ij = cofactor(j, i) / det
+            const int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+            const double cr = A.re[mi(r0, c0)] * A.re[mi(r1, c1)] - A.im[mi(r0, c0)] * A.im[mi(r1, c1)] -
+                              (A.re[mi(r0, c1)] * A.re[mi(r1, c0)] - A.im[mi(r0, c1)] * A.im[mi(r1, c0)]);
+            const double ci = A.re[mi(r0, c0)] * A.im[mi(r1, c1)] + A.im[mi(r0, c0)] * A.re[mi(r1, c1)] -
+                              (A.re[mi(r0, c1)] * A.im[mi(r1, c0)] + A.im[mi(r0, c1)] * A.re[mi(r1, c0)]);
+            B.re[mi(i, j)] = cr * ir - ci * ii;
+            B.im[mi(i, j)] = cr * ii + ci * ir;
         }
-    jacobi_herm(K, W);
-    M3 tmp, H2;
-    mul_nn(Hc, W, tmp);
-    mul_dn(W, tmp, H2);
-    jacobi_herm(H2, W2);
-    mul_nn(W, W2, T);
-    // phases of the diagonal of T^dag U T
-    M3 UT, D;
-    mul_nn(U, T, UT);
-    mul_dn(T, UT, D);
+}
+
+// Principal traceless logarithm: returns A^a with exp(i C A^a t^a) = U (the meaning of the SU(2)
+// MatrixILog, SC/SUNc/SU2/AlgebraOperations.cpp:92-107).  Through the Cayley transform: Y = i^k U with
+// k in {0,1,2,3} maximising |det(1 + Y)| (no eigenvalue of Y near -1; one of the four always keeps
+// |det| >= 0.44), C = i (1 - Y)(1 + Y)^-1 is Hermitian with eigenvalues tan(psi_j / 2), monotonic in
+// Y's eigenphases psi_j, so distinct phases stay distinct in C and its Jacobi eigenvectors are accurate
+// even near the identity.  Phases psi_j - k pi/2 wrapped to (-pi, pi]; when they sum to +-2 pi the
+// largest (smallest) is moved by 2 pi.
+GHD void ilog(double C, const M3& U, double* out) {
+    int kb = 0;
+    double db = -1.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        M3 A;
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            // i^k U
+            const double r = U.re[e], m = U.im[e];
+            A.re[e] = k == 0 ? r : (k == 1 ? -m : (k == 2 ? -r : m));
+            A.im[e] = k == 0 ? m : (k == 1 ? r : (k == 2 ? -m : -r));
+        }
+        A.re[0] += 1.0;
+        A.re[4] += 1.0;
+        A.re[8] += 1.0;
+        double dr, di;
+        det3(A, dr, di);
+        const double d = dr * dr + di * di;
+        if (d > db) { db = d; kb = k; }
+    }
+    M3 Y, Ap, Am, Ai, Cm, T;
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+        const double r = U.re[e], m = U.im[e];
+        Y.re[e] = kb == 0 ? r : (kb == 1 ? -m : (kb == 2 ? -r : m));
+        Y.im[e] = kb == 0 ? m : (kb == 1 ? r : (kb == 2 ? -m : -r));
+        Ap.re[e] = Y.re[e];
+        Ap.im[e] = Y.im[e];
+        Am.re[e] = -Y.re[e];
+        Am.im[e] = -Y.im[e];
+    }
+#pragma unroll
+    for (int d = 0; d < 9; d += 4) {
+        Ap.re[d] += 1.0;
+        Am.re[d] += 1.0;
+    }
+    inv3(Ap, Ai);
+    M3 P;
+    mul_nn(Am, Ai, P);
+    // Cm = i P, made exactly Hermitian
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            // (i P)_ij = -Im P_ij + i Re P_ij ; Hermitian part (X + X^dag)/2
+            const double xr = -P.im[mi(i, j)], xi = P.re[mi(i, j)];
+            const double yr = -P.im[mi(j, i)], yi = P.re[mi(j, i)];
+            Cm.re[mi(i, j)] = 0.5 * (xr + yr);
+            Cm.im[mi(i, j)] = 0.5 * (xi - yi);
+        }
+    jacobi_herm(Cm, T);
+    // phases of the diagonal of T^dag Y T
+    M3 YT, D;
+    mul_nn(Y, T, YT);
+    mul_dn(T, YT, D);
     double ph[3];
 #pragma unroll
-    for (int j = 0; j < 3; j++) ph[j] = atan2(D.im[mi(j, j)], D.re[mi(j, j)]);
+    for (int j = 0; j < 3; j++) {
+        double p = atan2(D.im[mi(j, j)], D.re[mi(j, j)]) - kb * (0.5 * kPi);
+        if (p > kPi) p -= 2.0 * kPi;
+        if (p <= -kPi) p += 2.0 * kPi;
+        ph[j] = p;
+    }
     const double s = ph[0] + ph[1] + ph[2];
     if (s > kPi) {
         int m = (ph[0] >= ph[1] && ph[0] >= ph[2]) ? 0 : (ph[1] >= ph[2] ? 1 : 2);
