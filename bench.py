@@ -120,7 +120,9 @@ def main():
     init = args.init
     if init == "mv":
         try:
-            xl, xr = Glasma.draw_charges(1, N[0], N[1], 0.5, 0.5)
+            # left nucleus from seed 1, right nucleus from seed 2, each drawn as the reference draws it
+            xl, _ = Glasma.draw_charges(1, N[0], N[1], 0.5, 0.5)
+            _, xr = Glasma.draw_charges(2, N[0], N[1], 0.5, 0.5)
             mv = Glasma.mv_params(lam=1.65, meff=0.1, sigma_l=8.0, sigma_r=8.0,
                                   center_l=int(0.85 * N[2] / 2 - 1), center_r=int(1.15 * N[2] / 2 - 1))
             t0 = time.perf_counter()
@@ -170,7 +172,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (" + ("MV collision initial state, seed 1" if init == "mv" else "random SU(3) fields") + ")",
+        "data": "synthetic (" + ("MV collision initial state built on the device, g2mu 0.5, sigma 8, m 0.1, "
+                                 "Lambda 1.65, charge seeds 1 / 2" if init == "mv" else "random SU(3) fields") + ")",
         "config": {"workload": "large single-event FP64 collision, 256x256x1024, a=1, dt=a/4, open z",
                    "lattice": list(N), "sites": S, "init": init, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -79,10 +79,18 @@ def build(verbose: bool = False, jobs: int = 8, out_dir: str | None = None) -> s
     if os.path.exists(main):
         _run([HIPCC, *flags, "-x", "hip", main, "-o", os.path.join(out, "glasma_solver"),
               "-L", out, "-lglasma", "-Wl,-rpath,$ORIGIN"])
-    hc = os.path.join(CSRC, "hostcheck.cpp")
-    if os.path.exists(hc) and shutil.which("g++"):
-        _run(["g++", *HOST_CXX_FLAGS, "-shared", "-o", os.path.join(out, "libglasma_hostcheck.so"), hc])
+    build_hostcheck(out)
     return lib
+
+
+def build_hostcheck(out: str | None = None) -> str:
+    """Host (g++) build of the per-site device code, for CPU tests against the oracle."""
+    out = os.path.abspath(out) if out else OUT
+    os.makedirs(out, exist_ok=True)
+    path = os.path.join(out, "libglasma_hostcheck.so")
+    if shutil.which("g++"):
+        _run(["g++", *HOST_CXX_FLAGS, "-shared", "-o", path, os.path.join(CSRC, "hostcheck.cpp")])
+    return path
 
 
 if __name__ == "__main__":

@@ -67,3 +67,28 @@ def fields_maxdiff(d: dict, f: O.Fields) -> dict:
         "JpS": float(np.abs(d["JpS"] - f.JpS).max()),
         "JmS": float(np.abs(d["JmS"] - f.JmS).max()),
     }
+
+
+_hc = None
+
+
+def hostcheck_lib():
+    """The engine's per-site code compiled for the host (libglasma_hostcheck.so), built on demand with g++."""
+    global _hc
+    if _hc is None:
+        import ctypes
+        import os
+        from soerenpragya3dglasma_amd import build as B
+        path = os.path.join(B.OUT, "libglasma_hostcheck.so")
+        if not os.path.exists(path):
+            B.build_hostcheck()
+        lib = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        cd = ctypes.c_double
+        for name, args in {"hc_su3_expi": [cd, dp, dp], "hc_su3_expi_m1": [cd, dp, dp], "hc_su3_ilog": [cd, dp, dp],
+                           "hc_su3_adj_mul": [dp, dp, dp], "hc_su3_inv_adj_mul": [dp, dp, dp],
+                           "hc_su3_retrigenu": [dp, dp],
+                           "hc_step": [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_int, cd, dp, dp, dp, dp, dp, dp]}.items():
+            getattr(lib, name).argtypes = args
+        _hc = lib
+    return _hc
