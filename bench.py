@@ -70,6 +70,46 @@ def cpu_baseline(sample_n, steps):
     }
 
 
+def batched_ensemble(nevents: int, steps: int) -> dict:
+    """The ensemble workload: events drawn with master seed 12345 (numpy RandomState), N_perp uniform in
+    {32, 48, 64, 96}, Nz uniform in {128, 192, 256, 384}, g^2 mu uniform in [0.3, 0.8], charge seeds 2k+1
+    (left) and 2k+2 (right), MV collision initial states built on the device; `steps` leapfrog steps of
+    every event, batched (one launch per step over all events, replayed from a HIP graph) against serial
+    per-event launches.  Both runs start from the same initial states."""
+    from soerenpragya3dglasma_amd import Glasma, GlasmaBatch
+    rng = np.random.RandomState(12345)
+    specs, qs = [], []
+    for _ in range(nevents):
+        nperp = int(rng.choice([32, 48, 64, 96]))
+        nz = int(rng.choice([128, 192, 256, 384]))
+        qs.append(float(rng.uniform(0.3, 0.8)))
+        specs.append(((nperp, nperp, nz), (1.0, 1.0, 1.0), 0.25))
+    sites = sum(sp[0][0] * sp[0][1] * sp[0][2] for sp in specs)
+    log(f"[bench] ensemble: {nevents} events, {sites} sites in total, {steps} steps")
+    out = {"events": nevents, "steps": steps, "sites_total": sites}
+    with GlasmaBatch(specs) as b:
+        t0 = time.perf_counter()
+        for k, (ev, sp, q) in enumerate(zip(b.events, specs, qs)):
+            N = sp[0]
+            xl, _ = Glasma.draw_charges(2 * k + 1, N[0], N[1], q, q)
+            _, xr = Glasma.draw_charges(2 * k + 2, N[0], N[1], q, q)
+            mv = Glasma.mv_params(lam=1.65, meff=0.1, sigma_l=4.0, sigma_r=4.0,
+                                  center_l=int(0.85 * N[2] / 2 - 1), center_r=int(1.15 * N[2] / 2 - 1))
+            ev.mv_initial_state(0, mv, xl, xr)
+        log(f"[bench] ensemble initial states in {time.perf_counter() - t0:.1f} s")
+        b.update(1)  # graph capture, warm-up
+        b.update_serial(1)
+        b.update(steps)
+        t_batch = b.last_ms() * 1e-3
+        b.update_serial(steps)
+        t_serial = b.last_ms() * 1e-3
+    out.update({"batched_s": t_batch, "serial_s": t_serial, "events_per_s_batched": nevents / t_batch,
+                "events_per_s_serial": nevents / t_serial, "batched_over_serial": t_serial / t_batch,
+                "site_updates_per_s_batched": sites * steps / t_batch})
+    log(f"[bench] ensemble: batched {t_batch:.2f} s, serial {t_serial:.2f} s, ratio {t_serial / t_batch:.2f}")
+    return out
+
+
 def calibrate_copy(gib: float) -> None:
     """A plain device-to-device copy kernel of known size, for scaling the HBM byte counters of rocprofv3.
 
@@ -99,6 +139,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, nargs=3, default=[256, 256, 128])
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-events", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.calibrate_copy is not None:
@@ -147,6 +190,10 @@ def main():
         f"unitarity max {umax:.2e}")
     g.close()
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_ensemble(args.batch_events, args.batch_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle on the host)")
@@ -175,6 +222,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_step", "kernel_ms": kern_ms, "bytes_per_site_update": BYTES_PER_SITE_UPDATE},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

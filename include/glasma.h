@@ -157,6 +157,25 @@ int glasma_fft_host(double* data, int nd, const int* dims, long batch, int sign)
  * J-, J+stat, J-stat with uniform values in [-scale, scale), from a counter-based hash of (seed, site). */
 int glasma_fill_random(glasma_ctx* ctx, unsigned long seed, double link_scale, double e_scale, double j_scale);
 
+/* ---- ensembles of events (no counterpart in the reference, which runs one event per process) ------- */
+/* An ensemble of independent events, each an ordinary context (its own lattice, spacings and time step)
+ * reached through glasma_batch_context for fields, initial conditions and observables.
+ * glasma_batch_update advances every event by nsteps leapfrog steps (Dynamics::SUNc::Update,
+ * DYNAMICS/SUNcDynamics.cpp:353-378) with one kernel launch per step over all events, the launches
+ * captured once per step count into a HIP graph.  Each event's fields are bit-identical to advancing it
+ * alone.  glasma_batch_update_serial advances the events one after another with per-event launches
+ * (the baseline).  glasma_batch_last_ms: device time of the last batched call, wall time of the last
+ * serial call.  Each event needs fewer than 2^32 sites. */
+typedef struct glasma_batch glasma_batch;
+int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out);   /* p[nevents] */
+int glasma_batch_destroy(glasma_batch* b);
+int glasma_batch_size(const glasma_batch* b, int* nevents);
+int glasma_batch_context(glasma_batch* b, int e, glasma_ctx** ctx);
+const char* glasma_batch_last_error(const glasma_batch* b);
+int glasma_batch_update(glasma_batch* b, int nsteps);
+int glasma_batch_update_serial(glasma_batch* b, int nsteps);
+int glasma_batch_last_ms(const glasma_batch* b, double* ms);
+
 /* ---- per-site SU(3) algebra on the device, batched (tests and tools) ------------------------- */
 /* op: 0 exp(i c alpha t) [in: n*8 alpha, out: n*18], 1 log [in: n*18 U, out: n*8],
  *     2 V alpha V^dag [in: n*18 V, in2: n*8, out: n*8], 3 V^dag alpha V [same],

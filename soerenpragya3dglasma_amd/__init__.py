@@ -4,6 +4,6 @@ The device library is the only execution path: importing works without a GPU, bu
 goes through libglasma.so and raises GlasmaError if it is not built or no device is present.
 """
 from ._lib import GlasmaError, header_functions, lib  # noqa: F401
-from .engine import Glasma, bulk_from_raw, fft_host, su3_batch  # noqa: F401
+from .engine import Glasma, GlasmaBatch, bulk_from_raw, fft_host, su3_batch  # noqa: F401
 
-__all__ = ["Glasma", "GlasmaError", "bulk_from_raw", "fft_host", "su3_batch", "lib", "header_functions"]
+__all__ = ["Glasma", "GlasmaBatch", "GlasmaError", "bulk_from_raw", "fft_host", "su3_batch", "lib", "header_functions"]

@@ -78,6 +78,14 @@ def lib():
         "glasma_fft_host": ([dp, c_int, ip, c_long, c_int], c_int),
         "glasma_fill_random": ([vp, c_ulong, c_double, c_double, c_double], c_int),
         "glasma_su3_batch": ([c_int, c_long, c_double, dp, dp, dp], c_int),
+        "glasma_batch_create": ([ctypes.POINTER(Params), c_int, ctypes.POINTER(vp)], c_int),
+        "glasma_batch_destroy": ([vp], c_int),
+        "glasma_batch_size": ([vp, ctypes.POINTER(c_int)], c_int),
+        "glasma_batch_context": ([vp, c_int, ctypes.POINTER(vp)], c_int),
+        "glasma_batch_last_error": ([vp], ctypes.c_char_p),
+        "glasma_batch_update": ([vp, c_int], c_int),
+        "glasma_batch_update_serial": ([vp, c_int], c_int),
+        "glasma_batch_last_ms": ([vp, dp], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name, None)

@@ -77,6 +77,13 @@ __global__ void k_set_identity(double* __restrict__ U, long S, int nmat);
 
 // FFT over complex arrays in device memory: lines of length n with element stride `es`; line l
 // starts at (l % b1) * s1 + (l / b1) * s2 (units of complex entries).  sign -1 forward, +1 backward.
+// leapfrog launches (leapfrog.hip)
+int launch_step(glasma_ctx* ctx, bool write_old);
+StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old);
+int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
+                      long nblocks);
+TileMap default_tile_map(const Geom& g);
+
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign);
 // complex array c[(comp*nz + z)*P + y*nx + x] of `ncomp` components: 1D along z, 2D in (x,y), 3D.
 int fft_z(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);

@@ -13,30 +13,21 @@
 // and walks the planes in order; rows y-1, y, y+1 are then processed by the same XCD close in time
 // (L2 hits) and a plane's band is re-read by the next plane shortly afterwards (MALL hits).
 //
-// GLASMA_STEP_WAVES (2 or 3: register budget of 256 or 168 per lane) and GLASMA_STEP_XCD (0/1)
-// select variants for measurement sweeps; the defaults are the fastest measured on MI355X.
+// GLASMA_STEP_WAVES (2 or 3: register budget of 256 or 168 per lane), GLASMA_STEP_FOUR and
+// GLASMA_STEP_MAP / _BW / _BH select variants for measurement sweeps; the defaults are the fastest
+// measured on MI355X.
 #include <cstdlib>
 
 #include "internal.h"
 
 namespace glasma {
 
-constexpr int kXcds = 8;
-
-// first site of this block: with bpb > 0, XCD-aware band mapping with bpb tiles per band per plane
-__device__ __forceinline__ long tile_first_site(const Geom& g, int bpb) {
-    const long b = blockIdx.x;
-    if (bpb <= 0) return b * kTile;
-    const long k = b % kXcds, i = b / kXcds;
-    const long z = i / bpb, j = i % bpb;
-    return z * g.P + k * (g.P / kXcds) + j * kTile;
-}
 
 template <class I, int W>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))) k_step(StepArgs A, int bpb) {
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))) k_step(StepArgs A, TileMap tm) {
     __shared__ double sh[4][8][kTile];  // J+stat'(x), J-stat'(x), J+stat'(x+z), J-stat'(x+z)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long sl = tile_first_site(A.g, bpb) + lane;
+    const long sl = tile_site(A.g, tm, blockIdx.x) + lane;
     const bool valid = sl < A.g.S;
     const I s = (I)(valid ? sl : 0);
     int x, y, z;
@@ -84,11 +75,12 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))
 // Four-wavefront variant: waves 0-2 own the link directions, wave 3 forms J+-stat'(x) and J+-stat'(x+z)
 // and, after the barrier, applies the J advection with U_h,z handed over through LDS.  Two such blocks
 // fill the 8 wavefront slots of a CU at two waves per SIMD.
-template <class I, int W>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) k_step4(StepArgs A, int bpb) {
+// the body of the four-wavefront step for the tile starting at site `first`
+template <class I>
+__device__ __forceinline__ void step_tile4(const StepArgs& A, long first) {
     __shared__ double shU[18][kTile];  // U_h,z
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long sl = tile_first_site(A.g, bpb) + lane;
+    const long sl = first + lane;
     const bool valid = sl < A.g.S;
     const I s = (I)(valid ? sl : 0);
     int x, y, z;
@@ -126,39 +118,103 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))
     }
 }
 
+// Four-wavefront variant: waves 0-2 own the link directions, wave 3 forms J+-stat'(x) and J+-stat'(x+z)
+// and, after the barrier, applies the J advection with U_h,z handed over through LDS.  Two such blocks
+// fill the 8 wavefront slots of a CU at two waves per SIMD.
+template <class I, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) k_step4(StepArgs A, TileMap tm) {
+    step_tile4<I>(A, tile_site(A.g, tm, blockIdx.x));
+}
+
+// Ensemble step: one launch advances every event of a batch.  Block b works on the tile starting at site
+// block_site[b] of event e = block_event[b] (each event's blocks in the order tile_site gives it alone);
+// the event's arguments come from a device table (uniform per block, so they are read with scalar
+// loads).  The per-site arithmetic is that of k_step4.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_step_batch(const StepArgs* __restrict__ table, const int* __restrict__ block_event,
+             const unsigned* __restrict__ block_site) {
+    const int e = block_event[blockIdx.x];
+    const StepArgs A = table[e];
+    step_tile4<unsigned int>(A, (long)block_site[blockIdx.x]);
+}
+
+int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
+                      long nblocks) {
+    if (nblocks <= 0 || nblocks > 0x7fffffffL) return GLASMA_ERR_ARG;
+    hipLaunchKernelGGL(k_step_batch, dim3((unsigned)nblocks), dim3(256), 0, st, table, block_event, block_site);
+    return hipGetLastError() == hipSuccess ? 0 : GLASMA_ERR_HIP;
+}
+
+// the arguments of one step of a context (current buffers, optional UOld)
+StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old) {
+    StepArgs A;
+    A.U = ctx->U[cur];
+    A.Un = ctx->U[cur ^ 1];
+    A.E = ctx->E;
+    A.J = ctx->J[cur];
+    A.Jn = ctx->J[cur ^ 1];
+    A.UOld = write_old ? ctx->UOld : nullptr;
+    A.g = ctx->g;
+    fill_step_coefficients(A, ctx->p.a, ctx->p.dt);
+    return A;
+}
+
 static int env_int(const char* name, int dflt) {
     const char* v = std::getenv(name);
     return v ? std::atoi(v) : dflt;
 }
 
 template <class I>
-static void launch(int waves, int four, int bpb, const StepArgs& A, hipStream_t st) {
+static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hipStream_t st) {
     dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
     if (four) {
-        hipLaunchKernelGGL((k_step4<I, 2>), grid, dim3(256), 0, st, A, bpb);
+        hipLaunchKernelGGL((k_step4<I, 2>), grid, dim3(256), 0, st, A, tm);
         return;
     }
-    if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, dim3(192), 0, st, A, bpb);
-    else hipLaunchKernelGGL((k_step<I, 2>), grid, dim3(192), 0, st, A, bpb);
+    if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, dim3(192), 0, st, A, tm);
+    else hipLaunchKernelGGL((k_step<I, 2>), grid, dim3(192), 0, st, A, tm);
+}
+
+// tile order for a geometry: the requested mode when the lattice fits it, else the next simpler one
+TileMap tile_map(const Geom& g, int mode, int bw, int bh) {
+    TileMap m = {0, 0, 0, 0, 0, 0};
+    if (mode == 2 && g.nx % kTile == 0 && bw > 0 && bh > 0) {
+        const int tpr = g.nx / kTile;
+        if (tpr % bw == 0 && g.ny % bh == 0) {
+            const long bands = (long)(tpr / bw) * (g.ny / bh);
+            if (bands % kXcds == 0) {
+                m.mode = 2;
+                m.bw = bw;
+                m.bh = bh;
+                m.nbx = tpr / bw;
+                m.tpbp = bw * bh;
+                return m;
+            }
+        }
+    }
+    if (mode >= 1 && g.P % (kXcds * kTile) == 0) {
+        m.mode = 1;
+        m.bpb = (int)(g.P / (kXcds * kTile));
+    }
+    return m;
+}
+
+TileMap default_tile_map(const Geom& g) {
+    // GLASMA_STEP_MAP = mode (0, 1, 2) and, for mode 2, the band width in tiles and height in rows
+    // (mode 2 with 4 x 4 bands measured fastest at 256 x 256 x 1024: 33.9 ms against 36.6 ms for mode 1)
+    static const int map_mode = env_int("GLASMA_STEP_MAP", 2);
+    static const int map_bw = env_int("GLASMA_STEP_BW", 4);
+    static const int map_bh = env_int("GLASMA_STEP_BH", 4);
+    return tile_map(g, map_mode, map_bw, map_bh);
 }
 
 int launch_step(glasma_ctx* ctx, bool write_old) {
     static const int waves = env_int("GLASMA_STEP_WAVES", 2);
-    static const int xcd = env_int("GLASMA_STEP_XCD", 1);
     static const int four = env_int("GLASMA_STEP_FOUR", 1);
-    StepArgs A;
-    A.U = ctx->U[ctx->cur];
-    A.Un = ctx->U[ctx->cur ^ 1];
-    A.E = ctx->E;
-    A.J = ctx->J[ctx->cur];
-    A.Jn = ctx->J[ctx->cur ^ 1];
-    A.UOld = write_old ? ctx->UOld : nullptr;
-    A.g = ctx->g;
-    fill_step_coefficients(A, ctx->p.a, ctx->p.dt);
-    // the band mapping needs whole tiles per band
-    const int bpb = (xcd && ctx->g.P % (kXcds * kTile) == 0) ? (int)(ctx->g.P / (kXcds * kTile)) : 0;
-    if (A.g.S < (1L << 32)) launch<unsigned int>(waves, four, bpb, A, ctx->stream);
-    else launch<unsigned long>(waves, four, bpb, A, ctx->stream);
+    const StepArgs A = step_args(ctx, ctx->cur, write_old);
+    const TileMap tm = default_tile_map(ctx->g);
+    if (A.g.S < (1L << 32)) launch<unsigned int>(waves, four, tm, A, ctx->stream);
+    else launch<unsigned long>(waves, four, tm, A, ctx->stream);
     ctx->cur ^= 1;
     return 0;
 }

@@ -207,6 +207,36 @@ GHD void plaq_mat(const double* __restrict__ U, int kind, int m1, I s1, int m2, 
 // reference recomputes it from identical inputs), as U + (X - 1) U with X - 1 formed directly, which
 // keeps rounding from biasing U away from SU(3).
 // ------------------------------------------------------------------------------------------------
+// Order in which the blocks of a step visit the tiles.  Workgroups are dealt to the 8 XCDs round-robin
+// (block b runs on XCD b % 8), each XCD with its own L2.
+//   mode 0: linear, tile b.
+//   mode 1: XCD k owns the k-th band of P/8 sites of every z plane and walks the planes in order.
+//   mode 2: column bands of bw tiles (64 x-sites each) by bh rows; XCD k takes bands k, k+8, ... and
+//           walks each band through all z planes before the next one, so the few planes of one band that
+//           are in flight at once share their z neighbours in L2.
+constexpr int kXcds = 8;
+struct TileMap {
+    int mode, bpb;
+    int bw, bh, nbx, tpbp;
+};
+
+// first site of block b
+GHD long tile_site(const Geom& g, const TileMap& m, long b) {
+    if (m.mode == 0) return b * kTile;
+    const long k = b % kXcds, i = b / kXcds;
+    if (m.mode == 1) {
+        const long z = i / m.bpb, j = i % m.bpb;
+        return z * g.P + k * (g.P / kXcds) + j * kTile;
+    }
+    const long per_band = (long)g.nz * m.tpbp;
+    const long mb = i / per_band, r = i - mb * per_band;
+    const long z = r / m.tpbp, t = r - z * m.tpbp;
+    const long band = mb * kXcds + k;
+    const long bxo = (band % m.nbx) * m.bw, byo = (band / m.nbx) * m.bh;
+    const long tx = t % m.bw, ty = t / m.bw;
+    return z * g.P + (byo + ty) * g.nx + (bxo + tx) * kTile;
+}
+
 struct StepArgs {
     const double* __restrict__ U;  // time t links
     double* __restrict__ Un;       // time t+dt links

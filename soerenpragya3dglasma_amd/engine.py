@@ -60,7 +60,7 @@ def bulk_from_raw(raw, a) -> dict:
 class Glasma:
     """One SU(3) lattice on the current GPU."""
 
-    def __init__(self, N, a=(1.0, 1.0, 1.0), dt: float = 0.25, bc: int = 0):
+    def __init__(self, N, a=(1.0, 1.0, 1.0), dt: float = 0.25, bc: int = 0, _handle=None):
         self.N = tuple(int(n) for n in N)
         self.a = tuple(float(x) for x in a)
         self.dt = float(dt)
@@ -72,6 +72,11 @@ class Glasma:
             p.a[i] = self.a[i]
         p.dt = self.dt
         p.bc_z = self.bc
+        self.params = p
+        self._owned = _handle is None
+        if _handle is not None:  # an event of a GlasmaBatch: the batch owns the context
+            self._h = _handle
+            return
         h = ctypes.c_void_p()
         rc = _lib.lib().glasma_create(ctypes.byref(p), ctypes.byref(h))
         if rc != 0:
@@ -81,7 +86,8 @@ class Glasma:
     # -- lifetime -----------------------------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            _lib.lib().glasma_destroy(self._h)
+            if getattr(self, "_owned", True):
+                _lib.lib().glasma_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -242,6 +248,71 @@ class Glasma:
     def set_gauge_transformation(self, G):
         G = np.ascontiguousarray(G, np.complex128)
         self._chk(_lib.lib().glasma_set_gauge_transformation(self._h, _p(G)), "glasma_set_gauge_transformation")
+
+
+class GlasmaBatch:
+    """An ensemble of independent events advanced together (one launch per step over all events, the
+    steps replayed from a HIP graph).  ``events[e]`` is a ``Glasma`` view of event e for fields, initial
+    conditions and observables."""
+
+    def __init__(self, specs):
+        """specs: list of (N, a, dt) or (N, a, dt, bc)."""
+        n = len(specs)
+        arr = (Params * n)()
+        for i, sp in enumerate(specs):
+            N, a, dt = sp[:3]
+            bc = sp[3] if len(sp) > 3 else 0
+            arr[i].nc = 3
+            for k in range(3):
+                arr[i].n[k] = int(N[k])
+                arr[i].a[k] = float(a[k])
+            arr[i].dt = float(dt)
+            arr[i].bc_z = int(bc)
+        h = ctypes.c_void_p()
+        rc = _lib.lib().glasma_batch_create(arr, n, ctypes.byref(h))
+        if rc != 0:
+            raise GlasmaError(f"glasma_batch_create failed ({rc})")
+        self._h = h
+        self.events = []
+        for i, sp in enumerate(specs):
+            c = ctypes.c_void_p()
+            self._chk(_lib.lib().glasma_batch_context(self._h, i, ctypes.byref(c)), "glasma_batch_context")
+            self.events.append(Glasma(sp[0], sp[1], sp[2], sp[3] if len(sp) > 3 else 0, _handle=c))
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = _lib.lib().glasma_batch_last_error(self._h)
+            raise GlasmaError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}")
+
+    def update(self, nsteps: int):
+        self._chk(_lib.lib().glasma_batch_update(self._h, int(nsteps)), "glasma_batch_update")
+
+    def update_serial(self, nsteps: int):
+        self._chk(_lib.lib().glasma_batch_update_serial(self._h, int(nsteps)), "glasma_batch_update_serial")
+
+    def last_ms(self) -> float:
+        ms = ctypes.c_double()
+        self._chk(_lib.lib().glasma_batch_last_ms(self._h, ctypes.byref(ms)), "glasma_batch_last_ms")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for ev in self.events:
+                ev._h = None
+            _lib.lib().glasma_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def fft_host(data: np.ndarray, dims, batch: int, sign: int) -> np.ndarray:

@@ -36,6 +36,19 @@ def test_step_fields_match_oracle(N, a, bc):
     assert np.abs(uold - f.UOld).max() < 1e-12
 
 
+@pytest.mark.parametrize("N", [(256, 32, 4), (128, 64, 3), (48, 8, 5)])
+def test_step_tile_orders_match_oracle(N):
+    """Geometries that take the column-band (256 x 32), plane-band (128 x 64) and linear (48 x 8) tile orders."""
+    f = random_fields(N, seed=13)
+    with _glasma(N, (1.0, 0.9, 0.8), 0.1) as g:
+        upload(g, f)
+        g.Update(2)
+        d = g.get_fields()
+    O.update(f, (1.0, 0.9, 0.8), 0.1, 2)
+    diff = fields_maxdiff(d, f)
+    assert max(diff.values()) < 1e-12, diff
+
+
 def test_multi_step_and_time():
     N = (8, 8, 8)
     f = random_fields(N, seed=5)
