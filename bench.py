@@ -110,6 +110,30 @@ def batched_ensemble(nevents: int, steps: int) -> dict:
     return out
 
 
+def adjoint_cost(N=(48, 48, 128), steps: int = 200, ckpt: int = 20) -> dict:
+    """Gradient of the slab energy (|z - Nz/2| < 8) after `steps` steps of a left-moving MV sheet with
+    respect to its charge density rho(x), against the forward evaluation of the same energy."""
+    from soerenpragya3dglasma_amd import Glasma
+    xl, _ = Glasma.draw_charges(1, N[0], N[1], 0.5, 0.5)
+    z = np.arange(N[2])
+    center = int(0.85 * N[2] / 2 - 1)
+    prof = np.exp(-0.5 * ((z - center) / 4.0) ** 2) / np.sqrt(2 * np.pi * 16.0)
+    rho = prof[:, None, None, None] * xl[None]
+    mv = Glasma.mv_params(lam=1.65, meff=0.1, sigma_l=4.0, sigma_r=4.0, center_l=center, center_r=center)
+    with Glasma(N, (1.0, 1.0, 1.0), 0.25) as g:
+        g.slab_energy_from_charge(mv, rho, 2, N[2] // 2, 8)  # warm-up
+        t0 = time.perf_counter()
+        e = g.slab_energy_from_charge(mv, rho, steps, N[2] // 2, 8)
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        e2, grad = g.slab_energy_gradient(mv, rho, steps, ckpt, N[2] // 2, 8)
+        t_adj = time.perf_counter() - t0
+    log(f"[bench] adjoint: forward {t_fwd:.3f} s, forward+gradient {t_adj:.3f} s, ratio {t_adj / t_fwd:.1f}")
+    return {"lattice": list(N), "steps": steps, "checkpoint_every": ckpt, "energy": e, "forward_s": t_fwd,
+            "gradient_s": t_adj, "gradient_over_forward": t_adj / t_fwd, "grad_norm": float(np.linalg.norm(grad)),
+            "energies_agree": abs(e - e2) <= 1e-12 * abs(e)}
+
+
 def calibrate_copy(gib: float) -> None:
     """A plain device-to-device copy kernel of known size, for scaling the HBM byte counters of rocprofv3.
 
@@ -142,6 +166,7 @@ def main():
     ap.add_argument("--batch-events", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.calibrate_copy is not None:
@@ -194,6 +219,10 @@ def main():
     if not args.no_batch:
         batched = batched_ensemble(args.batch_events, args.batch_steps)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_cost()
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (oracle on the host)")
@@ -223,6 +252,7 @@ def main():
                      "kernel": "k_step", "kernel_ms": kern_ms, "bytes_per_site_update": BYTES_PER_SITE_UPDATE},
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

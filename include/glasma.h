@@ -176,6 +176,28 @@ int glasma_batch_update(glasma_batch* b, int nsteps);
 int glasma_batch_update_serial(glasma_batch* b, int nsteps);
 int glasma_batch_last_ms(const glasma_batch* b, double* ms);
 
+/* ---- reverse-mode derivatives (no counterpart in the reference, which has no gradient code) ------------ */
+/* Slab energy of the current fields: 1/2 sum over sites with |z - zc| < half of sum_mu c_mu (E_mu^a E_mu^a
+ * + 4 Re tr(1 - P_mu)), c_mu = (a_mu / a^3)^2, P_0 = P_yz, P_1 = P_zx, P_2 = P_xy rooted at the site: the
+ * local form of Observables::Bulk's T00 (OBSERVABLES/BulkObservables.cpp:22-97). */
+int glasma_slab_energy(glasma_ctx* ctx, int zc, int half, double* energy);
+/* One leapfrog step's vector-Jacobian product: the arrays hold cotangents of the fields after one step
+ * (reference AoS layout with ghosts, ghosts ignored) and are overwritten with the cotangents of the fields
+ * before it.  The context's fields are the linearisation point and are left unchanged (UOld is not). */
+int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double* Jmb, double* JpSb, double* JmSb);
+/* The left-moving MV sheet over all z (InitialConditions::SingleSheet with 'L',
+ * INITIALCONDITIONS/SUNCCURRENTS/RevSingleNucleus.cpp:7-210) built from a given charge density rho (the
+ * density CollidingObjects::SetChargeDensities produces, COLLIDINGOBJECTS/CollidingObjects.cpp:34-102;
+ * a + 8*Index3D with ghosts, ghosts ignored; smoothing with mv->lambda, mv->m_eff), then nsteps leapfrog
+ * steps; energy = glasma_slab_energy of the result. */
+int glasma_slab_energy_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int zc,
+                                   int half, double* energy);
+/* The same energy and its gradient with respect to rho (grad_rho: same layout, ghosts zero), by
+ * reverse-mode differentiation of the steps (forward states checkpointed every ckpt steps, each segment
+ * recomputed once) and of the initial conditions.  The context holds the initial state afterwards. */
+int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int ckpt,
+                                int zc, int half, double* energy, double* grad_rho);
+
 /* ---- per-site SU(3) algebra on the device, batched (tests and tools) ------------------------- */
 /* op: 0 exp(i c alpha t) [in: n*8 alpha, out: n*18], 1 log [in: n*18 U, out: n*8],
  *     2 V alpha V^dag [in: n*18 V, in2: n*8, out: n*8], 3 V^dag alpha V [same],

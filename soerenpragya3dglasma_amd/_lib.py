@@ -86,6 +86,10 @@ def lib():
         "glasma_batch_update": ([vp, c_int], c_int),
         "glasma_batch_update_serial": ([vp, c_int], c_int),
         "glasma_batch_last_ms": ([vp, dp], c_int),
+        "glasma_slab_energy": ([vp, c_int, c_int, dp], c_int),
+        "glasma_step_vjp": ([vp] + [dp] * 6, c_int),
+        "glasma_slab_energy_from_charge": ([vp, ctypes.POINTER(MVParams), dp, c_int, c_int, c_int, dp], c_int),
+        "glasma_slab_energy_gradient": ([vp, ctypes.POINTER(MVParams), dp, c_int, c_int, c_int, c_int, dp, dp], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name, None)

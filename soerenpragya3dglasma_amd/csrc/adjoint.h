@@ -1,0 +1,392 @@
+// Reverse-mode derivative (vector-Jacobian product) of the leapfrog step and of the slab energy, per site.
+//
+// Conventions: a complex matrix M has the cotangent Mbar with dL = <Mbar, dM> = Re tr(Mbar^dag dM); an
+// algebra vector v (8 reals) has the cotangent vbar with dL = vbar . dv.  Links are treated as general
+// complex matrices (every formula of the step is defined for those), so the products below are exact
+// adjoints and finite differences along arbitrary directions check them.
+//
+// Forward step (site.h): E' = E + gc Re tr(i t U Sigma) - dtz (J+ - J-)/sqrt2 [z],
+// J+-stat' = J+-stat -+ dtz (J+- - Ad(U_z^dag(x-z)) J+-(x-z)), X = exp(i cU E' t), U_h = X U,
+// J+-' = J+- -+ dtz (Ad(U_h,z) J+-stat'(x+z) - J+-stat'), U' = X U_h.  The adjoint runs in three passes:
+//   A (site-local): cotangents of X, U_h, U (direct part), E (through the exponential), J+-stat', J+-;
+//   B (gather from x+z): the J+-(x-z), U_z(x-z) terms of the J-stat' update;
+//   C (scatter): the staple terms of the E update into the links of neighbouring sites.
+// The exponential's adjoint: with A = i cU E' t, dexp_A[H] = e^A sum_k (-ad_A)^k H/(k+1)!, so
+// Zbar = sum_k ad_A^k (X^dag Xbar)/(k+1)! and Ebar^a += cU Re tr(i t^a Zbar^dag).
+#pragma once
+
+#include "site.h"
+
+namespace glasma {
+
+GHD void m3_zero(M3& A) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) { A.re[k] = 0.0; A.im[k] = 0.0; }
+}
+GHD void m3_axpy(double c, const M3& A, M3& C) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) { C.re[k] = fma(c, A.re[k], C.re[k]); C.im[k] = fma(c, A.im[k], C.im[k]); }
+}
+// Mbar = -i c H(v)  (cotangent of M from r = c Re tr(i t^a M) with cotangent v)
+GHD void m_from_retr_cot(double c, const double* v, M3& Mbar) {
+    M3 H;
+    herm_from_alg(c, v, H);
+#pragma unroll
+    for (int k = 0; k < 9; k++) { Mbar.re[k] = H.im[k]; Mbar.im[k] = -H.re[k]; }
+}
+// r = Re tr(i t^a Z^dag)
+GHD void retr_igen_dag(const M3& Z, double* r) {
+    M3 D;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) { D.re[mi(i, j)] = Z.re[mi(j, i)]; D.im[mi(i, j)] = -Z.im[mi(j, i)]; }
+    retr_igen(D, r);
+}
+// Vbar += s * 4 L V R  for Hermitian L = H(l), R = H(r): the V-cotangent of adj_mul (L = H(obar), R = H(alpha))
+// and of inv_adj_mul (L = H(alpha), R = H(obar)) with output cotangent obar
+GHD void acc_lvr(double s, const double* l, const M3& V, const double* r, M3& Vbar) {
+    M3 L, R, T, P;
+    herm_from_alg(1.0, l, L);
+    herm_from_alg(1.0, r, R);
+    mul_nn(L, V, T);
+    mul_nn(T, R, P);
+    m3_axpy(4.0 * s, P, Vbar);
+}
+
+// ebar += cU Re tr(i t^a Zbar^dag), Zbar = sum_k ad_A^k(X^dag Xbar)/(k+1)!, A = i cU e^a t^a
+GHD void dexp_adjoint(double cU, const double* e, const M3& X, const M3& Xbar, double* ebar) {
+    M3 Hm, A, Y, Z, T1, T2;
+    herm_from_alg(cU, e, Hm);
+#pragma unroll
+    for (int k = 0; k < 9; k++) { A.re[k] = -Hm.im[k]; A.im[k] = Hm.re[k]; }  // i H
+    mul_dn(X, Xbar, Y);
+    Z = Y;
+    double fact = 1.0;
+    for (int k = 1; k <= 40; k++) {
+        mul_nn(A, Y, T1);
+        mul_nn(Y, A, T2);
+        double mx = 0.0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            Y.re[q] = T1.re[q] - T2.re[q];
+            Y.im[q] = T1.im[q] - T2.im[q];
+            mx = fmax(mx, fmax(fabs(Y.re[q]), fabs(Y.im[q])));
+        }
+        fact /= (double)(k + 1);
+        m3_axpy(fact, Y, Z);
+        if (mx * fact < 1e-19) break;
+    }
+    double r[8];
+    retr_igen_dag(Z, r);
+#pragma unroll
+    for (int a = 0; a < 8; a++) ebar[a] = fma(cU, r[a], ebar[a]);
+}
+
+// cotangent buffers of one backward step (tiled layouts like the fields)
+struct StepAdjArgs {
+    const double* __restrict__ U;    // links at time t
+    const double* __restrict__ Uh;   // half-step links U_h (the step's UOld)
+    const double* __restrict__ E1;   // E at t+dt (= E')
+    const double* __restrict__ J;    // currents at t
+    const double* __restrict__ J1;   // currents at t+dt
+    const double* __restrict__ Ub1;  // cotangent of U(t+dt)
+    double* __restrict__ Ub;         // cotangent of U(t): written by pass A, accumulated by B and C
+    double* __restrict__ Eb;         // in: cotangent of E(t+dt); out: cotangent of E(t) (in place)
+    const double* __restrict__ Jb1;  // cotangent of J(t+dt)
+    double* __restrict__ Jb;         // cotangent of J(t)
+    Geom g;
+    double gc[3], cE[3], cU[3], dtz;
+};
+
+// the site whose J+-stat' site s reads as "x+z" (jstat_new_up): above, or s itself on the open top
+template <class I>
+GHD I up_site(const Geom& g, I s, int z) {
+    const int zp = zup(g, z);
+    return zp != z ? (I)((long)s + g.P * (long)(zp - z)) : s;
+}
+// the site whose J+-, U_z site s reads as "x-z" (jstat_new): below, or s itself on the open bottom
+template <class I>
+GHD I down_site(const Geom& g, I s, int z) {
+    const int zm = zdn(g, z);
+    return (I)((long)s + g.P * (long)(zm - z));
+}
+
+// Pass A at site s.
+template <class I>
+GHD void step_vjp_local(const StepAdjArgs& A, I s) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    const double k = A.dtz;
+    double jbp[8], jbm[8], jsbp[8], jsbm[8], t8[8];
+    ldv<I>(A.Jb1, kJPlanes, JP * 8, s, jbp);
+    ldv<I>(A.Jb1, kJPlanes, JM * 8, s, jbm);
+    ldv<I>(A.Jb1, kJPlanes, JPS * 8, s, jsbp);
+    ldv<I>(A.Jb1, kJPlanes, JMS * 8, s, jsbm);
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        jsbp[a] += k * jbp[a];
+        jsbm[a] -= k * jbm[a];
+    }
+    // J+-stat'(s) is read as "x+z" by the site below (and by s itself on the open top layer)
+    {
+        const int zm = zdn(g, z);
+        if (zm != z) {
+            const I sb = (I)((long)s + g.P * (long)(zm - z));
+            if (up_site<I>(g, sb, zm) == s) {
+                M3 Uhb;
+                double jb[8];
+                ldm<I>(A.Uh, kUPlanes, 36, sb, Uhb);
+                ldv<I>(A.Jb1, kJPlanes, JP * 8, sb, jb);
+                inv_adj_mul(Uhb, jb, t8);
+                for (int a = 0; a < 8; a++) jsbp[a] -= k * t8[a];
+                ldv<I>(A.Jb1, kJPlanes, JM * 8, sb, jb);
+                inv_adj_mul(Uhb, jb, t8);
+                for (int a = 0; a < 8; a++) jsbm[a] += k * t8[a];
+            }
+        }
+        if (up_site<I>(g, s, z) == s) {
+            M3 Uhs;
+            ldm<I>(A.Uh, kUPlanes, 36, s, Uhs);
+            inv_adj_mul(Uhs, jbp, t8);
+            for (int a = 0; a < 8; a++) jsbp[a] -= k * t8[a];
+            inv_adj_mul(Uhs, jbm, t8);
+            for (int a = 0; a < 8; a++) jsbm[a] += k * t8[a];
+        }
+    }
+    // U_h,z cotangent from the J advection at s
+    M3 Uhbz;
+    m3_zero(Uhbz);
+    {
+        M3 Uhz;
+        double su[8];
+        ldm<I>(A.Uh, kUPlanes, 36, s, Uhz);
+        const I su_s = up_site<I>(g, s, z);
+        ldv<I>(A.J1, kJPlanes, JPS * 8, su_s, su);
+        acc_lvr(-k, jbp, Uhz, su, Uhbz);
+        ldv<I>(A.J1, kJPlanes, JMS * 8, su_s, su);
+        acc_lvr(k, jbm, Uhz, su, Uhbz);
+    }
+    // the three links: U' = X U_h, U_h = X U, X = exp(i cU E' t)
+    double ebz[8];
+    for (int mu = 0; mu < 3; mu++) {
+        M3 Ub1, Uh, U0, X, Xb, Uhbar, T;
+        double e1[8], eb[8];
+        ldm<I>(A.Ub1, kUPlanes, 18 * mu, s, Ub1);
+        ldm<I>(A.Uh, kUPlanes, 18 * mu, s, Uh);
+        ldm<I>(A.U, kUPlanes, 18 * mu, s, U0);
+        ldv<I>(A.E1, kEPlanes, 8 * mu, s, e1);
+        ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+        expi(A.cU[mu], e1, X);
+        mul_nd(Ub1, Uh, Xb);
+        mul_dn(X, Ub1, Uhbar);
+        if (mu == 2) m3_axpy(1.0, Uhbz, Uhbar);
+        mul_nd(Uhbar, U0, T);
+        m3_axpy(1.0, T, Xb);
+        mul_dn(X, Uhbar, T);
+        stm<I>(A.Ub, kUPlanes, 18 * mu, s, T);
+        dexp_adjoint(A.cU[mu], e1, X, Xb, eb);
+        stv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+        if (mu == 2)
+            for (int a = 0; a < 8; a++) ebz[a] = eb[a];
+    }
+    const double ks = k / kSqrt2;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        jbp[a] += -k * jsbp[a] - ks * ebz[a];
+        jbm[a] += k * jsbm[a] + ks * ebz[a];
+    }
+    stv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
+    stv<I>(A.Jb, kJPlanes, JM * 8, s, jbm);
+    stv<I>(A.Jb, kJPlanes, JPS * 8, s, jsbp);
+    stv<I>(A.Jb, kJPlanes, JMS * 8, s, jsbm);
+}
+
+// Pass B at site s: s is read as "x-z" by the site above (and by itself on the open bottom layer).
+template <class I>
+GHD void step_vjp_down(const StepAdjArgs& A, I s) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    const double k = A.dtz;
+    M3 Uz, Ubz;
+    ldm<I>(A.U, kUPlanes, 36, s, Uz);
+    ldm<I>(A.Ub, kUPlanes, 36, s, Ubz);
+    double jp[8], jm[8], jbp[8], jbm[8], t8[8];
+    ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
+    ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
+    ldv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
+    ldv<I>(A.Jb, kJPlanes, JM * 8, s, jbm);
+    for (int src = 0; src < 2; src++) {
+        I sx;
+        if (src == 0) {
+            const int zp = zup(g, z);
+            if (zp == z) continue;
+            sx = (I)((long)s + g.P * (long)(zp - z));
+            if (down_site<I>(g, sx, zp) != s) continue;
+        } else {
+            if (down_site<I>(g, s, z) != s) continue;
+            sx = s;
+        }
+        double jsb[8];
+        ldv<I>(A.Jb, kJPlanes, JPS * 8, sx, jsb);
+        adj_mul(Uz, jsb, t8);
+        for (int a = 0; a < 8; a++) jbp[a] += k * t8[a];
+        acc_lvr(k, jp, Uz, jsb, Ubz);
+        ldv<I>(A.Jb, kJPlanes, JMS * 8, sx, jsb);
+        adj_mul(Uz, jsb, t8);
+        for (int a = 0; a < 8; a++) jbm[a] -= k * t8[a];
+        acc_lvr(-k, jm, Uz, jsb, Ubz);
+    }
+    stv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
+    stv<I>(A.Jb, kJPlanes, JM * 8, s, jbm);
+    stm<I>(A.Ub, kUPlanes, 36, s, Ubz);
+}
+
+// Pass C at site s: staple terms of the E update, scattered with ADD (atomic on the device).
+template <class I, class ADD>
+GHD void step_vjp_staples(const StepAdjArgs& A, I s, ADD add) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    for (int mu = 0; mu < 3; mu++) {
+        double eb[8];
+        ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+        M3 Mb, Umu, Sig, Sigb, T;
+        m_from_retr_cot(A.gc[mu], eb, Mb);
+        ldm<I>(A.U, kUPlanes, 18 * mu, s, Umu);
+        m3_zero(Sig);
+        for (int q = 1; q <= 2; q++) {
+            const int nu = (mu + q) % 3;
+            M3 P1, P2, Tq;
+            ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, 0), P1);
+            ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, nu, 1, mu, 0), P2);
+            mul_nd(P1, P2, Tq);
+            ldm<I>(A.U, kUPlanes, 18 * nu, s, P1);
+            acc_mul_nd(A.cE[nu], Tq, P1, Sig);
+            const I sm = nbr<I>(g, x, y, z, nu, -1, mu, 0);
+            ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, -1), P1);
+            ldm<I>(A.U, kUPlanes, 18 * mu, sm, P2);
+            mul_dd(P1, P2, Tq);
+            ldm<I>(A.U, kUPlanes, 18 * nu, sm, P1);
+            acc_mul_nn(A.cE[nu], Tq, P1, Sig);
+        }
+        mul_nd(Mb, Sig, T);
+        add(mu, s, T);
+        mul_dn(Umu, Mb, Sigb);
+        for (int q = 1; q <= 2; q++) {
+            const int nu = (mu + q) % 3;
+            M3 Sb, P, Q, R, T1, T2;
+            Sb = Sigb;
+            for (int kk = 0; kk < 9; kk++) { Sb.re[kk] *= A.cE[nu]; Sb.im[kk] *= A.cE[nu]; }
+            // forward staple P Q^dag R^dag: P = U_nu(x+mu), Q = U_mu(x+nu), R = U_nu(x)
+            const I sP = nbr<I>(g, x, y, z, mu, 1, nu, 0), sQ = nbr<I>(g, x, y, z, nu, 1, mu, 0);
+            ldm<I>(A.U, kUPlanes, 18 * nu, sP, P);
+            ldm<I>(A.U, kUPlanes, 18 * mu, sQ, Q);
+            ldm<I>(A.U, kUPlanes, 18 * nu, s, R);
+            mul_nn(Sb, R, T1);
+            mul_nn(T1, Q, T2);
+            add(nu, sP, T2);  // Pbar = Sb R Q
+            mul_dd(R, Sb, T1);
+            mul_nn(T1, P, T2);
+            add(mu, sQ, T2);  // Qbar = R^dag Sb^dag P
+            mul_dn(Sb, P, T1);
+            mul_nd(T1, Q, T2);
+            add(nu, s, T2);  // Rbar = Sb^dag P Q^dag
+            // backward staple P^dag Q^dag R: P = U_nu(x+mu-nu), Q = U_mu(x-nu), R = U_nu(x-nu)
+            const I sP2 = nbr<I>(g, x, y, z, mu, 1, nu, -1), sm = nbr<I>(g, x, y, z, nu, -1, mu, 0);
+            ldm<I>(A.U, kUPlanes, 18 * nu, sP2, P);
+            ldm<I>(A.U, kUPlanes, 18 * mu, sm, Q);
+            ldm<I>(A.U, kUPlanes, 18 * nu, sm, R);
+            mul_dn(Q, R, T1);
+            mul_nd(T1, Sb, T2);
+            add(nu, sP2, T2);  // Pbar = Q^dag R Sb^dag
+            mul_nd(R, Sb, T1);
+            mul_nd(T1, P, T2);
+            add(mu, sm, T2);  // Qbar = R Sb^dag P^dag
+            mul_nn(Q, P, T1);
+            mul_nn(T1, Sb, T2);
+            add(nu, sm, T2);  // Rbar = Q P Sb
+        }
+    }
+}
+
+// Slab energy 1/2 sum_{|z - zc| < half} sum_mu c_mu (E_mu^2 + 4 Re tr(1 - P_mu)), c_mu = (a_mu/a^3)^2,
+// P_0 = P_yz, P_1 = P_zx, P_2 = P_xy rooted at x (the local form of Observables::Bulk's T00,
+// SC/OBSERVABLES/BulkObservables.cpp:22-97): value at site s and, with ADD, its cotangents
+// (Ebar written at s, links scattered).
+struct SlabArgs {
+    const double* __restrict__ U;
+    const double* __restrict__ E;
+    double* __restrict__ Eb;
+    Geom g;
+    double c[3];
+    int zc, half;
+};
+GHD bool in_slab(const SlabArgs& A, int z) { return (z - A.zc) < A.half && (A.zc - z) < A.half; }
+
+template <class I>
+GHD double slab_energy_site(const SlabArgs& A, I s) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    if (!in_slab(A, z)) return 0.0;
+    double v = 0.0;
+    for (int mu = 0; mu < 3; mu++) {
+        double e[8];
+        ldv<I>(A.E, kEPlanes, 8 * mu, s, e);
+        double e2 = 0.0;
+        for (int a = 0; a < 8; a++) e2 = fma(e[a], e[a], e2);
+        const int j = (mu + 1) % 3, kk = (mu + 2) % 3;
+        M3 Pq;
+        plaq_mat<I>(A.U, 0, j, s, kk, nbr<I>(g, x, y, z, j, 1, kk, 0), j, nbr<I>(g, x, y, z, kk, 1, j, 0), kk, s, Pq);
+        v += 0.5 * A.c[mu] * (e2 + 4.0 * retr_one_minus(Pq));
+    }
+    return v;
+}
+
+template <class I, class ADD>
+GHD void slab_energy_vjp_site(const SlabArgs& A, I s, double scale, ADD add) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    const bool in = in_slab(A, z);
+    for (int mu = 0; mu < 3; mu++) {
+        double e[8];
+        ldv<I>(A.E, kEPlanes, 8 * mu, s, e);
+        for (int a = 0; a < 8; a++) e[a] = in ? scale * A.c[mu] * e[a] : 0.0;
+        stv<I>(A.Eb, kEPlanes, 8 * mu, s, e);
+        if (!in) continue;
+        // P = a b c^dag d^dag: a = U_j(x), b = U_k(x+j), c = U_j(x+k), d = U_k(x); Pbar = -2 c_mu scale 1
+        const int j = (mu + 1) % 3, kk = (mu + 2) % 3;
+        const I sb = nbr<I>(g, x, y, z, j, 1, kk, 0), sc = nbr<I>(g, x, y, z, kk, 1, j, 0);
+        M3 Ma, Mb, Mc, Md, T1, T2, T3;
+        ldm<I>(A.U, kUPlanes, 18 * j, s, Ma);
+        ldm<I>(A.U, kUPlanes, 18 * kk, sb, Mb);
+        ldm<I>(A.U, kUPlanes, 18 * j, sc, Mc);
+        ldm<I>(A.U, kUPlanes, 18 * kk, s, Md);
+        const double w = -2.0 * A.c[mu] * scale;
+        // abar = w d c b^dag
+        mul_nn(Md, Mc, T1);
+        mul_nd(T1, Mb, T2);
+        for (int q = 0; q < 9; q++) { T2.re[q] *= w; T2.im[q] *= w; }
+        add(j, s, T2);
+        // bbar = w a^dag d c
+        mul_dn(Ma, Md, T1);
+        mul_nn(T1, Mc, T2);
+        for (int q = 0; q < 9; q++) { T2.re[q] *= w; T2.im[q] *= w; }
+        add(kk, sb, T2);
+        // cbar = w d^dag a b
+        mul_nn(Ma, Mb, T3);
+        mul_dn(Md, T3, T2);
+        for (int q = 0; q < 9; q++) { T2.re[q] *= w; T2.im[q] *= w; }
+        add(j, sc, T2);
+        // dbar = w a b c^dag
+        mul_nd(T3, Mc, T2);
+        for (int q = 0; q < 9; q++) { T2.re[q] *= w; T2.im[q] *= w; }
+        add(kk, s, T2);
+    }
+}
+
+}  // namespace glasma

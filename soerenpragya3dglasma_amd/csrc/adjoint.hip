@@ -1,0 +1,364 @@
+// Reverse-mode derivatives on the device (no counterpart in the reference, which has no gradient code):
+// the leapfrog step (three passes of adjoint.h per step), the slab energy, and the whole map
+// charge density -> left-moving MV sheet -> nsteps leapfrog steps -> slab energy, with the forward run
+// checkpointed every ckpt steps and each segment recomputed once during the backward sweep.
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+#include "adjoint.h"
+
+namespace glasma {
+namespace {
+
+constexpr int kAdjBlock = 256;
+inline dim3 agrid(long n) { return dim3((unsigned)((n + kAdjBlock - 1) / kAdjBlock)); }
+
+struct AtomicAddU {
+    double* Ub;
+    __device__ void operator()(int mu, unsigned s, const M3& M) const {
+        double* q = Ub + lidx<unsigned>(kUPlanes, 18 * mu, s);
+        for (int k = 0; k < 9; k++) {
+            unsafeAtomicAdd(q + k * 2 * kTile, M.re[k]);
+            unsafeAtomicAdd(q + k * 2 * kTile + 1, M.im[k]);
+        }
+    }
+};
+
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_local(StepAdjArgs A) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < A.g.S) step_vjp_local<unsigned>(A, (unsigned)s);
+}
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < A.g.S) step_vjp_down<unsigned>(A, (unsigned)s);
+}
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_staples(StepAdjArgs A) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < A.g.S) step_vjp_staples<unsigned>(A, (unsigned)s, AtomicAddU{A.Ub});
+}
+
+// slab energy: per-block partial sums, then one block sums them in a fixed order
+__global__ void __launch_bounds__(kAdjBlock) k_slab_energy(SlabArgs A, double* part) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = s < A.g.S ? slab_energy_site<unsigned>(A, (unsigned)s) : 0.0;
+    __shared__ double sh[kAdjBlock / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kAdjBlock / 64; w++) t += sh[w];
+        part[blockIdx.x] = t;
+    }
+}
+__global__ void __launch_bounds__(kAdjBlock) k_sum_parts(const double* part, long n, double* out) {
+    double v = 0.0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
+    __shared__ double sh[kAdjBlock / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kAdjBlock / 64; w++) t += sh[w];
+        *out = t;
+    }
+}
+__global__ void __launch_bounds__(kAdjBlock) k_slab_vjp(SlabArgs A, double* Ub) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < A.g.S) slab_energy_vjp_site<unsigned>(A, (unsigned)s, 1.0, AtomicAddU{Ub});
+}
+__global__ void k_fill0(double* p, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+SlabArgs slab_args(const glasma_ctx* ctx, int zc, int half) {
+    SlabArgs A;
+    A.U = ctx->U[ctx->cur];
+    A.E = ctx->E;
+    A.Eb = nullptr;
+    A.g = ctx->g;
+    const double* a = ctx->p.a;
+    const double acube = a[0] * a[1] * a[2];
+    for (int m = 0; m < 3; m++) A.c[m] = (a[m] / acube) * (a[m] / acube);
+    A.zc = zc;
+    A.half = half;
+    return A;
+}
+
+int slab_energy(glasma_ctx* ctx, int zc, int half, double* out) {
+    const long nb = (ctx->g.S + kAdjBlock - 1) / kAdjBlock;
+    int rc = ensure_red(ctx, (size_t)nb + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_slab_energy, dim3((unsigned)nb), dim3(kAdjBlock), 0, ctx->stream, slab_args(ctx, zc, half), ctx->red);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kAdjBlock), 0, ctx->stream, ctx->red, nb, ctx->red + nb);
+    GL_HIP(ctx, hipMemcpyAsync(out, ctx->red + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+StepAdjArgs adj_args(const glasma_ctx* ctx) {
+    StepArgs F = step_args(ctx, ctx->cur, false);
+    StepAdjArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.g = ctx->g;
+    for (int m = 0; m < 3; m++) {
+        A.gc[m] = F.gc[m];
+        A.cE[m] = F.cE[m];
+        A.cU[m] = F.cU[m];
+    }
+    A.dtz = F.dtz;
+    return A;
+}
+
+int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A) {
+    const long S = ctx->g.S;
+    hipLaunchKernelGGL(k_vjp_local, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_vjp_down, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_vjp_staples, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+// device buffers of one field state (tiled U, E, J) and helpers to move the context's state in and out
+struct State {
+    double *U, *E, *J;
+};
+int save_state(glasma_ctx* ctx, const State& s) {
+    const size_t Sp = pad_sites(ctx->g.S);
+    GL_HIP(ctx, hipMemcpyAsync(s.U, ctx->U[ctx->cur], kUPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    GL_HIP(ctx, hipMemcpyAsync(s.E, ctx->E, kEPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    GL_HIP(ctx, hipMemcpyAsync(s.J, ctx->J[ctx->cur], kJPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+int load_state(glasma_ctx* ctx, const State& s) {
+    const size_t Sp = pad_sites(ctx->g.S);
+    GL_HIP(ctx, hipMemcpyAsync(ctx->U[ctx->cur], s.U, kUPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    GL_HIP(ctx, hipMemcpyAsync(ctx->E, s.E, kEPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    GL_HIP(ctx, hipMemcpyAsync(ctx->J[ctx->cur], s.J, kJPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+
+// device memory of one call, freed on scope exit
+struct DevBuf {
+    std::vector<double*> p;
+    ~DevBuf() {
+        for (double* q : p) hipFree(q);
+    }
+    double* get(glasma_ctx* ctx, size_t doubles, int* rc) {
+        double* q = nullptr;
+        if (hipMalloc(&q, doubles * sizeof(double)) != hipSuccess) {
+            *rc = set_error(ctx, GLASMA_ERR_NOMEM, "device allocation failed in the adjoint");
+            return nullptr;
+        }
+        p.push_back(q);
+        return q;
+    }
+};
+
+// host charge density (a + 8*Index3D with ghosts) <-> device (a*S + s)
+void rho_to_dev_layout(const glasma_ctx* ctx, const double* h, std::vector<double>& out) {
+    const long S = ctx->g.S, P = ctx->g.P;
+    out.resize(8 * (size_t)S);
+    for (long s = 0; s < S; s++)
+        for (int a = 0; a < 8; a++) out[(size_t)a * S + s] = h[8 * (s + P) + a];
+}
+void rho_from_dev_layout(const glasma_ctx* ctx, const std::vector<double>& in, double* h) {
+    const long S = ctx->g.S, P = ctx->g.P;
+    std::memset(h, 0, sizeof(double) * 8 * (size_t)(S + 2 * P));
+    for (long s = 0; s < S; s++)
+        for (int a = 0; a < 8; a++) h[8 * (s + P) + a] = in[(size_t)a * S + s];
+}
+
+bool mv_ok(const glasma_mv_params* mv) { return mv && mv->lambda > 0.0 && mv->m_eff >= 0.0; }
+
+}  // namespace
+}  // namespace glasma
+
+using namespace glasma;
+
+extern "C" {
+
+int glasma_slab_energy(glasma_ctx* ctx, int zc, int half, double* energy) {
+    if (!ctx || !energy || half < 0) return GLASMA_ERR_ARG;
+    return slab_energy(ctx, zc, half, energy);
+}
+
+int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double* Jmb, double* JpSb, double* JmSb) {
+    if (!ctx || !Ub || !Eb || !Jpb || !Jmb || !JpSb || !JmSb) return GLASMA_ERR_ARG;
+    const size_t Sp = pad_sites(ctx->g.S);
+    int rc = 0;
+    DevBuf b;
+    double* s0U = b.get(ctx, kUPlanes * Sp, &rc);
+    double* s0E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+    double* s0J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    double* cU1 = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+    double* cU0 = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+    double* cE = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+    double* cJ1 = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    double* cJ0 = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    if (rc) return rc;
+    GL_HIP(ctx, hipMemsetAsync(cU1, 0, kUPlanes * Sp * sizeof(double), ctx->stream));
+    GL_HIP(ctx, hipMemsetAsync(cE, 0, kEPlanes * Sp * sizeof(double), ctx->stream));
+    GL_HIP(ctx, hipMemsetAsync(cJ1, 0, kJPlanes * Sp * sizeof(double), ctx->stream));
+    if ((rc = upload_aos(ctx, Ub, cU1, kUPlanes))) return rc;
+    if ((rc = upload_aos(ctx, Eb, cE, kEPlanes))) return rc;
+    const double* jin[4] = {Jpb, Jmb, JpSb, JmSb};
+    for (int w = 0; w < 4; w++)
+        if ((rc = upload_aos(ctx, jin[w], cJ1, 8, kJPlanes, 8 * w))) return rc;
+    const State s0 = {s0U, s0E, s0J};
+    if ((rc = save_state(ctx, s0))) return rc;
+    const double t = ctx->t;
+    const long ts = ctx->tsteps;
+    launch_step(ctx, true);
+    StepAdjArgs A = adj_args(ctx);
+    A.U = s0U;
+    A.Uh = ctx->UOld;
+    A.E1 = ctx->E;
+    A.J = s0J;
+    A.J1 = ctx->J[ctx->cur];
+    A.Ub1 = cU1;
+    A.Ub = cU0;
+    A.Eb = cE;
+    A.Jb1 = cJ1;
+    A.Jb = cJ0;
+    if ((rc = run_vjp(ctx, A))) return rc;
+    // back to the state at t
+    ctx->cur ^= 1;
+    if ((rc = load_state(ctx, s0))) return rc;
+    ctx->t = t;
+    ctx->tsteps = ts;
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = download_aos(ctx, cU0, Ub, kUPlanes))) return rc;
+    if ((rc = download_aos(ctx, cE, Eb, kEPlanes))) return rc;
+    double* jout[4] = {Jpb, Jmb, JpSb, JmSb};
+    for (int w = 0; w < 4; w++)
+        if ((rc = download_aos(ctx, cJ0, jout[w], 8, kJPlanes, 8 * w))) return rc;
+    return 0;
+}
+
+int glasma_slab_energy_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int zc,
+                                   int half, double* energy) {
+    if (!ctx || !rho || !energy || nsteps < 0 || half < 0 || !mv_ok(mv)) return GLASMA_ERR_ARG;
+    const long S = ctx->g.S, Sp = pad_sites(S);
+    int rc = 0;
+    DevBuf b;
+    IcTape tape;
+    double* rho_d = b.get(ctx, 8 * S, &rc);
+    tape.rho_s = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    tape.A0 = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    tape.A1 = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    tape.V0 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
+    tape.V1 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
+    if (rc) return rc;
+    std::vector<double> h;
+    rho_to_dev_layout(ctx, rho, h);
+    GL_HIP(ctx, hipMemcpy(rho_d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = ic_left_from_charge(ctx, mv, rho_d, tape))) return rc;
+    for (int n = 0; n < nsteps; n++) {
+        launch_step(ctx, n == nsteps - 1);
+        ctx->tsteps++;
+        ctx->t += ctx->p.dt;
+    }
+    GL_HIP(ctx, hipGetLastError());
+    return slab_energy(ctx, zc, half, energy);
+}
+
+int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int ckpt,
+                                int zc, int half, double* energy, double* grad_rho) {
+    if (!ctx || !rho || !energy || !grad_rho || nsteps < 0 || ckpt <= 0 || half < 0 || !mv_ok(mv)) return GLASMA_ERR_ARG;
+    const long S = ctx->g.S, Sp = pad_sites(S);
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    const int seglen = nsteps < ckpt ? nsteps : ckpt;
+    int rc = 0;
+    DevBuf b;
+    IcTape tape;
+    double* rho_d = b.get(ctx, 8 * S, &rc);
+    tape.rho_s = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    tape.A0 = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    tape.A1 = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    tape.V0 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
+    tape.V1 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
+    std::vector<State> ck(nseg > 0 ? nseg : 0), seg(seglen + 1);
+    for (auto& s : ck) {
+        s.U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+        s.E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+        s.J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    }
+    for (auto& s : seg) {
+        s.U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+        s.E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+        s.J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    }
+    std::vector<double*> Uh(seglen > 0 ? seglen : 0);
+    for (auto& p : Uh) p = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+    double* cU[2];
+    double* cJ[2];
+    cU[0] = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+    cU[1] = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+    cJ[0] = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    cJ[1] = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    double* cE = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+    double* rho_bar = rc ? nullptr : b.get(ctx, 8 * S, &rc);
+    double* work = rc ? nullptr : b.get(ctx, 54 * Sp + 36 * Sp + 32 * S, &rc);
+    if (rc) return rc;
+    std::vector<double> h;
+    rho_to_dev_layout(ctx, rho, h);
+    GL_HIP(ctx, hipMemcpy(rho_d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    // forward with checkpoints
+    if ((rc = ic_left_from_charge(ctx, mv, rho_d, tape))) return rc;
+    for (int n = 0; n < nsteps; n++) {
+        if (n % ckpt == 0 && (rc = save_state(ctx, ck[n / ckpt]))) return rc;
+        launch_step(ctx, false);
+    }
+    GL_HIP(ctx, hipGetLastError());
+    if ((rc = slab_energy(ctx, zc, half, energy))) return rc;
+    // cotangent of the final state
+    hipLaunchKernelGGL(k_fill0, agrid(kUPlanes * Sp), dim3(kAdjBlock), 0, ctx->stream, cU[0], kUPlanes * Sp);
+    hipLaunchKernelGGL(k_fill0, agrid(kJPlanes * Sp), dim3(kAdjBlock), 0, ctx->stream, cJ[0], kJPlanes * Sp);
+    {
+        SlabArgs A = slab_args(ctx, zc, half);
+        A.Eb = cE;
+        hipLaunchKernelGGL(k_slab_vjp, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, cU[0]);
+    }
+    int cb = 0;
+    // backward sweep, one segment at a time
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int n0 = sg * ckpt, n1 = (n0 + ckpt < nsteps) ? n0 + ckpt : nsteps;
+        if ((rc = load_state(ctx, ck[sg]))) return rc;
+        if ((rc = save_state(ctx, seg[0]))) return rc;
+        for (int n = n0; n < n1; n++) {
+            launch_step(ctx, true);
+            GL_HIP(ctx, hipMemcpyAsync(Uh[n - n0], ctx->UOld, kUPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+            if ((rc = save_state(ctx, seg[n - n0 + 1]))) return rc;
+        }
+        for (int n = n1 - 1; n >= n0; n--) {
+            const int i = n - n0;
+            StepAdjArgs A = adj_args(ctx);
+            A.U = seg[i].U;
+            A.Uh = Uh[i];
+            A.E1 = seg[i + 1].E;
+            A.J = seg[i].J;
+            A.J1 = seg[i + 1].J;
+            A.Ub1 = cU[cb];
+            A.Ub = cU[cb ^ 1];
+            A.Eb = cE;
+            A.Jb1 = cJ[cb];
+            A.Jb = cJ[cb ^ 1];
+            if ((rc = run_vjp(ctx, A))) return rc;
+            cb ^= 1;
+        }
+    }
+    // the initial state again (its fields are the linearisation point of the initial-condition map)
+    if ((rc = ic_left_from_charge(ctx, mv, rho_d, tape))) return rc;
+    if ((rc = ic_left_vjp(ctx, mv, tape, cU[cb], cE, cJ[cb], rho_bar, work))) return rc;
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    GL_HIP(ctx, hipMemcpy(h.data(), rho_bar, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    rho_from_dev_layout(ctx, h, grad_rho);
+    return 0;
+}
+
+}  // extern "C"

@@ -3,7 +3,7 @@
 // device library is the only execution path of the engine.  Field arrays use the device's tiled layout.
 #include <cstring>
 
-#include "site.h"
+#include "adjoint.h"
 
 using namespace glasma;
 
@@ -33,6 +33,63 @@ void hc_step(const int* n, const double* a, int bc, double dt, const double* U, 
     A.g.S = A.g.P * n[2];
     fill_step_coefficients(A, a, dt);
     for (long s = 0; s < A.g.S; s++) step_site<unsigned long>(A, (unsigned long)s);
+}
+
+
+// one backward step: passes A, B, C over all sites (C accumulates serially)
+void hc_step_vjp(const int* n, const double* a, int bc, double dt, const double* U, const double* Uh, const double* E1,
+                 const double* J, const double* J1, const double* Ub1, double* Ub, double* Eb, const double* Jb1,
+                 double* Jb) {
+    StepArgs F;
+    fill_step_coefficients(F, a, dt);
+    StepAdjArgs A;
+    A.U = U; A.Uh = Uh; A.E1 = E1; A.J = J; A.J1 = J1; A.Ub1 = Ub1; A.Ub = Ub; A.Eb = Eb; A.Jb1 = Jb1; A.Jb = Jb;
+    A.g.nx = n[0]; A.g.ny = n[1]; A.g.nz = n[2]; A.g.bc = bc;
+    A.g.P = (long)n[0] * n[1];
+    A.g.S = A.g.P * n[2];
+    for (int m = 0; m < 3; m++) { A.gc[m] = F.gc[m]; A.cE[m] = F.cE[m]; A.cU[m] = F.cU[m]; }
+    A.dtz = F.dtz;
+    for (long s = 0; s < A.g.S; s++) step_vjp_local<unsigned long>(A, (unsigned long)s);
+    for (long s = 0; s < A.g.S; s++) step_vjp_down<unsigned long>(A, (unsigned long)s);
+    auto add = [&](int mu, unsigned long site, const M3& M) {
+        for (int k = 0; k < 9; k++) {
+            Ub[lidx<unsigned long>(kUPlanes, 18 * mu + 2 * k, site)] += M.re[k];
+            Ub[lidx<unsigned long>(kUPlanes, 18 * mu + 2 * k + 1, site)] += M.im[k];
+        }
+    };
+    for (long s = 0; s < A.g.S; s++) step_vjp_staples<unsigned long>(A, (unsigned long)s, add);
+}
+
+static SlabArgs mk_slab(const int* n, const double* a, int bc, const double* U, const double* E, double* Eb, int zc,
+                        int half) {
+    SlabArgs A;
+    A.U = U; A.E = E; A.Eb = Eb;
+    A.g.nx = n[0]; A.g.ny = n[1]; A.g.nz = n[2]; A.g.bc = bc;
+    A.g.P = (long)n[0] * n[1];
+    A.g.S = A.g.P * n[2];
+    const double acube = a[0] * a[1] * a[2];
+    for (int m = 0; m < 3; m++) A.c[m] = (a[m] / acube) * (a[m] / acube);
+    A.zc = zc; A.half = half;
+    return A;
+}
+
+double hc_slab_energy(const int* n, const double* a, int bc, const double* U, const double* E, int zc, int half) {
+    SlabArgs A = mk_slab(n, a, bc, U, E, nullptr, zc, half);
+    double v = 0.0;
+    for (long s = 0; s < A.g.S; s++) v += slab_energy_site<unsigned long>(A, (unsigned long)s);
+    return v;
+}
+
+void hc_slab_energy_vjp(const int* n, const double* a, int bc, const double* U, const double* E, int zc, int half,
+                        double* Ub, double* Eb) {
+    SlabArgs A = mk_slab(n, a, bc, U, E, Eb, zc, half);
+    auto add = [&](int mu, unsigned long site, const M3& M) {
+        for (int k = 0; k < 9; k++) {
+            Ub[lidx<unsigned long>(kUPlanes, 18 * mu + 2 * k, site)] += M.re[k];
+            Ub[lidx<unsigned long>(kUPlanes, 18 * mu + 2 * k + 1, site)] += M.im[k];
+        }
+    };
+    for (long s = 0; s < A.g.S; s++) slab_energy_vjp_site<unsigned long>(A, (unsigned long)s, 1.0, add);
 }
 
 }

@@ -84,6 +84,18 @@ int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_ev
                       long nblocks);
 TileMap default_tile_map(const Geom& g);
 
+// the left-moving MV sheet from a given charge density, and its reverse-mode derivative (mv.hip)
+struct IcTape {
+    double* rho_s;  // 8*S
+    double* A0;     // 8*S
+    double* A1;     // 8*S
+    double* V0;     // 18 planes, tiled
+    double* V1;     // 18 planes, tiled
+};
+int ic_left_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, IcTape& tape);
+int ic_left_vjp(glasma_ctx* ctx, const glasma_mv_params* mv, const IcTape& tape, double* Ub, double* Eb,
+                const double* Jb, double* rho_bar, double* work);
+
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign);
 // complex array c[(comp*nz + z)*P + y*nx + x] of `ncomp` components: 1D along z, 2D in (x,y), 3D.
 int fft_z(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "adjoint.h"
 
 namespace glasma {
 
@@ -229,7 +230,8 @@ __global__ void k_sheet_jstat(const double* __restrict__ U, const double* __rest
 
 // continuity inversion in z momentum space (:131-209):
 //   J~(0) = J~stat(0); J~(N/2) = 0; otherwise J~ = J~stat (e^{-+ i omega dt} - 1) / (dt pZD)
-__global__ void k_jinv(double* __restrict__ F, Geom g, double az, double dt, int right) {
+// (conj = 1: the complex-conjugate filter, i.e. the transpose of the real-linear map)
+__global__ void k_jinv(double* __restrict__ F, Geom g, double az, double dt, int right, int conj) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.S) return;
     const int pz = (int)(s / g.P);
@@ -248,6 +250,11 @@ __global__ void k_jinv(double* __restrict__ F, Geom g, double az, double dt, int
     } else {
         zr = -zr; zi = -zi;
         pr = cm - 1.0; pi_ = sm; nr = cp - 1.0; ni = sp;
+    }
+    if (conj) {
+        zi = -zi;
+        pi_ = -pi_;
+        ni = -ni;
     }
     const double dr = dt * zr, di = dt * zi;  // dt * pZD
     const double den = dr * dr + di * di;
@@ -378,16 +385,331 @@ int single_sheet(glasma_ctx* ctx, Wk& w, bool right, int zlo, int zhi, int wstat
     // J-stat from the transverse Gauss law and J from the z continuity inversion (:98-209)
     hipLaunchKernelGGL(k_sheet_jstat, grid1(g.S), dim3(256), 0, ctx->stream, U, ctx->E, J, w.F, g, zlo, zhi, wstat);
     GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, -1));
-    hipLaunchKernelGGL(k_jinv, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[2], dt, right ? 1 : 0);
+    hipLaunchKernelGGL(k_jinv, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[2], dt, right ? 1 : 0, 0);
     GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, +1));
     hipLaunchKernelGGL(k_store_j, grid1(g.S), dim3(256), 0, ctx->stream, w.F, J, g, wj);
     GL_HIP(ctx, hipGetLastError());
     return 0;
 }
 
+// ------------------------------------------------------------------------------------------------
+// The left-moving sheet from a given charge density, with its intermediates kept, and the reverse-mode
+// derivative of that map (cotangents of U, E, J+, J+stat at t = 0 -> cotangent of the charge density).
+// Forward: rho_s = smooth(rho); A0 = poisson(rho_s); V0 = scan(A0); UOld = links(V0);
+// rho_ev = evolve_L(rho_s); A1 = poisson(rho_ev); V1 = scan(A1); U = links(V1); E = ilog(U UOld^dag);
+// J+stat = sqrt2 sum_i (E_i - Ad(U_i^dag(x-i)) E_i(x-i)); J+ = jinv_L(J+stat).
+// The smoothing and Poisson filters are real and even in p (self-adjoint maps); the z-evolution and the
+// current inversion transpose to their complex-conjugate filters (evolve_L^T = evolve_R).
+// ------------------------------------------------------------------------------------------------
+// F = (values of tiled planes p0..p0+7 of a field with NP planes, 0)
+__global__ void k_load_planes(double* __restrict__ F, const double* __restrict__ X, Geom g, int NP, int p0) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        F[2 * i] = X[lidx<long>(NP, p0 + a, s)];
+        F[2 * i + 1] = 0.0;
+    }
+}
+// out[a*S + s] = x[a*S + s] + c * Re F
+__global__ void k_axpy_real(double* __restrict__ out, const double* __restrict__ x, const double* __restrict__ F,
+                            double c, Geom g) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 8 * g.S) return;
+    out[i] = (x ? x[i] : 0.0) + c * F[2 * i];
+}
+__global__ void k_copy_real(double* __restrict__ out, const double* __restrict__ F, Geom g) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 8 * g.S) return;
+    out[i] = F[2 * i];
+}
+__global__ void k_zero(double* __restrict__ p, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// J+stat cotangent jsb (real, a*S + s) -> E_x, E_y and U_x, U_y cotangents (gathered at y from y + i)
+__global__ void k_ic_jstat_vjp(const double* __restrict__ U, const double* __restrict__ E,
+                               const double* __restrict__ jsb, double* __restrict__ Ub, double* __restrict__ Eb,
+                               Geom g) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    double own[8];
+    for (int a = 0; a < 8; a++) own[a] = jsb[(long)a * g.S + s];
+    for (int mu = 0; mu < 2; mu++) {
+        const long sn = mu == 0 ? sidx(g, x + 1, y, z) : sidx(g, x, y + 1, z);
+        double up[8], e[8], eb[8], t8[8];
+        for (int a = 0; a < 8; a++) up[a] = jsb[(long)a * g.S + sn];
+        M3 Um, Ubm;
+        ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+        ldm<long>(Ub, kUPlanes, 18 * mu, s, Ubm);
+        ldv<long>(E, kEPlanes, 8 * mu, s, e);
+        ldv<long>(Eb, kEPlanes, 8 * mu, s, eb);
+        adj_mul(Um, up, t8);
+        for (int a = 0; a < 8; a++) eb[a] += kSqrt2 * own[a] - kSqrt2 * t8[a];
+        acc_lvr(-kSqrt2, e, Um, up, Ubm);
+        stv<long>(Eb, kEPlanes, 8 * mu, s, eb);
+        stm<long>(Ub, kUPlanes, 18 * mu, s, Ubm);
+    }
+}
+
+// 8x8 linear solve by Gaussian elimination with partial pivoting (A is overwritten); returns x in b
+__device__ void solve8(double (&A)[8][8], double* b) {
+    for (int c = 0; c < 8; c++) {
+        int p = c;
+        for (int r = c + 1; r < 8; r++)
+            if (fabs(A[r][c]) > fabs(A[p][c])) p = r;
+        if (p != c) {
+            for (int k = 0; k < 8; k++) { const double t = A[c][k]; A[c][k] = A[p][k]; A[p][k] = t; }
+            const double t = b[c]; b[c] = b[p]; b[p] = t;
+        }
+        const double inv = 1.0 / A[c][c];
+        for (int r = c + 1; r < 8; r++) {
+            const double f = A[r][c] * inv;
+            for (int k = c; k < 8; k++) A[r][k] -= f * A[c][k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int c = 7; c >= 0; c--) {
+        double v = b[c];
+        for (int k = c + 1; k < 8; k++) v -= A[c][k] * b[k];
+        b[c] = v / A[c][c];
+    }
+}
+
+// E_mu = ilog(coff, M), M = U_mu UOld_mu^dag (mu = x, y): with A = i coff E t and the left-trivialised
+// derivative G = sum_k (-ad_A)^k/(k+1)!, M^dag dM = G(i coff t dE); in components kappa = -2 Re tr(i t M^dag dM)
+// = Ghat coff dE, so kappabar = Ghat^-T Ebar / coff and Mbar = 2 i M H(kappabar).  Ubar += Mbar UOld,
+// UOldbar = Mbar^dag U.
+__global__ void k_ic_ilog_vjp(const double* __restrict__ U, const double* __restrict__ UO,
+                              const double* __restrict__ E, const double* __restrict__ Eb, double* __restrict__ Ub,
+                              double* __restrict__ UOb, Geom g, double coff0, double coff1) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    for (int mu = 0; mu < 2; mu++) {
+        const double coff = mu == 0 ? coff0 : coff1;
+        M3 Um, Uo, M, Hm, A;
+        ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+        ldm<long>(UO, kUPlanes, 18 * mu, s, Uo);
+        mul_nd(Um, Uo, M);
+        double e[8], eb[8];
+        ldv<long>(E, kEPlanes, 8 * mu, s, e);
+        ldv<long>(Eb, kEPlanes, 8 * mu, s, eb);
+        herm_from_alg(coff, e, Hm);
+        for (int k = 0; k < 9; k++) { A.re[k] = -Hm.im[k]; A.im[k] = Hm.re[k]; }
+        double G[8][8];
+        for (int b = 0; b < 8; b++) {
+            double unit[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            unit[b] = 1.0;
+            M3 Y, Z, T1, T2, Hb;
+            herm_from_alg(1.0, unit, Hb);
+            for (int k = 0; k < 9; k++) { Y.re[k] = -Hb.im[k]; Y.im[k] = Hb.re[k]; }  // i t^b
+            Z = Y;
+            double fact = 1.0;
+            for (int kk = 1; kk <= 40; kk++) {
+                mul_nn(A, Y, T1);
+                mul_nn(Y, A, T2);
+                double mx = 0.0;
+                for (int q = 0; q < 9; q++) {
+                    Y.re[q] = T2.re[q] - T1.re[q];  // -ad_A
+                    Y.im[q] = T2.im[q] - T1.im[q];
+                    mx = fmax(mx, fmax(fabs(Y.re[q]), fabs(Y.im[q])));
+                }
+                fact /= (double)(kk + 1);
+                m3_axpy(fact, Y, Z);
+                if (mx * fact < 1e-19) break;
+            }
+            double col[8];
+            retr_igen(Z, col);
+            for (int a = 0; a < 8; a++) G[b][a] = -2.0 * col[a];  // transpose: row b = column b of Ghat
+        }
+        double kb[8];
+        for (int a = 0; a < 8; a++) kb[a] = eb[a] / coff;
+        solve8(G, kb);
+        M3 Hk, Mb, T;
+        herm_from_alg(1.0, kb, Hk);
+        mul_nn(M, Hk, T);
+        for (int k = 0; k < 9; k++) { Mb.re[k] = -2.0 * T.im[k]; Mb.im[k] = 2.0 * T.re[k]; }  // 2 i M H
+        M3 Ubm, Ta;
+        ldm<long>(Ub, kUPlanes, 18 * mu, s, Ubm);
+        mul_nn(Mb, Uo, Ta);
+        m3_axpy(1.0, Ta, Ubm);
+        stm<long>(Ub, kUPlanes, 18 * mu, s, Ubm);
+        mul_dn(Mb, Um, Ta);
+        stm<long>(UOb, kUPlanes, 18 * mu, s, Ta);
+    }
+}
+
+// V cotangent from the link cotangents of U_mu = V^dag(x) V(x + mu), mu = x, y (gathered at y)
+__global__ void k_links_vjp(const double* __restrict__ V, const double* __restrict__ Ub, double* __restrict__ Vb,
+                            Geom g) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    M3 acc, Vn, Bm, T;
+    m3_zero(acc);
+    for (int mu = 0; mu < 2; mu++) {
+        const long sp = mu == 0 ? sidx(g, x + 1, y, z) : sidx(g, x, y + 1, z);
+        const long sm = mu == 0 ? sidx(g, x - 1, y, z) : sidx(g, x, y - 1, z);
+        ldm<long>(V, kGPlanes, 0, sp, Vn);
+        ldm<long>(Ub, kUPlanes, 18 * mu, s, Bm);
+        mul_nd(Vn, Bm, T);  // V(y+mu) Ubar_mu(y)^dag
+        m3_axpy(1.0, T, acc);
+        ldm<long>(V, kGPlanes, 0, sm, Vn);
+        ldm<long>(Ub, kUPlanes, 18 * mu, sm, Bm);
+        mul_nn(Vn, Bm, T);  // V(y-mu) Ubar_mu(y-mu)
+        m3_axpy(1.0, T, acc);
+    }
+    stm<long>(Vb, kGPlanes, 0, s, acc);
+}
+
+// reverse of the left-moving Wilson-line scan V(z) = X(z) V(z+1), X(z) = exp(-i A(z) t / sqrt2):
+// Abar (real, a*S + s) from Vbar, one thread per transverse site walking z upwards
+__global__ void k_scan_vjp_left(const double* __restrict__ A, const double* __restrict__ V,
+                                const double* __restrict__ Vb, double* __restrict__ Ab, Geom g) {
+    long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s2 >= g.P) return;
+    const double c = -1.0 / kSqrt2;
+    M3 carry, Wb, Wn, X, Xb, T;
+    m3_zero(carry);
+    for (int z = 0; z < g.nz; z++) {
+        const long s = s2 + g.P * z;
+        ldm<long>(Vb, kGPlanes, 0, s, Wb);
+        m3_axpy(1.0, carry, Wb);
+        if (z + 1 < g.nz) ldm<long>(V, kGPlanes, 0, s + g.P, Wn);
+        else set_unit(Wn);
+        double a[8], ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 8; q++) a[q] = A[(long)q * g.S + s];
+        expi(c, a, X);
+        mul_nd(Wb, Wn, Xb);  // Xbar = Wbar W(z+1)^dag
+        dexp_adjoint(c, a, X, Xb, ab);
+        for (int q = 0; q < 8; q++) Ab[(long)q * g.S + s] = ab[q];
+        mul_dn(X, Wb, T);  // carried into W(z+1)
+        carry = T;
+    }
+}
+
 bool mv_valid(const glasma_mv_params* mv) {
     return mv && mv->sigma_l > 0.0 && mv->sigma_r > 0.0 && mv->lambda > 0.0 && mv->m_eff >= 0.0;
 }
+
+}  // namespace
+
+// Left-moving sheet over all z from a given charge density rho (device, real, a*S + s; the density
+// SetChargeDensities would produce), keeping the intermediates in tape (device arrays of 8*S doubles
+// for rho_s, A0, A1 and tiled 18-plane Wilson lines V0, V1).  Resets the fields and the time first.
+int ic_left_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, IcTape& tape) {
+    const Geom& g = ctx->g;
+    int rc = reset_fields(ctx);
+    if (rc) return rc;
+    rc = ensure_fftbuf(ctx, 16 * (size_t)g.S);
+    if (rc) return rc;
+    double* F = ctx->fftbuf;
+    hipStream_t st = ctx->stream;
+    double* U = ctx->U[ctx->cur];
+    double* J = ctx->J[ctx->cur];
+    const double dt = ctx->p.dt;
+    hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, st, F, rho, g);
+    GL_FFT(ctx, fft_xy(st, F, g, 8, g.nz, -1));
+    hipLaunchKernelGGL(k_transverse_filter, grid1(g.S), dim3(256), 0, st, F, g, ctx->p.a[0], ctx->p.a[1], mv->lambda,
+                       mv->m_eff, 0);
+    GL_FFT(ctx, fft_xy(st, F, g, 8, g.nz, +1));
+    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, st, F, tape.rho_s, g);
+    GL_RC(poisson_all(ctx, F));
+    hipLaunchKernelGGL(k_copy_real, grid1(8 * g.S), dim3(256), 0, st, tape.A0, F, g);
+    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, st, F, tape.V0, g, 0);
+    hipLaunchKernelGGL(k_sheet_links, grid1(g.S), dim3(256), 0, st, tape.V0, ctx->UOld, g, 0, g.nz - 1);
+    hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, st, F, tape.rho_s, g);
+    GL_FFT(ctx, fft_z(st, F, g, 8, -1));
+    hipLaunchKernelGGL(k_evolve_phase, grid1(g.S), dim3(256), 0, st, F, g, ctx->p.a[2], dt, 0);
+    GL_FFT(ctx, fft_z(st, F, g, 8, +1));
+    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, st, F, (double*)nullptr, g);
+    GL_RC(poisson_all(ctx, F));
+    hipLaunchKernelGGL(k_copy_real, grid1(8 * g.S), dim3(256), 0, st, tape.A1, F, g);
+    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, st, F, tape.V1, g, 0);
+    hipLaunchKernelGGL(k_sheet_links, grid1(g.S), dim3(256), 0, st, tape.V1, U, g, 0, g.nz - 1);
+    const double acube = ctx->p.a[0] * ctx->p.a[1] * ctx->p.a[2];
+    const double coff0 = -dt * ctx->p.a[0] * ctx->p.a[0] / acube;
+    const double coff1 = -dt * ctx->p.a[1] * ctx->p.a[1] / acube;
+    hipLaunchKernelGGL(k_sheet_efield, grid1(g.S), dim3(256), 0, st, U, ctx->UOld, ctx->E, g, 0, g.nz - 1, coff0, coff1);
+    hipLaunchKernelGGL(k_sheet_jstat, grid1(g.S), dim3(256), 0, st, U, ctx->E, J, F, g, 0, g.nz - 1, JPS);
+    GL_FFT(ctx, fft_z(st, F, g, 8, -1));
+    hipLaunchKernelGGL(k_jinv, grid1(g.S), dim3(256), 0, st, F, g, ctx->p.a[2], dt, 0, 0);
+    GL_FFT(ctx, fft_z(st, F, g, 8, +1));
+    hipLaunchKernelGGL(k_store_j, grid1(g.S), dim3(256), 0, st, F, J, g, JP);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+// Reverse-mode derivative of ic_left_from_charge.  In: cotangents of the sheet's U (Ub, tiled 54 planes;
+// modified), E (Eb, tiled 24; modified) and J (Jb, tiled 32: J+ and J+stat used), the fields and tape of
+// the same call.  Out: rho_bar (device, real, a*S + s).  work: 54*Sp + 2*18*Sp + 4*8*S doubles.
+int ic_left_vjp(glasma_ctx* ctx, const glasma_mv_params* mv, const IcTape& tape, double* Ub, double* Eb,
+                const double* Jb, double* rho_bar, double* work) {
+    const Geom& g = ctx->g;
+    const long Sp = pad_sites(g.S);
+    hipStream_t st = ctx->stream;
+    int rc = ensure_fftbuf(ctx, 16 * (size_t)g.S);
+    if (rc) return rc;
+    double* F = ctx->fftbuf;
+    double* UOb = work;
+    double* V0b = UOb + 54 * Sp;
+    double* V1b = V0b + 18 * Sp;
+    double* jsb = V1b + 18 * Sp;
+    double* A0b = jsb + 8 * g.S;
+    double* A1b = A0b + 8 * g.S;
+    double* tmp = A1b + 8 * g.S;
+    const double* U = ctx->U[ctx->cur];
+    const double dt = ctx->p.dt;
+    // J+ = jinv(J+stat): J+stat cotangent += jinv^T(J+ cotangent)
+    hipLaunchKernelGGL(k_load_planes, grid1(g.S), dim3(256), 0, st, F, Jb, g, kJPlanes, 8 * JP);
+    GL_FFT(ctx, fft_z(st, F, g, 8, -1));
+    hipLaunchKernelGGL(k_jinv, grid1(g.S), dim3(256), 0, st, F, g, ctx->p.a[2], dt, 0, 1);
+    GL_FFT(ctx, fft_z(st, F, g, 8, +1));
+    hipLaunchKernelGGL(k_copy_real, grid1(8 * g.S), dim3(256), 0, st, tmp, F, g);
+    hipLaunchKernelGGL(k_load_planes, grid1(g.S), dim3(256), 0, st, F, Jb, g, kJPlanes, 8 * JPS);
+    hipLaunchKernelGGL(k_axpy_real, grid1(8 * g.S), dim3(256), 0, st, jsb, tmp, F, 1.0, g);
+    // J+stat from E and U
+    hipLaunchKernelGGL(k_ic_jstat_vjp, grid1(g.S), dim3(256), 0, st, U, ctx->E, jsb, Ub, Eb, g);
+    // E = ilog(U UOld^dag)
+    const double acube = ctx->p.a[0] * ctx->p.a[1] * ctx->p.a[2];
+    const double coff0 = -dt * ctx->p.a[0] * ctx->p.a[0] / acube;
+    const double coff1 = -dt * ctx->p.a[1] * ctx->p.a[1] / acube;
+    hipLaunchKernelGGL(k_zero, grid1(54 * Sp), dim3(256), 0, st, UOb, 54 * Sp);
+    hipLaunchKernelGGL(k_ic_ilog_vjp, grid1(g.S), dim3(256), 0, st, U, ctx->UOld, ctx->E, Eb, Ub, UOb, g, coff0, coff1);
+    // links -> Wilson lines -> Poisson solutions
+    hipLaunchKernelGGL(k_links_vjp, grid1(g.S), dim3(256), 0, st, tape.V1, Ub, V1b, g);
+    hipLaunchKernelGGL(k_links_vjp, grid1(g.S), dim3(256), 0, st, tape.V0, UOb, V0b, g);
+    hipLaunchKernelGGL(k_scan_vjp_left, grid1(g.P), dim3(256), 0, st, tape.A1, tape.V1, V1b, A1b, g);
+    hipLaunchKernelGGL(k_scan_vjp_left, grid1(g.P), dim3(256), 0, st, tape.A0, tape.V0, V0b, A0b, g);
+    // rho_ev cotangent = poisson(A1bar); rho_s cotangent = evolve_R(that) + poisson(A0bar)
+    hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, st, F, A1b, g);
+    GL_RC(poisson_all(ctx, F));
+    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, st, F, (double*)nullptr, g);
+    GL_FFT(ctx, fft_z(st, F, g, 8, -1));
+    hipLaunchKernelGGL(k_evolve_phase, grid1(g.S), dim3(256), 0, st, F, g, ctx->p.a[2], dt, 1);
+    GL_FFT(ctx, fft_z(st, F, g, 8, +1));
+    hipLaunchKernelGGL(k_copy_real, grid1(8 * g.S), dim3(256), 0, st, tmp, F, g);
+    hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, st, F, A0b, g);
+    GL_RC(poisson_all(ctx, F));
+    hipLaunchKernelGGL(k_axpy_real, grid1(8 * g.S), dim3(256), 0, st, tmp, tmp, F, 1.0, g);
+    // rho cotangent = smooth(rho_s cotangent)
+    hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, st, F, tmp, g);
+    GL_FFT(ctx, fft_xy(st, F, g, 8, g.nz, -1));
+    hipLaunchKernelGGL(k_transverse_filter, grid1(g.S), dim3(256), 0, st, F, g, ctx->p.a[0], ctx->p.a[1], mv->lambda,
+                       mv->m_eff, 0);
+    GL_FFT(ctx, fft_xy(st, F, g, 8, g.nz, +1));
+    hipLaunchKernelGGL(k_copy_real, grid1(8 * g.S), dim3(256), 0, st, rho_bar, F, g);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
+namespace {
 
 }  // namespace
 

@@ -227,6 +227,49 @@ class Glasma:
                                                      _p(V), _p(rho)), "glasma_set_wilson_lines")
         return V, rho
 
+    # -- reverse-mode derivatives ---------------------------------------------------------------------
+    def slab_energy(self, zc: int, half: int) -> float:
+        """Energy of the layers |z - zc| < half (the local form of the bulk T00)."""
+        e = ctypes.c_double()
+        self._chk(_lib.lib().glasma_slab_energy(self._h, int(zc), int(half), ctypes.byref(e)), "glasma_slab_energy")
+        return e.value
+
+    def step_vjp(self, cot: dict) -> dict:
+        """Cotangents of the fields before one leapfrog step from those after it (keys U, E, Jp, Jm, JpS, JmS)."""
+        d = {k: np.ascontiguousarray(cot[k]).copy() for k in ("U", "E", "Jp", "Jm", "JpS", "JmS")}
+        self._chk(_lib.lib().glasma_step_vjp(self._h, *[_p(d[k]) for k in ("U", "E", "Jp", "Jm", "JpS", "JmS")]),
+                  "glasma_step_vjp")
+        return d
+
+    def _rho(self, rho):
+        Nx, Ny, Nz = self.N
+        r = np.zeros((Nz + 2, Ny, Nx, 8))
+        rho = np.asarray(rho, np.float64)
+        if rho.shape == (Nz, Ny, Nx, 8):
+            r[1:-1] = rho
+        else:
+            r[...] = rho
+        return r
+
+    def slab_energy_from_charge(self, mv: MVParams, rho, nsteps: int, zc: int, half: int) -> float:
+        """Left-moving MV sheet from the charge density rho (Nz, Ny, Nx, 8), nsteps steps, slab energy."""
+        e = ctypes.c_double()
+        r = self._rho(rho)
+        self._chk(_lib.lib().glasma_slab_energy_from_charge(self._h, ctypes.byref(mv), _p(r), int(nsteps), int(zc),
+                                                            int(half), ctypes.byref(e)),
+                  "glasma_slab_energy_from_charge")
+        return e.value
+
+    def slab_energy_gradient(self, mv: MVParams, rho, nsteps: int, ckpt: int, zc: int, half: int):
+        """(energy, d energy / d rho) with rho and the gradient shaped (Nz, Ny, Nx, 8)."""
+        e = ctypes.c_double()
+        r = self._rho(rho)
+        gr = np.zeros_like(r)
+        self._chk(_lib.lib().glasma_slab_energy_gradient(self._h, ctypes.byref(mv), _p(r), int(nsteps), int(ckpt),
+                                                         int(zc), int(half), ctypes.byref(e), _p(gr)),
+                  "glasma_slab_energy_gradient")
+        return e.value, gr[1:-1].copy()
+
     # -- gauge fixing -------------------------------------------------------------------------------
     def coulomb_gauge(self, tol: float = 1e-15, maxit: int = 1000, alpha: float = 2.0) -> np.ndarray:
         stats = np.zeros(7)

@@ -1,0 +1,85 @@
+"""The per-site reverse-mode code of the engine (adjoint.h, built for the host) against central finite
+differences of the per-site step code (site.h, host build), on small lattices.  CPU only.
+
+One leapfrog step L(state) = <w, step(state)> with random cotangent w: <VJP(w), v> equals
+(L(s + eps v) - L(s - eps v)) / (2 eps) to 1e-8 relative (eps = 1e-6), for open and periodic z, Nz = 1 and
+Nz = 2 (where the boundary rules of J-stat'(x+z) and (x-z) coincide).  Same for the slab energy.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from soerenpragya3dglasma_amd import layout as LY
+from tests.helpers import hostcheck_lib, random_fields
+
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _lib():
+    hc = hostcheck_lib()
+    hc.hc_step_vjp.argtypes = [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_int, ctypes.c_double] + [dp] * 10
+    hc.hc_slab_energy.argtypes = [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int]
+    hc.hc_slab_energy.restype = ctypes.c_double
+    hc.hc_slab_energy_vjp.argtypes = [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_int, dp, dp, ctypes.c_int,
+                                      ctypes.c_int, dp, dp]
+    return hc
+
+
+def p(x):
+    return x.ctypes.data_as(dp)
+
+
+@pytest.mark.parametrize("N,bc", [((4, 3, 5), 0), ((4, 4, 4), 1), ((3, 3, 1), 0), ((4, 2, 2), 0), ((3, 2, 2), 1)])
+def test_step_vjp_against_finite_differences(N, bc):
+    hc = _lib()
+    a, dt = (1.0, 0.9, 0.7), 0.1
+    f = random_fields(N, seed=2, bc=bc)
+    S = N[0] * N[1] * N[2]
+    U = LY.tile(LY.u_aos_to_soa(f.U))
+    E = LY.tile(LY.v_aos_to_soa(f.E, 24))
+    J = LY.tile(np.concatenate([LY.v_aos_to_soa(x, 8) for x in (f.Jp, f.Jm, f.JpS, f.JmS)]))
+    n = (ctypes.c_int * 3)(*N)
+    av = (ctypes.c_double * 3)(*a)
+
+    def step(U, E, J):
+        E = E.copy()
+        Un, Jn, Uh = np.zeros_like(U), np.zeros_like(J), np.zeros_like(U)
+        hc.hc_step(n, av, bc, dt, p(U), p(Un), p(E), p(J), p(Jn), p(Uh))
+        return Un, E, Jn, Uh
+
+    rng = np.random.default_rng(7)
+    mask = {NP: LY.tile(np.ones((NP, S))) for NP in (54, 24, 32)}
+    w = [rng.normal(size=U.shape) * mask[54], rng.normal(size=E.shape) * mask[24], rng.normal(size=J.shape) * mask[32]]
+    L = lambda st: sum(float((wi * si).sum()) for wi, si in zip(w, st[:3]))  # noqa: E731
+    Un, E1, Jn, Uh = step(U, E, J)
+    Ub, Eb, Jb = np.zeros_like(U), w[1].copy(), np.zeros_like(J)
+    hc.hc_step_vjp(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(w[0]), p(Ub), p(Eb), p(w[2]), p(Jb))
+    for _ in range(3):
+        v = [rng.normal(size=U.shape) * mask[54], rng.normal(size=E.shape) * mask[24], rng.normal(size=J.shape) * mask[32]]
+        eps = 1e-6
+        fd = (L(step(U + eps * v[0], E + eps * v[1], J + eps * v[2])) -
+              L(step(U - eps * v[0], E - eps * v[1], J - eps * v[2]))) / (2 * eps)
+        ad = float((Ub * v[0]).sum() + (Eb * v[1]).sum() + (Jb * v[2]).sum())
+        assert abs(fd - ad) < 1e-8 * abs(fd), (fd, ad)
+
+
+def test_slab_energy_vjp_against_finite_differences():
+    hc = _lib()
+    N, bc, a = (4, 3, 6), 0, (1.0, 0.9, 0.7)
+    f = random_fields(N, seed=3)
+    S = N[0] * N[1] * N[2]
+    U = LY.tile(LY.u_aos_to_soa(f.U))
+    E = LY.tile(LY.v_aos_to_soa(f.E, 24))
+    n = (ctypes.c_int * 3)(*N)
+    av = (ctypes.c_double * 3)(*a)
+    Ub, Eb = np.zeros_like(U), np.zeros_like(E)
+    hc.hc_slab_energy_vjp(n, av, bc, p(U), p(E), 3, 2, p(Ub), p(Eb))
+    rng = np.random.default_rng(1)
+    vU = rng.normal(size=U.shape) * LY.tile(np.ones((54, S)))
+    vE = rng.normal(size=E.shape) * LY.tile(np.ones((24, S)))
+    eps = 1e-6
+    fd = (hc.hc_slab_energy(n, av, bc, p(U + eps * vU), p(E + eps * vE), 3, 2) -
+          hc.hc_slab_energy(n, av, bc, p(U - eps * vU), p(E - eps * vE), 3, 2)) / (2 * eps)
+    ad = float((Ub * vU).sum() + (Eb * vE).sum())
+    assert abs(fd - ad) < 1e-8 * abs(fd)
