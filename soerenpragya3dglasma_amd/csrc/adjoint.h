@@ -10,7 +10,9 @@
 // J+-' = J+- -+ dtz (Ad(U_h,z) J+-stat'(x+z) - J+-stat'), U' = X U_h.  The adjoint runs in three passes:
 //   A (site-local): cotangents of X, U_h, U (direct part), E (through the exponential), J+-stat', J+-;
 //   B (gather from x+z): the J+-(x-z), U_z(x-z) terms of the J-stat' update;
-//   C (scatter): the staple terms of the E update into the links of neighbouring sites.
+//   C (staples): the staple terms of the E update, either scattered into the links of neighbouring sites
+//     (step_vjp_staples, with an accumulation functor) or, without atomics, as C1 (step_vjp_sigbar) +
+//     C2 (step_vjp_gather), which the device uses.
 // The exponential's adjoint: with A = i cU E' t, dexp_A[H] = e^A sum_k (-ad_A)^k H/(k+1)!, so
 // Zbar = sum_k ad_A^k (X^dag Xbar)/(k+1)! and Ebar^a += cU Re tr(i t^a Zbar^dag).
 #pragma once
@@ -309,6 +311,170 @@ GHD void step_vjp_staples(const StepAdjArgs& A, I s, ADD add) {
             mul_nn(T1, Sb, T2);
             add(nu, sm, T2);  // Rbar = Q P Sb
         }
+    }
+}
+
+// ---- pass C without atomics: C1 stores Sigbar_mu(x) = U_mu(x)^dag Mbar_mu(x) and adds the own-link term
+// Mbar Sigma^dag at x; C2 gathers, for each link (y, lam), the staple terms of every site x whose E update
+// reads it.  The sources of a target are the preimages of the forward neighbour map, so the clamped
+// neighbours of open z boundaries contribute exactly as in the scatter form.
+template <class I>
+GHD void step_vjp_sigbar(const StepAdjArgs& A, I s, double* __restrict__ Sb) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    for (int mu = 0; mu < 3; mu++) {
+        double eb[8];
+        ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+        M3 Mb, Umu, Sig, T, Ub;
+        m_from_retr_cot(A.gc[mu], eb, Mb);
+        ldm<I>(A.U, kUPlanes, 18 * mu, s, Umu);
+        mul_dn(Umu, Mb, T);
+        stm<I>(Sb, kUPlanes, 18 * mu, s, T);
+        m3_zero(Sig);
+        for (int q = 1; q <= 2; q++) {
+            const int nu = (mu + q) % 3;
+            M3 P1, P2, Tq;
+            ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, 0), P1);
+            ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, nu, 1, mu, 0), P2);
+            mul_nd(P1, P2, Tq);
+            ldm<I>(A.U, kUPlanes, 18 * nu, s, P1);
+            acc_mul_nd(A.cE[nu], Tq, P1, Sig);
+            const I sm = nbr<I>(g, x, y, z, nu, -1, mu, 0);
+            ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, -1), P1);
+            ldm<I>(A.U, kUPlanes, 18 * mu, sm, P2);
+            mul_dd(P1, P2, Tq);
+            ldm<I>(A.U, kUPlanes, 18 * nu, sm, P1);
+            acc_mul_nn(A.cE[nu], Tq, P1, Sig);
+        }
+        mul_nd(Mb, Sig, T);
+        ldm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
+        m3_axpy(1.0, T, Ub);
+        stm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
+    }
+}
+
+// source z layers of a target layer zy under a z shift dz of the forward neighbour map (up to two with
+// open boundaries: the regular one and the clamped layer itself)
+GHD int z_sources(const Geom& g, int zy, int dz, int* zs) {
+    if (dz == 0) {
+        zs[0] = zy;
+        return 1;
+    }
+    int n = 0;
+    const int c0 = dz > 0 ? (zy > 0 ? zy - 1 : g.nz - 1) : (zy + 1 < g.nz ? zy + 1 : 0);
+    const int cand[2] = {c0, zy};
+    for (int k = 0; k < 2; k++) {
+        const int c = cand[k];
+        const int t = dz > 0 ? zup(g, c) : zdn(g, c);
+        if (t == zy && (n == 0 || zs[0] != c)) zs[n++] = c;
+    }
+    return n;
+}
+
+template <class I>
+GHD void step_vjp_gather(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
+    const Geom& g = A.g;
+    int x, y, z;
+    site_coords<I>(g, s, x, y, z);
+    for (int lam = 0; lam < 3; lam++) {
+        M3 acc, Sbar, P, Q, R, T1, T2;
+        ldm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
+        for (int k = 1; k <= 2; k++) {
+            const int mu = (lam + k) % 3;  // staples of E_mu in which U_lam is a nu = lam link
+            const double c = A.cE[lam];
+            // forward R: x = y; Rbar = Sb^dag P Q^dag, P = U_lam(x+mu), Q = U_mu(x+lam)
+            ldm<I>(Sb, kUPlanes, 18 * mu, s, Sbar);
+            ldm<I>(A.U, kUPlanes, 18 * lam, nbr<I>(g, x, y, z, mu, 1, lam, 0), P);
+            ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, lam, 1, mu, 0), Q);
+            mul_dn(Sbar, P, T1);
+            mul_nd(T1, Q, T2);
+            m3_axpy(c, T2, acc);
+            int zs[2];
+            // forward P: y = x + mu; Pbar = Sb R Q, R = U_lam(x), Q = U_mu(x+lam)
+            {
+                const int nzs = z_sources(g, z, mu == 2 ? 1 : 0, zs);
+                const int xs = mu == 0 ? x - 1 : x, ys = mu == 1 ? y - 1 : y;
+                for (int iz = 0; iz < nzs; iz++) {
+                    const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
+                    const I sx = (I)sidx(g, xx, yy, zz);
+                    ldm<I>(Sb, kUPlanes, 18 * mu, sx, Sbar);
+                    ldm<I>(A.U, kUPlanes, 18 * lam, sx, R);
+                    ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, xx, yy, zz, lam, 1, mu, 0), Q);
+                    mul_nn(Sbar, R, T1);
+                    mul_nn(T1, Q, T2);
+                    m3_axpy(c, T2, acc);
+                }
+            }
+            // backward P: y = x + mu - lam; Pbar = Q^dag R Sb^dag, Q = U_mu(x-lam), R = U_lam(x-lam)
+            {
+                const int dz = (mu == 2 ? 1 : 0) + (lam == 2 ? -1 : 0);
+                const int nzs = z_sources(g, z, dz, zs);
+                const int xs = x - (mu == 0 ? 1 : 0) + (lam == 0 ? 1 : 0), ys = y - (mu == 1 ? 1 : 0) + (lam == 1 ? 1 : 0);
+                for (int iz = 0; iz < nzs; iz++) {
+                    const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
+                    const I sx = (I)sidx(g, xx, yy, zz);
+                    const I sm = nbr<I>(g, xx, yy, zz, lam, -1, mu, 0);
+                    ldm<I>(Sb, kUPlanes, 18 * mu, sx, Sbar);
+                    ldm<I>(A.U, kUPlanes, 18 * mu, sm, Q);
+                    ldm<I>(A.U, kUPlanes, 18 * lam, sm, R);
+                    mul_dn(Q, R, T1);
+                    mul_nd(T1, Sbar, T2);
+                    m3_axpy(c, T2, acc);
+                }
+            }
+            // backward R: y = x - lam; Rbar = Q P Sb, Q = U_mu(x-lam), P = U_lam(x+mu-lam)
+            {
+                const int nzs = z_sources(g, z, lam == 2 ? -1 : 0, zs);
+                const int xs = lam == 0 ? x + 1 : x, ys = lam == 1 ? y + 1 : y;
+                for (int iz = 0; iz < nzs; iz++) {
+                    const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
+                    const I sx = (I)sidx(g, xx, yy, zz);
+                    ldm<I>(Sb, kUPlanes, 18 * mu, sx, Sbar);
+                    ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, xx, yy, zz, lam, -1, mu, 0), Q);
+                    ldm<I>(A.U, kUPlanes, 18 * lam, nbr<I>(g, xx, yy, zz, mu, 1, lam, -1), P);
+                    mul_nn(Q, P, T1);
+                    mul_nn(T1, Sbar, T2);
+                    m3_axpy(c, T2, acc);
+                }
+            }
+        }
+        for (int k = 1; k <= 2; k++) {
+            const int nu = (lam + k) % 3;  // staples of E_lam in which U_lam(x +- nu) is the Q link
+            const double c = A.cE[nu];
+            int zs[2];
+            // forward Q: y = x + nu; Qbar = R^dag Sb^dag P, R = U_nu(x), P = U_nu(x+lam)
+            {
+                const int nzs = z_sources(g, z, nu == 2 ? 1 : 0, zs);
+                const int xs = nu == 0 ? x - 1 : x, ys = nu == 1 ? y - 1 : y;
+                for (int iz = 0; iz < nzs; iz++) {
+                    const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
+                    const I sx = (I)sidx(g, xx, yy, zz);
+                    ldm<I>(Sb, kUPlanes, 18 * lam, sx, Sbar);
+                    ldm<I>(A.U, kUPlanes, 18 * nu, sx, R);
+                    ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, lam, 1, nu, 0), P);
+                    mul_dd(R, Sbar, T1);
+                    mul_nn(T1, P, T2);
+                    m3_axpy(c, T2, acc);
+                }
+            }
+            // backward Q: y = x - nu; Qbar = R Sb^dag P^dag, R = U_nu(x-nu), P = U_nu(x+lam-nu)
+            {
+                const int nzs = z_sources(g, z, nu == 2 ? -1 : 0, zs);
+                const int xs = nu == 0 ? x + 1 : x, ys = nu == 1 ? y + 1 : y;
+                for (int iz = 0; iz < nzs; iz++) {
+                    const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
+                    const I sx = (I)sidx(g, xx, yy, zz);
+                    ldm<I>(Sb, kUPlanes, 18 * lam, sx, Sbar);
+                    ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, nu, -1, lam, 0), R);
+                    ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, lam, 1, nu, -1), P);
+                    mul_nd(R, Sbar, T1);
+                    mul_nd(T1, P, T2);
+                    m3_axpy(c, T2, acc);
+                }
+            }
+        }
+        stm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
     }
 }
 

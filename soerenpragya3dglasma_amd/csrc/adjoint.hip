@@ -1,5 +1,5 @@
 // Reverse-mode derivatives on the device (no counterpart in the reference, which has no gradient code):
-// the leapfrog step (three passes of adjoint.h per step), the slab energy, and the whole map
+// the leapfrog step (passes A, B, C1, C2 of adjoint.h per step, no atomics), the slab energy, and the whole map
 // charge density -> left-moving MV sheet -> nsteps leapfrog steps -> slab energy, with the forward run
 // checkpointed every ckpt steps and each segment recomputed once during the backward sweep.
 #include <cstring>
@@ -33,9 +33,13 @@ __global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s < A.g.S) step_vjp_down<unsigned>(A, (unsigned)s);
 }
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_staples(StepAdjArgs A) {
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_sigbar(StepAdjArgs A, double* Sb) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < A.g.S) step_vjp_staples<unsigned>(A, (unsigned)s, AtomicAddU{A.Ub});
+    if (s < A.g.S) step_vjp_sigbar<unsigned>(A, (unsigned)s, Sb);
+}
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < A.g.S) step_vjp_gather<unsigned>(A, Sb, (unsigned)s);
 }
 
 // slab energy: per-block partial sums, then one block sums them in a fixed order
@@ -113,11 +117,13 @@ StepAdjArgs adj_args(const glasma_ctx* ctx) {
     return A;
 }
 
-int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A) {
+// passes A, B, C1, C2 of one backward step; Sb: 54-plane scratch for the staple cotangents
+int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A, double* Sb) {
     const long S = ctx->g.S;
     hipLaunchKernelGGL(k_vjp_local, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
     hipLaunchKernelGGL(k_vjp_down, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_vjp_staples, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_vjp_sigbar, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
+    hipLaunchKernelGGL(k_vjp_gather, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
     GL_HIP(ctx, hipGetLastError());
     return 0;
 }
@@ -199,6 +205,7 @@ int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double
     double* cE = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
     double* cJ1 = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
     double* cJ0 = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    double* Sb = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
     if (rc) return rc;
     GL_HIP(ctx, hipMemsetAsync(cU1, 0, kUPlanes * Sp * sizeof(double), ctx->stream));
     GL_HIP(ctx, hipMemsetAsync(cE, 0, kEPlanes * Sp * sizeof(double), ctx->stream));
@@ -224,7 +231,7 @@ int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double
     A.Eb = cE;
     A.Jb1 = cJ1;
     A.Jb = cJ0;
-    if ((rc = run_vjp(ctx, A))) return rc;
+    if ((rc = run_vjp(ctx, A, Sb))) return rc;
     // back to the state at t
     ctx->cur ^= 1;
     if ((rc = load_state(ctx, s0))) return rc;
@@ -303,6 +310,7 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     double* cE = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
     double* rho_bar = rc ? nullptr : b.get(ctx, 8 * S, &rc);
     double* work = rc ? nullptr : b.get(ctx, 54 * Sp + 36 * Sp + 32 * S, &rc);
+    double* Sb = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
     if (rc) return rc;
     std::vector<double> h;
     rho_to_dev_layout(ctx, rho, h);
@@ -348,7 +356,7 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
             A.Eb = cE;
             A.Jb1 = cJ[cb];
             A.Jb = cJ[cb ^ 1];
-            if ((rc = run_vjp(ctx, A))) return rc;
+            if ((rc = run_vjp(ctx, A, Sb))) return rc;
             cb ^= 1;
         }
     }

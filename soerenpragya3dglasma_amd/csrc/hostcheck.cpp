@@ -92,4 +92,24 @@ void hc_slab_energy_vjp(const int* n, const double* a, int bc, const double* U, 
     for (long s = 0; s < A.g.S; s++) slab_energy_vjp_site<unsigned long>(A, (unsigned long)s, 1.0, add);
 }
 
+
+// one backward step with the gather form of pass C (C1 + C2); Sb: 54-plane scratch
+void hc_step_vjp_gather(const int* n, const double* a, int bc, double dt, const double* U, const double* Uh,
+                        const double* E1, const double* J, const double* J1, const double* Ub1, double* Ub, double* Eb,
+                        const double* Jb1, double* Jb, double* Sb) {
+    StepArgs F;
+    fill_step_coefficients(F, a, dt);
+    StepAdjArgs A;
+    A.U = U; A.Uh = Uh; A.E1 = E1; A.J = J; A.J1 = J1; A.Ub1 = Ub1; A.Ub = Ub; A.Eb = Eb; A.Jb1 = Jb1; A.Jb = Jb;
+    A.g.nx = n[0]; A.g.ny = n[1]; A.g.nz = n[2]; A.g.bc = bc;
+    A.g.P = (long)n[0] * n[1];
+    A.g.S = A.g.P * n[2];
+    for (int m = 0; m < 3; m++) { A.gc[m] = F.gc[m]; A.cE[m] = F.cE[m]; A.cU[m] = F.cU[m]; }
+    A.dtz = F.dtz;
+    for (long s = 0; s < A.g.S; s++) step_vjp_local<unsigned long>(A, (unsigned long)s);
+    for (long s = 0; s < A.g.S; s++) step_vjp_down<unsigned long>(A, (unsigned long)s);
+    for (long s = 0; s < A.g.S; s++) step_vjp_sigbar<unsigned long>(A, (unsigned long)s, Sb);
+    for (long s = 0; s < A.g.S; s++) step_vjp_gather<unsigned long>(A, Sb, (unsigned long)s);
+}
+
 }

@@ -168,7 +168,9 @@ template <class I>
 static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hipStream_t st) {
     dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
     if (four) {
-        hipLaunchKernelGGL((k_step4<I, 2>), grid, dim3(256), 0, st, A, tm);
+        if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3>), grid, dim3(256), 0, st, A, tm);
+        else if (waves == 4) hipLaunchKernelGGL((k_step4<I, 4>), grid, dim3(256), 0, st, A, tm);
+        else hipLaunchKernelGGL((k_step4<I, 2>), grid, dim3(256), 0, st, A, tm);
         return;
     }
     if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, dim3(192), 0, st, A, tm);

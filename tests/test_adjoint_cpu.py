@@ -83,3 +83,29 @@ def test_slab_energy_vjp_against_finite_differences():
           hc.hc_slab_energy(n, av, bc, p(U - eps * vU), p(E - eps * vE), 3, 2)) / (2 * eps)
     ad = float((Ub * vU).sum() + (Eb * vE).sum())
     assert abs(fd - ad) < 1e-8 * abs(fd)
+
+
+@pytest.mark.parametrize("N,bc", [((4, 3, 5), 0), ((4, 4, 4), 1), ((3, 3, 1), 0), ((2, 2, 3), 0), ((2, 3, 1), 1)])
+def test_gather_form_equals_scatter_form(N, bc):
+    """The device's atomic-free staple adjoint (C1 + C2) equals the scatter form, boundary layers included."""
+    hc = _lib()
+    hc.hc_step_vjp_gather.argtypes = [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_int, ctypes.c_double] + [dp] * 11
+    a, dt = (1.0, 0.9, 0.7), 0.1
+    f = random_fields(N, seed=9, bc=bc)
+    S = N[0] * N[1] * N[2]
+    U = LY.tile(LY.u_aos_to_soa(f.U))
+    E = LY.tile(LY.v_aos_to_soa(f.E, 24))
+    J = LY.tile(np.concatenate([LY.v_aos_to_soa(x, 8) for x in (f.Jp, f.Jm, f.JpS, f.JmS)]))
+    n = (ctypes.c_int * 3)(*N)
+    av = (ctypes.c_double * 3)(*a)
+    E1, Un, Jn, Uh = E.copy(), np.zeros_like(U), np.zeros_like(J), np.zeros_like(U)
+    hc.hc_step(n, av, bc, dt, p(U), p(Un), p(E1), p(J), p(Jn), p(Uh))
+    rng = np.random.default_rng(5)
+    wU, wE, wJ = rng.normal(size=U.shape), rng.normal(size=E.shape), rng.normal(size=J.shape)
+    Ub1, Eb1, Jb1 = np.zeros_like(U), wE.copy(), np.zeros_like(J)
+    hc.hc_step_vjp(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub1), p(Eb1), p(wJ), p(Jb1))
+    Ub2, Eb2, Jb2, Sb = np.zeros_like(U), wE.copy(), np.zeros_like(J), np.zeros_like(U)
+    hc.hc_step_vjp_gather(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub2), p(Eb2), p(wJ), p(Jb2), p(Sb))
+    m = LY.tile(np.ones((54, S))) > 0
+    assert np.abs(Ub1 - Ub2)[m].max() < 1e-14 * np.abs(Ub1[m]).max()
+    assert np.array_equal(Eb1, Eb2) and np.array_equal(Jb1, Jb2)
