@@ -72,58 +72,173 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))
     }
 }
 
-// Four-wavefront variant: waves 0-2 own the link directions, wave 3 forms J+-stat'(x) and J+-stat'(x+z)
-// and, after the barrier, applies the J advection with U_h,z handed over through LDS.  Two such blocks
-// fill the 8 wavefront slots of a CU at two waves per SIMD.
-// the body of the four-wavefront step for the tile starting at site `first`
-template <class I>
-__device__ __forceinline__ void step_tile4(const StepArgs& A, long first) {
-    __shared__ double shU[18][kTile];  // U_h,z
+// One row of a tile (64 consecutive x at fixed y, z, when nx is a multiple of 64) with a halo site at each
+// end: slot 0 is x0-1, slots 1..64 the tile, slot 65 is x0+64 (x periodic).
+constexpr int kRow = kTile + 2;
+
+// LDS of one block of the four-wavefront step
+struct StepShared {
+    double row[3][18][kRow];  // U_mu on the tile's row, plane pairs as in ldm (re, im of element k)
+    double j[6][8][kTile];    // J+stat'(x), J-stat'(x), J+stat'(x+z), J-stat'(x+z), J+(x), J-(x)
+    int ready;
+};
+
+__device__ __forceinline__ void ld_row(const double (*r)[kRow], int slot, M3& M) {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        M.re[k] = r[2 * k][slot];
+        M.im[k] = r[2 * k + 1][slot];
+    }
+}
+
+// The two staples of link MU in transverse direction NU (link_update's loop body) with the operands on
+// the tile's own row -- U_nu(x), U_nu(x+x) for MU = x, U_mu(x+-x) and U_x(x-x) for NU = x -- read from
+// the LDS row copy instead of global memory.  Same products in the same order as link_update.
+template <class I, int MU, int NU>
+__device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y, int z, int lane,
+                                            const double (*row)[18][kRow], M3& Sig) {
+    const Geom& g = A.g;
+    const double cn = A.cE[NU];
+    M3 P1, P2, T;
+    // forward staple U_nu(x+mu) U_mu^dag(x+nu) U_nu^dag(x)
+    if (MU == 0) ld_row(row[NU], lane + 2, P1);
+    else ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, MU, 1, NU, 0), P1);
+    if (NU == 0) ld_row(row[MU], lane + 2, P2);
+    else ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(g, x, y, z, NU, 1, MU, 0), P2);
+    mul_nd(P1, P2, T);
+    ld_row(row[NU], lane + 1, P1);
+    acc_mul_nd(cn, T, P1, Sig);
+    // backward staple U_nu^dag(x+mu-nu) U_mu^dag(x-nu) U_nu(x-nu)
+    ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, MU, 1, NU, -1), P1);
+    if (NU == 0) {
+        ld_row(row[MU], lane, P2);
+    } else {
+        ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(g, x, y, z, NU, -1, MU, 0), P2);
+    }
+    mul_dd(P1, P2, T);
+    if (NU == 0) ld_row(row[0], lane, P1);
+    else ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, NU, -1, MU, 0), P1);
+    acc_mul_nn(cn, T, P1, Sig);
+}
+
+// link_update with the row operands from LDS
+template <class I, int MU>
+__device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, int y, int z, int lane,
+                                                const double (*row)[18][kRow], M3& Uh) {
+    M3 Sig;
+#pragma unroll
+    for (int k = 0; k < 9; k++) { Sig.re[k] = 0.0; Sig.im[k] = 0.0; }
+    staples_row<I, MU, (MU + 1) % 3>(A, s, x, y, z, lane, row, Sig);
+    staples_row<I, MU, (MU + 2) % 3>(A, s, x, y, z, lane, row, Sig);
+    M3 Uo;
+    ld_row(row[MU], lane + 1, Uo);
+    link_finish<I, MU>(A, s, Uo, Sig, Uh);
+}
+
+// Four-wavefront step of the tile starting at site `first`.  Waves 0-2 update the link directions; wave 3
+// forms J+-stat'(x) and J+-stat'(x+z) (the cheaper work) and hands them, with the J+-(x) it read, to
+// wave 2 through LDS.  Wave 2 applies the J advection with its own U_h,z as soon as its link is done, so
+// the block ends with its slowest link wavefront instead of a serial advection after a block barrier.
+// The hand-over is a workgroup-scope flag: wave 3 sets it unconditionally once its stores to LDS are
+// complete, so wave 2's wait always ends; the wait is capped all the same.
+//
+// ROW (nx a multiple of 64, so a tile is one row): waves 0-2 first copy U_w of the row and its two halo
+// sites into LDS, and the 18 of 41 link operands per site that lie on the row are read from there.
+// This takes about a third of the vector-memory accesses off the L1, whose pipeline is what the
+// kernel waits on.
+template <class I, bool ROW>
+__device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepShared& sh) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long sl = first + lane;
     const bool valid = sl < A.g.S;
     const I s = (I)(valid ? sl : 0);
     int x, y, z;
     site_coords<I>(A.g, s, x, y, z);
-    double jps[8], jms[8], jpsU[8], jmsU[8];
-    M3 Uh;
-    if (wave == 0) {
-        if (valid) link_update<I, 0>(A, s, x, y, z, Uh);
-    } else if (wave == 1) {
-        if (valid) link_update<I, 1>(A, s, x, y, z, Uh);
-    } else if (wave == 2) {
-        if (valid) {
-            link_update<I, 2>(A, s, x, y, z, Uh);
+    if (ROW && wave < 3) {
+        // every lane of a row tile is a lattice site
+        M3 M;
+        ldm<I>(A.U, kUPlanes, 18 * wave, s, M);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            sh.row[wave][2 * k][lane + 1] = M.re[k];
+            sh.row[wave][2 * k + 1][lane + 1] = M.im[k];
+        }
+        if (lane < 2) {
+            const int x0 = x - lane;  // the tile's first x
+            const int xh = lane == 0 ? xwrap(x0 - 1, A.g.nx) : xwrap(x0 + kTile, A.g.nx);
+            ldm<I>(A.U, kUPlanes, 18 * wave, (I)(s - (I)x + (I)xh), M);
+            const int slot = lane == 0 ? 0 : kRow - 1;
+#pragma unroll
             for (int k = 0; k < 9; k++) {
-                shU[2 * k][lane] = Uh.re[k];
-                shU[2 * k + 1][lane] = Uh.im[k];
+                sh.row[wave][2 * k][slot] = M.re[k];
+                sh.row[wave][2 * k + 1][slot] = M.im[k];
             }
         }
-    } else {
+    }
+    if (threadIdx.x == 0) sh.ready = 0;
+    __syncthreads();
+    if (wave == 0) {
+        M3 Uh;
+        if (ROW) link_update_row<I, 0>(A, s, x, y, z, lane, sh.row, Uh);
+        else if (valid) link_update<I, 0>(A, s, x, y, z, Uh);
+    } else if (wave == 1) {
+        M3 Uh;
+        if (ROW) link_update_row<I, 1>(A, s, x, y, z, lane, sh.row, Uh);
+        else if (valid) link_update<I, 1>(A, s, x, y, z, Uh);
+    } else if (wave == 3) {
         if (valid) {
+            double jps[8], jms[8], jp[8], jm[8];
             const int zm = zdn(A.g, z);
-            jstat_new<I>(A, s, (I)(sl + A.g.P * (long)(zm - z)), jps, jms);
+            jstat_new_keep<I>(A, s, (I)(sl + A.g.P * (long)(zm - z)), jps, jms, jp, jm);
             stv<I>(A.Jn, kJPlanes, JPS * 8, s, jps);
             stv<I>(A.Jn, kJPlanes, JMS * 8, s, jms);
-            jstat_new_up<I>(A, s, z, jpsU, jmsU);
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                sh.j[0][a][lane] = jps[a];
+                sh.j[1][a][lane] = jms[a];
+                sh.j[4][a][lane] = jp[a];
+                sh.j[5][a][lane] = jm[a];
+            }
+            jstat_new_up<I>(A, s, z, jps, jms);
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                sh.j[2][a][lane] = jps[a];
+                sh.j[3][a][lane] = jms[a];
+            }
         }
-    }
-    __syncthreads();
-    if (wave == 3 && valid) {
-        for (int k = 0; k < 9; k++) {
-            Uh.re[k] = shU[2 * k][lane];
-            Uh.im[k] = shU[2 * k + 1][lane];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&sh.ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        M3 Uh;
+        if (ROW) link_update_row<I, 2>(A, s, x, y, z, lane, sh.row, Uh);
+        else if (valid) link_update<I, 2>(A, s, x, y, z, Uh);
+        for (int spin = 0; spin < (1 << 24); spin++) {
+            if (__hip_atomic_load(&sh.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+            __builtin_amdgcn_s_sleep(2);
         }
-        j_advect<I>(A, s, Uh, jps, jms, jpsU, jmsU);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (valid) {
+            double jps[8], jms[8], jpsU[8], jmsU[8], jp[8], jm[8];
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                jps[a] = sh.j[0][a][lane];
+                jms[a] = sh.j[1][a][lane];
+                jpsU[a] = sh.j[2][a][lane];
+                jmsU[a] = sh.j[3][a][lane];
+                jp[a] = sh.j[4][a][lane];
+                jm[a] = sh.j[5][a][lane];
+            }
+            j_advect_with<I>(A, s, Uh, jps, jms, jpsU, jmsU, jp, jm);
+        }
     }
 }
 
-// Four-wavefront variant: waves 0-2 own the link directions, wave 3 forms J+-stat'(x) and J+-stat'(x+z)
-// and, after the barrier, applies the J advection with U_h,z handed over through LDS.  Two such blocks
-// fill the 8 wavefront slots of a CU at two waves per SIMD.
-template <class I, int W>
+// Four-wavefront variant (step_tile4); two such blocks fill the 8 wavefront slots of a CU at two waves
+// per SIMD.
+template <class I, int W, bool ROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) k_step4(StepArgs A, TileMap tm) {
-    step_tile4<I>(A, tile_site(A.g, tm, blockIdx.x));
+    __shared__ StepShared sh;
+    step_tile4<I, ROW>(A, tile_site(A.g, tm, blockIdx.x), sh);
 }
 
 // Ensemble step: one launch advances every event of a batch.  Block b works on the tile starting at site
@@ -133,9 +248,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_step_batch(const StepArgs* __restrict__ table, const int* __restrict__ block_event,
              const unsigned* __restrict__ block_site) {
+    __shared__ StepShared sh;
     const int e = block_event[blockIdx.x];
     const StepArgs A = table[e];
-    step_tile4<unsigned int>(A, (long)block_site[blockIdx.x]);
+    if (A.g.nx % kTile == 0) step_tile4<unsigned int, true>(A, (long)block_site[blockIdx.x], sh);
+    else step_tile4<unsigned int, false>(A, (long)block_site[blockIdx.x], sh);
 }
 
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
@@ -168,9 +285,14 @@ template <class I>
 static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hipStream_t st) {
     dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
     if (four) {
-        if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3>), grid, dim3(256), 0, st, A, tm);
-        else if (waves == 4) hipLaunchKernelGGL((k_step4<I, 4>), grid, dim3(256), 0, st, A, tm);
-        else hipLaunchKernelGGL((k_step4<I, 2>), grid, dim3(256), 0, st, A, tm);
+        // the row-staged form needs whole rows per tile; GLASMA_STEP_FOUR=2 forces the plain form
+        if (A.g.nx % kTile == 0 && four == 1) {
+            if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
+            else hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
+        } else {
+            if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, false>), grid, dim3(256), 0, st, A, tm);
+            else hipLaunchKernelGGL((k_step4<I, 2, false>), grid, dim3(256), 0, st, A, tm);
+        }
         return;
     }
     if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, dim3(192), 0, st, A, tm);

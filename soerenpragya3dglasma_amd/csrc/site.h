@@ -282,6 +282,26 @@ GHD void jstat_new(const StepArgs& A, I xs, I xd, double* jps, double* jms) {
     for (int a = 0; a < 8; a++) jms[a] += A.dtz * (jv[a] - down[a]);
 }
 
+// jstat_new at xs = x that also returns the J+-(x) it read, for the J advection of the same site
+template <class I>
+GHD void jstat_new_keep(const StepArgs& A, I xs, I xd, double* jps, double* jms, double* jp, double* jm) {
+    M3 Ud;
+    double jdn[8], down[8];
+    ldm<I>(A.U, kUPlanes, 36, xd, Ud);
+    ldv<I>(A.J, kJPlanes, JPS * 8, xs, jps);
+    ldv<I>(A.J, kJPlanes, JP * 8, xd, jdn);
+    inv_adj_mul(Ud, jdn, down);
+    ldv<I>(A.J, kJPlanes, JP * 8, xs, jp);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jps[a] += -A.dtz * (jp[a] - down[a]);
+    ldv<I>(A.J, kJPlanes, JMS * 8, xs, jms);
+    ldv<I>(A.J, kJPlanes, JM * 8, xd, jdn);
+    inv_adj_mul(Ud, jdn, down);
+    ldv<I>(A.J, kJPlanes, JM * 8, xs, jm);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jms[a] += A.dtz * (jm[a] - down[a]);
+}
+
 // J+-stat' at the site above (x+z), or at x itself on the open top layer (the reference's ghost copy)
 template <class I>
 GHD void jstat_new_up(const StepArgs& A, I s, int z, double* jpsU, double* jmsU) {
@@ -292,6 +312,34 @@ GHD void jstat_new_up(const StepArgs& A, I s, int z, double* jpsU, double* jmsU)
         const int zm = zdn(A.g, z);
         jstat_new<I>(A, s, (I)((long)s + A.g.P * (long)(zm - z)), jpsU, jmsU);
     }
+}
+
+// The rest of link_update once the staple sum Sig is formed: E'_MU, the exponential, U_h (and UOld),
+// U'_MU, with Uo = U_MU(x).
+template <class I, int MU>
+GHD void link_finish(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh) {
+    double Em[8];
+    {
+        double F[8];
+        retr_igen_prod(Uo, Sig, F);
+        ldv<I>(A.E, kEPlanes, 8 * MU, s, Em);
+#pragma unroll
+        for (int a = 0; a < 8; a++) Em[a] += A.gc[MU] * F[a];
+    }
+    if (MU == 2) {
+        double jp[8], jm[8];
+        ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
+        ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
+#pragma unroll
+        for (int a = 0; a < 8; a++) Em[a] += -A.dtz * ((jp[a] - jm[a]) / kSqrt2);
+    }
+    stv<I>(A.E, kEPlanes, 8 * MU, s, Em);
+    M3 D;
+    expi_m1(A.cU[MU], Em, D);
+    add_mul(D, Uo, Uh);
+    if (A.UOld) stm<I>(A.UOld, kUPlanes, 18 * MU, s, Uh);
+    add_mul(D, Uh, Uo);
+    stm<I>(A.Un, kUPlanes, 18 * MU, s, Uo);
 }
 
 // Link direction MU of site s: E'_MU, the exponential, U_h (and UOld), U'_MU.  Returns U_h in Uh.
@@ -322,45 +370,33 @@ GHD void link_update(const StepArgs& A, I s, int x, int y, int z, M3& Uh) {
     }
     M3 Uo;
     ldm<I>(A.U, kUPlanes, 18 * MU, s, Uo);
-    double Em[8];
-    {
-        double F[8];
-        retr_igen_prod(Uo, Sig, F);
-        ldv<I>(A.E, kEPlanes, 8 * MU, s, Em);
-#pragma unroll
-        for (int a = 0; a < 8; a++) Em[a] += A.gc[MU] * F[a];
-    }
-    if (MU == 2) {
-        double jp[8], jm[8];
-        ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
-        ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
-#pragma unroll
-        for (int a = 0; a < 8; a++) Em[a] += -A.dtz * ((jp[a] - jm[a]) / kSqrt2);
-    }
-    stv<I>(A.E, kEPlanes, 8 * MU, s, Em);
-    M3 D;
-    expi_m1(A.cU[MU], Em, D);
-    add_mul(D, Uo, Uh);
-    if (A.UOld) stm<I>(A.UOld, kUPlanes, 18 * MU, s, Uh);
-    add_mul(D, Uh, Uo);
-    stm<I>(A.Un, kUPlanes, 18 * MU, s, Uo);
+    link_finish<I, MU>(A, s, Uo, Sig, Uh);
 }
 
+
 // J+-'(x) = J+-(x) -+ (dt/a_z) [U_h,z(x) J+-stat'(x+z) U_h,z^dag(x) - J+-stat'(x)]  (:238-269)
+// the same with J+-(x) already at hand (jp, jm are overwritten)
+template <class I>
+GHD void j_advect_with(const StepArgs& A, I s, const M3& Uhz, const double* jps, const double* jms,
+                       const double* jpsU, const double* jmsU, double* jp, double* jm) {
+    double up[8];
+    adj_mul(Uhz, jpsU, up);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jp[a] += -A.dtz * (up[a] - jps[a]);
+    stv<I>(A.Jn, kJPlanes, JP * 8, s, jp);
+    adj_mul(Uhz, jmsU, up);
+#pragma unroll
+    for (int a = 0; a < 8; a++) jm[a] += A.dtz * (up[a] - jms[a]);
+    stv<I>(A.Jn, kJPlanes, JM * 8, s, jm);
+}
+
 template <class I>
 GHD void j_advect(const StepArgs& A, I s, const M3& Uhz, const double* jps, const double* jms, const double* jpsU,
                   const double* jmsU) {
-    double up[8], jv[8];
-    adj_mul(Uhz, jpsU, up);
-    ldv<I>(A.J, kJPlanes, JP * 8, s, jv);
-#pragma unroll
-    for (int a = 0; a < 8; a++) jv[a] += -A.dtz * (up[a] - jps[a]);
-    stv<I>(A.Jn, kJPlanes, JP * 8, s, jv);
-    adj_mul(Uhz, jmsU, up);
-    ldv<I>(A.J, kJPlanes, JM * 8, s, jv);
-#pragma unroll
-    for (int a = 0; a < 8; a++) jv[a] += A.dtz * (up[a] - jms[a]);
-    stv<I>(A.Jn, kJPlanes, JM * 8, s, jv);
+    double jp[8], jm[8];
+    ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
+    ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
+    j_advect_with<I>(A, s, Uhz, jps, jms, jpsU, jmsU, jp, jm);
 }
 
 // Whole step of one site, in the order the direction-split kernel performs it (host equivalent).
