@@ -1,7 +1,7 @@
 // Ensembles of independent events advanced together (no counterpart in the reference, which runs one
 // event per process): every event is an ordinary context -- fields, initial conditions and observables
 // go through the single-event entry points -- and glasma_batch_update advances all of them with one
-// kernel launch per step over the combined (event, tile) work list.  The nsteps launches are captured
+// kernel launch per step and step form over the combined (event, tile) work list.  The nsteps launches are captured
 // once into a HIP graph per step count and replayed, so a call costs one graph launch.
 //
 // The per-event step arguments live in small device tables rebuilt from each context's current buffer
@@ -21,6 +21,7 @@ struct glasma_batch {
     int* block_event = nullptr;
     unsigned* block_site = nullptr;
     long nblocks = 0;
+    long nrow = 0;  // the first nrow blocks belong to events with nx a multiple of 64 (row-staged form)
     std::map<int, hipGraphExec_t> graphs;  // by step count
     double last_ms = 0.0;
     std::string err;
@@ -62,7 +63,8 @@ int graph_for(glasma_batch* b, int nsteps, hipGraphExec_t* out) {
     int rc = 0;
     for (int i = 0; i < nsteps && rc == 0; i++) {
         const int ph = i & 1, old = (i == nsteps - 1) ? 1 : 0;
-        rc = launch_step_batch(b->stream, b->table + (2 * ph + old) * n, b->block_event, b->block_site, b->nblocks);
+        rc = launch_step_batch(b->stream, b->table + (2 * ph + old) * n, b->block_event, b->block_site, b->nrow,
+                               b->nblocks - b->nrow);
     }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -125,12 +127,17 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
         }
         b->ev.push_back(c);
     }
+    // events with whole rows per tile first (the row-staged form), then the rest
     std::vector<long> t0(nevents);
-    long nb = 0;
-    for (int e = 0; e < nevents; e++) {
-        t0[e] = nb;
-        nb += pad_sites(b->ev[e]->g.S) / kTile;
-    }
+    long nb = 0, nrow = 0;
+    for (int pass = 0; pass < 2; pass++)
+        for (int e = 0; e < nevents; e++) {
+            const bool row = b->ev[e]->g.nx % kTile == 0;
+            if (row != (pass == 0)) continue;
+            t0[e] = nb;
+            nb += pad_sites(b->ev[e]->g.S) / kTile;
+            if (row) nrow = nb;
+        }
     // each event's blocks in the tile order it would take alone
     std::vector<int> be(nb);
     std::vector<unsigned> bs(nb);
@@ -144,6 +151,7 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
         }
     }
     b->nblocks = nb;
+    b->nrow = nrow;
     hipError_t err = hipStreamCreate(&b->stream);
     if (err == hipSuccess) err = hipEventCreate(&b->e0);
     if (err == hipSuccess) err = hipEventCreate(&b->e1);

@@ -80,8 +80,9 @@ __global__ void k_set_identity(double* __restrict__ U, long S, int nmat);
 // leapfrog launches (leapfrog.hip)
 int launch_step(glasma_ctx* ctx, bool write_old);
 StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old);
+// ensemble step over nrow row-form blocks followed by nplain plain-form blocks
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
-                      long nblocks);
+                      long nrow, long nplain);
 TileMap default_tile_map(const Geom& g);
 
 // the left-moving MV sheet from a given charge density, and its reverse-mode derivative (mv.hip)

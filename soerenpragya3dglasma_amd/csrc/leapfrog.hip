@@ -1,17 +1,20 @@
 // The fused FP64 leapfrog kernel: one launch per Dynamics::SUNc::Update (SC/DYNAMICS/SUNcDynamics.cpp:353-378).
 //
-// A block of 192 threads advances one tile of 64 sites; wavefront w (0, 1, 2) owns link direction w
-// of every site (link_update in site.h).  The x wave also forms J+-stat'(x) and the y wave
-// J+-stat'(x+z); they hand them to the z wave through LDS, which applies the J advection with its
-// half-step link after one barrier.  Splitting a site over three wavefronts keeps each wavefront's
-// live state to a few matrices, so two of them fit on a SIMD without spilling.
+// A block of 256 threads advances one tile of 64 sites (step_tile4): wavefronts 0, 1, 2 own link
+// direction 0, 1, 2 of every site (link_update in site.h), wavefront 3 forms J+-stat'(x) and
+// J+-stat'(x+z) and hands them to wavefront 2 through LDS, which applies the J advection with its own
+// half-step link.  Splitting a site over wavefronts keeps each wavefront's live state to a few matrices.
+// When nx is a multiple of 64 a tile is one lattice row, and the link operands on that row (own links
+// and their x neighbours, 18 of the 41 matrix loads per site) come from an LDS copy of the row made once
+// per block; that form runs three wavefronts per SIMD, the plain form two.  The older three-wavefront
+// kernel (k_step) is kept for measurements.
 //
 // HBM traffic per site-update: read U (432 B) + E (192 B) + J (256 B), write the same: 1760 B, the
 // compulsory minimum.  The neighbour link matrices are re-read through L2 / MALL, so the order in
 // which the 8 XCDs (each with its own 4 MB L2) visit the lattice matters: workgroups are dealt to the
-// XCDs round-robin, so with the XCD-aware mapping XCD k owns the k-th band of rows of every z plane
-// and walks the planes in order; rows y-1, y, y+1 are then processed by the same XCD close in time
-// (L2 hits) and a plane's band is re-read by the next plane shortly afterwards (MALL hits).
+// XCDs round-robin, and with the default column-band order (TileMap mode 2 in site.h) each XCD walks
+// its bands of 4 x 4 tiles through z, so the y and z neighbours of a tile are read by the same XCD
+// shortly before or after it (L2 hits); HBM reads come to 1.25x the compulsory bytes.
 //
 // GLASMA_STEP_WAVES (2 or 3: register budget of 256 or 168 per lane), GLASMA_STEP_FOUR and
 // GLASMA_STEP_MAP / _BW / _BH select variants for measurement sweeps; the defaults are the fastest
@@ -21,6 +24,11 @@
 #include "internal.h"
 
 namespace glasma {
+
+static int env_int(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
 
 
 template <class I, int W>
@@ -241,24 +249,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))
     step_tile4<I, ROW>(A, tile_site(A.g, tm, blockIdx.x), sh);
 }
 
-// Ensemble step: one launch advances every event of a batch.  Block b works on the tile starting at site
-// block_site[b] of event e = block_event[b] (each event's blocks in the order tile_site gives it alone);
-// the event's arguments come from a device table (uniform per block, so they are read with scalar
-// loads).  The per-site arithmetic is that of k_step4.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// Ensemble step: one launch per form advances every event of a batch.  Block b works on the tile starting
+// at site block_site[b] of event e = block_event[b] (each event's blocks in the order tile_site gives it
+// alone); the event's arguments come from a device table (uniform per block, so they are read with scalar
+// loads).  The per-site arithmetic is that of k_step4.  The blocks of events with nx a multiple of 64
+// come first and run the row-staged form at three waves per SIMD, the rest the plain form at two, as
+// each runs alone.
+template <int W, bool ROW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_step_batch(const StepArgs* __restrict__ table, const int* __restrict__ block_event,
              const unsigned* __restrict__ block_site) {
     __shared__ StepShared sh;
-    const int e = block_event[blockIdx.x];
-    const StepArgs A = table[e];
-    if (A.g.nx % kTile == 0) step_tile4<unsigned int, true>(A, (long)block_site[blockIdx.x], sh);
-    else step_tile4<unsigned int, false>(A, (long)block_site[blockIdx.x], sh);
+    const StepArgs A = table[block_event[blockIdx.x]];
+    step_tile4<unsigned int, ROW>(A, (long)block_site[blockIdx.x], sh);
 }
 
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
-                      long nblocks) {
-    if (nblocks <= 0 || nblocks > 0x7fffffffL) return GLASMA_ERR_ARG;
-    hipLaunchKernelGGL(k_step_batch, dim3((unsigned)nblocks), dim3(256), 0, st, table, block_event, block_site);
+                      long nrow, long nplain) {
+    if (nrow < 0 || nplain < 0 || nrow + nplain <= 0 || nrow > 0x7fffffffL || nplain > 0x7fffffffL)
+        return GLASMA_ERR_ARG;
+    if (nrow > 0)
+        hipLaunchKernelGGL((k_step_batch<3, true>), dim3((unsigned)nrow), dim3(256), 0, st, table, block_event,
+                           block_site);
+    if (nplain > 0)
+        hipLaunchKernelGGL((k_step_batch<2, false>), dim3((unsigned)nplain), dim3(256), 0, st, table,
+                           block_event + nrow, block_site + nrow);
     return hipGetLastError() == hipSuccess ? 0 : GLASMA_ERR_HIP;
 }
 
@@ -276,19 +291,16 @@ StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old) {
     return A;
 }
 
-static int env_int(const char* name, int dflt) {
-    const char* v = std::getenv(name);
-    return v ? std::atoi(v) : dflt;
-}
-
 template <class I>
 static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hipStream_t st) {
     dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
     if (four) {
         // the row-staged form needs whole rows per tile; GLASMA_STEP_FOUR=2 forces the plain form
+        // three waves per SIMD (168 registers, a few spilled) measured faster for the row-staged form
+        // (30.9 against 31.8 ms at 256 x 256 x 1024); the plain form spills too much at three
         if (A.g.nx % kTile == 0 && four == 1) {
-            if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
-            else hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
+            if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
+            else hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
         } else {
             if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, false>), grid, dim3(256), 0, st, A, tm);
             else hipLaunchKernelGGL((k_step4<I, 2, false>), grid, dim3(256), 0, st, A, tm);
@@ -333,7 +345,7 @@ TileMap default_tile_map(const Geom& g) {
 }
 
 int launch_step(glasma_ctx* ctx, bool write_old) {
-    static const int waves = env_int("GLASMA_STEP_WAVES", 2);
+    static const int waves = env_int("GLASMA_STEP_WAVES", 0);  // 0: the measured best of each form
     static const int four = env_int("GLASMA_STEP_FOUR", 1);
     const StepArgs A = step_args(ctx, ctx->cur, write_old);
     const TileMap tm = default_tile_map(ctx->g);
