@@ -136,8 +136,13 @@ __device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, i
     M3 Sig;
 #pragma unroll
     for (int k = 0; k < 9; k++) { Sig.re[k] = 0.0; Sig.im[k] = 0.0; }
+    // scheduling barriers keep the compiler from hoisting the next staple pair's loads into this one:
+    // that fits the form into 168 registers without spills (three waves per SIMD, 31.3 -> 30.1 ms at
+    // 256 x 256 x 1024), the waves hiding the latency instead
     staples_row<I, MU, (MU + 1) % 3>(A, s, x, y, z, lane, row, Sig);
+    __builtin_amdgcn_sched_barrier(0);
     staples_row<I, MU, (MU + 2) % 3>(A, s, x, y, z, lane, row, Sig);
+    __builtin_amdgcn_sched_barrier(0);
     M3 Uo;
     ld_row(row[MU], lane + 1, Uo);
     link_finish<I, MU>(A, s, Uo, Sig, Uh);
@@ -296,8 +301,8 @@ static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hi
     dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
     if (four) {
         // the row-staged form needs whole rows per tile; GLASMA_STEP_FOUR=2 forces the plain form
-        // three waves per SIMD (168 registers, a few spilled) measured faster for the row-staged form
-        // (30.9 against 31.8 ms at 256 x 256 x 1024); the plain form spills too much at three
+        // three waves per SIMD (168 registers) measured faster for the row-staged form (30.1 against
+        // 31.8 ms at 256 x 256 x 1024); the plain form spills too much at three
         if (A.g.nx % kTile == 0 && four == 1) {
             if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
             else hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
