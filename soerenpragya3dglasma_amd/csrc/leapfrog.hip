@@ -129,6 +129,15 @@ __device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y
     acc_mul_nn(cn, T, P1, Sig);
 }
 
+// wait for the J wavefront's hand-over (set unconditionally once its LDS stores are complete; capped)
+__device__ __forceinline__ void wait_ready(StepShared& sh) {
+    for (int spin = 0; spin < (1 << 24); spin++) {
+        if (__hip_atomic_load(&sh.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // link_update with the row operands from LDS
 template <class I, int MU>
 __device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, int y, int z, int lane,
@@ -212,7 +221,18 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
                 sh.j[4][a][lane] = jp[a];
                 sh.j[5][a][lane] = jm[a];
             }
-            jstat_new_up<I>(A, s, z, jps, jms);
+            if (ROW) {
+                // U_z(x) of J+-stat'(x+z) from the row copy; on the open top layer J+-stat'(x+z) is
+                // J+-stat'(x) itself (jstat_new_up), already in jps, jms
+                const int zp = zup(A.g, z);
+                if (zp != z) {
+                    M3 Uz;
+                    ld_row(sh.row[2], lane + 1, Uz);
+                    jstat_core<I>(A, (I)(sl + A.g.P * (long)(zp - z)), s, Uz, jps, jms);
+                }
+            } else {
+                jstat_new_up<I>(A, s, z, jps, jms);
+            }
 #pragma unroll
             for (int a = 0; a < 8; a++) {
                 sh.j[2][a][lane] = jps[a];
@@ -225,11 +245,7 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
         M3 Uh;
         if (ROW) link_update_row<I, 2>(A, s, x, y, z, lane, sh.row, Uh);
         else if (valid) link_update<I, 2>(A, s, x, y, z, Uh);
-        for (int spin = 0; spin < (1 << 24); spin++) {
-            if (__hip_atomic_load(&sh.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        wait_ready(sh);
         if (valid) {
             double jps[8], jms[8], jpsU[8], jmsU[8], jp[8], jm[8];
 #pragma unroll

@@ -283,10 +283,8 @@ GHD void fill_step_coefficients(StepArgs& A, const double* a, double dt) {
 
 // J+-stat' of site xs from time-t data, xd = xs - z (SC/DYNAMICS/SUNcDynamics.cpp:205-236)
 template <class I>
-GHD void jstat_new(const StepArgs& A, I xs, I xd, double* jps, double* jms) {
-    M3 Ud;
+GHD void jstat_core(const StepArgs& A, I xs, I xd, const M3& Ud, double* jps, double* jms) {
     double jv[8], jdn[8], down[8];
-    ldm<I>(A.U, kUPlanes, 36, xd, Ud);
     ldv<I>(A.J, kJPlanes, JPS * 8, xs, jps);
     ldv<I>(A.J, kJPlanes, JP * 8, xd, jdn);
     inv_adj_mul(Ud, jdn, down);
@@ -299,6 +297,12 @@ GHD void jstat_new(const StepArgs& A, I xs, I xd, double* jps, double* jms) {
     ldv<I>(A.J, kJPlanes, JM * 8, xs, jv);
 #pragma unroll
     for (int a = 0; a < 8; a++) jms[a] += A.dtz * (jv[a] - down[a]);
+}
+template <class I>
+GHD void jstat_new(const StepArgs& A, I xs, I xd, double* jps, double* jms) {
+    M3 Ud;
+    ldm<I>(A.U, kUPlanes, 36, xd, Ud);
+    jstat_core<I>(A, xs, xd, Ud, jps, jms);
 }
 
 // jstat_new at xs = x that also returns the J+-(x) it read, for the J advection of the same site
@@ -334,9 +338,9 @@ GHD void jstat_new_up(const StepArgs& A, I s, int z, double* jpsU, double* jmsU)
 }
 
 // The rest of link_update once the staple sum Sig is formed: E'_MU, the exponential, U_h (and UOld),
-// U'_MU, with Uo = U_MU(x).
+// U'_MU, with Uo = U_MU(x) and, for the z link, J+-(x) given.
 template <class I, int MU>
-GHD void link_finish(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh) {
+GHD void link_finish_j(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh, const double* jp, const double* jm) {
     double Em[8];
     {
         double F[8];
@@ -346,9 +350,6 @@ GHD void link_finish(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh) {
         for (int a = 0; a < 8; a++) Em[a] += A.gc[MU] * F[a];
     }
     if (MU == 2) {
-        double jp[8], jm[8];
-        ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
-        ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
 #pragma unroll
         for (int a = 0; a < 8; a++) Em[a] += -A.dtz * ((jp[a] - jm[a]) / kSqrt2);
     }
@@ -359,6 +360,16 @@ GHD void link_finish(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh) {
     if (A.UOld) stm<I>(A.UOld, kUPlanes, 18 * MU, s, Uh);
     add_mul(D, Uh, Uo);
     stm<I>(A.Un, kUPlanes, 18 * MU, s, Uo);
+}
+// the same with J+-(x) (the E source of the z link) read here
+template <class I, int MU>
+GHD void link_finish(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh) {
+    double jp[8] = {}, jm[8] = {};
+    if (MU == 2) {
+        ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
+        ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
+    }
+    link_finish_j<I, MU>(A, s, Uo, Sig, Uh, jp, jm);
 }
 
 // Link direction MU of site s: E'_MU, the exponential, U_h (and UOld), U'_MU.  Returns U_h in Uh.
