@@ -147,8 +147,17 @@ def calibrate_copy(gib: float) -> None:
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
+    # the copy's own rate: the bandwidth a plain read-and-write stream reaches on this device
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
     print(json.dumps({"calibration": "copy", "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
-                      "launches": 3}), flush=True)
+                      "launches": 13, "ms_per_copy": ms, "read_plus_write_GBps": 2 * n * 8 / ms / 1e6}),
+          flush=True)
 
 
 def main():
