@@ -294,10 +294,10 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
         s.E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
         s.J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
     }
-    for (auto& s : seg) {
-        s.U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
-        s.E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
-        s.J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    for (size_t i = 1; i < seg.size(); i++) {  // seg[0] is the segment's checkpoint itself
+        seg[i].U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+        seg[i].E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+        seg[i].J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
     }
     std::vector<double*> Uh(seglen > 0 ? seglen : 0);
     for (auto& p : Uh) p = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
@@ -335,13 +335,21 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     // backward sweep, one segment at a time
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int n0 = sg * ckpt, n1 = (n0 + ckpt < nsteps) ? n0 + ckpt : nsteps;
-        if ((rc = load_state(ctx, ck[sg]))) return rc;
-        if ((rc = save_state(ctx, seg[0]))) return rc;
+        // recompute the segment straight into its slots: U and J ping-pong from slot i to slot i+1, UOld
+        // lands in Uh[i], and E (updated in place by the step) is copied into slot i+1 first
+        seg[0] = ck[sg];
         for (int n = n0; n < n1; n++) {
-            launch_step(ctx, true);
-            GL_HIP(ctx, hipMemcpyAsync(Uh[n - n0], ctx->UOld, kUPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice,
+            const int i = n - n0;
+            GL_HIP(ctx, hipMemcpyAsync(seg[i + 1].E, seg[i].E, kEPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice,
                                        ctx->stream));
-            if ((rc = save_state(ctx, seg[n - n0 + 1]))) return rc;
+            StepArgs F = step_args(ctx, ctx->cur, true);
+            F.U = seg[i].U;
+            F.Un = seg[i + 1].U;
+            F.E = seg[i + 1].E;
+            F.J = seg[i].J;
+            F.Jn = seg[i + 1].J;
+            F.UOld = Uh[i];
+            if ((rc = launch_step_args(ctx, F))) return rc;
         }
         for (int n = n1 - 1; n >= n0; n--) {
             const int i = n - n0;

@@ -79,6 +79,7 @@ __global__ void k_set_identity(double* __restrict__ U, long S, int nmat);
 // starts at (l % b1) * s1 + (l / b1) * s2 (units of complex entries).  sign -1 forward, +1 backward.
 // leapfrog launches (leapfrog.hip)
 int launch_step(glasma_ctx* ctx, bool write_old);
+int launch_step_args(glasma_ctx* ctx, const StepArgs& A);  // explicit buffers; ctx->cur is left alone
 StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old);
 // ensemble step over nrow row-form blocks followed by nplain plain-form blocks
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,

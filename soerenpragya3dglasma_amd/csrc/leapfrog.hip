@@ -365,15 +365,20 @@ TileMap default_tile_map(const Geom& g) {
     return tile_map(g, map_mode, map_bw, map_bh);
 }
 
-int launch_step(glasma_ctx* ctx, bool write_old) {
+// one step with explicit buffers (the context supplies geometry, stream and tile order)
+int launch_step_args(glasma_ctx* ctx, const StepArgs& A) {
     static const int waves = env_int("GLASMA_STEP_WAVES", 0);  // 0: the measured best of each form
     static const int four = env_int("GLASMA_STEP_FOUR", 1);
-    const StepArgs A = step_args(ctx, ctx->cur, write_old);
     const TileMap tm = default_tile_map(ctx->g);
     if (A.g.S < (1L << 32)) launch<unsigned int>(waves, four, tm, A, ctx->stream);
     else launch<unsigned long>(waves, four, tm, A, ctx->stream);
-    ctx->cur ^= 1;
     return 0;
+}
+
+int launch_step(glasma_ctx* ctx, bool write_old) {
+    const int rc = launch_step_args(ctx, step_args(ctx, ctx->cur, write_old));
+    ctx->cur ^= 1;
+    return rc;
 }
 
 }  // namespace glasma
