@@ -162,8 +162,8 @@ int glasma_fill_random(glasma_ctx* ctx, unsigned long seed, double link_scale, d
  * reached through glasma_batch_context for fields, initial conditions and observables.
  * glasma_batch_update advances every event by nsteps leapfrog steps (Dynamics::SUNc::Update,
  * DYNAMICS/SUNcDynamics.cpp:353-378) with one kernel launch per step over all events (two when the
- * ensemble mixes lattices whose Nx is and is not a multiple of 64), the launches captured once per
- * step count into a HIP graph.  Each event's fields are bit-identical to advancing it
+ * ensemble mixes lattices whose Nx is and is not a multiple or a divisor of 64), the launches captured
+ * once per step count into a HIP graph.  Each event's fields are bit-identical to advancing it
  * alone.  glasma_batch_update_serial advances the events one after another with per-event launches
  * (the baseline).  glasma_batch_last_ms: device time of the last batched call, wall time of the last
  * serial call.  Each event needs fewer than 2^32 sites. */

@@ -21,7 +21,7 @@ struct glasma_batch {
     int* block_event = nullptr;
     unsigned* block_site = nullptr;
     long nblocks = 0;
-    long nrow = 0;  // the first nrow blocks belong to events with nx a multiple of 64 (row-staged form)
+    long nrow = 0;  // the first nrow blocks belong to events whose tiles are whole rows (row-staged form)
     std::map<int, hipGraphExec_t> graphs;  // by step count
     double last_ms = 0.0;
     std::string err;
@@ -132,7 +132,7 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
     long nb = 0, nrow = 0;
     for (int pass = 0; pass < 2; pass++)
         for (int e = 0; e < nevents; e++) {
-            const bool row = b->ev[e]->g.nx % kTile == 0;
+            const bool row = row_tiles(b->ev[e]->g);
             if (row != (pass == 0)) continue;
             t0[e] = nb;
             nb += pad_sites(b->ev[e]->g.S) / kTile;

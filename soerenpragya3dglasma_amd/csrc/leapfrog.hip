@@ -4,9 +4,9 @@
 // direction 0, 1, 2 of every site (link_update in site.h), wavefront 3 forms J+-stat'(x) and
 // J+-stat'(x+z) and hands them to wavefront 2 through LDS, which applies the J advection with its own
 // half-step link.  Splitting a site over wavefronts keeps each wavefront's live state to a few matrices.
-// When nx is a multiple of 64 a tile is one lattice row, and the link operands on that row (own links
-// and their x neighbours, 18 of the 41 matrix loads per site) come from an LDS copy of the row made once
-// per block; that form runs three wavefronts per SIMD, the plain form two.  The older three-wavefront
+// When a tile is made of whole lattice rows (nx a multiple or a divisor of 64), the link operands on those
+// rows (own links and their x neighbours, 18 of the 41 matrix loads per site) come from an LDS copy made
+// once per block; that form runs three wavefronts per SIMD, the plain form two.  The older three-wavefront
 // kernel (k_step) is kept for measurements.
 //
 // HBM traffic per site-update: read U (432 B) + E (192 B) + J (256 B), write the same: 1760 B, the
@@ -80,8 +80,9 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))
     }
 }
 
-// One row of a tile (64 consecutive x at fixed y, z, when nx is a multiple of 64) with a halo site at each
-// end: slot 0 is x0-1, slots 1..64 the tile, slot 65 is x0+64 (x periodic).
+// The links of a tile's sites in LDS, slot 1 + lane, when the tile is made of whole rows: nx a multiple of
+// 64 (one row; slot 0 then holds x0-1 and slot 65 x0+64, x periodic) or a divisor of 64 (64 / nx rows,
+// whose x neighbours all lie in the tile).
 constexpr int kRow = kTile + 2;
 
 // LDS of one block of the four-wavefront step
@@ -100,18 +101,19 @@ __device__ __forceinline__ void ld_row(const double (*r)[kRow], int slot, M3& M)
 }
 
 // The two staples of link MU in transverse direction NU (link_update's loop body) with the operands on
-// the tile's own row -- U_nu(x), U_nu(x+x) for MU = x, U_mu(x+-x) and U_x(x-x) for NU = x -- read from
-// the LDS row copy instead of global memory.  Same products in the same order as link_update.
+// the tile's own rows -- U_nu(x), U_nu(x+x) for MU = x, U_mu(x+-x) and U_x(x-x) for NU = x -- read from
+// the LDS copy instead of global memory (slots: own 1 + lane, x+1 at sp, x-1 at sm).  Same products in the
+// same order as link_update.
 template <class I, int MU, int NU>
-__device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y, int z, int lane,
+__device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y, int z, int lane, int sp, int sm,
                                             const double (*row)[18][kRow], M3& Sig) {
     const Geom& g = A.g;
     const double cn = A.cE[NU];
     M3 P1, P2, T;
     // forward staple U_nu(x+mu) U_mu^dag(x+nu) U_nu^dag(x)
-    if (MU == 0) ld_row(row[NU], lane + 2, P1);
+    if (MU == 0) ld_row(row[NU], sp, P1);
     else ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, MU, 1, NU, 0), P1);
-    if (NU == 0) ld_row(row[MU], lane + 2, P2);
+    if (NU == 0) ld_row(row[MU], sp, P2);
     else ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(g, x, y, z, NU, 1, MU, 0), P2);
     mul_nd(P1, P2, T);
     ld_row(row[NU], lane + 1, P1);
@@ -119,12 +121,12 @@ __device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y
     // backward staple U_nu^dag(x+mu-nu) U_mu^dag(x-nu) U_nu(x-nu)
     ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, MU, 1, NU, -1), P1);
     if (NU == 0) {
-        ld_row(row[MU], lane, P2);
+        ld_row(row[MU], sm, P2);
     } else {
         ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(g, x, y, z, NU, -1, MU, 0), P2);
     }
     mul_dd(P1, P2, T);
-    if (NU == 0) ld_row(row[0], lane, P1);
+    if (NU == 0) ld_row(row[0], sm, P1);
     else ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, NU, -1, MU, 0), P1);
     acc_mul_nn(cn, T, P1, Sig);
 }
@@ -140,17 +142,17 @@ __device__ __forceinline__ void wait_ready(StepShared& sh) {
 
 // link_update with the row operands from LDS
 template <class I, int MU>
-__device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, int y, int z, int lane,
-                                                const double (*row)[18][kRow], M3& Uh) {
+__device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, int y, int z, int lane, int sp,
+                                                int sm, const double (*row)[18][kRow], M3& Uh) {
     M3 Sig;
 #pragma unroll
     for (int k = 0; k < 9; k++) { Sig.re[k] = 0.0; Sig.im[k] = 0.0; }
     // scheduling barriers keep the compiler from hoisting the next staple pair's loads into this one:
     // that fits the form into 168 registers without spills (three waves per SIMD, 31.3 -> 30.1 ms at
     // 256 x 256 x 1024), the waves hiding the latency instead
-    staples_row<I, MU, (MU + 1) % 3>(A, s, x, y, z, lane, row, Sig);
+    staples_row<I, MU, (MU + 1) % 3>(A, s, x, y, z, lane, sp, sm, row, Sig);
     __builtin_amdgcn_sched_barrier(0);
-    staples_row<I, MU, (MU + 2) % 3>(A, s, x, y, z, lane, row, Sig);
+    staples_row<I, MU, (MU + 2) % 3>(A, s, x, y, z, lane, sp, sm, row, Sig);
     __builtin_amdgcn_sched_barrier(0);
     M3 Uo;
     ld_row(row[MU], lane + 1, Uo);
@@ -164,8 +166,9 @@ __device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, i
 // The hand-over is a workgroup-scope flag: wave 3 sets it unconditionally once its stores to LDS are
 // complete, so wave 2's wait always ends; the wait is capped all the same.
 //
-// ROW (nx a multiple of 64, so a tile is one row): waves 0-2 first copy U_w of the row and its two halo
-// sites into LDS, and the 18 of 41 link operands per site that lie on the row are read from there.
+// ROW (row_tiles: a tile is whole rows): waves 0-2 first copy U_w of the tile (and, for one-row tiles, the
+// two halo sites) into LDS, and the 18 of 41 link operands per site that lie on those rows are read from
+// there.
 // This takes about a third of the vector-memory accesses off the L1, whose pipeline is what the
 // kernel waits on.
 template <class I, bool ROW>
@@ -176,8 +179,13 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
     const I s = (I)(valid ? sl : 0);
     int x, y, z;
     site_coords<I>(A.g, s, x, y, z);
+    // LDS slots of x+1 and x-1: the next / previous lane, or the halo slots at the ends of a one-row tile;
+    // in a tile of several short rows, the lane of the periodic neighbour in the same row
+    const bool one_row = A.g.nx % kTile == 0;
+    const int sp = one_row ? lane + 2 : lane + 1 + (x + 1 == A.g.nx ? 1 - A.g.nx : 1);
+    const int sm = one_row ? lane : lane + 1 + (x == 0 ? A.g.nx - 1 : -1);
     if (ROW && wave < 3) {
-        // every lane of a row tile is a lattice site
+        // lanes past the end of the lattice (last tile only) copy site 0; no valid lane reads their slots
         M3 M;
         ldm<I>(A.U, kUPlanes, 18 * wave, s, M);
 #pragma unroll
@@ -185,7 +193,7 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
             sh.row[wave][2 * k][lane + 1] = M.re[k];
             sh.row[wave][2 * k + 1][lane + 1] = M.im[k];
         }
-        if (lane < 2) {
+        if (one_row && lane < 2) {
             const int x0 = x - lane;  // the tile's first x
             const int xh = lane == 0 ? xwrap(x0 - 1, A.g.nx) : xwrap(x0 + kTile, A.g.nx);
             ldm<I>(A.U, kUPlanes, 18 * wave, (I)(s - (I)x + (I)xh), M);
@@ -201,12 +209,18 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
     __syncthreads();
     if (wave == 0) {
         M3 Uh;
-        if (ROW) link_update_row<I, 0>(A, s, x, y, z, lane, sh.row, Uh);
-        else if (valid) link_update<I, 0>(A, s, x, y, z, Uh);
+        if (ROW) {
+            if (valid) link_update_row<I, 0>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
+        } else if (valid) {
+            link_update<I, 0>(A, s, x, y, z, Uh);
+        }
     } else if (wave == 1) {
         M3 Uh;
-        if (ROW) link_update_row<I, 1>(A, s, x, y, z, lane, sh.row, Uh);
-        else if (valid) link_update<I, 1>(A, s, x, y, z, Uh);
+        if (ROW) {
+            if (valid) link_update_row<I, 1>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
+        } else if (valid) {
+            link_update<I, 1>(A, s, x, y, z, Uh);
+        }
     } else if (wave == 3) {
         if (valid) {
             double jps[8], jms[8], jp[8], jm[8];
@@ -243,8 +257,11 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
         if (lane == 0) __hip_atomic_store(&sh.ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
         M3 Uh;
-        if (ROW) link_update_row<I, 2>(A, s, x, y, z, lane, sh.row, Uh);
-        else if (valid) link_update<I, 2>(A, s, x, y, z, Uh);
+        if (ROW) {
+            if (valid) link_update_row<I, 2>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
+        } else if (valid) {
+            link_update<I, 2>(A, s, x, y, z, Uh);
+        }
         wait_ready(sh);
         if (valid) {
             double jps[8], jms[8], jpsU[8], jmsU[8], jp[8], jm[8];
@@ -273,9 +290,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))
 // Ensemble step: one launch per form advances every event of a batch.  Block b works on the tile starting
 // at site block_site[b] of event e = block_event[b] (each event's blocks in the order tile_site gives it
 // alone); the event's arguments come from a device table (uniform per block, so they are read with scalar
-// loads).  The per-site arithmetic is that of k_step4.  The blocks of events with nx a multiple of 64
-// come first and run the row-staged form at three waves per SIMD, the rest the plain form at two, as
-// each runs alone.
+// loads).  The per-site arithmetic is that of k_step4.  The blocks of events whose tiles are whole rows
+// (row_tiles) come first and run the row-staged form at three waves per SIMD, the rest the plain form at
+// two, as each runs alone.
 template <int W, bool ROW>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_step_batch(const StepArgs* __restrict__ table, const int* __restrict__ block_event,
@@ -319,7 +336,7 @@ static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hi
         // the row-staged form needs whole rows per tile; GLASMA_STEP_FOUR=2 forces the plain form
         // three waves per SIMD (168 registers) measured faster for the row-staged form (30.1 against
         // 31.8 ms at 256 x 256 x 1024); the plain form spills too much at three
-        if (A.g.nx % kTile == 0 && four == 1) {
+        if (row_tiles(A.g) && four == 1) {
             if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
             else hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
         } else {

@@ -234,6 +234,10 @@ GHD void plaq_mat(const double* __restrict__ U, int kind, int m1, I s1, int m2, 
 //           walks each band through all z planes before the next one, so the few planes of one band that
 //           are in flight at once share their z neighbours in L2.
 constexpr int kXcds = 8;
+
+// whether a geometry's 64-site tiles are made of whole rows: nx a multiple of 64 (one row per tile) or a
+// divisor of 64; the step then takes its row operands from an LDS copy of the tile
+GHD bool row_tiles(const Geom& g) { return g.nx % kTile == 0 || kTile % g.nx == 0; }
 struct TileMap {
     int mode, bpb;
     int bw, bh, nbx, tpbp;
