@@ -97,6 +97,9 @@ def batched_ensemble(nevents: int, steps: int) -> dict:
                                   center_l=int(0.85 * N[2] / 2 - 1), center_r=int(1.15 * N[2] / 2 - 1))
             ev.mv_initial_state(0, mv, xl, xr)
         log(f"[bench] ensemble initial states in {time.perf_counter() - t0:.1f} s")
+        t0 = time.perf_counter()
+        b.tune(reps=2)  # one tile order per distinct lattice, used by the batched and the serial runs
+        log(f"[bench] ensemble tile orders tuned in {time.perf_counter() - t0:.1f} s")
         b.update(1)  # graph capture, warm-up
         b.update_serial(1)
         b.update(steps)
@@ -176,6 +179,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile order instead of tuning it")
     args = ap.parse_args()
 
     if args.calibrate_copy is not None:
@@ -205,6 +209,12 @@ def main():
         log(f"[bench] MV collision initial state built on the device in {time.perf_counter() - t0:.2f} s")
     else:
         g.fill_random(seed=1, link_scale=0.5, e_scale=0.1, j_scale=0.05)
+    tune = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tune = g.tune_step(reps=3)
+        log(f"[bench] tile order tuned in {time.perf_counter() - t0:.2f} s: {tune['chosen']} "
+            f"({', '.join(f'{k} {v:.2f}' for k, v in tune['candidates_ms'].items())} ms)")
     log(f"[bench] warmup {args.warmup} steps")
     g.update_async(args.warmup)
     g.synchronize()
@@ -220,6 +230,7 @@ def main():
     value = S * args.steps / wall
     achieved = BYTES_PER_SITE_UPDATE * S / (kern_ms * 1e-3) / 1e9
     umax, _ = g.unitarity()
+    g_order = g.tile_order()[0]
     log(f"[bench] {ms_per_step:.3f} ms/step wall, {kern_ms:.3f} ms/kernel, {achieved:.0f} GB/s algorithmic, "
         f"unitarity max {umax:.2e}")
     g.close()
@@ -255,7 +266,8 @@ def main():
         "data": "synthetic (" + ("MV collision initial state built on the device, g2mu 0.5, sigma 8, m 0.1, "
                                  "Lambda 1.65, charge seeds 1 / 2" if init == "mv" else "random SU(3) fields") + ")",
         "config": {"workload": "large single-event FP64 collision, 256x256x1024, a=1, dt=a/4, open z",
-                   "lattice": list(N), "sites": S, "init": init, "parallelism": "single GPU"},
+                   "lattice": list(N), "sites": S, "init": init, "parallelism": "single GPU",
+                   "tile_order": g_order, "tile_orders_ms": tune["candidates_ms"] if tune else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_step", "kernel_ms": kern_ms, "bytes_per_site_update": BYTES_PER_SITE_UPDATE},

@@ -177,6 +177,22 @@ int glasma_batch_update(glasma_batch* b, int nsteps);
 int glasma_batch_update_serial(glasma_batch* b, int nsteps);
 int glasma_batch_last_ms(const glasma_batch* b, double* ms);
 
+/* ---- tile-order autotuner (no counterpart in the reference, which has no GPU kernels) -------------------- */
+/* The order in which the step's blocks visit the lattice sets how much neighbour-link traffic the XCD-local
+ * L2 caches absorb.  glasma_tune_step times the step on the context's current fields under every order
+ * that fits the lattice (XCD column bands of several shapes walked in z, plane bands, linear), each with one
+ * warm-up and `reps` timed launches into scratch buffers -- the fields, UOld and the clock are unchanged, and
+ * no order changes any result -- and keeps the fastest for the context's later steps.  Optional outputs
+ * (arrays of at least GLASMA_TUNE_MAX entries): the candidates' codes (mode*10000 + band width in tiles *100
+ * + band height in rows), their ms per step, the index of the chosen one.  Needs 880 B per site of scratch
+ * device memory while it runs.  glasma_tile_order reports the code in use and whether it was tuned.
+ * glasma_batch_tune tunes each distinct lattice of an ensemble once (events of the same lattice share the
+ * order) and rebuilds the ensemble's work list. */
+#define GLASMA_TUNE_MAX 16
+int glasma_tune_step(glasma_ctx* ctx, int reps, int* ncand, int* codes, double* ms, int* best);
+int glasma_tile_order(const glasma_ctx* ctx, int* code, int* tuned);
+int glasma_batch_tune(glasma_batch* b, int reps);
+
 /* ---- reverse-mode derivatives (no counterpart in the reference, which has no gradient code) ------------ */
 /* Slab energy of the current fields: 1/2 sum over sites with |z - zc| < half of sum_mu c_mu (E_mu^a E_mu^a
  * + 4 Re tr(1 - P_mu)), c_mu = (a_mu / a^3)^2, P_0 = P_yz, P_1 = P_zx, P_2 = P_xy rooted at the site: the

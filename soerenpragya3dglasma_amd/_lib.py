@@ -86,6 +86,9 @@ def lib():
         "glasma_batch_update": ([vp, c_int], c_int),
         "glasma_batch_update_serial": ([vp, c_int], c_int),
         "glasma_batch_last_ms": ([vp, dp], c_int),
+        "glasma_tune_step": ([vp, c_int, ip, ip, dp, ip], c_int),
+        "glasma_tile_order": ([vp, ip, ip], c_int),
+        "glasma_batch_tune": ([vp, c_int], c_int),
         "glasma_slab_energy": ([vp, c_int, c_int, dp], c_int),
         "glasma_step_vjp": ([vp] + [dp] * 6, c_int),
         "glasma_slab_energy_from_charge": ([vp, ctypes.POINTER(MVParams), dp, c_int, c_int, c_int, dp], c_int),

@@ -21,6 +21,7 @@ struct glasma_batch {
     int* block_event = nullptr;
     unsigned* block_site = nullptr;
     long nblocks = 0;
+    std::vector<long> t0;  // first block of each event
     long nrow = 0;  // the first nrow blocks belong to events whose tiles are whole rows (row-staged form)
     std::map<int, hipGraphExec_t> graphs;  // by step count
     double last_ms = 0.0;
@@ -82,6 +83,24 @@ int graph_for(glasma_batch* b, int nsteps, hipGraphExec_t* out) {
     return 0;
 }
 
+// the (event, tile) work list: each event's blocks in the tile order it takes alone (its context's order)
+hipError_t upload_work_list(glasma_batch* b) {
+    std::vector<int> be(b->nblocks);
+    std::vector<unsigned> bs(b->nblocks);
+    for (size_t e = 0; e < b->ev.size(); e++) {
+        const Geom& g = b->ev[e]->g;
+        const TileMap tm = ctx_tile_map(b->ev[e]);
+        const long ntile = pad_sites(g.S) / kTile;
+        for (long k = 0; k < ntile; k++) {
+            be[b->t0[e] + k] = (int)e;
+            bs[b->t0[e] + k] = (unsigned)tile_site(g, tm, k);
+        }
+    }
+    hipError_t err = hipMemcpy(b->block_event, be.data(), be.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(b->block_site, bs.data(), bs.size() * sizeof(unsigned), hipMemcpyHostToDevice);
+    return err;
+}
+
 void advance_clocks(glasma_batch* b, int nsteps) {
     for (glasma_ctx* c : b->ev) {
         c->cur ^= (nsteps & 1);
@@ -138,18 +157,7 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
             nb += pad_sites(b->ev[e]->g.S) / kTile;
             if (row) nrow = nb;
         }
-    // each event's blocks in the tile order it would take alone
-    std::vector<int> be(nb);
-    std::vector<unsigned> bs(nb);
-    for (int e = 0; e < nevents; e++) {
-        const Geom& g = b->ev[e]->g;
-        const TileMap tm = default_tile_map(g);
-        const long ntile = pad_sites(g.S) / kTile;
-        for (long k = 0; k < ntile; k++) {
-            be[t0[e] + k] = e;
-            bs[t0[e] + k] = (unsigned)tile_site(g, tm, k);
-        }
-    }
+    b->t0 = t0;
     b->nblocks = nb;
     b->nrow = nrow;
     hipError_t err = hipStreamCreate(&b->stream);
@@ -158,8 +166,7 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
     if (err == hipSuccess) err = hipMalloc(&b->table, 4 * (size_t)nevents * sizeof(StepArgs));
     if (err == hipSuccess) err = hipMalloc(&b->block_event, (size_t)nb * sizeof(int));
     if (err == hipSuccess) err = hipMalloc(&b->block_site, (size_t)nb * sizeof(unsigned));
-    if (err == hipSuccess) err = hipMemcpy(b->block_event, be.data(), (size_t)nb * sizeof(int), hipMemcpyHostToDevice);
-    if (err == hipSuccess) err = hipMemcpy(b->block_site, bs.data(), (size_t)nb * sizeof(unsigned), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = upload_work_list(b);
     if (err != hipSuccess) {
         int code = err == hipErrorOutOfMemory ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP;
         glasma_batch_destroy(b);
@@ -210,6 +217,29 @@ int glasma_batch_update_serial(glasma_batch* b, int nsteps) {
         if (rc) return bfail(b, rc, std::string("event update failed: ") + glasma_last_error(c));
     }
     b->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+int glasma_batch_tune(glasma_batch* b, int reps) {
+    if (!b || reps <= 0) return GLASMA_ERR_ARG;
+    // one tuning per distinct lattice; events of the same lattice share its order
+    for (size_t e = 0; e < b->ev.size(); e++) {
+        glasma_ctx* c = b->ev[e];
+        bool done = false;
+        for (size_t f = 0; f < e && !done; f++) {
+            const Geom& g = b->ev[f]->g;
+            if (g.nx == c->g.nx && g.ny == c->g.ny && g.nz == c->g.nz) {
+                c->tm = b->ev[f]->tm;
+                c->tm_tuned = b->ev[f]->tm_tuned;
+                done = true;
+            }
+        }
+        if (done) continue;
+        const int rc = glasma_tune_step(c, reps, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return bfail(b, rc, std::string("tuning event failed: ") + glasma_last_error(c));
+    }
+    GB_HIP(b, hipStreamSynchronize(b->stream));
+    GB_HIP(b, upload_work_list(b));
     return 0;
 }
 

@@ -58,6 +58,9 @@ struct glasma_ctx {
     double* EG = nullptr;      // gauge-fixed electric field
     double* fftbuf = nullptr;  // complex work array (2 doubles per entry)
     size_t fftbuf_cap = 0;     // doubles
+    // block -> tile order of the step, when glasma_tune_step has chosen one (else the default order)
+    glasma::TileMap tm{};
+    bool tm_tuned = false;
     std::string err;
 };
 
@@ -85,6 +88,9 @@ StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old);
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
                       long nrow, long nplain);
 TileMap default_tile_map(const Geom& g);
+TileMap tile_map(const Geom& g, int mode, int bw, int bh);  // the requested order, or the next simpler one
+TileMap ctx_tile_map(const glasma_ctx* ctx);                // the context's tuned order, or the default
+int launch_step_map(glasma_ctx* ctx, const StepArgs& A, const TileMap& tm);
 
 // the left-moving MV sheet from a given charge density, and its reverse-mode derivative (mv.hip)
 struct IcTape {

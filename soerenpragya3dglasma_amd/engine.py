@@ -29,6 +29,7 @@ from . import _lib
 from ._lib import GlasmaError, MVParams, Params, check
 
 OBS_NAMES = ("T00", "TXX", "TYY", "TZZ", "T0X", "T0Y", "T0Z", "TXY", "TYZ", "TZX")
+_TUNE_MAX = 16  # GLASMA_TUNE_MAX of glasma.h
 
 
 def _p(a: np.ndarray):
@@ -167,6 +168,32 @@ class Glasma:
         ms = ctypes.c_double()
         self._chk(_lib.lib().glasma_last_step_kernel_ms(self._h, ctypes.byref(ms)), "glasma_last_step_kernel_ms")
         return ms.value
+
+    # -- tile-order autotuner (no counterpart in the reference) --------------------------------------
+    @staticmethod
+    def _order(code: int) -> str:
+        mode, bw, bh = code // 10000, (code // 100) % 100, code % 100
+        return f"bands {bw}x{bh}" if mode == 2 else ("plane bands" if mode == 1 else "linear")
+
+    def tune_step(self, reps: int = 3) -> dict:
+        """Time the step under every block -> tile order that fits the lattice (on scratch outputs: the
+        fields are unchanged) and keep the fastest for later steps.  Returns the candidates' ms per step
+        and the chosen order."""
+        n = ctypes.c_int()
+        codes = np.zeros(_TUNE_MAX, dtype=np.int32)
+        ms = np.zeros(_TUNE_MAX)
+        best = ctypes.c_int()
+        self._chk(_lib.lib().glasma_tune_step(self._h, int(reps), ctypes.byref(n),
+                                              codes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(ms),
+                                              ctypes.byref(best)), "glasma_tune_step")
+        cand = {self._order(int(c)): float(t) for c, t in zip(codes[:n.value], ms[:n.value])}
+        return {"candidates_ms": cand, "chosen": self._order(int(codes[best.value]))}
+
+    def tile_order(self) -> tuple:
+        """(order in use, whether it was tuned)."""
+        code, tuned = ctypes.c_int(), ctypes.c_int()
+        self._chk(_lib.lib().glasma_tile_order(self._h, ctypes.byref(code), ctypes.byref(tuned)), "glasma_tile_order")
+        return self._order(code.value), bool(tuned.value)
 
     # -- observables --------------------------------------------------------------------------------
     def bulk_raw(self) -> np.ndarray:
@@ -337,6 +364,11 @@ class GlasmaBatch:
         ms = ctypes.c_double()
         self._chk(_lib.lib().glasma_batch_last_ms(self._h, ctypes.byref(ms)), "glasma_batch_last_ms")
         return ms.value
+
+    def tune(self, reps: int = 3):
+        """Tune the step's tile order once per distinct lattice of the ensemble (Glasma.tune_step) and
+        rebuild the work list; the events' fields are unchanged."""
+        self._chk(_lib.lib().glasma_batch_tune(self._h, int(reps)), "glasma_batch_tune")
 
     def close(self):
         if getattr(self, "_h", None):
