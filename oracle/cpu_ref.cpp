@@ -25,6 +25,7 @@
 //   * MT19937                                        -> 10000th output KAT 4123659995, numpy RandomState
 //   * Gaussian draw (GSL gsl_ran_gaussian)           -> parity unpinned (GSL absent; published polar method)
 //   * Poisson / smoothing / z-evolution              -> single-mode closed forms
+//   * sheet from a given charge density              -> equals the MV left mover for the SetChargeDensities density
 //   * Coulomb gauge fixing                           -> gauge orbit recovery, residual <= tol, invariants
 #include <cmath>
 #include <complex>
@@ -960,21 +961,21 @@ static void SetNewWilsonLines(std::vector<cd>& F, const MVParams& P, int z, char
 
 // CollidingObjects::SetWilsonLines(rhoOld, rho, V, D, Center, QsA)  :318-371 with the charges xi already
 // drawn (xi[a + 8*(x + Nx*y)] = QsA*Gauss(), drawn in y,x,a order).  rhoOld/rho/V carry z ghost layers.
-static void SetWilsonLines(const MVParams& P, const double* xi, char D, int center, double* rhoOld, double* rho, cd* V) {
+// The per-z part of CollidingObjects::SetWilsonLines (COLLIDINGOBJECTS/CollidingObjects.cpp:318-371) for a
+// given charge density rho_in (a + 8*(x + Nx*(y + Ny*(z+1))), the density SetChargeDensities :34-102
+// produces): rhoOld = rho_in, smoothing, Poisson solve (SolveLaplaceEquation :131-176) and the Wilson-line
+// product (SetNewWilsonLines :179-226), scanning z down for 'L' and up for 'R'.
+static void SetWilsonLinesRho(const MVParams& P, const double* rho_in, char D, double* rhoOld, double* rho, cd* V) {
     int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
     long np = (long)Nx * Ny;
     std::vector<cd> Wil(9 * np);
     for (long i = 0; i < np; i++) SU3::Unit(&Wil[9 * i]);
     std::vector<cd> F(8 * np);
-    double norm = std::sqrt(P.a[0] * P.a[1]);
     double nf = 1.0 / ((double)Nx * Ny);
     auto one_z = [&](int z) {
-        // SetChargeDensities :34-102
-        double prof = LongitudinalProfile(z, center, D == 'L' ? P.sigma_l : P.sigma_r);
-        double mult = (D == 'L') ? P.nucleus_l : P.nucleus_r;
         for (long i = 0; i < np; i++)
             for (int a = 0; a < 8; a++) {
-                double v = mult * norm * prof * xi[a + 8 * i];
+                double v = rho_in[a + 8 * (i + np * (z + 1))];
                 rhoOld[a + 8 * (i + np * (z + 1))] = v;
                 F[a + 8 * i] = v;
             }
@@ -997,6 +998,22 @@ static void SetWilsonLines(const MVParams& P, const double* xi, char D, int cent
     };
     if (D == 'R') for (int z = 0; z < Nz; z++) one_z(z);
     else for (int z = Nz - 1; z >= 0; z--) one_z(z);
+}
+
+// CollidingObjects::SetWilsonLines: the charge of SetChargeDensities :34-102 (mult * sqrt(a0 a1) *
+// LongitudinalProfile(z) * xi, LongitudinalProfile :30-32), then SetWilsonLinesRho.
+static void SetWilsonLines(const MVParams& P, const double* xi, char D, int center, double* rhoOld, double* rho, cd* V) {
+    int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    long np = (long)Nx * Ny;
+    double norm = std::sqrt(P.a[0] * P.a[1]);
+    double mult = (D == 'L') ? P.nucleus_l : P.nucleus_r;
+    std::vector<double> rin(8 * np * (Nz + 2), 0.0);
+    for (int z = 0; z < Nz; z++) {
+        double prof = LongitudinalProfile(z, center, D == 'L' ? P.sigma_l : P.sigma_r);
+        for (long i = 0; i < np; i++)
+            for (int a = 0; a < 8; a++) rin[a + 8 * (i + np * (z + 1))] = mult * norm * prof * xi[a + 8 * i];
+    }
+    SetWilsonLinesRho(P, rin.data(), D, rhoOld, rho, V);
 }
 
 // SetWilsonLinesForFirstStep :377-411 (Poisson solve of the evolved charge, no smoothing)
@@ -1383,6 +1400,22 @@ void ref_mv_initial_state(int driver, const int* N, const double* a, int bc, dou
         SingleSheet(P, rhoR.data(), 0, N[2] - 1, RhoRT1.data(), VR.data(), 'R', JmS, Jm, (cd*)U, (cd*)UOld, E);
     }
     L.syncJ(JpS); L.syncJ(Jp); L.syncJ(JmS); L.syncJ(Jm);
+    L.syncU((cd*)U); L.syncU((cd*)UOld); L.syncE(E);
+}
+// The left-moving sheet over all z (the driver of SC/LeftNucleusSimulation.cpp:357, i.e. SingleSheet with 'L'
+// over [0, Nz-1]) built from a given charge density rho_in instead of MV draws.  Fields pre-initialised
+// as for ref_mv_initial_state.
+void ref_left_sheet_from_charge(const int* N, const double* a, int bc, double dt, double lambda, double meff,
+                                const double* rho_in, double* U, double* UOld, double* E, double* Jp, double* JpS) {
+    Lat<SU3> L = mk3(N, a, bc);
+    MVParams P{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, dt, bc, lambda, meff, 0, 0, 1, 1};
+    long np = L.plane(), nzg = N[2] + 2;
+    std::vector<double> rhoOld(8 * np * nzg, 0.0), rho(8 * np * nzg, 0.0), RhoT1(8 * np * nzg, 0.0);
+    std::vector<cd> V(9 * np * nzg);
+    for (long i = 0; i < np * nzg; i++) SU3::Unit(&V[9 * i]);
+    SetWilsonLinesRho(P, rho_in, 'L', rhoOld.data(), rho.data(), V.data());
+    SingleSheet(P, rho.data(), 0, N[2] - 1, RhoT1.data(), V.data(), 'L', JpS, Jp, (cd*)U, (cd*)UOld, E);
+    L.syncJ(JpS); L.syncJ(Jp);
     L.syncU((cd*)U); L.syncU((cd*)UOld); L.syncE(E);
 }
 // --- gauge fixing ---

@@ -64,6 +64,8 @@ def lib():
         L.ref_mv_initial_state.argtypes = ([ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp] + [_dp] * 7)
+        L.ref_left_sheet_from_charge.argtypes = [_ip, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_double, _dp] + [_dp] * 5
         L.ref_coulomb_gauge.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_double, _dp]
         L.ref_gauge_transform.argtypes = [_ip, _dp, ctypes.c_int, _dp] + [_dp] * 6
@@ -246,6 +248,19 @@ def mv_initial_state(driver: int, N, a, dt, xi_l, xi_r, *, lam, meff, sigma_l, s
     lib().ref_mv_initial_state(driver, ivec(N), dvec(a), bc, dt, lam, meff, sigma_l, sigma_r, mult_l, mult_r,
                                int(center_l), int(center_r), P(xl), P(xr), P(f.U), P(f.UOld), P(f.E), P(f.Jp),
                                P(f.Jm), P(f.JpS), P(f.JmS))
+    return f
+
+
+def left_sheet_from_charge(N, a, dt, rho, *, lam, meff, bc=0) -> Fields:
+    """The left-moving sheet over all z (SingleSheet 'L', the driver of SC/LeftNucleusSimulation.cpp:357)
+    built from a given charge density rho shaped (Nz, Ny, Nx, 8): the density SetChargeDensities
+    (COLLIDINGOBJECTS/CollidingObjects.cpp:34-102) would produce, before smoothing."""
+    f = Fields(N, 3)
+    Nx, Ny, Nz = N
+    r = np.zeros((Nz + 2, Ny, Nx, 8))
+    r[1:-1] = np.asarray(rho, np.float64).reshape(Nz, Ny, Nx, 8)
+    lib().ref_left_sheet_from_charge(ivec(N), dvec(a), bc, dt, lam, meff, P(r), P(f.U), P(f.UOld), P(f.E), P(f.Jp),
+                                     P(f.JpS))
     return f
 
 

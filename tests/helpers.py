@@ -92,3 +92,38 @@ def hostcheck_lib():
             getattr(lib, name).argtypes = args
         _hc = lib
     return _hc
+
+
+def slab_energy_np(f, a, zc, half):
+    """Energy of the layers |z - zc| < half from host fields in the reference layout (numpy, independent of the
+    engine): 1/2 sum over those sites of sum_mu c_mu (E_mu^a E_mu^a + 4 Re tr(1 - P_mu)), c_mu = (a_mu / a^3)^2,
+    P_mu the plaquette in the plane (mu+1, mu+2) rooted at the site, z neighbours from the ghost layers."""
+    Nx, Ny, Nz = f.N
+    U = np.swapaxes(f.U, -1, -2)  # [z+1, y, x, mu, i, j]
+    acube = a[0] * a[1] * a[2]
+    c = [(a[m] / acube) ** 2 for m in range(3)]
+
+    def at(mu, zg, d=None):
+        """U_mu on layer zg, shifted by +1 in direction d (0: x, 1: y, 2: z)."""
+        if d == 2:
+            return U[zg + 1, :, :, mu]
+        M = U[zg, :, :, mu]
+        if d == 0:
+            return np.roll(M, -1, axis=1)
+        if d == 1:
+            return np.roll(M, -1, axis=0)
+        return M
+
+    dag = lambda M: np.conj(np.swapaxes(M, -1, -2))
+    tot = 0.0
+    for z in range(Nz):
+        if not abs(z - zc) < half:
+            continue
+        zg = z + 1
+        for mu in range(3):
+            j, k = (mu + 1) % 3, (mu + 2) % 3
+            P = at(j, zg) @ at(k, zg, j) @ dag(at(j, zg, k)) @ dag(at(k, zg))
+            retr = 3.0 - np.trace(P, axis1=-2, axis2=-1).real
+            e2 = (f.E[zg, :, :, mu, :] ** 2).sum(axis=-1)
+            tot += 0.5 * c[mu] * float((e2 + 4.0 * retr).sum())
+    return tot

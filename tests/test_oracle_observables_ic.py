@@ -13,7 +13,10 @@ SUNcDynamics.cpp:275-350):
 * smoothing of a single mode: rho exp(-p^2/2 Lambda^2) p^2/(p^2 + m^2);
 * EvolveRhoViaFourier of a single z mode: cos(k z -+ w dt) (pure phase);
 * the Wilson line of one transverse site against scipy.linalg.expm of the Poisson solution;
-* the sheet satisfies the Gauss law exactly.
+* the sheet satisfies the Gauss law exactly;
+* the sheet from a given charge density (the oracle side of the gradient checks) equals the MV left mover
+  for the density SetChargeDensities forms;
+* the numpy slab energy of the gradient checks, over all layers, equals the oracle's bulk T00.
 Coulomb gauge fixing (CoulombGaugeFixing.cpp, FourierAcceleration.cpp): converges for Nz = 1, keeps
 gauge-invariant observables, recovers a gauge-rotated copy.
 """
@@ -228,3 +231,29 @@ def test_coulomb_gauge_recovers_rotated_copy():
     assert np.allclose(O.bulk(f2, a)[[0, 1, 2, 5]], O.bulk(fc, a)[[0, 1, 2, 5]], rtol=1e-12)
     # the Coulomb-gauge copy is unique up to a global rotation: plaquette traces per site agree
     assert abs(st2[6] - st[6]) < 1e-10 * abs(st[6])
+
+
+def test_left_sheet_from_charge_is_the_mv_left_mover():
+    """The oracle's sheet from a given charge density equals its MV left mover when the density is the one
+    SetChargeDensities forms from the same draws (mult * sqrt(a0 a1) * profile(z) * xi)."""
+    N, a, dt, c, sig = (6, 5, 12), (1.0, 0.9, 0.8), 0.25, 4, 2.0
+    xl, xr = O.draw_charges(3, N[0], N[1], 0.6, 0.6)
+    f1 = O.mv_initial_state(1, N, a, dt, xl, xr, lam=1.65, meff=0.2, sigma_l=sig, sigma_r=sig, center_l=c, center_r=c)
+    z = np.arange(N[2])
+    prof = np.exp(-0.5 * ((z - c) / sig) ** 2) / np.sqrt(2 * np.pi * sig * sig)
+    rho = np.sqrt(a[0] * a[1]) * prof[:, None, None, None] * np.asarray(xl).reshape(N[1], N[0], 8)[None]
+    f2 = O.left_sheet_from_charge(N, a, dt, rho, lam=1.65, meff=0.2)
+    for k in ("U", "UOld", "E", "Jp", "JpS"):
+        assert np.abs(getattr(f1, k) - getattr(f2, k)).max() < 1e-13, k
+
+
+def test_numpy_slab_energy_covering_the_lattice_is_bulk_T00():
+    from tests.helpers import random_fields, slab_energy_np
+    N, a = (5, 4, 6), (1.0, 0.9, 0.8)
+    f = random_fields(N, seed=2)
+    O.update(f, a, 0.1, 1)
+    T00 = O.bulk_T(O.bulk(f, a), a)["T00"]
+    assert abs(slab_energy_np(f, a, 3, 10) - T00) < 1e-12 * abs(T00)
+    # a slab is the sum of its layers
+    parts = sum(slab_energy_np(f, a, z, 1) for z in range(N[2]))
+    assert abs(parts - T00) < 1e-12 * abs(T00)
