@@ -270,7 +270,7 @@ def main():
                    "tile_order": g_order, "tile_orders_ms": tune["candidates_ms"] if tune else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_step", "kernel_ms": kern_ms, "bytes_per_site_update": BYTES_PER_SITE_UPDATE},
+                     "kernel": "k_step4", "kernel_ms": kern_ms, "bytes_per_site_update": BYTES_PER_SITE_UPDATE},
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,

@@ -6,8 +6,7 @@
 // half-step link.  Splitting a site over wavefronts keeps each wavefront's live state to a few matrices.
 // When a tile is made of whole lattice rows (nx a multiple or a divisor of 64), the link operands on those
 // rows (own links and their x neighbours, 18 of the 41 matrix loads per site) come from an LDS copy made
-// once per block; that form runs three wavefronts per SIMD, the plain form two.  The older three-wavefront
-// kernel (k_step) is kept for measurements.
+// once per block; that form runs three wavefronts per SIMD, the plain form two.
 //
 // HBM traffic per site-update: read U (432 B) + E (192 B) + J (256 B), write the same: 1760 B, the
 // compulsory minimum.  The neighbour link matrices are re-read through L2 / MALL, so the order in
@@ -16,7 +15,7 @@
 // its bands of 4 x 4 tiles through z, so the y and z neighbours of a tile are read by the same XCD
 // shortly before or after it (L2 hits); HBM reads come to 1.25x the compulsory bytes.
 //
-// GLASMA_STEP_WAVES (2 or 3: register budget of 256 or 168 per lane), GLASMA_STEP_FOUR and
+// GLASMA_STEP_WAVES (2 or 3: register budget of 256 or 168 per lane), GLASMA_STEP_FOUR=2 (plain form only) and
 // GLASMA_STEP_MAP / _BW / _BH select variants for measurement sweeps; the defaults are the fastest
 // measured on MI355X.
 #include <cstdlib>
@@ -30,55 +29,6 @@ static int env_int(const char* name, int dflt) {
     return v ? std::atoi(v) : dflt;
 }
 
-
-template <class I, int W>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, W))) k_step(StepArgs A, TileMap tm) {
-    __shared__ double sh[4][8][kTile];  // J+stat'(x), J-stat'(x), J+stat'(x+z), J-stat'(x+z)
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long sl = tile_site(A.g, tm, blockIdx.x) + lane;
-    const bool valid = sl < A.g.S;
-    const I s = (I)(valid ? sl : 0);
-    int x, y, z;
-    site_coords<I>(A.g, s, x, y, z);
-    M3 Uh;
-    if (wave == 0) {
-        if (valid) {
-            link_update<I, 0>(A, s, x, y, z, Uh);
-            double jps[8], jms[8];
-            const int zm = zdn(A.g, z);
-            jstat_new<I>(A, s, (I)(sl + A.g.P * (long)(zm - z)), jps, jms);
-            stv<I>(A.Jn, kJPlanes, JPS * 8, s, jps);
-            stv<I>(A.Jn, kJPlanes, JMS * 8, s, jms);
-            for (int a = 0; a < 8; a++) {
-                sh[0][a][lane] = jps[a];
-                sh[1][a][lane] = jms[a];
-            }
-        }
-    } else if (wave == 1) {
-        if (valid) {
-            link_update<I, 1>(A, s, x, y, z, Uh);
-            double ju[8], jv[8];
-            jstat_new_up<I>(A, s, z, ju, jv);
-            for (int a = 0; a < 8; a++) {
-                sh[2][a][lane] = ju[a];
-                sh[3][a][lane] = jv[a];
-            }
-        }
-    } else {
-        if (valid) link_update<I, 2>(A, s, x, y, z, Uh);
-    }
-    __syncthreads();
-    if (wave == 2 && valid) {
-        double jps[8], jms[8], jpsU[8], jmsU[8];
-        for (int a = 0; a < 8; a++) {
-            jps[a] = sh[0][a][lane];
-            jms[a] = sh[1][a][lane];
-            jpsU[a] = sh[2][a][lane];
-            jmsU[a] = sh[3][a][lane];
-        }
-        j_advect<I>(A, s, Uh, jps, jms, jpsU, jmsU);
-    }
-}
 
 // The links of a tile's sites in LDS, slot 1 + lane, when the tile is made of whole rows: nx a multiple of
 // 64 (one row; slot 0 then holds x0-1 and slot 65 x0+64, x periodic) or a divisor of 64 (64 / nx rows,
@@ -332,21 +282,16 @@ StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old) {
 template <class I>
 static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hipStream_t st) {
     dim3 grid((unsigned)(pad_sites(A.g.S) / kTile));
-    if (four) {
-        // the row-staged form needs whole rows per tile; GLASMA_STEP_FOUR=2 forces the plain form
-        // three waves per SIMD (168 registers) measured faster for the row-staged form (30.1 against
-        // 31.8 ms at 256 x 256 x 1024); the plain form spills too much at three
-        if (row_tiles(A.g) && four == 1) {
-            if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
-            else hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
-        } else {
-            if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, false>), grid, dim3(256), 0, st, A, tm);
-            else hipLaunchKernelGGL((k_step4<I, 2, false>), grid, dim3(256), 0, st, A, tm);
-        }
-        return;
+    // the row-staged form needs whole rows per tile (GLASMA_STEP_FOUR=2 forces the plain form); three
+    // waves per SIMD (168 registers) measured faster for it (30.1 against 31.8 ms at 256 x 256 x 1024),
+    // the plain form spills at three and runs at two
+    if (row_tiles(A.g) && four != 2) {
+        if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
+        else hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
+    } else {
+        if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, false>), grid, dim3(256), 0, st, A, tm);
+        else hipLaunchKernelGGL((k_step4<I, 2, false>), grid, dim3(256), 0, st, A, tm);
     }
-    if (waves == 3) hipLaunchKernelGGL((k_step<I, 3>), grid, dim3(192), 0, st, A, tm);
-    else hipLaunchKernelGGL((k_step<I, 2>), grid, dim3(192), 0, st, A, tm);
 }
 
 // tile order for a geometry: the requested mode when the lattice fits it, else the next simpler one
