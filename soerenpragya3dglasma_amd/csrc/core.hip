@@ -461,21 +461,27 @@ extern "C" {
 
 const char* glasma_version(void) { return "glasma-mi355x 0.1 (gfx950, FP64)"; }
 
+// the message of the last failed glasma_create on this thread (there is no context to hold it)
+static thread_local std::string g_create_err;
+
+static int create_fail(int code, const std::string& msg) {
+    g_create_err = msg;
+    return code;
+}
+
 const char* glasma_last_error(const glasma_ctx* ctx) {
-    static thread_local std::string create_err;
-    if (!ctx) return create_err.c_str();
+    if (!ctx) return g_create_err.c_str();
     return ctx->err.c_str();
 }
 
-static thread_local std::string g_create_err;
-
 int glasma_create(const glasma_params* p, glasma_ctx** out) {
-    if (!p || !out) return GLASMA_ERR_ARG;
+    if (!p || !out) return create_fail(GLASMA_ERR_ARG, "glasma_create: null argument");
     *out = nullptr;
-    if (p->nc != 3) return GLASMA_ERR_ARG;
+    if (p->nc != 3) return create_fail(GLASMA_ERR_ARG, "glasma_create: only SU(3) (nc = 3) is supported");
     for (int m = 0; m < 3; m++)
-        if (p->n[m] <= 0 || !(p->a[m] > 0.0)) return GLASMA_ERR_ARG;
-    if (p->bc_z != 0 && p->bc_z != 1) return GLASMA_ERR_ARG;
+        if (p->n[m] <= 0 || !(p->a[m] > 0.0))
+            return create_fail(GLASMA_ERR_ARG, "glasma_create: lattice sizes and spacings must be positive");
+    if (p->bc_z != 0 && p->bc_z != 1) return create_fail(GLASMA_ERR_ARG, "glasma_create: bc_z must be 0 or 1");
     glasma_ctx* ctx = new glasma_ctx();
     ctx->p = *p;
     ctx->g.nx = p->n[0];
@@ -498,13 +504,15 @@ int glasma_create(const glasma_params* p, glasma_ctx** out) {
     if (e != hipSuccess) {
         int code = (e == hipErrorOutOfMemory) ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP;
         glasma_destroy(ctx);
-        return code;
+        return create_fail(code, std::string("glasma_create: ") + hipGetErrorString(e));
     }
     int rc = reset_fields(ctx);
     if (rc) {
+        const std::string msg = std::string("glasma_create: ") + ctx->err;
         glasma_destroy(ctx);
-        return rc;
+        return create_fail(rc, msg);
     }
+    g_create_err.clear();
     *out = ctx;
     return 0;
 }

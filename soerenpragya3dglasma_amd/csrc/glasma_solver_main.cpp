@@ -1,0 +1,240 @@
+// glasma_solver: the reference's command-line simulation drivers on top of the C ABI (include/glasma.h).
+//
+// One program for the three drivers the reference selects at compile time (SUNcSolver.cpp:16-24):
+// collision (Simulation.cpp), left-moving (LeftNucleusSimulation.cpp) and right-moving
+// (RightNucleusSimulation.cpp) nucleus.  Command-line keys, defaults, the run cadence and the text
+// outputs follow the reference:
+//   keys      CommandlineArguments.cpp:4-116 (-o -N -Nx -Ny -Nz -a -ax -ay -az -dT -RlOverAzL -RlOverAzR -QsA
+//             -MEff -Lambda -SEED -NucleusL -NucleusR), defaults LATTICE/3DGrid.cpp:20-23,
+//             LATTICE/Parameters.cpp:4-48, DYNAMICS/SUNcDynamics.cpp:30;
+//   run       Simulation.cpp:327-393 (charges of both nuclei from one stream, seed -SEED, scaled by Qs0A_L;
+//             initial state; initial Gauss profile, unitarity, EMT map and Gauss record; Evolve(1100));
+//   evolve    Simulation.cpp:212-322: every step Update; every 20 steps the bulk observables into
+//             EnergyID<seed>.txt and the Gauss-law record into GausCheck<seed>.txt; every 100 steps the
+//             energy-momentum profile EMTDynT<t>ID<seed>.txt; every 1000 steps the unitarity check.
+// Project keys: -driver collision|left|right, -QsAL (the charge scale, Parameters::Qs0A_L, which the
+// reference's command line does not reach), -tmax (Evolve's end time, 1100), -steps (stop after this many
+// steps), -bc 0|1 (open / periodic z), -tune 0|1 (time the step's tile orders first), -block n (steps
+// per glasma_update call between observable points; at most 20).
+// Errors print the library's message and exit with status 1 (the reference exits with 0 on errors).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/glasma.h"
+
+namespace {
+
+struct Args {
+    std::map<std::string, std::string> kv;
+    bool has(const char* k) const { return kv.count(k) != 0; }
+    double num(const char* k, double d) const { return has(k) ? std::atof(kv.at(k).c_str()) : d; }
+    long integer(const char* k, long d) const { return has(k) ? std::atol(kv.at(k).c_str()) : d; }
+    std::string str(const char* k, const char* d) const { return has(k) ? kv.at(k) : std::string(d); }
+};
+
+[[noreturn]] void die(const std::string& what, const glasma_ctx* ctx, int rc = 0) {
+    std::cerr << "#ERROR " << what << " (status " << rc << "): " << glasma_last_error(ctx) << std::endl;
+    std::exit(1);
+}
+
+// the reference's StringManipulation::StringCast: default stream formatting of each piece
+template <class... T>
+std::string cat(const T&... x) {
+    std::stringstream ss;
+    (ss << ... << x);
+    return ss.str();
+}
+
+struct Bulk {
+    double T00, TXX, TYY, TZZ, SOURCE;
+};
+
+// Observables::Bulk's T00..SOURCE from the raw sums (OBSERVABLES/BulkObservables.cpp:22-62)
+Bulk bulk(glasma_ctx* ctx, const double* a) {
+    double raw[9];
+    if (glasma_bulk(ctx, raw)) die("glasma_bulk", ctx);
+    const double acube = a[0] * a[1] * a[2];
+    double t[3], src = 0.0;
+    for (int m = 0; m < 3; m++) {
+        const double c = (a[m] / acube) * (a[m] / acube);
+        t[m] = c * raw[m] + c * raw[3 + m];
+        src += a[m] / (acube * acube) * raw[6 + m];
+    }
+    return {0.5 * (t[0] + t[1] + t[2]), 0.5 * (t[1] + t[2] - t[0]), 0.5 * (t[0] + t[2] - t[1]),
+            0.5 * (t[0] + t[1] - t[2]), src};
+}
+
+double now(const glasma_ctx* ctx) {
+    double t;
+    long n;
+    glasma_get_time(ctx, &t, &n);
+    return t;
+}
+
+long steps_done(const glasma_ctx* ctx) {
+    double t;
+    long n;
+    glasma_get_time(ctx, &t, &n);
+    return n;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Args A;
+    for (int i = 1; i + 1 < argc; i += 2) {
+        if (argv[i][0] != '-') {
+            std::cerr << "#ERROR expected -key value pairs, got " << argv[i] << std::endl;
+            return 2;
+        }
+        A.kv[argv[i] + 1] = argv[i + 1];
+    }
+    // lattice (LATTICE/3DGrid.cpp:20-23; -N sets all three, -Nx/-Ny/-Nz override)
+    const long Ns = A.integer("N", -1);
+    glasma_params p;
+    p.nc = 3;
+    const long n_def[3] = {128, 128, 2048};
+    const char* nkey[3] = {"Nx", "Ny", "Nz"};
+    for (int d = 0; d < 3; d++) p.n[d] = (int)A.integer(nkey[d], Ns > 0 ? Ns : n_def[d]);
+    const double as = A.num("a", -1.0);
+    const double a_def[3] = {1.0, 1.0, 0.25};
+    const char* akey[3] = {"ax", "ay", "az"};
+    for (int d = 0; d < 3; d++) p.a[d] = A.num(akey[d], as > 0 ? as : a_def[d]);
+    p.dt = A.num("dT", 0.08);
+    p.bc_z = (int)A.integer("bc", 0);
+    // MV parameters (LATTICE/Parameters.cpp:4-28: Qs0A = 0.33, Lambda = 5 Qs0A, MEff = Qs0A at the default
+    // spacings; the charges are scaled by Qs0A_L = 0.125 / sqrt(ax ay))
+    const double qs0a = A.num("QsA", 0.33);
+    const double qsl = A.num("QsAL", 0.125 / std::sqrt(p.a[0] * p.a[1]));
+    glasma_mv_params mv;
+    mv.lambda = A.num("Lambda", 5 * 0.33);
+    mv.m_eff = A.num("MEff", 0.33);
+    mv.sigma_l = A.num("RlOverAzL", 16.0);
+    mv.sigma_r = A.num("RlOverAzR", 16.0);
+    mv.nucleus_l = A.num("NucleusL", 1.0);
+    mv.nucleus_r = A.num("NucleusR", 1.0);
+    mv.center_l = (int)(0.85 * p.n[2] / 2 - 1);
+    mv.center_r = (int)(1.15 * p.n[2] / 2 - 1);
+    const long seed = A.integer("SEED", 0);
+    const std::string drv = A.str("driver", "collision");
+    const int driver = drv == "left" ? 1 : (drv == "right" ? 2 : 0);
+    if (driver == 0 && drv != "collision") {
+        std::cerr << "#ERROR -driver must be collision, left or right" << std::endl;
+        return 2;
+    }
+    const double tmax = A.num("tmax", 1100.0);
+    const long max_steps = A.integer("steps", -1);
+    const int block = (int)std::max(1L, std::min(20L, A.integer("block", 20)));
+    std::string out = A.str("o", "OUTPUT");
+    std::filesystem::create_directories(out);
+    if (!out.empty() && out.back() != '/') out += '/';
+
+    std::cerr << "#GAUGE GROUP IS SU(3)" << std::endl;
+    std::cerr << "## LATTICE SIZE IS " << p.n[0] << "x" << p.n[1] << "x" << p.n[2] << std::endl;
+    std::cerr << "## LATTICE SPACINGS ARE " << p.a[0] << " " << p.a[1] << " " << p.a[2] << std::endl;
+    std::cerr << "## DYNAMICAL TIME STEP IS " << p.dt << std::endl;
+    glasma_ctx* ctx = nullptr;
+    if (const int rc = glasma_create(&p, &ctx)) die("glasma_create", ctx, rc);
+    std::cerr << "#STARTING ID=" << seed << " (" << drv << ")" << std::endl;
+
+    // info file (Simulation.cpp:133-158)
+    {
+        std::ofstream f(cat(out, "InfoID", seed, ".txt"));
+        f.precision(15);
+        f << "#LATTICE DATA" << std::endl;
+        f << "#Nx,Ny,Nz= " << p.n[0] << " " << p.n[1] << " " << p.n[2] << std::endl;
+        f << "#ax,ay,az= " << p.a[0] << " " << p.a[1] << " " << p.a[2] << std::endl;
+        f << "#dT= " << p.dt << std::endl;
+        f << "    " << std::endl;
+        f << "QsAperp= " << qs0a << std::endl;
+        f << "Qs0A_L (charge scale)= " << qsl << std::endl;
+        f << "MEff aka Infrared Cutoff= " << mv.m_eff / qs0a << " (times QsAperp)" << std::endl;
+        f << "Lambda aka UV Cutoff= " << mv.lambda / qs0a << " (times QsAperp)" << std::endl;
+        f << "SigmaLOverAz L= " << mv.sigma_l << std::endl;
+        f << "SigmaLOverAz R= " << mv.sigma_r << std::endl;
+        f << "driver= " << drv << std::endl;
+    }
+
+    // initial state (Simulation.cpp:337-378)
+    std::vector<double> xl(8L * p.n[0] * p.n[1]), xr(8L * p.n[0] * p.n[1]);
+    if (glasma_draw_charges((unsigned long)seed, p.n[0], p.n[1], qsl, qsl, xl.data(), xr.data()))
+        die("glasma_draw_charges", ctx);
+    if (glasma_mv_initial_state(ctx, driver, &mv, xl.data(), xr.data())) die("glasma_mv_initial_state", ctx);
+    if (A.integer("tune", 0)) {
+        if (glasma_tune_step(ctx, 3, nullptr, nullptr, nullptr, nullptr)) die("glasma_tune_step", ctx);
+    }
+    const long Nz = p.n[2];
+    std::vector<double> per_z(Nz), emt(10 * Nz);
+    double gmax, grms, umax, urms;
+    auto gauss_record = [&]() {  // GaussLaw::SaveViolation("GausCheck") (GaussLawViolation.cpp:141-164)
+        if (glasma_gauss(ctx, nullptr, &gmax, &grms)) die("glasma_gauss", ctx);
+        std::ofstream f(cat(out, "GausCheck", seed, ".txt"), std::ios_base::app);
+        f.precision(15);
+        f << now(ctx) << " " << gmax << "  " << grms << std::endl;
+    };
+    auto emt_map = [&]() {  // EnergyMomentumTensor::CreateMap("EMTDynT<t>") (EnergyMomentumTensor.cpp:92-170)
+        if (glasma_emt(ctx, emt.data())) die("glasma_emt", ctx);
+        std::ofstream f(cat(out, "EMTDynT", now(ctx), "ID", seed, ".txt"));
+        f.precision(15);
+        for (long z = 0; z < Nz; z++) {
+            f << z;
+            for (int k = 0; k < 10; k++) f << "  " << emt[10 * z + k];
+            f << std::endl;
+        }
+    };
+    {  // GaussLaw::CheckViolation("GVDynT<t>") per z (GaussLawViolation.cpp:16-74)
+        if (glasma_gauss(ctx, per_z.data(), &gmax, &grms)) die("glasma_gauss", ctx);
+        std::ofstream f(cat(out, "GVDynT", now(ctx), "ID", seed, ".txt"));
+        f.precision(15);
+        for (long z = 0; z < Nz; z++) f << z << "  " << per_z[z] << std::endl;
+        std::cerr << "#GAUSS LAW VIOLATION AT T=" << now(ctx) << " MAX=" << gmax << " AVG=" << grms << std::endl;
+    }
+    if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);
+    std::cerr << "#UNITARITY VIOLATION AT T=" << now(ctx) << " MAX=" << umax << " AVG=" << urms << std::endl;
+    emt_map();
+    gauss_record();
+
+    // Evolve (Simulation.cpp:212-322)
+    std::ofstream energy(cat(out, "EnergyID", seed, ".txt"));
+    energy.precision(15);
+    {
+        const Bulk b = bulk(ctx, p.a);
+        energy << now(ctx) << " " << b.T00 << " " << b.TXX << " " << b.TYY << " " << b.TZZ << " " << b.SOURCE << std::endl;
+    }
+    std::cerr << "#STARTING TIME EVOLUTION" << std::endl;
+    while (now(ctx) < tmax && (max_steps < 0 || steps_done(ctx) < max_steps)) {
+        // advance to the next observable point (a multiple of `block` steps), or by one step near the end
+        long n = block - steps_done(ctx) % block;
+        if (max_steps >= 0) n = std::min(n, max_steps - steps_done(ctx));
+        if (now(ctx) + (n - 1) * p.dt >= tmax) n = 1;
+        if (glasma_update(ctx, (int)n)) die("glasma_update", ctx);
+        const long s = steps_done(ctx);
+        if (s % 20 == 0) {
+            const Bulk b = bulk(ctx, p.a);
+            std::cerr << s << "  " << now(ctx) << " " << b.T00 << " " << b.SOURCE << std::endl;
+            energy << s << "  " << now(ctx) << "  " << b.T00 << "  " << b.SOURCE << std::endl;
+            gauss_record();
+        }
+        if (s % 100 == 0) {
+            std::cerr << "#SAVING OBSERVABLES AT T=" << s << "  " << now(ctx) << std::endl;
+            emt_map();
+        }
+        if (s % 1000 == 0) {
+            if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);
+            std::cerr << "#UNITARITY VIOLATION AT T=" << now(ctx) << " MAX=" << umax << " AVG=" << urms << std::endl;
+        }
+    }
+    energy.close();
+    std::cerr << "#COMPLETED ID=" << seed << " after " << steps_done(ctx) << " steps, t=" << now(ctx) << std::endl;
+    glasma_destroy(ctx);
+    return 0;
+}

@@ -40,3 +40,23 @@ def test_argument_errors_are_reported_without_a_device():
     assert L.glasma_draw_charges(1, 0, 4, 0.5, 0.5, x.ctypes.data_as(_lib.dp), x.ctypes.data_as(_lib.dp)) == -1
     assert L.glasma_version().decode().startswith("glasma")
     assert L.glasma_destroy(None) == 0
+
+
+def test_creation_errors_are_reported_without_a_context():
+    """glasma_create checks its parameters before touching a device, and glasma_last_error(NULL) explains
+    a failed creation."""
+    import ctypes
+    from soerenpragya3dglasma_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    p = _lib.Params()
+    p.nc = 2
+    for k in range(3):
+        p.n[k], p.a[k] = 4, 1.0
+    p.dt = 0.1
+    assert L.glasma_create(ctypes.byref(p), ctypes.byref(h)) == -1  # GLASMA_ERR_ARG
+    assert b"SU(3)" in L.glasma_last_error(None)
+    p.nc, p.n[2] = 3, 0
+    assert L.glasma_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"positive" in L.glasma_last_error(None)
+    assert not h.value
