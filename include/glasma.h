@@ -40,7 +40,8 @@ typedef enum glasma_status {
     GLASMA_ERR_HIP = -2,     /* HIP runtime error (text in glasma_last_error) */
     GLASMA_ERR_NOMEM = -3,   /* device or host allocation failed */
     GLASMA_ERR_STATE = -4,   /* call out of order (e.g. observable before fields are set) */
-    GLASMA_ERR_NUMERIC = -5  /* non-finite value or non-unitary update detected */
+    GLASMA_ERR_NUMERIC = -5, /* non-finite value or non-unitary update detected */
+    GLASMA_ERR_IO = -6       /* a file could not be read or written, or does not match the context */
 } glasma_status;
 
 /* Lattice and dynamics parameters: Lattice::N/a (LATTICE/3DGrid.cpp:14-34), Dynamics::dTStep
@@ -176,6 +177,23 @@ const char* glasma_batch_last_error(const glasma_batch* b);
 int glasma_batch_update(glasma_batch* b, int nsteps);
 int glasma_batch_update_serial(glasma_batch* b, int nsteps);
 int glasma_batch_last_ms(const glasma_batch* b, double* ms);
+
+/* ---- configuration files -------------------------------------------------------------------------------- */
+/* The reference's text configurations: IO::SaveConfiguration (U and E files) and SaveConfigurationJ (J file),
+ * IO/SaveConfiguration.cpp:6-93, and IO::LoadConfiguration, IO/LoadConfiguration.cpp:3-202, with
+ * Dynamics::LoadedTime (DYNAMICS/SUNcDynamics.cpp:42-45).  Any file name may be NULL (that file is skipped /
+ * that field kept).  Records "x y z mu" + 9 complex entries (U), "x y z mu" + 8 components (E), "x y z a
+ * J+stat J+ J-stat J-" (J), precision 15, z over the ghost layers -1..Nz (U, E of periodic lattices:
+ * 0..Nz-1).  Loading checks record order and count and returns GLASMA_ERR_IO with a message instead of
+ * exiting.  Fixed reference bugs: a J record sets its own colour only (LoadCurrents' `for(aIn==0;aIn<3;...)`
+ * wrote colours a..2), and the step count of a loaded time is round(t/dt) (LoadedTime computed it from the
+ * old time).  The text keeps 15 digits; for a bit-identical restart use glasma_save_state / glasma_load_state
+ * (no counterpart in the reference): an exact binary file of U, UOld, E, the four currents and the clock,
+ * checked against the context's lattice, spacings, time step and boundary on loading. */
+int glasma_save_configuration(glasma_ctx* ctx, const char* ufile, const char* efile, const char* jfile);
+int glasma_load_configuration(glasma_ctx* ctx, const char* ufile, const char* efile, const char* jfile, double t);
+int glasma_save_state(glasma_ctx* ctx, const char* file);
+int glasma_load_state(glasma_ctx* ctx, const char* file);
 
 /* ---- tile-order autotuner (no counterpart in the reference, which has no GPU kernels) -------------------- */
 /* The order in which the step's blocks visit the lattice sets how much neighbour-link traffic the XCD-local

@@ -86,6 +86,10 @@ def lib():
         "glasma_batch_update": ([vp, c_int], c_int),
         "glasma_batch_update_serial": ([vp, c_int], c_int),
         "glasma_batch_last_ms": ([vp, dp], c_int),
+        "glasma_save_configuration": ([vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p], c_int),
+        "glasma_load_configuration": ([vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, c_double], c_int),
+        "glasma_save_state": ([vp, ctypes.c_char_p], c_int),
+        "glasma_load_state": ([vp, ctypes.c_char_p], c_int),
         "glasma_tune_step": ([vp, c_int, ip, ip, dp, ip], c_int),
         "glasma_tile_order": ([vp, ip, ip], c_int),
         "glasma_batch_tune": ([vp, c_int], c_int),

@@ -12,10 +12,14 @@
 //   evolve    Simulation.cpp:212-322: every step Update; every 20 steps the bulk observables into
 //             EnergyID<seed>.txt and the Gauss-law record into GausCheck<seed>.txt; every 100 steps the
 //             energy-momentum profile EMTDynT<t>ID<seed>.txt; every 1000 steps the unitarity check.
+//   restart   -Loaded 1 -i1 U -i2 E -i3 J -Time t (Simulation.cpp:396-421): the fields from the reference's
+//             text configuration files, the configuration saved again as UDynT/EDynT/JDynT<t>ID<seed>.txt,
+//             then Evolve(100).
 // Project keys: -driver collision|left|right, -QsAL (the charge scale, Parameters::Qs0A_L, which the
-// reference's command line does not reach), -tmax (Evolve's end time, 1100), -steps (stop after this many
-// steps), -bc 0|1 (open / periodic z), -tune 0|1 (time the step's tile orders first), -block n (steps
-// per glasma_update call between observable points; at most 20).
+// reference's command line does not reach), -tmax (Evolve's end time: 1100, or 100 after -Loaded),
+// -steps (stop after this many steps), -bc 0|1 (open / periodic z), -tune 0|1 (time the step's tile
+// orders first), -block n (steps per glasma_update call between observable points; at most 20),
+// -state_in / -state_out (an exact binary state instead of the initial conditions / at the end).
 // Errors print the library's message and exit with status 1 (the reference exits with 0 on errors).
 #include <cmath>
 #include <cstdio>
@@ -131,7 +135,9 @@ int main(int argc, char** argv) {
         std::cerr << "#ERROR -driver must be collision, left or right" << std::endl;
         return 2;
     }
-    const double tmax = A.num("tmax", 1100.0);
+    const bool loaded = A.integer("Loaded", 0) != 0;
+    const std::string state_in = A.str("state_in", ""), state_out = A.str("state_out", "");
+    const double tmax = A.num("tmax", loaded ? 100.0 : 1100.0);
     const long max_steps = A.integer("steps", -1);
     const int block = (int)std::max(1L, std::min(20L, A.integer("block", 20)));
     std::string out = A.str("o", "OUTPUT");
@@ -164,17 +170,31 @@ int main(int argc, char** argv) {
         f << "driver= " << drv << std::endl;
     }
 
-    // initial state (Simulation.cpp:337-378)
-    std::vector<double> xl(8L * p.n[0] * p.n[1]), xr(8L * p.n[0] * p.n[1]);
-    if (glasma_draw_charges((unsigned long)seed, p.n[0], p.n[1], qsl, qsl, xl.data(), xr.data()))
-        die("glasma_draw_charges", ctx);
-    if (glasma_mv_initial_state(ctx, driver, &mv, xl.data(), xr.data())) die("glasma_mv_initial_state", ctx);
-    if (A.integer("tune", 0)) {
-        if (glasma_tune_step(ctx, 3, nullptr, nullptr, nullptr, nullptr)) die("glasma_tune_step", ctx);
-    }
     const long Nz = p.n[2];
     std::vector<double> per_z(Nz), emt(10 * Nz);
     double gmax, grms, umax, urms;
+    const bool fresh = !loaded && state_in.empty();
+    if (loaded) {
+        // Simulation::Run(f1, f2, f3, t) (Simulation.cpp:396-421)
+        const std::string i1 = A.str("i1", ""), i2 = A.str("i2", ""), i3 = A.str("i3", "");
+        const double T = A.num("Time", 0.0);
+        auto name = [](const std::string& f) { return f.empty() ? nullptr : f.c_str(); };
+        if (int rc = glasma_load_configuration(ctx, name(i1), name(i2), name(i3), T)) die("glasma_load_configuration", ctx, rc);
+        const std::string u = cat(out, "UDynT", now(ctx), "ID", seed, ".txt"), e = cat(out, "EDynT", now(ctx), "ID", seed, ".txt"),
+                          j = cat(out, "JDynT", now(ctx), "ID", seed, ".txt");
+        if (int rc = glasma_save_configuration(ctx, u.c_str(), e.c_str(), j.c_str())) die("glasma_save_configuration", ctx, rc);
+    } else if (!state_in.empty()) {
+        if (int rc = glasma_load_state(ctx, state_in.c_str())) die("glasma_load_state", ctx, rc);
+    } else {
+        // initial state (Simulation.cpp:337-378)
+        std::vector<double> xl(8L * p.n[0] * p.n[1]), xr(8L * p.n[0] * p.n[1]);
+        if (glasma_draw_charges((unsigned long)seed, p.n[0], p.n[1], qsl, qsl, xl.data(), xr.data()))
+            die("glasma_draw_charges", ctx);
+        if (glasma_mv_initial_state(ctx, driver, &mv, xl.data(), xr.data())) die("glasma_mv_initial_state", ctx);
+    }
+    if (A.integer("tune", 0)) {
+        if (glasma_tune_step(ctx, 3, nullptr, nullptr, nullptr, nullptr)) die("glasma_tune_step", ctx);
+    }
     auto gauss_record = [&]() {  // GaussLaw::SaveViolation("GausCheck") (GaussLawViolation.cpp:141-164)
         if (glasma_gauss(ctx, nullptr, &gmax, &grms)) die("glasma_gauss", ctx);
         std::ofstream f(cat(out, "GausCheck", seed, ".txt"), std::ios_base::app);
@@ -191,17 +211,19 @@ int main(int argc, char** argv) {
             f << std::endl;
         }
     };
-    {  // GaussLaw::CheckViolation("GVDynT<t>") per z (GaussLawViolation.cpp:16-74)
-        if (glasma_gauss(ctx, per_z.data(), &gmax, &grms)) die("glasma_gauss", ctx);
-        std::ofstream f(cat(out, "GVDynT", now(ctx), "ID", seed, ".txt"));
-        f.precision(15);
-        for (long z = 0; z < Nz; z++) f << z << "  " << per_z[z] << std::endl;
-        std::cerr << "#GAUSS LAW VIOLATION AT T=" << now(ctx) << " MAX=" << gmax << " AVG=" << grms << std::endl;
+    if (fresh) {
+        {  // GaussLaw::CheckViolation("GVDynT<t>") per z (GaussLawViolation.cpp:16-74)
+            if (glasma_gauss(ctx, per_z.data(), &gmax, &grms)) die("glasma_gauss", ctx);
+            std::ofstream f(cat(out, "GVDynT", now(ctx), "ID", seed, ".txt"));
+            f.precision(15);
+            for (long z = 0; z < Nz; z++) f << z << "  " << per_z[z] << std::endl;
+            std::cerr << "#GAUSS LAW VIOLATION AT T=" << now(ctx) << " MAX=" << gmax << " AVG=" << grms << std::endl;
+        }
+        if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);
+        std::cerr << "#UNITARITY VIOLATION AT T=" << now(ctx) << " MAX=" << umax << " AVG=" << urms << std::endl;
+        emt_map();
+        gauss_record();
     }
-    if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);
-    std::cerr << "#UNITARITY VIOLATION AT T=" << now(ctx) << " MAX=" << umax << " AVG=" << urms << std::endl;
-    emt_map();
-    gauss_record();
 
     // Evolve (Simulation.cpp:212-322)
     std::ofstream energy(cat(out, "EnergyID", seed, ".txt"));
@@ -234,6 +256,9 @@ int main(int argc, char** argv) {
         }
     }
     energy.close();
+    if (!state_out.empty()) {
+        if (int rc = glasma_save_state(ctx, state_out.c_str())) die("glasma_save_state", ctx, rc);
+    }
     std::cerr << "#COMPLETED ID=" << seed << " after " << steps_done(ctx) << " steps, t=" << now(ctx) << std::endl;
     glasma_destroy(ctx);
     return 0;

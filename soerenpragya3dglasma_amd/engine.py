@@ -22,6 +22,7 @@ Host arrays follow the reference layout with z ghost layers (see ``layout.py``).
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -168,6 +169,28 @@ class Glasma:
         ms = ctypes.c_double()
         self._chk(_lib.lib().glasma_last_step_kernel_ms(self._h, ctypes.byref(ms)), "glasma_last_step_kernel_ms")
         return ms.value
+
+    # -- configuration files (IO/SaveConfiguration.cpp, IO/LoadConfiguration.cpp) ---------------------
+    @staticmethod
+    def _path(p):
+        return None if p is None else os.fsencode(p)
+
+    def save_configuration(self, ufile=None, efile=None, jfile=None):
+        """The reference's text configuration files (U, E and J; any may be None)."""
+        self._chk(_lib.lib().glasma_save_configuration(self._h, self._path(ufile), self._path(efile),
+                                                       self._path(jfile)), "glasma_save_configuration")
+
+    def load_configuration(self, ufile=None, efile=None, jfile=None, t: float = 0.0):
+        """Fields from the reference's text configuration files; the clock is set to t."""
+        self._chk(_lib.lib().glasma_load_configuration(self._h, self._path(ufile), self._path(efile),
+                                                       self._path(jfile), float(t)), "glasma_load_configuration")
+
+    def save_state(self, path):
+        """Exact binary state (fields, UOld, clock) for a bit-identical restart."""
+        self._chk(_lib.lib().glasma_save_state(self._h, self._path(path)), "glasma_save_state")
+
+    def load_state(self, path):
+        self._chk(_lib.lib().glasma_load_state(self._h, self._path(path)), "glasma_load_state")
 
     # -- tile-order autotuner (no counterpart in the reference) --------------------------------------
     @staticmethod
