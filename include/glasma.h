@@ -113,6 +113,18 @@ int glasma_unitarity(glasma_ctx* ctx, double* umax, double* urms);
 /* EnergyMomentumTensor::CreateMap (OBSERVABLES/EnergyMomentumTensor.cpp:92-170): out[10*z + k] =
  * T00 TXX TYY TZZ T0X T0Y T0Z TXY TYZ TZX transverse averages per z. */
 int glasma_emt(glasma_ctx* ctx, double* out);
+/* IO::SaveFields (IO/SaveConfiguration.cpp:112-158): out[4*z + k], z = 0..Nz-1, the file's columns
+ * 0.5 (B_x^2 + B_y^2), 0.5 B_z^2, 0.5 (E_x^2 + E_y^2), 0.5 E_z^2, transverse averages from the
+ * elementary plaquettes of the current links and the local E. */
+int glasma_field_profile(glasma_ctx* ctx, double* out);
+/* IO::SaveTransverselyIntegratedCurrents (IO/SaveConfiguration.cpp:160-204): out[4*r + k] for the
+ * (Nz + 2) layers z = r - 1 = -1..Nz (the ghost layers as SynchronizeGhostCells fills them), the
+ * file's columns 0.5 <Jm^2>, 0.5 <JmStat^2>, 0.5 <Jp^2>, 0.5 <JpStat^2> (transverse averages). */
+int glasma_current_profile(glasma_ctx* ctx, double* out);
+/* EnergyMomentumTensor::CreateMapBfCollision (OBSERVABLES/EnergyMomentumTensor.cpp:10-90): out[2*(x +
+ * Nx*y) + side], the energy density summed over z / Nz for z <= (Nz - 2) / 2 (side 0, "T00Left") and
+ * above (side 1, "T00Right"). */
+int glasma_transverse_energy_map(glasma_ctx* ctx, double* out);
 
 /* ---- MV initial conditions ---------------------------------------------------------------------- */
 /* Charges of both nuclei from one MT19937 stream seeded as gsl_rng_set (seed 0 -> 4357), Gaussian

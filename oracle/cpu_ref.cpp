@@ -21,6 +21,7 @@
 //                                                      conservation ~dt^2, time reversal, free Maxwell
 //                                                      dispersion, exact J shift at dt = a_z
 //   * observables                                    -> numpy brute force on tiny lattices
+//   * output-file profiles and transverse map        -> numpy brute force; totals equal the bulk sums
 //   * FFT                                            -> numpy.fft
 //   * MT19937                                        -> 10000th output KAT 4123659995, numpy RandomState
 //   * Gaussian draw (GSL gsl_ran_gaussian)           -> parity unpinned (GSL absent; published polar method)
@@ -794,6 +795,84 @@ template <class G> static void EMT(const Lat<G>& L, typename G::elem* U, typenam
     }
 }
 
+// IO::SaveFields (SC/IO/SaveConfiguration.cpp:112-158): per z, terms divided by N0 N1 before summing.
+// out[4 z + k] = 0.5 (B0 + B1), 0.5 B2, 0.5 (E0 + E1), 0.5 E2.  Ghost layers are used as the caller holds them.
+template <class G> static void FieldProfile(const Lat<G>& L, typename G::elem* U, double* E, double* out) {
+    typedef typename G::elem elem;
+    const double NP = (double)L.N[0] * L.N[1];
+#pragma omp parallel for
+    for (int z = 0; z < L.N[2]; z++) {
+        double EE0 = 0, EE1 = 0, EE2 = 0, BB0 = 0, BB1 = 0, BB2 = 0;
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++) {
+                elem Uxy[G::MS], Uyz[G::MS], Uzx[G::MS];  // COMPUTE_ELEMENTARY_PLAQUETTES, SC/CALC/Plaquettes.cpp:83-88
+                UUDD<G>(L.Ul(U, x, y, z, 0), L.Ul(U, x + 1, y, z, 1), L.Ul(U, x, y + 1, z, 0), L.Ul(U, x, y, z, 1), Uxy);
+                UUDD<G>(L.Ul(U, x, y, z, 1), L.Ul(U, x, y + 1, z, 2), L.Ul(U, x, y, z + 1, 1), L.Ul(U, x, y, z, 2), Uyz);
+                UUDD<G>(L.Ul(U, x, y, z, 2), L.Ul(U, x, y, z + 1, 0), L.Ul(U, x + 1, y, z, 2), L.Ul(U, x, y, z, 0), Uzx);
+                BB0 += 4.0 * G::ReTrIDMinusU(Uyz) / NP;
+                BB1 += 4.0 * G::ReTrIDMinusU(Uzx) / NP;
+                BB2 += 4.0 * G::ReTrIDMinusU(Uxy) / NP;
+                for (int a = 0; a < G::NG; a++) {
+                    EE0 += SQR(L.El(E, x, y, z, 0)[a]) / NP;
+                    EE1 += SQR(L.El(E, x, y, z, 1)[a]) / NP;
+                    EE2 += SQR(L.El(E, x, y, z, 2)[a]) / NP;
+                }
+            }
+        double* o = out + 4L * z;
+        o[0] = 0.5 * (BB0 + BB1); o[1] = 0.5 * BB2; o[2] = 0.5 * (EE0 + EE1); o[3] = 0.5 * EE2;
+    }
+}
+
+// IO::SaveTransverselyIntegratedCurrents (SC/IO/SaveConfiguration.cpp:160-204): rows z = -1..Nz,
+// out[4 (z + 1) + k] = 0.5 <Jm^2>, 0.5 <JmStat^2>, 0.5 <Jp^2>, 0.5 <JpStat^2>.
+template <class G> static void CurrentProfile(const Lat<G>& L, double* Jp, double* Jm, double* JpS, double* JmS, double* out) {
+    const double NP = (double)L.N[0] * L.N[1];
+    for (int z = -1; z <= L.N[2]; z++) {
+        double JmJm = 0, JmStJmSt = 0, JpJp = 0, JpStJpSt = 0;
+        for (int y = 0; y < L.N[1]; y++)
+            for (int x = 0; x < L.N[0]; x++)
+                for (int a = 0; a < G::NG; a++) {
+                    JmJm += SQR(L.Jl(Jm, x, y, z)[a]);
+                    JmStJmSt += SQR(L.Jl(JmS, x, y, z)[a]);
+                    JpJp += SQR(L.Jl(Jp, x, y, z)[a]);
+                    JpStJpSt += SQR(L.Jl(JpS, x, y, z)[a]);
+                }
+        double* o = out + 4L * (z + 1);
+        o[0] = 0.5 * JmJm / NP; o[1] = 0.5 * JmStJmSt / NP; o[2] = 0.5 * JpJp / NP; o[3] = 0.5 * JpStJpSt / NP;
+    }
+}
+
+// EnergyMomentumTensor::CreateMapBfCollision (SC/OBSERVABLES/EnergyMomentumTensor.cpp:10-77) over the
+// whole lattice: out[2 (x + N0 y) + side], side 0 for z <= (zHigh - 1) / 2, zHigh = N2 - 1.
+template <class G> static void TransverseMap(const Lat<G>& L, typename G::elem* U, double* E, double* out) {
+    typedef typename G::elem elem;
+    double cB[3], cE[3];
+    for (int mu = 0; mu < 3; mu++) { cB[mu] = SQR(L.a[mu] / L.aCube); cE[mu] = cB[mu]; }  // aScale = 1
+    const int zHigh = L.N[2] - 1;
+#pragma omp parallel for
+    for (int y = 0; y < L.N[1]; y++)
+        for (int x = 0; x < L.N[0]; x++) {
+            double T00Left = 0.0, T00Right = 0.0;
+            for (int z = 0; z <= zHigh; z++) {
+                elem Uxy[G::MS], Uyz[G::MS], Uzx[G::MS];
+                UUDD<G>(L.Ul(U, x, y, z, 0), L.Ul(U, x + 1, y, z, 1), L.Ul(U, x, y + 1, z, 0), L.Ul(U, x, y, z, 1), Uxy);
+                UUDD<G>(L.Ul(U, x, y, z, 1), L.Ul(U, x, y + 1, z, 2), L.Ul(U, x, y, z + 1, 1), L.Ul(U, x, y, z, 2), Uyz);
+                UUDD<G>(L.Ul(U, x, y, z, 2), L.Ul(U, x, y, z + 1, 0), L.Ul(U, x + 1, y, z, 2), L.Ul(U, x, y, z, 0), Uzx);
+                const double BB0 = 4.0 * G::ReTrIDMinusU(Uyz), BB1 = 4.0 * G::ReTrIDMinusU(Uzx), BB2 = 4.0 * G::ReTrIDMinusU(Uxy);
+                double EE0 = 0, EE1 = 0, EE2 = 0;
+                for (int a = 0; a < G::NG; a++) {
+                    EE0 += SQR(L.El(E, x, y, z, 0)[a]);
+                    EE1 += SQR(L.El(E, x, y, z, 1)[a]);
+                    EE2 += SQR(L.El(E, x, y, z, 2)[a]);
+                }
+                const double t = 0.5 * (cE[0] * EE0 + cE[1] * EE1 + cE[2] * EE2 + cB[0] * BB0 + cB[1] * BB1 + cB[2] * BB2) / L.N[2];
+                if (z <= (zHigh - 1) / 2) T00Left += t; else T00Right += t;
+            }
+            out[2 * (x + (long)L.N[0] * y)] = T00Left;
+            out[2 * (x + (long)L.N[0] * y) + 1] = T00Right;
+        }
+}
+
 // ============================================================================================
 //  FFT (oracle's own mixed-radix recursion; FFTW conventions of SC/FFT/FFT*D.cpp: forward e^{-2 pi i},
 //  backward e^{+2 pi i}, both unnormalised).
@@ -1330,6 +1409,19 @@ void ref_unitarity(int nc, const int* N, const double* a, int bc, double* U, dou
 void ref_emt(int nc, const int* N, const double* a, int bc, double* U, double* UOld, double* E, double* out) {
     if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); EMT<SU3>(L, (cd*)U, (cd*)UOld, E, out); }
     else { Lat<SU2> L = mk2(N, a, bc); EMT<SU2>(L, U, UOld, E, out); }
+}
+void ref_field_profile(int nc, const int* N, const double* a, int bc, double* U, double* E, double* out) {
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); FieldProfile<SU3>(L, (cd*)U, E, out); }
+    else { Lat<SU2> L = mk2(N, a, bc); FieldProfile<SU2>(L, U, E, out); }
+}
+void ref_current_profile(int nc, const int* N, double* Jp, double* Jm, double* JpS, double* JmS, double* out) {
+    const double a[3] = {1, 1, 1};
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, 0); CurrentProfile<SU3>(L, Jp, Jm, JpS, JmS, out); }
+    else { Lat<SU2> L = mk2(N, a, 0); CurrentProfile<SU2>(L, Jp, Jm, JpS, JmS, out); }
+}
+void ref_transverse_map(int nc, const int* N, const double* a, int bc, double* U, double* E, double* out) {
+    if (nc == 3) { Lat<SU3> L = mk3(N, a, bc); TransverseMap<SU3>(L, (cd*)U, E, out); }
+    else { Lat<SU2> L = mk2(N, a, bc); TransverseMap<SU2>(L, U, E, out); }
 }
 // --- FFT / RNG ---
 void ref_fft_nd(double* data, int nd, const int* dims, long batch, int sign) { fft_nd((cd*)data, nd, dims, batch, sign); }

@@ -51,6 +51,9 @@ def lib():
         L.ref_gauss.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int] + [_dp] * 7
         L.ref_unitarity.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int] + [_dp] * 3
         L.ref_emt.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int] + [_dp] * 4
+        L.ref_field_profile.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int] + [_dp] * 3
+        L.ref_current_profile.argtypes = [ctypes.c_int, _ip] + [_dp] * 5
+        L.ref_transverse_map.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int] + [_dp] * 3
         L.ref_fft_nd.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_long, ctypes.c_int]
         L.ref_mt_raw.argtypes = [ctypes.c_ulong, ctypes.c_int, ctypes.c_long, ctypes.POINTER(ctypes.c_uint32)]
         L.ref_gauss_draws.argtypes = [ctypes.c_ulong, ctypes.c_long, _dp]
@@ -168,6 +171,27 @@ def unitarity(f: Fields, a, bc: int = 0):
 def emt(f: Fields, a, bc: int = 0) -> np.ndarray:
     out = np.zeros((f.N[2], 10))
     lib().ref_emt(f.nc, ivec(f.N), dvec(a), bc, P(f.U), P(f.UOld), P(f.E), P(out))
+    return out
+
+
+def field_profile(f: Fields, a, bc: int = 0) -> np.ndarray:
+    """IO::SaveFields columns per z (SC/IO/SaveConfiguration.cpp:112-158), shape (Nz, 4)."""
+    out = np.zeros((f.N[2], 4))
+    lib().ref_field_profile(f.nc, ivec(f.N), dvec(a), bc, P(f.U), P(f.E), P(out))
+    return out
+
+
+def current_profile(f: Fields) -> np.ndarray:
+    """IO::SaveTransverselyIntegratedCurrents rows z = -1..Nz (SC/IO/SaveConfiguration.cpp:160-204), (Nz + 2, 4)."""
+    out = np.zeros((f.N[2] + 2, 4))
+    lib().ref_current_profile(f.nc, ivec(f.N), P(f.Jp), P(f.Jm), P(f.JpS), P(f.JmS), P(out))
+    return out
+
+
+def transverse_map(f: Fields, a, bc: int = 0) -> np.ndarray:
+    """EnergyMomentumTensor::CreateMapBfCollision (SC/OBSERVABLES/EnergyMomentumTensor.cpp:10-77), (Ny, Nx, 2)."""
+    out = np.zeros((f.N[1], f.N[0], 2))
+    lib().ref_transverse_map(f.nc, ivec(f.N), dvec(a), bc, P(f.U), P(f.E), P(out))
     return out
 
 

@@ -68,6 +68,9 @@ def lib():
         "glasma_gauss": ([vp, dp, dp, dp], c_int),
         "glasma_unitarity": ([vp, dp, dp], c_int),
         "glasma_emt": ([vp, dp], c_int),
+        "glasma_field_profile": ([vp, dp], c_int),
+        "glasma_current_profile": ([vp, dp], c_int),
+        "glasma_transverse_energy_map": ([vp, dp], c_int),
         "glasma_draw_charges": ([c_ulong, c_int, c_int, c_double, c_double, dp, dp], c_int),
         "glasma_mv_initial_state": ([vp, c_int, ctypes.POINTER(MVParams), dp, dp], c_int),
         "glasma_set_wilson_lines": ([vp, c_char, ctypes.POINTER(MVParams), dp, dp, dp], c_int),

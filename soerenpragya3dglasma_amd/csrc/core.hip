@@ -409,6 +409,77 @@ __global__ void __launch_bounds__(256) k_emt(ObsArgs A) {
     block_store<10>(v, A.red + ((long)blockIdx.y * A.bpp + blockIdx.x) * 10);
 }
 
+// IO::SaveTransverselyIntegratedCurrents (SC/IO/SaveConfiguration.cpp:160-204): per layer
+// sum_{x,y,a} of Jm^2, JmStat^2, Jp^2, JpStat^2.  K = 4.
+__global__ void __launch_bounds__(256) k_currents(ObsArgs A) {
+    double v[4] = {0, 0, 0, 0};
+    int x, y, z;
+    long s;
+    if (site_of(A, x, y, z, s)) {
+        for (int a = 0; a < 8; a++) {
+            const double jm = Jv(A.J, JM, a, s), jms = Jv(A.J, JMS, a, s), jp = Jv(A.J, JP, a, s),
+                         jps = Jv(A.J, JPS, a, s);
+            v[0] += jm * jm;
+            v[1] += jms * jms;
+            v[2] += jp * jp;
+            v[3] += jps * jps;
+        }
+    }
+    block_store<4>(v, A.red + ((long)blockIdx.y * A.bpp + blockIdx.x) * 4);
+}
+
+// EnergyMomentumTensor::CreateMapBfCollision (SC/OBSERVABLES/EnergyMomentumTensor.cpp:10-77): for each
+// transverse site the energy density summed over z, split at z <= (Nz - 2) / 2 (left) and above (right),
+// from the elementary plaquettes of the current links and the local E.  One thread per (x, y) and chunk of
+// kMapChunk layers; part[(chunk * P + i) * 2 + side].
+constexpr int kMapChunk = 32;
+
+__global__ void __launch_bounds__(256) k_tmap(ObsArgs A, double* __restrict__ part) {
+    const Geom& g = A.g;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.P) return;
+    const int x = (int)(i % g.nx), y = (int)(i / g.nx);
+    const int z0 = blockIdx.y * kMapChunk, z1 = min(z0 + kMapChunk, g.nz);
+    const int zsplit = (g.nz - 2) / 2;  // (zHigh - 1) / 2 with zHigh = Nz - 1
+    const double ac = A.a[0] * A.a[1] * A.a[2];
+    double c[3];
+    for (int m = 0; m < 3; m++) c[m] = (A.a[m] / ac) * (A.a[m] / ac);  // cB = cE = (a_mu aScale^2 / aCube)^2
+    const double inv_nz = 1.0 / (double)g.nz;
+    double left = 0.0, right = 0.0;
+    for (int z = z0; z < z1; z++) {
+        const long s = i + g.P * z;
+        const long sxp = sidx(g, x + 1, y, z), syp = sidx(g, x, y + 1, z), szp = sidx(g, x, y, zup(g, z));
+        M3 Pq;
+        plaq_full(A.U, g.S, 0, 0, s, 1, sxp, 0, syp, 1, s, Pq);
+        const double B2 = 4.0 * retr_one_minus(Pq);
+        plaq_full(A.U, g.S, 0, 1, s, 2, syp, 1, szp, 2, s, Pq);
+        const double B0 = 4.0 * retr_one_minus(Pq);
+        plaq_full(A.U, g.S, 0, 2, s, 0, szp, 2, sxp, 0, s, Pq);
+        const double B1 = 4.0 * retr_one_minus(Pq);
+        double E0 = 0.0, E1 = 0.0, E2 = 0.0;
+        for (int a = 0; a < 8; a++) {
+            const double e0 = Ev(A.E, 0, a, s), e1 = Ev(A.E, 1, a, s), e2 = Ev(A.E, 2, a, s);
+            E0 += e0 * e0;
+            E1 += e1 * e1;
+            E2 += e2 * e2;
+        }
+        const double t = 0.5 * (c[0] * E0 + c[1] * E1 + c[2] * E2 + c[0] * B0 + c[1] * B1 + c[2] * B2) * inv_nz;
+        if (z <= zsplit) left += t; else right += t;
+    }
+    double* o = part + ((long)blockIdx.y * g.P + i) * 2;
+    o[0] = left;
+    o[1] = right;
+}
+
+// fixed-order sum of the chunks: out[i * 2 + side] = sum_c part[(c * P + i) * 2 + side]
+__global__ void k_tmap_finalize(const double* __restrict__ part, double* __restrict__ out, long P, int nchunk) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * P) return;
+    double acc = 0.0;
+    for (int c = 0; c < nchunk; c++) acc += part[(long)c * 2 * P + i];
+    out[i] = acc;
+}
+
 // per-z fixed-order finalisation: out[z*K + k] = sum_b red[(z*bpp + b)*K + k] (or max for the
 // entries flagged in maxmask)
 __global__ void k_finalize(const double* __restrict__ red, double* __restrict__ out, int nz, int bpp, int K,
@@ -689,6 +760,63 @@ int glasma_emt(glasma_ctx* ctx, double* out) {
     int rc = run_obs(ctx, k_emt, 10, 0u, pz, true);
     if (rc) return rc;
     std::memcpy(out, pz.data(), pz.size() * sizeof(double));
+    return 0;
+}
+
+int glasma_field_profile(glasma_ctx* ctx, double* out) {
+    if (!ctx || !out) return GLASMA_ERR_ARG;
+    std::vector<double> pz;
+    int rc = run_obs(ctx, k_bulk, 9, 0u, pz, false);
+    if (rc) return rc;
+    const double inv = 1.0 / (double)ctx->g.P;
+    for (int z = 0; z < ctx->g.nz; z++) {
+        const double* r = &pz[(size_t)z * 9];  // E0^2 E1^2 E2^2 B0 B1 B2 (sums over the layer)
+        out[4 * z + 0] = 0.5 * (r[3] * inv + r[4] * inv);
+        out[4 * z + 1] = 0.5 * (r[5] * inv);
+        out[4 * z + 2] = 0.5 * (r[0] * inv + r[1] * inv);
+        out[4 * z + 3] = 0.5 * (r[2] * inv);
+    }
+    return 0;
+}
+
+int glasma_current_profile(glasma_ctx* ctx, double* out) {
+    if (!ctx || !out) return GLASMA_ERR_ARG;
+    std::vector<double> pz;
+    int rc = run_obs(ctx, k_currents, 4, 0u, pz, false);
+    if (rc) return rc;
+    const int nz = ctx->g.nz;
+    const double inv = 1.0 / (double)ctx->g.P;
+    // rows z = -1 .. nz; the ghost layers hold what SynchronizeGhostCells puts there
+    for (int r = 0; r < nz + 2; r++) {
+        const int z = r - 1;
+        const int src = z < 0 ? (ctx->g.bc ? nz - 1 : 0) : (z >= nz ? (ctx->g.bc ? 0 : nz - 1) : z);
+        for (int k = 0; k < 4; k++) out[4 * r + k] = 0.5 * pz[(size_t)src * 4 + k] * inv;
+    }
+    return 0;
+}
+
+int glasma_transverse_energy_map(glasma_ctx* ctx, double* out) {
+    if (!ctx || !out) return GLASMA_ERR_ARG;
+    const Geom& g = ctx->g;
+    const int bpp = (int)((g.P + 255) / 256);
+    const int nchunk = (g.nz + kMapChunk - 1) / kMapChunk;
+    int rc = ensure_red(ctx, (size_t)(nchunk + 1) * 2 * g.P);
+    if (rc) return rc;
+    ObsArgs A;
+    A.U = ctx->U[ctx->cur];
+    A.UOld = nullptr;
+    A.E = ctx->E;
+    A.J = ctx->J[ctx->cur];
+    A.red = nullptr;
+    A.g = g;
+    A.bpp = bpp;
+    for (int m = 0; m < 3; m++) A.a[m] = ctx->p.a[m];
+    double* fin = ctx->red + (size_t)nchunk * 2 * g.P;
+    hipLaunchKernelGGL(k_tmap, dim3(bpp, nchunk), dim3(256), 0, ctx->stream, A, ctx->red);
+    hipLaunchKernelGGL(k_tmap_finalize, dim3((unsigned)((2 * g.P + 255) / 256)), dim3(256), 0, ctx->stream, ctx->red,
+                       fin, (long)g.P, nchunk);
+    GL_HIP(ctx, hipMemcpyAsync(out, fin, 2 * g.P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

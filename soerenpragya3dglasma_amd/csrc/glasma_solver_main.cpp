@@ -19,7 +19,10 @@
 // reference's command line does not reach), -tmax (Evolve's end time: 1100, or 100 after -Loaded),
 // -steps (stop after this many steps), -bc 0|1 (open / periodic z), -tune 0|1 (time the step's tile
 // orders first), -block n (steps per glasma_update call between observable points; at most 20),
-// -state_in / -state_out (an exact binary state instead of the initial conditions / at the end).
+// -state_in / -state_out (an exact binary state instead of the initial conditions / at the end), -maps 1
+// (the output routines no reference driver calls: CreateMapBfCollision of the initial state as
+// MapBfCollisionID<seed>.txt, and with every EMT profile SaveFields / SaveTransverselyIntegratedCurrents
+// as FieldsDynT<t>ID<seed>.txt / CurrentsDynT<t>ID<seed>.txt).
 // Errors print the library's message and exit with status 1 (the reference exits with 0 on errors).
 #include <cmath>
 #include <cstdio>
@@ -211,6 +214,37 @@ int main(int argc, char** argv) {
             f << std::endl;
         }
     };
+    const bool maps = A.integer("maps", 0) != 0;
+    auto profiles = [&]() {
+        if (!maps) return;
+        {  // IO::SaveFields (IO/SaveConfiguration.cpp:112-158)
+            std::vector<double> v(4 * Nz);
+            if (glasma_field_profile(ctx, v.data())) die("glasma_field_profile", ctx);
+            std::ofstream f(cat(out, "FieldsDynT", now(ctx), "ID", seed, ".txt"));
+            f.precision(15);
+            for (long z = 0; z < Nz; z++)
+                f << z << "  " << v[4 * z] << " " << v[4 * z + 1] << "  " << v[4 * z + 2] << "  " << v[4 * z + 3] << std::endl;
+        }
+        {  // IO::SaveTransverselyIntegratedCurrents (IO/SaveConfiguration.cpp:160-204)
+            std::vector<double> v(4 * (Nz + 2));
+            if (glasma_current_profile(ctx, v.data())) die("glasma_current_profile", ctx);
+            std::ofstream f(cat(out, "CurrentsDynT", now(ctx), "ID", seed, ".txt"));
+            f.precision(15);
+            for (long r = 0; r < Nz + 2; r++)
+                f << r - 1 << "  " << v[4 * r] << "  " << v[4 * r + 1] << "  " << v[4 * r + 2] << "  " << v[4 * r + 3] << std::endl;
+        }
+    };
+    if (fresh && maps) {  // EnergyMomentumTensor::CreateMapBfCollision (EnergyMomentumTensor.cpp:10-77)
+        std::vector<double> m(2L * p.n[0] * p.n[1]);
+        if (glasma_transverse_energy_map(ctx, m.data())) die("glasma_transverse_energy_map", ctx);
+        std::ofstream f(cat(out, "MapBfCollisionID", seed, ".txt"));
+        f.precision(15);
+        for (long y = 0; y < p.n[1]; y++)
+            for (long x = 0; x < p.n[0]; x++) {
+                const long i = 2 * (x + (long)p.n[0] * y);
+                f << x << "  " << y << "  " << m[i] << "  " << m[i + 1] << std::endl;
+            }
+    }
     if (fresh) {
         {  // GaussLaw::CheckViolation("GVDynT<t>") per z (GaussLawViolation.cpp:16-74)
             if (glasma_gauss(ctx, per_z.data(), &gmax, &grms)) die("glasma_gauss", ctx);
@@ -222,6 +256,7 @@ int main(int argc, char** argv) {
         if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);
         std::cerr << "#UNITARITY VIOLATION AT T=" << now(ctx) << " MAX=" << umax << " AVG=" << urms << std::endl;
         emt_map();
+        profiles();
         gauss_record();
     }
 
@@ -249,6 +284,7 @@ int main(int argc, char** argv) {
         if (s % 100 == 0) {
             std::cerr << "#SAVING OBSERVABLES AT T=" << s << "  " << now(ctx) << std::endl;
             emt_map();
+            profiles();
         }
         if (s % 1000 == 0) {
             if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);

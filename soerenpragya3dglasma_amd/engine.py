@@ -245,6 +245,25 @@ class Glasma:
         self._chk(_lib.lib().glasma_emt(self._h, _p(out)), "glasma_emt")
         return out
 
+    def field_profile(self) -> np.ndarray:
+        """IO::SaveFields columns per z (Nz, 4): 0.5(Bx^2+By^2), 0.5 Bz^2, 0.5(Ex^2+Ey^2), 0.5 Ez^2."""
+        out = np.zeros((self.N[2], 4))
+        self._chk(_lib.lib().glasma_field_profile(self._h, _p(out)), "glasma_field_profile")
+        return out
+
+    def current_profile(self) -> np.ndarray:
+        """IO::SaveTransverselyIntegratedCurrents columns for z = -1..Nz (Nz + 2, 4): 0.5<Jm^2>,
+        0.5<JmStat^2>, 0.5<Jp^2>, 0.5<JpStat^2>."""
+        out = np.zeros((self.N[2] + 2, 4))
+        self._chk(_lib.lib().glasma_current_profile(self._h, _p(out)), "glasma_current_profile")
+        return out
+
+    def transverse_energy_map(self) -> np.ndarray:
+        """EnergyMomentumTensor::CreateMapBfCollision values (Ny, Nx, 2): T00Left, T00Right."""
+        out = np.zeros((self.N[1], self.N[0], 2))
+        self._chk(_lib.lib().glasma_transverse_energy_map(self._h, _p(out)), "glasma_transverse_energy_map")
+        return out
+
     # -- initial conditions -------------------------------------------------------------------------
     @staticmethod
     def draw_charges(seed: int, Nx: int, Ny: int, qs_l: float, qs_r: float):

@@ -127,3 +127,31 @@ def slab_energy_np(f, a, zc, half):
             e2 = (f.E[zg, :, :, mu, :] ** 2).sum(axis=-1)
             tot += 0.5 * c[mu] * float((e2 + 4.0 * retr).sum())
     return tot
+
+
+def site_fields_np(f):
+    """Per-site squared fields from host fields in the reference layout (numpy, independent of the engine):
+    (E2, B), each shaped (Nz, Ny, Nx, 3); E2[..., mu] = sum_a E_mu^a E_mu^a and B[..., mu] = 4 Re tr(1 - P_mu)
+    with P_mu the elementary plaquette in the plane (mu+1, mu+2) rooted at the site (z neighbours from the
+    ghost layers)."""
+    Nx, Ny, Nz = f.N
+    U = np.swapaxes(f.U, -1, -2)  # [z+1, y, x, mu, i, j]
+    dag = lambda M: np.conj(np.swapaxes(M, -1, -2))  # noqa: E731
+
+    def at(mu, d=None):
+        if d == 2:
+            return U[2:, :, :, mu]
+        M = U[1:-1, :, :, mu]
+        if d == 0:
+            return np.roll(M, -1, axis=2)
+        if d == 1:
+            return np.roll(M, -1, axis=1)
+        return M
+
+    B = np.zeros((Nz, Ny, Nx, 3))
+    for mu in range(3):
+        j, k = (mu + 1) % 3, (mu + 2) % 3
+        P = at(j) @ at(k, j) @ dag(at(j, k)) @ dag(at(k))
+        B[..., mu] = 4.0 * (3.0 - np.trace(P, axis1=-2, axis2=-1).real)
+    E2 = (f.E[1:-1] ** 2).sum(axis=-1)
+    return E2, B

@@ -1,0 +1,73 @@
+"""GPU parity of the output-file observables against the oracle: IO::SaveFields (glasma_field_profile),
+IO::SaveTransverselyIntegratedCurrents (glasma_current_profile) and EnergyMomentumTensor::CreateMapBfCollision
+(glasma_transverse_energy_map), after a few steps so that both sides hold evolved fields with synchronised
+ghost layers.  Tolerance 1e-12 relative (sums in different orders)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import random_fields, relerr, upload
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N,a,bc", [((8, 6, 10), (1.0, 1.0, 0.5), 0),  # one z chunk of the map kernel
+                                    ((70, 3, 40), (1.0, 0.9, 0.7), 0),  # ragged plane (70 * 3 sites), two chunks
+                                    ((8, 4, 70), (1.0, 1.0, 1.0), 1)])  # periodic z, three chunks
+def test_output_profiles_match_oracle(N, a, bc):
+    from soerenpragya3dglasma_amd import Glasma
+    dt = 0.1
+    f = random_fields(N, seed=31, bc=bc)
+    with Glasma(N, a, dt, bc) as g:
+        upload(g, f)
+        g.Update(3)
+        prof, cur, tmap = g.field_profile(), g.current_profile(), g.transverse_energy_map()
+    O.update(f, a, dt, 3, bc)
+    prof_o, cur_o, tmap_o = O.field_profile(f, a, bc), O.current_profile(f), O.transverse_map(f, a, bc)
+    assert prof.shape == (N[2], 4) and cur.shape == (N[2] + 2, 4) and tmap.shape == (N[1], N[0], 2)
+    assert relerr(prof, prof_o) < 1e-12
+    assert relerr(cur, cur_o) < 1e-12
+    assert relerr(tmap, tmap_o) < 1e-12
+    # the ghost rows are the layers SynchronizeGhostCells copies
+    lo, hi = (N[2] - 1, 0) if bc else (0, N[2] - 1)
+    assert np.array_equal(cur[0], cur[lo + 1]) and np.array_equal(cur[-1], cur[hi + 1])
+
+
+def test_transverse_map_of_the_mv_collision_separates_the_nuclei():
+    """Before the collision the left half holds the left nucleus and the right half the right one: with
+    only the left charges drawn, the right column sums vanish."""
+    from soerenpragya3dglasma_amd import Glasma
+    N, a, dt = (8, 8, 32), (1.0, 1.0, 1.0), 0.25
+    xl, xr = Glasma.draw_charges(4, N[0], N[1], 0.125, 0.125)
+    mv = Glasma.mv_params(lam=1.65, meff=0.33, sigma_l=2.0, sigma_r=2.0, center_l=int(0.85 * N[2] / 2 - 1),
+                          center_r=int(1.15 * N[2] / 2 - 1))
+    with Glasma(N, a, dt) as g:
+        g.mv_initial_state(0, mv, xl, np.zeros_like(xr))
+        m = g.transverse_energy_map()
+        T00 = g.bulk()["T00"]
+    assert m[..., 0].min() > 0 and np.abs(m[..., 1]).max() < 1e-14 * m[..., 0].max()
+    assert abs(m.sum() * N[2] - T00) < 1e-12 * T00
+
+
+def test_driver_writes_the_output_files(tmp_path):
+    """glasma_solver -maps 1: MapBfCollision of the initial state, and SaveFields / currents profiles with every
+    EMT profile; with a = 1 their totals are the T00 of the energy record."""
+    import os
+    import subprocess
+    solver = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "soerenpragya3dglasma_amd",
+                          "_lib", "glasma_solver")
+    out = tmp_path / "out"
+    r = subprocess.run([solver, "-o", str(out), "-Nx", "8", "-Ny", "8", "-Nz", "16", "-a", "1", "-dT", "0.25",
+                        "-SEED", "3", "-steps", "100", "-maps", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    energy = [[float(v) for v in l.split()] for l in open(out / "EnergyID3.txt").read().splitlines()]
+    m = np.loadtxt(out / "MapBfCollisionID3.txt")
+    assert m.shape == (64, 4) and np.array_equal(m[:8, 0], np.arange(8)) and np.all(m[:8, 1] == 0)
+    assert m[:, 2:].sum() * 16 == pytest.approx(energy[0][1], rel=1e-12)
+    for t, T00 in ((0, energy[0][1]), (25, energy[-1][2])):
+        fld = np.loadtxt(out / f"FieldsDynT{t}ID3.txt")
+        assert fld.shape == (16, 5) and np.array_equal(fld[:, 0], np.arange(16))
+        assert fld[:, 1:].sum() * 64 == pytest.approx(T00, rel=1e-12)
+        cur = np.loadtxt(out / f"CurrentsDynT{t}ID3.txt")
+        assert cur.shape == (18, 5) and np.array_equal(cur[:, 0], np.arange(-1, 17))
+        assert np.array_equal(cur[0, 1:], cur[1, 1:]) and cur[:, 1:].max() > 0

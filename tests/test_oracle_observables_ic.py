@@ -7,6 +7,9 @@ Observables (SC/OBSERVABLES/*.cpp):
   TXX, TYY, TZZ exactly (every plaquette sits in four clovers, every E link in two averages);
 * closed forms with U = 1 and uniform E (T0i = 0, Tij = q_i q_j e_i . e_j);
 * unitarity and Gauss law against numpy.
+* the output-file profiles (IO::SaveFields, IO::SaveTransverselyIntegratedCurrents,
+  EnergyMomentumTensor::CreateMapBfCollision) against numpy plaquettes and J sums, and their totals against
+  the bulk sums.
 MV initial conditions (SC/COLLIDINGOBJECTS/CollidingObjects.cpp, RevSingleNucleus.cpp,
 SUNcDynamics.cpp:275-350):
 * Poisson solve of a single transverse mode: A = rho / p^2 with the lattice p^2;
@@ -257,3 +260,55 @@ def test_numpy_slab_energy_covering_the_lattice_is_bulk_T00():
     # a slab is the sum of its layers
     parts = sum(slab_energy_np(f, a, z, 1) for z in range(N[2]))
     assert abs(parts - T00) < 1e-12 * abs(T00)
+
+
+# ------------------------------------------------------------------------------------------------
+# Output-file observables: IO::SaveFields, IO::SaveTransverselyIntegratedCurrents,
+# EnergyMomentumTensor::CreateMapBfCollision
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("bc", [0, 1])
+def test_field_profile_against_numpy_and_bulk(bc):
+    from tests.helpers import site_fields_np
+    N = (4, 3, 5)
+    a = (1.0, 0.9, 0.7)
+    f = random_fields(N, seed=21, bc=bc)
+    prof = O.field_profile(f, a, bc)
+    E2, B = site_fields_np(f)
+    P = N[0] * N[1]
+    ref = np.stack([0.5 * (B[..., 0] + B[..., 1]).sum(axis=(1, 2)) / P, 0.5 * B[..., 2].sum(axis=(1, 2)) / P,
+                    0.5 * (E2[..., 0] + E2[..., 1]).sum(axis=(1, 2)) / P, 0.5 * E2[..., 2].sum(axis=(1, 2)) / P], axis=1)
+    assert np.allclose(prof, ref, rtol=1e-13, atol=0)
+    raw = O.bulk(f, a, bc)  # the layer sums add up to the bulk sums
+    tot = 2 * P * prof.sum(axis=0)
+    assert np.allclose(tot, [raw[3] + raw[4], raw[5], raw[0] + raw[1], raw[2]], rtol=1e-13)
+
+
+def test_current_profile_rows_and_columns():
+    N = (3, 4, 5)
+    f = random_fields(N, seed=22)
+    for k, A in enumerate((f.Jp, f.Jm, f.JpS, f.JmS)):  # distinct scales expose a swapped column
+        A *= 1.0 + k
+    f.Jm[0] *= 3.0  # the ghost layers enter as stored
+    out = O.current_profile(f)
+    P = N[0] * N[1]
+    for r in range(N[2] + 2):
+        ref = [0.5 * (A[r] ** 2).sum() / P for A in (f.Jm, f.JmS, f.Jp, f.JpS)]
+        assert np.allclose(out[r], ref, rtol=1e-14), r
+
+
+@pytest.mark.parametrize("Nz", [6, 7])
+def test_transverse_map_against_numpy_and_bulk(Nz):
+    from tests.helpers import site_fields_np
+    N = (4, 3, Nz)
+    a = (1.0, 0.9, 0.7)
+    f = random_fields(N, seed=23)
+    m = O.transverse_map(f, a)
+    E2, B = site_fields_np(f)
+    acube = np.prod(a)
+    c = (np.array(a) / acube) ** 2
+    dens = 0.5 * ((E2 + B) * c).sum(axis=-1) / Nz  # (Nz, Ny, Nx)
+    zs = (Nz - 2) // 2  # the last left layer
+    assert np.allclose(m[..., 0], dens[: zs + 1].sum(axis=0), rtol=1e-13, atol=0)
+    assert np.allclose(m[..., 1], dens[zs + 1:].sum(axis=0), rtol=1e-13, atol=0)
+    T = O.bulk_T(O.bulk(f, a), a)
+    assert abs(m.sum() * Nz - T["T00"]) < 1e-12 * T["T00"]
