@@ -151,6 +151,14 @@ int glasma_set_wilson_lines(glasma_ctx* ctx, char dir, const glasma_mv_params* m
  * alpha = 2).  The gauge-fixed copies U^G, E^G are kept in the context; the dynamical fields are not
  * modified.  stats7 = iterations, max dev, avg dev, max step, avg step, min functional, mean functional. */
 int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, double* stats7);
+/* The same with the update the reference selects by #define (CoulombGaugeFixing.cpp:4,55-67):
+ * GLASMA_GAUGE_FOURIER (param = alpha; identical to glasma_coulomb_gauge), GLASMA_GAUGE_JACOBI
+ * (ALGORITHMS/Jacobi.cpp:13-74, param = step size, 0.3 in the reference) or GLASMA_GAUGE_LOS_ALAMOS
+ * (ALGORITHMS/LosAlamos.cpp:13-87, even then odd checkerboard sites, MaxTraceProjection of w; param = the
+ * over-relaxation probability, which must be 0, the reference's value). */
+enum { GLASMA_GAUGE_FOURIER = 0, GLASMA_GAUGE_JACOBI = 1, GLASMA_GAUGE_LOS_ALAMOS = 2 };
+int glasma_coulomb_gauge_algorithm(glasma_ctx* ctx, int algorithm, double tol, int maxit, double param,
+                                   double* stats7);
 /* GaugeTransformation::Reset / SetIdentity (CoulombGaugeFixing.cpp:26-30) */
 int glasma_gauge_reset(glasma_ctx* ctx);
 /* Gauge-fixed copies (GaugeFixedVariables::U/E, GAUGETRANSFORMATION/GaugeTransformation.cpp:8-21) and G

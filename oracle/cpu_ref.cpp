@@ -1248,7 +1248,44 @@ static void SingleSheet(const MVParams& P, double* rho, int zLow, int zHigh, dou
 //  periodic momenta in all three directions, and the zero mode passed through unnormalised.
 //  stats: [0] iterations, [1] max dev, [2] avg dev, [3] max step, [4] avg step, [5] min functional, [6] sum functional
 // ============================================================================================
-static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, double* EG, double tol, int maxit, double alpha, double* stats) {
+// SUNcGroup::Extended::MaxTraceProjection (SC/SUNc/SU3/GroupOperations.cpp:358-378): seven Cabibbo-Marinari
+// sweeps over the SU(2) subgroups (0,1), (0,2), (1,2) (SC/SUNc/SU3/util/CabbiboMarinariProjection.cpp:4-96);
+// each left-multiplies G and VNew by the subgroup element that makes VNew's 2x2 block proportional to a
+// positive matrix.  VT = G^dagger.
+static void MaxTraceProjection(const cd* V, cd* VT) {
+    auto MI = SU3::MI;
+    cd G[9], VNew[9], A[9], B[9];
+    SU3::Unit(G);
+    std::memcpy(VNew, V, sizeof(VNew));
+    const int sub[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int step = 0; step < 7; step++)
+        for (int k = 0; k < 3; k++) {
+            const int p = sub[k][0], q = sub[k][1];
+            const cd app = std::conj(VNew[MI(p, p)]) + VNew[MI(q, q)];
+            const cd apq = std::conj(VNew[MI(q, p)]) - VNew[MI(p, q)];
+            const double inv = 1.0 / std::sqrt(std::norm(app) + std::norm(apq));
+            SU3::Unit(A);
+            A[MI(p, p)] = inv * app;
+            A[MI(p, q)] = inv * apq;
+            A[MI(q, p)] = inv * (std::conj(VNew[MI(p, q)]) - VNew[MI(q, p)]);
+            A[MI(q, q)] = inv * (std::conj(VNew[MI(q, q)]) + VNew[MI(p, p)]);
+            SU3::UU(A, G, B); std::memcpy(G, B, sizeof(B));
+            SU3::UU(A, VNew, B); std::memcpy(VNew, B, sizeof(B));
+        }
+    SU3::Dag(G, VT);
+}
+
+// CoulombGaugeFixing::SetCoulombGauge (SC/GAUGETRANSFORMATION/CoulombGaugeFixing.cpp:32-88) with one of the
+// three update algorithms the reference selects by #define (:4, :55-67):
+//   algo 0  FourierAccelerationAlgorithm (ALGORITHMS/FourierAcceleration.cpp:15-148), param = alpha;
+//   algo 1  JacobiAlgorithm (ALGORITHMS/Jacobi.cpp:13-74): g <- exp(-i param Delta t)^dag g at every site at
+//           once, param = alphaStepSize (0.3 in the reference);
+//   algo 2  LosAlamosAlgorithm (ALGORITHMS/LosAlamos.cpp:13-87): even then odd checkerboard sites ((x+y+z)
+//           % 2), g <- MaxTraceProjection(w)^dag g, the links re-transformed after each half sweep; the
+//           monitors accumulate over both halves.  Over-relaxation probability 0 (the reference's value).
+// Monitors (CoulombGaugeDeviation.cpp:43-70): max / RMS of Delta, max / RMS of |tr(1 - wTilde)|, min / mean
+// of Re tr w.  The Jacobi sweep's unreduced MaxStepSize (a data race in the reference) is taken as the max.
+static void CoulombGaugeAlgo(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, double* EG, int algo, double tol, int maxit, double param, double* stats) {
     int Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
     long np = L.plane(), V = np * Nz;
     auto syncG = [&]() { L.sync(Gt, 9L); };
@@ -1265,6 +1302,20 @@ static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, do
                 }
         syncG();
     };
+    // GET_LOCAL_GAUGE_VIOLATION (CoulombGaugeDeviation.cpp:23-41): w and Delta^a = Re tr(i t^a w)
+    auto local_w = [&](int x, int y, int z, cd* w, double* dv) {
+        cd d0[9], d1[9], d2[9];
+        SU3::Dag(L.Ul(UG, x - 1, y, z, 0), d0);
+        SU3::Dag(L.Ul(UG, x, y - 1, z, 1), d1);
+        SU3::Dag(L.Ul(UG, x, y, z - 1, 2), d2);
+        const cd* u0 = L.Ul(UG, x, y, z, 0);
+        const cd* u1 = L.Ul(UG, x, y, z, 1);
+        const cd* u2 = L.Ul(UG, x, y, z, 2);
+        double zf = (Nz > 1) ? 1.0 : 0.0;
+        for (int k = 0; k < 9; k++)
+            w[k] = SQR(1.0 / L.a[0]) * (u0[k] + d0[k]) + SQR(1.0 / L.a[1]) * (u1[k] + d1[k]) + zf * SQR(1.0 / L.a[2]) * (u2[k] + d2[k]);
+        SU3::ReTrIGenU(w, dv);
+    };
     // CopyFields: only the interior (the ghosts of UG keep their identity / previous contents)
     std::memcpy(L.Ul(UG, 0, 0, 0, 0), L.Ul(U, 0, 0, 0, 0), 27 * V * sizeof(cd));
     std::memcpy(L.El(EG, 0, 0, 0, 0), L.El(E, 0, 0, 0, 0), 24 * V * sizeof(double));
@@ -1272,66 +1323,87 @@ static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, do
     int dims[3] = {Nx, Ny, Nz};
     std::vector<cd> F(8 * V);
     double gmax = 1.0, gavg = 0, smax = 0, savg = 0, fmin = 0, fsum = 0;
+    const double fm0 = 6.0 * (1.0 / SQR(L.a[0]) + 1.0 / SQR(L.a[1]) + 1.0 / SQR(L.a[2]));
     int n = 0;
     while (n < maxit && gmax > tol) {
-        double mx = 0.0, sq = 0.0, fs = 0.0;
-        double fm = 6.0 * (1.0 / SQR(L.a[0]) + 1.0 / SQR(L.a[1]) + 1.0 / SQR(L.a[2]));
+        double mx = 0.0, sq = 0.0, fs = 0.0, fm = fm0, sm = 0.0, ssq = 0.0;
+        if (algo == 2) {
+            for (int parity = 0; parity < 2; parity++) {
+#pragma omp parallel for reduction(+ : sq, fs, ssq) reduction(max : mx, sm) reduction(min : fm)
+                for (int z = 0; z < Nz; z++)
+                    for (int y = 0; y < Ny; y++)
+                        for (int x = 0; x < Nx; x++) {
+                            if ((x + y + z) % 2 != parity) continue;
+                            cd w[9], wt[9], gn[9];
+                            double dv[8];
+                            local_w(x, y, z, w, dv);
+                            for (int a = 0; a < 8; a++) {
+                                mx = std::max(mx, std::fabs(dv[a]));
+                                sq += SQR(dv[a]);
+                            }
+                            double lf = SU3::ReTr(w);
+                            fm = std::min(fm, lf);
+                            fs += lf;
+                            MaxTraceProjection(w, wt);
+                            double st = std::abs(3.0 - (wt[0] + wt[4] + wt[8]));
+                            sm = std::max(sm, st);
+                            ssq += st * st;
+                            cd* g = Gt + 9 * L.idx3(x, y, z);
+                            SU3::DU(wt, g, gn);
+                            std::memcpy(g, gn, sizeof(gn));
+                        }
+                transform_links();
+            }
+        } else {
 #pragma omp parallel for reduction(+ : sq, fs) reduction(max : mx) reduction(min : fm)
-        for (int z = 0; z < Nz; z++)
-            for (int y = 0; y < Ny; y++)
-                for (int x = 0; x < Nx; x++) {
-                    cd d0[9], d1[9], d2[9], w[9];
-                    SU3::Dag(L.Ul(UG, x - 1, y, z, 0), d0);
-                    SU3::Dag(L.Ul(UG, x, y - 1, z, 1), d1);
-                    SU3::Dag(L.Ul(UG, x, y, z - 1, 2), d2);
-                    const cd* u0 = L.Ul(UG, x, y, z, 0);
-                    const cd* u1 = L.Ul(UG, x, y, z, 1);
-                    const cd* u2 = L.Ul(UG, x, y, z, 2);
-                    double zf = (Nz > 1) ? 1.0 : 0.0;
-                    for (int k = 0; k < 9; k++)
-                        w[k] = SQR(1.0 / L.a[0]) * (u0[k] + d0[k]) + SQR(1.0 / L.a[1]) * (u1[k] + d1[k]) + zf * SQR(1.0 / L.a[2]) * (u2[k] + d2[k]);
-                    double dv[8];
-                    SU3::ReTrIGenU(w, dv);
-                    for (int a = 0; a < 8; a++) {
-                        mx = std::max(mx, std::fabs(dv[a]));
-                        sq += SQR(dv[a]);
+            for (int z = 0; z < Nz; z++)
+                for (int y = 0; y < Ny; y++)
+                    for (int x = 0; x < Nx; x++) {
+                        cd w[9];
+                        double dv[8];
+                        local_w(x, y, z, w, dv);
+                        for (int a = 0; a < 8; a++) {
+                            mx = std::max(mx, std::fabs(dv[a]));
+                            sq += SQR(dv[a]);
+                        }
+                        double lf = SU3::ReTr(w);
+                        fm = std::min(fm, lf);
+                        fs += lf;
+                        long s = x + (long)Nx * (y + (long)Ny * z);
+                        for (int a = 0; a < 8; a++) F[a + 8 * s] = dv[a];
                     }
-                    double lf = SU3::ReTr(w);
-                    fm = std::min(fm, lf);
-                    fs += lf;
-                    long s = x + (long)Nx * (y + (long)Ny * z);
-                    for (int a = 0; a < 8; a++) F[a + 8 * s] = dv[a];
-                }
-        fft_nd(F.data(), 3, dims, 8, -1);
-        double nf = 1.0 / ((double)V);
-        for (int pz = 0; pz < Nz; pz++)
-            for (int py = 0; py < Ny; py++)
-                for (int px = 0; px < Nx; px++) {
-                    double p2 = (2.0 - 2.0 * std::cos(2.0 * PI * px / (double)Nx)) / SQR(L.a[0]) + (2.0 - 2.0 * std::cos(2.0 * PI * py / (double)Ny)) / SQR(L.a[1]) + (2.0 - 2.0 * std::cos(2.0 * PI * pz / (double)Nz)) / SQR(L.a[2]);
-                    if (p2 > 0.0) {
-                        long s = px + (long)Nx * (py + (long)Ny * pz);
-                        for (int a = 0; a < 8; a++) F[a + 8 * s] = nf * F[a + 8 * s] / p2;
-                    }
-                }
-        fft_nd(F.data(), 3, dims, 8, +1);
-        double sm = 0.0, ssq = 0.0;
+            if (algo == 0) {
+                fft_nd(F.data(), 3, dims, 8, -1);
+                double nf = 1.0 / ((double)V);
+                for (int pz = 0; pz < Nz; pz++)
+                    for (int py = 0; py < Ny; py++)
+                        for (int px = 0; px < Nx; px++) {
+                            double p2 = (2.0 - 2.0 * std::cos(2.0 * PI * px / (double)Nx)) / SQR(L.a[0]) + (2.0 - 2.0 * std::cos(2.0 * PI * py / (double)Ny)) / SQR(L.a[1]) + (2.0 - 2.0 * std::cos(2.0 * PI * pz / (double)Nz)) / SQR(L.a[2]);
+                            if (p2 > 0.0) {
+                                long s = px + (long)Nx * (py + (long)Ny * pz);
+                                for (int a = 0; a < 8; a++) F[a + 8 * s] = nf * F[a + 8 * s] / p2;
+                            }
+                        }
+                fft_nd(F.data(), 3, dims, 8, +1);
+            }
 #pragma omp parallel for reduction(+ : ssq) reduction(max : sm)
-        for (int z = 0; z < Nz; z++)
-            for (int y = 0; y < Ny; y++)
-                for (int x = 0; x < Nx; x++) {
-                    long s = x + (long)Nx * (y + (long)Ny * z);
-                    double ub[8];
-                    for (int a = 0; a < 8; a++) ub[a] = F[a + 8 * s].real();
-                    cd wt[9], gn[9];
-                    SU3::MatrixIExp(-alpha, ub, wt);
-                    double st = std::abs(3.0 - (wt[0] + wt[4] + wt[8]));
-                    sm = std::max(sm, st);
-                    ssq += st * st;
-                    cd* g = Gt + 9 * L.idx3(x, y, z);
-                    SU3::DU(wt, g, gn);
-                    std::memcpy(g, gn, sizeof(gn));
-                }
-        transform_links();
+            for (int z = 0; z < Nz; z++)
+                for (int y = 0; y < Ny; y++)
+                    for (int x = 0; x < Nx; x++) {
+                        long s = x + (long)Nx * (y + (long)Ny * z);
+                        double ub[8];
+                        for (int a = 0; a < 8; a++) ub[a] = F[a + 8 * s].real();
+                        cd wt[9], gn[9];
+                        SU3::MatrixIExp(-param, ub, wt);
+                        double st = std::abs(3.0 - (wt[0] + wt[4] + wt[8]));
+                        sm = std::max(sm, st);
+                        ssq += st * st;
+                        cd* g = Gt + 9 * L.idx3(x, y, z);
+                        SU3::DU(wt, g, gn);
+                        std::memcpy(g, gn, sizeof(gn));
+                    }
+            transform_links();
+        }
         gmax = mx; gavg = std::sqrt(sq / (V * 8.0));
         smax = sm; savg = std::sqrt(ssq / (V * 8.0));
         fmin = fm; fsum = fs / V;
@@ -1346,6 +1418,10 @@ static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, do
                     SU3::AdjointMultiplication(Gt + 9 * L.idx3(x, y, z), L.El(E, x, y, z, mu), L.El(EG, x, y, z, mu));
     syncG();
     stats[0] = n; stats[1] = gmax; stats[2] = gavg; stats[3] = smax; stats[4] = savg; stats[5] = fmin; stats[6] = fsum;
+}
+
+static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, double* EG, double tol, int maxit, double alpha, double* stats) {
+    CoulombGaugeAlgo(L, U, E, Gt, UG, EG, 0, tol, maxit, alpha, stats);
 }
 
 // ============================================================================================
@@ -1516,6 +1592,12 @@ void ref_coulomb_gauge(const int* N, const double* a, double* U, double* E, doub
     Lat<SU3> L = mk3(N, a, 0);
     CoulombGauge(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, tol, maxit, alpha, stats);
 }
+void ref_coulomb_gauge_algo(const int* N, const double* a, double* U, double* E, double* G, double* UG, double* EG,
+                            int algo, double tol, int maxit, double param, double* stats) {
+    Lat<SU3> L = mk3(N, a, 0);
+    CoulombGaugeAlgo(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, algo, tol, maxit, param, stats);
+}
+void ref_max_trace_projection(const double* V, double* VT) { MaxTraceProjection((const cd*)V, (cd*)VT); }
 // Gauge transformation of every field with a site-dependent g (with ghosts): U -> g U g^dag(x+mu), E, J -> g E g^dag.
 void ref_gauge_transform(const int* N, const double* a, int bc, const double* G, double* U, double* E, double* Jp, double* Jm, double* JpS, double* JmS) {
     Lat<SU3> L = mk3(N, a, bc);

@@ -72,6 +72,9 @@ def lib():
         L.ref_coulomb_gauge.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                                         ctypes.c_double, _dp]
         L.ref_gauge_transform.argtypes = [_ip, _dp, ctypes.c_int, _dp] + [_dp] * 6
+        L.ref_coulomb_gauge_algo.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double, _dp]
+        L.ref_max_trace_projection.argtypes = [_dp, _dp]
         L.ref_sync_ghosts.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_long]
         L.ref_num_threads.restype = ctypes.c_int
         L.ref_set_threads.argtypes = [ctypes.c_int]
@@ -296,6 +299,28 @@ def coulomb_gauge(f: Fields, a, G: np.ndarray, tol: float, maxit: int, alpha: fl
     stats = np.zeros(7)
     lib().ref_coulomb_gauge(ivec(f.N), dvec(a), P(f.U), P(f.E), P(G), P(UG), P(EG), tol, maxit, alpha, P(stats))
     return UG, EG, stats
+
+
+ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2}
+
+
+def coulomb_gauge_algo(f: Fields, a, G: np.ndarray, algo: str, tol: float, maxit: int, param: float):
+    """SetCoulombGauge with the Fourier-accelerated (param = alpha), Jacobi (param = step size) or Los Alamos
+    (param = over-relaxation probability, 0) update; G is updated in place."""
+    UG = Fields(f.N).U
+    EG = np.zeros_like(f.E)
+    stats = np.zeros(7)
+    lib().ref_coulomb_gauge_algo(ivec(f.N), dvec(a), P(f.U), P(f.E), P(G), P(UG), P(EG), ALGORITHMS[algo], tol, maxit,
+                                 param, P(stats))
+    return UG, EG, stats
+
+
+def max_trace_projection(V: np.ndarray) -> np.ndarray:
+    """SUNcGroup::Extended::MaxTraceProjection of a 3x3 complex matrix (reference column-major storage)."""
+    V = np.ascontiguousarray(V, np.complex128)
+    out = np.zeros(9, np.complex128)
+    lib().ref_max_trace_projection(P(V.reshape(9)), P(out))
+    return out
 
 
 def gauge_transform(f: Fields, a, G: np.ndarray, bc: int = 0):

@@ -12,6 +12,14 @@
 // U^G_mu(x) = g(x) U_mu(x) g^dag(x + mu).  Monitors (max / RMS of Delta and of the step, min / mean of the
 // functional Re tr w) come from deterministic two-level reductions.
 //
+// The reference's two other update algorithms (selected there by #define, CoulombGaugeFixing.cpp:4,55-67) share
+// w, Delta, the monitors and the link transformation:
+//   Jacobi      ALGORITHMS/Jacobi.cpp:13-74      g <- exp(-i alpha Delta t)^dag g at every site, no FFT;
+//   Los Alamos  ALGORITHMS/LosAlamos.cpp:13-87   even, then odd checkerboard sites ((x + y + z) % 2):
+//               g <- MaxTraceProjection(w)^dag g (seven Cabibbo-Marinari sweeps, SUNc/SU3/GroupOperations.cpp:358-378,
+//               SUNc/SU3/util/CabbiboMarinariProjection.cpp), links re-transformed after each half sweep; the
+//               monitors cover both halves.  Over-relaxation probability 0 only (the reference's value).
+//
 // Ghost-layer behaviour of the reference, reproduced: the gauge-fixed links' z ghost layers are never
 // written, so U^G(z = -1) is the identity; the gauge transformation's top ghost layer is synchronised
 // after each link transformation, so a transformation sees the top layer of the previous iteration.  Here
@@ -97,6 +105,29 @@ __device__ void block_partials(double s0, double s1, double m0, double m1, doubl
     }
 }
 
+// w(x) = sum_mu c_mu (U^G_mu(x) + U^G_mu(x - mu)^dag), the z term only for Nz > 1; below z = 0 the gauge-fixed
+// links' ghost layer is the identity (GET_LOCAL_GAUGE_VIOLATION, CoulombGaugeDeviation.cpp:23-41)
+__device__ __forceinline__ void local_w(const double* __restrict__ UG, const Geom& g, long s, int x, int y, int z,
+                                        double cx, double cy, double cz, M3& W) {
+    M3 A, B;
+    const double zf = g.nz > 1 ? 1.0 : 0.0;
+    for (int k = 0; k < 9; k++) { W.re[k] = 0.0; W.im[k] = 0.0; }
+    for (int mu = 0; mu < 3; mu++) {
+        const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
+        ldm<long>(UG, kUPlanes, 18 * mu, s, A);
+        if (mu == 0) ldm<long>(UG, kUPlanes, 0, sidx(g, x - 1, y, z), B);
+        else if (mu == 1) ldm<long>(UG, kUPlanes, 18, sidx(g, x, y - 1, z), B);
+        else if (z > 0) ldm<long>(UG, kUPlanes, 36, s - g.P, B);
+        else set_unit(B);
+        // w += c (A + B^dag), grouped as the reference adds the three directions
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                W.re[mi(i, j)] += c * (A.re[mi(i, j)] + B.re[mi(j, i)]);
+                W.im[mi(i, j)] += c * (A.im[mi(i, j)] - B.im[mi(j, i)]);
+            }
+    }
+}
+
 // Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52)
 __global__ __launch_bounds__(kCgBlock) void k_cg_dev(const double* __restrict__ UG, double* __restrict__ F,
                                                       double* __restrict__ part, Geom g, double cx, double cy,
@@ -107,23 +138,8 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev(const double* __restrict__ 
         const int z = (int)(s / g.P);
         const long s2 = s - (long)z * g.P;
         const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
-        M3 A, B, W;
-        const double zf = g.nz > 1 ? 1.0 : 0.0;
-        for (int k = 0; k < 9; k++) { W.re[k] = 0.0; W.im[k] = 0.0; }
-        for (int mu = 0; mu < 3; mu++) {
-            const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
-            ldm<long>(UG, kUPlanes, 18 * mu, s, A);
-            if (mu == 0) ldm<long>(UG, kUPlanes, 0, sidx(g, x - 1, y, z), B);
-            else if (mu == 1) ldm<long>(UG, kUPlanes, 18, sidx(g, x, y - 1, z), B);
-            else if (z > 0) ldm<long>(UG, kUPlanes, 36, s - g.P, B);
-            else set_unit(B);
-            // w += c (A + B^dag), grouped as the reference adds the three directions
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) {
-                    W.re[mi(i, j)] += c * (A.re[mi(i, j)] + B.re[mi(j, i)]);
-                    W.im[mi(i, j)] += c * (A.im[mi(i, j)] - B.im[mi(j, i)]);
-                }
-        }
+        M3 W;
+        local_w(UG, g, s, x, y, z, cx, cy, cz, W);
         double dv[8];
         retr_igen(W, dv);
         for (int a = 0; a < 8; a++) {
@@ -137,6 +153,87 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev(const double* __restrict__ 
         fs = lf;
     }
     block_partials(sq, fs, mx, fm, part + 4 * (long)blockIdx.x);
+}
+
+// rows P and Q of X <- A X, A the identity outside the SU(2) block [[a, b], [c, d]] of rows / columns P, Q
+template <int P, int Q>
+__device__ __forceinline__ void left_su2(M3& X, double ar, double ai, double br, double bi, double cr, double ci,
+                                         double dr, double di) {
+    for (int j = 0; j < 3; j++) {
+        const double pr = X.re[mi(P, j)], pi = X.im[mi(P, j)], qr = X.re[mi(Q, j)], qi = X.im[mi(Q, j)];
+        X.re[mi(P, j)] = (ar * pr - ai * pi) + (br * qr - bi * qi);
+        X.im[mi(P, j)] = (ar * pi + ai * pr) + (br * qi + bi * qr);
+        X.re[mi(Q, j)] = (cr * pr - ci * pi) + (dr * qr - di * qi);
+        X.im[mi(Q, j)] = (cr * pi + ci * pr) + (dr * qi + di * qr);
+    }
+}
+
+// one Cabibbo-Marinari step in subgroup (P, Q) (CabbiboMarinariProjection.cpp:4-96): the SU(2) element that
+// turns VNew's (P, Q) block into a positive multiple of the identity, applied to G and VNew
+template <int P, int Q>
+__device__ __forceinline__ void cm_step(M3& G, M3& V) {
+    const double ar = V.re[mi(P, P)] + V.re[mi(Q, Q)], ai = -V.im[mi(P, P)] + V.im[mi(Q, Q)];  // conj(Vpp) + Vqq
+    const double br = V.re[mi(Q, P)] - V.re[mi(P, Q)], bi = -V.im[mi(Q, P)] - V.im[mi(P, Q)];  // conj(Vqp) - Vpq
+    const double cr = V.re[mi(P, Q)] - V.re[mi(Q, P)], ci = -V.im[mi(P, Q)] - V.im[mi(Q, P)];  // conj(Vpq) - Vqp
+    const double dr = V.re[mi(Q, Q)] + V.re[mi(P, P)], di = -V.im[mi(Q, Q)] + V.im[mi(P, P)];  // conj(Vqq) + Vpp
+    const double inv = 1.0 / sqrt((ar * ar + ai * ai) + (br * br + bi * bi));
+    left_su2<P, Q>(G, inv * ar, inv * ai, inv * br, inv * bi, inv * cr, inv * ci, inv * dr, inv * di);
+    left_su2<P, Q>(V, inv * ar, inv * ai, inv * br, inv * bi, inv * cr, inv * ci, inv * dr, inv * di);
+}
+
+// SUNcGroup::Extended::MaxTraceProjection (SUNc/SU3/GroupOperations.cpp:358-378): Vt = G^dag after seven sweeps
+__device__ void max_trace_projection(const M3& V, M3& Vt) {
+    M3 G, W = V;
+    set_unit(G);
+#pragma unroll 1
+    for (int n = 0; n < 7; n++) {
+        cm_step<0, 1>(G, W);
+        cm_step<0, 2>(G, W);
+        cm_step<1, 2>(G, W);
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            Vt.re[mi(i, j)] = G.re[mi(j, i)];
+            Vt.im[mi(i, j)] = -G.im[mi(j, i)];
+        }
+}
+
+// Los Alamos half sweep over the sites of one checkerboard parity (LosAlamos.cpp:13-67): monitors of Delta and
+// of the functional into dpart, of the step |tr(1 - wTilde)| into spart; g <- wTilde^dag g
+__global__ __launch_bounds__(kCgBlock) void k_cg_losalamos(const double* __restrict__ UG, double* __restrict__ G,
+                                                            double* __restrict__ dpart, double* __restrict__ spart,
+                                                            Geom g, double cx, double cy, double cz, double fm0,
+                                                            int parity) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0, ssq = 0.0, sm = 0.0;
+    if (s < g.S) {
+        const int z = (int)(s / g.P);
+        const long s2 = s - (long)z * g.P;
+        const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+        if (((x + y + z) & 1) == parity) {
+            M3 W, Wt, Gx, Gn;
+            local_w(UG, g, s, x, y, z, cx, cy, cz, W);
+            double dv[8];
+            retr_igen(W, dv);
+            for (int a = 0; a < 8; a++) {
+                mx = fmax(mx, fabs(dv[a]));
+                sq += dv[a] * dv[a];
+            }
+            const double lf = W.re[0] + W.re[4] + W.re[8];
+            fm = fmin(fm, lf);
+            fs = lf;
+            max_trace_projection(W, Wt);
+            const double st = hypot(3.0 - (Wt.re[0] + Wt.re[4] + Wt.re[8]), -(Wt.im[0] + Wt.im[4] + Wt.im[8]));
+            sm = st;
+            ssq = st * st;
+            ldm<long>(G, kGPlanes, 0, s, Gx);
+            mul_dn(Wt, Gx, Gn);
+            stm<long>(G, kGPlanes, 0, s, Gn);
+        }
+    }
+    block_partials(sq, fs, mx, fm, dpart + 4 * (long)blockIdx.x);
+    __syncthreads();  // block_partials' shared scratch is reused
+    block_partials(ssq, 0.0, sm, 0.0, spart + 4 * (long)blockIdx.x);
 }
 
 // Fourier acceleration: F *= 1/(V p^2) for p^2 > 0 (FourierAcceleration.cpp:55-75)
@@ -239,24 +336,33 @@ int transform_links(glasma_ctx* ctx) {
 
 using namespace glasma;
 
-extern "C" int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, double* stats7) {
+namespace {
+
+int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param, double* stats7) {
     if (!ctx) return GLASMA_ERR_ARG;
     if (maxit < 0 || !(tol >= 0.0)) return set_error(ctx, GLASMA_ERR_ARG, "invalid tolerance or iteration cap");
+    if (algo < GLASMA_GAUGE_FOURIER || algo > GLASMA_GAUGE_LOS_ALAMOS)
+        return set_error(ctx, GLASMA_ERR_ARG, "unknown gauge-fixing algorithm");
+    if (algo == GLASMA_GAUGE_LOS_ALAMOS && param != 0.0)
+        return set_error(ctx, GLASMA_ERR_ARG,
+                         "Los Alamos over-relaxation needs the reference's shared random stream; only probability 0 "
+                         "is supported");
     int rc = ensure_gauge(ctx);
     if (!rc) rc = ensure_gauge_fixed(ctx);
     if (rc) return rc;
     const Geom& g = ctx->g;
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
-    rc = ensure_red(ctx, 4 * (size_t)nblk + 8);
-    if (!rc) rc = ensure_fftbuf(ctx, 16 * (size_t)g.S);
+    rc = ensure_red(ctx, 16 * (size_t)nblk + 8);
+    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 16 * (size_t)g.S);
     if (rc) return rc;
     const double* a = ctx->p.a;
     // weights (1/a_mu)^2 of w and the functional's starting minimum 6 sum_mu 1/a_mu^2, rounded as the
     // reference writes them
     const double cx = (1.0 / a[0]) * (1.0 / a[0]), cy = (1.0 / a[1]) * (1.0 / a[1]), cz = (1.0 / a[2]) * (1.0 / a[2]);
     const double fm0 = 6.0 * (1.0 / (a[0] * a[0]) + 1.0 / (a[1] * a[1]) + 1.0 / (a[2] * a[2]));
-    double* part = ctx->red;
-    double* fin = ctx->red + 4 * nblk;
+    double* part = ctx->red;                // 2 * nblk blocks of 4 (Los Alamos: both half sweeps)
+    double* spart = ctx->red + 8 * nblk;    // Los Alamos step monitors
+    double* fin = ctx->red + 16 * nblk;
     double* F = ctx->fftbuf;
     if ((rc = transform_links(ctx))) return rc;
     double gmax = 1.0, gavg = 0.0, smax = 0.0, savg = 0.0, fmin = 0.0, fsum = 0.0;
@@ -264,15 +370,28 @@ extern "C" int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, doub
     const double V = (double)g.S;
     while (n < maxit && gmax > tol) {
         double h[8];
-        hipLaunchKernelGGL(k_cg_dev, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->UG, F, part, g, cx, cy, cz,
-                           fm0);
-        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
-        if ((rc = fft_3d(ctx->stream, F, g, 8, -1))) return set_error(ctx, rc, "gauge fixing FFT failed");
-        hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1], a[2]);
-        if ((rc = fft_3d(ctx->stream, F, g, 8, +1))) return set_error(ctx, rc, "gauge fixing FFT failed");
-        hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g, alpha);
-        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
-        if ((rc = transform_links(ctx))) return rc;
+        if (algo == GLASMA_GAUGE_LOS_ALAMOS) {
+            for (int parity = 0; parity < 2; parity++) {
+                hipLaunchKernelGGL(k_cg_losalamos, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->UG, ctx->G,
+                                   part + 4 * parity * nblk, spart + 4 * parity * nblk, g, cx, cy, cz, fm0, parity);
+                if ((rc = transform_links(ctx))) return rc;
+            }
+            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, 2 * nblk, fin, fm0);
+            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, spart, 2 * nblk, fin + 4, 0.0);
+        } else {
+            hipLaunchKernelGGL(k_cg_dev, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->UG, F, part, g, cx, cy,
+                               cz, fm0);
+            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
+            if (algo == GLASMA_GAUGE_FOURIER) {
+                if ((rc = fft_3d(ctx->stream, F, g, 8, -1))) return set_error(ctx, rc, "gauge fixing FFT failed");
+                hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1], a[2]);
+                if ((rc = fft_3d(ctx->stream, F, g, 8, +1))) return set_error(ctx, rc, "gauge fixing FFT failed");
+            }
+            hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g,
+                               param);
+            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
+            if ((rc = transform_links(ctx))) return rc;
+        }
         GL_HIP(ctx, hipMemcpyAsync(h, fin, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
         gmax = h[2];
@@ -297,6 +416,17 @@ extern "C" int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, doub
         stats7[6] = fsum;
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, double* stats7) {
+    return coulomb_gauge(ctx, GLASMA_GAUGE_FOURIER, tol, maxit, alpha, stats7);
+}
+
+extern "C" int glasma_coulomb_gauge_algorithm(glasma_ctx* ctx, int algorithm, double tol, int maxit, double param,
+                                              double* stats7) {
+    return coulomb_gauge(ctx, algorithm, tol, maxit, param, stats7);
 }
 
 extern "C" int glasma_gauge_reset(glasma_ctx* ctx) {

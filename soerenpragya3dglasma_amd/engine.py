@@ -345,6 +345,20 @@ class Glasma:
         self._chk(_lib.lib().glasma_coulomb_gauge(self._h, tol, int(maxit), alpha, _p(stats)), "glasma_coulomb_gauge")
         return stats
 
+    GAUGE_ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2}
+
+    def coulomb_gauge_algorithm(self, algorithm: str, tol: float = 1e-15, maxit: int = 1000,
+                                param: float = None) -> np.ndarray:
+        """Coulomb gauge fixing with the reference's "fourier" (param = alpha, 2), "jacobi" (param = step size,
+        0.3) or "losalamos" (param = over-relaxation probability, 0) update."""
+        algo = self.GAUGE_ALGORITHMS[algorithm]
+        if param is None:
+            param = (2.0, 0.3, 0.0)[algo]
+        stats = np.zeros(7)
+        self._chk(_lib.lib().glasma_coulomb_gauge_algorithm(self._h, algo, tol, int(maxit), float(param), _p(stats)),
+                  "glasma_coulomb_gauge_algorithm")
+        return stats
+
     def gauge_reset(self):
         self._chk(_lib.lib().glasma_gauge_reset(self._h), "glasma_gauge_reset")
 

@@ -1,0 +1,83 @@
+"""GPU parity of the reference's alternative Coulomb gauge-fixing updates (ALGORITHMS/Jacobi.cpp,
+ALGORITHMS/LosAlamos.cpp) through glasma_coulomb_gauge_algorithm, against the oracle: a fixed number of
+iterations on open-z lattices (where the reference's iterations stall rather than converge), convergence
+on a transverse (Nz = 1) lattice, and the argument checks."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import random_fields, upload
+
+pytestmark = pytest.mark.gpu
+
+A1 = (1.0, 1.0, 1.0)
+
+
+def _identity_G(N):
+    G = np.zeros((N[2] + 2, N[1], N[0], 3, 3), complex)
+    G[..., 0, 0] = G[..., 1, 1] = G[..., 2, 2] = 1
+    return G
+
+
+@pytest.mark.parametrize("algo,param", [("jacobi", 0.3), ("losalamos", 0.0)])
+@pytest.mark.parametrize("N,a,iters", [((6, 6, 8), A1, 4), ((5, 6, 3), (1.0, 0.9, 0.6), 3), ((8, 6, 1), A1, 10)])
+def test_fixed_iterations_match_oracle(algo, param, N, a, iters):
+    from soerenpragya3dglasma_amd import Glasma
+    f = random_fields(N, seed=23, link_scale=0.3, e_scale=0.3)
+    G = _identity_G(N)
+    UG_o, EG_o, st_o = O.coulomb_gauge_algo(f, a, G, algo, 0.0, iters, param)
+    with Glasma(N, a, 0.1, 0) as g:
+        upload(g, f)
+        st = g.coulomb_gauge_algorithm(algo, 0.0, iters, param)
+        UG, EG, Gd = g.get_gauge_fixed()
+    print(algo, st, st_o)
+    assert st[0] == st_o[0] == iters
+    assert np.allclose(st[1:], st_o[1:], rtol=1e-10, atol=1e-13)
+    assert np.abs(Gd[1:-1] - G[1:-1]).max() < 1e-11
+    assert np.abs(UG[1:-1] - UG_o[1:-1]).max() < 1e-11
+    assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < 1e-11
+
+
+@pytest.mark.parametrize("algo,param,cap", [("jacobi", 0.3, 3000), ("losalamos", 0.0, 3000)])
+def test_transverse_gauge_converges(algo, param, cap):
+    from soerenpragya3dglasma_amd import Glasma
+    N = (8, 8, 1)
+    f = random_fields(N, seed=5, link_scale=0.3, e_scale=0.3)
+    _, _, st_o = O.coulomb_gauge_algo(f.copy(), A1, _identity_G(N), algo, 1e-12, cap, param)
+    with Glasma(N, A1, 0.1, 0) as g:
+        upload(g, f)
+        st = g.coulomb_gauge_algorithm(algo, 1e-12, cap, param)
+        UG, EG, _ = g.get_gauge_fixed()
+    print(algo, st, st_o)
+    assert st[1] <= 1e-12 and st[0] < cap
+    assert abs(st[0] - st_o[0]) <= 2
+    assert abs(st[6] - st_o[6]) < 1e-12 * abs(st_o[6])
+    fg = f.copy()
+    fg.U[1:-1] = UG[1:-1]
+    fg.E[1:-1] = EG[1:-1]
+    fg.sync(0)
+    b0, b1 = O.bulk(f, A1), O.bulk(fg, A1)
+    assert np.abs(b1[0:3] - b0[0:3]).max() < 1e-12 * b0[0:3].max()
+
+
+def test_fourier_through_the_algorithm_entry_is_the_same():
+    from soerenpragya3dglasma_amd import Glasma
+    N = (6, 4, 3)
+    f = random_fields(N, seed=2, link_scale=0.3)
+    with Glasma(N, A1, 0.1, 0) as g, Glasma(N, A1, 0.1, 0) as h:
+        upload(g, f)
+        upload(h, f)
+        s1 = g.coulomb_gauge(0.0, 3, 2.0)
+        s2 = h.coulomb_gauge_algorithm("fourier", 0.0, 3, 2.0)
+        assert np.array_equal(s1, s2)
+        assert np.array_equal(g.get_gauge_fixed()[2], h.get_gauge_fixed()[2])
+
+
+def test_argument_errors():
+    from soerenpragya3dglasma_amd import Glasma, GlasmaError
+    with Glasma((4, 4, 2), A1, 0.1, 0) as g:
+        with pytest.raises(GlasmaError, match="over-relaxation"):
+            g.coulomb_gauge_algorithm("losalamos", 1e-12, 10, 0.5)
+        from soerenpragya3dglasma_amd import _lib
+        assert _lib.lib().glasma_coulomb_gauge_algorithm(g._h, 7, 1e-12, 10, 0.0, None) != 0
+        assert b"unknown" in _lib.lib().glasma_last_error(g._h)

@@ -1,0 +1,161 @@
+"""Pins of the oracle's alternative Coulomb gauge-fixing updates (SC/GAUGETRANSFORMATION/ALGORITHMS/Jacobi.cpp,
+LosAlamos.cpp) and of SUNcGroup::Extended::MaxTraceProjection (SC/SUNc/SU3/GroupOperations.cpp:358-378).  CPU only.
+
+* MaxTraceProjection: an SU(2) element of each Cabibbo-Marinari subgroup, times c > 0, is recovered exactly
+  (one subgroup step solves it, the others then see the identity); the output is in SU(3); a scaled SU(3)
+  matrix is recovered to the iteration's convergence; for a generic matrix the result beats the identity and
+  random SU(3) elements in Re tr(X^dag V).
+* Jacobi: one iteration against numpy (w from explicit links, Delta^a = Re tr(i t^a w)) and scipy.linalg.expm,
+  with the monitors; the refactored Fourier path still equals the original entry point bit for bit.
+* Los Alamos: the gauge functional never decreases from one iteration to the next (each site update
+  maximises its local term); the iteration reaches the Coulomb condition and recovers a gauge-rotated copy.
+* Open z with Nz > 1: both iterations stall at a fixed point, as the reference's top ghost layer makes them.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from oracle import oracle as O
+from tests.helpers import T, herm, random_fields, random_su3
+
+A1 = (1.0, 1.0, 1.0)
+
+
+def _identity_G(N):
+    G = np.zeros((N[2] + 2, N[1], N[0], 3, 3), complex)
+    G[..., 0, 0] = G[..., 1, 1] = G[..., 2, 2] = 1
+    return G
+
+
+def _mtp(M):
+    """MaxTraceProjection of a row-major 3x3 matrix, returned row-major."""
+    return O.max_trace_projection(M.T.reshape(9)).reshape(3, 3).T
+
+
+@pytest.mark.parametrize("p,q", [(0, 1), (0, 2), (1, 2)])
+def test_max_trace_projection_su2_subgroups_exact(p, q):
+    """The (0,1) sweep comes first, so any SU(2) element there is undone at once; for (0,2) and (1,2) the
+    element has a real diagonal, which leaves the earlier subgroups' steps at the identity."""
+    rng = np.random.default_rng(p + 3 * q)
+    for _ in range(10):
+        a = rng.normal(size=4)
+        if (p, q) != (0, 1):
+            a[3] = 0.0
+        a /= np.linalg.norm(a)
+        s2 = np.array([[a[0] + 1j * a[3], a[2] + 1j * a[1]], [-a[2] + 1j * a[1], a[0] - 1j * a[3]]])
+        M = np.eye(3, dtype=complex)
+        M[np.ix_([p, q], [p, q])] = s2
+        Vt = _mtp(2.5 * M)
+        assert np.abs(Vt - M).max() < 2e-15
+
+
+def test_max_trace_projection_properties():
+    rng = np.random.default_rng(5)
+    assert np.abs(_mtp(3.0 * np.eye(3)) - np.eye(3)).max() == 0
+    for scale in (0.3, 1.0):
+        for M in random_su3(rng, (10,), scale):
+            assert np.abs(_mtp(1.7 * M) - M).max() < 1e-5  # seven sweeps: converged to ~1e-6
+    for _ in range(10):
+        V = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        Vt = _mtp(V)
+        assert np.abs(Vt @ Vt.conj().T - np.eye(3)).max() < 1e-14 and abs(np.linalg.det(Vt) - 1) < 1e-14
+        f0 = np.trace(Vt.conj().T @ V).real
+        assert f0 >= np.trace(V).real  # the sweeps start at the identity and never lose
+        for X in random_su3(rng, (20,), 1.0):
+            assert np.trace(X.conj().T @ V).real <= f0
+
+
+def test_fourier_path_unchanged():
+    N = (6, 4, 3)
+    f = random_fields(N, seed=2, link_scale=0.3)
+    G1, G2 = _identity_G(N), _identity_G(N)
+    UG1, EG1, s1 = O.coulomb_gauge(f.copy(), A1, G1, 1e-12, 5)
+    UG2, EG2, s2 = O.coulomb_gauge_algo(f.copy(), A1, G2, "fourier", 1e-12, 5, 2.0)
+    assert np.array_equal(G1, G2) and np.array_equal(UG1, UG2) and np.array_equal(s1, s2)
+
+
+@pytest.mark.parametrize("N", [(4, 3, 2), (5, 4, 1)])
+def test_jacobi_one_iteration_against_numpy(N):
+    a = (1.0, 0.9, 0.8)
+    alpha = 0.3
+    f = random_fields(N, seed=7, link_scale=0.4)
+    G = _identity_G(N)
+    UG, EG, st = O.coulomb_gauge_algo(f.copy(), a, G, "jacobi", 0.0, 1, alpha)
+    assert st[0] == 1
+    Nx, Ny, Nz = N
+    L = lambda z, y, x, mu: np.eye(3) if z < 0 else f.U[z + 1, y % Ny, x % Nx, mu].T  # noqa: E731
+    c = [1 / a[0] ** 2, 1 / a[1] ** 2, (1 / a[2] ** 2) if Nz > 1 else 0.0]
+    dmax = dsq = fsum = smax = 0.0
+    fmin = 6 * sum(1 / x ** 2 for x in a)
+    for z in range(Nz):
+        for y in range(Ny):
+            for x in range(Nx):
+                w = sum(c[mu] * (L(z, y, x, mu) + L(z - d[2], y - d[1], x - d[0], mu).conj().T)
+                        for mu, d in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))))
+                dv = np.array([np.trace(1j * T[k] @ w).real for k in range(8)])
+                dmax = max(dmax, np.abs(dv).max())
+                dsq += (dv ** 2).sum()
+                fsum += np.trace(w).real
+                fmin = min(fmin, np.trace(w).real)
+                W = sl.expm(-1j * alpha * herm(dv))
+                smax = max(smax, abs(np.trace(np.eye(3) - W)))
+                assert np.abs(G[z + 1, y, x].T - W.conj().T).max() < 1e-14
+    V = Nx * Ny * Nz
+    assert abs(st[1] - dmax) < 1e-14 and abs(st[2] - np.sqrt(dsq / (8 * V))) < 1e-14
+    assert abs(st[3] - smax) < 1e-14 and abs(st[5] - fmin) < 1e-13 and abs(st[6] - fsum / V) < 1e-13
+
+
+def test_los_alamos_functional_increases_and_converges():
+    """Nz = 1 (no z links in w): with even Nx, Ny the checkerboard is consistent and every site update raises
+    its local term Re tr(h w), so the functional cannot fall."""
+    N = (6, 4, 1)
+    f = random_fields(N, seed=9, link_scale=0.3)
+    G = _identity_G(N)
+    prev = -np.inf
+    for it in range(40):
+        _, _, st = O.coulomb_gauge_algo(f.copy(), A1, G, "losalamos", 0.0, 1, 0.0)  # G carries over
+        assert st[6] >= prev - 1e-13, (it, st[6], prev)
+        prev = st[6]
+    UG, EG, st = O.coulomb_gauge_algo(f.copy(), A1, G, "losalamos", 1e-12, 2000, 0.0)
+    assert st[1] <= 1e-12 and st[0] < 2000
+
+
+
+@pytest.mark.parametrize("algo,param", [("jacobi", 0.3), ("losalamos", 0.0)])
+def test_open_z_stalls_at_a_fixed_point(algo, param):
+    """With open z and Nz > 1 the last layer's z links are transformed by the same g at both ends (the top ghost
+    of g is a copy of layer Nz - 1), so the Coulomb condition cannot be reached: the reference's iterations
+    stall at a fixed point with a finite deviation.  The oracle reproduces that rather than converging."""
+    N = (6, 4, 3)
+    f = random_fields(N, seed=9, link_scale=0.3)
+    _, _, s1 = O.coulomb_gauge_algo(f.copy(), A1, _identity_G(N), algo, 0.0, 1000, param)
+    _, _, s2 = O.coulomb_gauge_algo(f.copy(), A1, _identity_G(N), algo, 0.0, 3000, param)
+    assert s1[1] > 1e-3 and abs(s2[2] - s1[2]) < 1e-9 and abs(s2[6] - s1[6]) < 1e-9
+
+
+@pytest.mark.parametrize("algo,param", [("jacobi", 0.3), ("losalamos", 0.0)])
+def test_alternative_algorithms_recover_rotated_copy(algo, param):
+    """Fixed by the Fourier-accelerated solver, rotated by a random g, fixed again by Jacobi / Los Alamos: the
+    gauge-invariant content and the Coulomb functional come back."""
+    N = (8, 6, 1)
+    a = A1
+    f = random_fields(N, seed=8, link_scale=0.3)
+    UG, EG, st = O.coulomb_gauge(f.copy(), a, _identity_G(N), 1e-13, 300)
+    fc = f.copy()
+    fc.U[1:-1] = UG[1:-1]
+    fc.E[1:-1] = EG[1:-1]
+    fc.sync(0)
+    rng = np.random.default_rng(3)
+    G = _identity_G(N)
+    G[1:-1] = np.swapaxes(random_su3(rng, (N[2], N[1], N[0]), 0.4), -1, -2)
+    G[0], G[-1] = G[1], G[-2]
+    fr = fc.copy()
+    O.gauge_transform(fr, a, G, 0)
+    UG2, EG2, st2 = O.coulomb_gauge_algo(fr.copy(), a, _identity_G(N), algo, 1e-11, 5000, param)
+    assert st2[1] <= 1e-11, st2
+    f2 = fr.copy()
+    f2.U[1:-1] = UG2[1:-1]
+    f2.E[1:-1] = EG2[1:-1]
+    f2.sync(0)
+    assert np.allclose(O.bulk(f2, a)[[0, 1, 2, 5]], O.bulk(fc, a)[[0, 1, 2, 5]], rtol=1e-12)
+    assert abs(st2[6] - st[6]) < 1e-10 * abs(st[6])
