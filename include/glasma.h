@@ -159,6 +159,14 @@ int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, d
 enum { GLASMA_GAUGE_FOURIER = 0, GLASMA_GAUGE_JACOBI = 1, GLASMA_GAUGE_LOS_ALAMOS = 2 };
 int glasma_coulomb_gauge_algorithm(glasma_ctx* ctx, int algorithm, double tol, int maxit, double param,
                                    double* stats7);
+/* Gauss-law restoration (GAUSSLAWFIXING/GaussRestore.cpp:134-182; transverse_only != 0:
+ * GaussRestore2D.cpp, E_x and E_y only): E relaxed in place toward the Gauss law with the static charges,
+ * three steps per iteration (Gamma = 7/24, 7/48, 7/72), until max |G| <= tol or maxit iterations
+ * (reference: 1e-14, 30000).  The violation's ghost layers are edge copies and the fields' ghosts are
+ * synchronised after each step, as in GaussRestore2D.cpp, for both variants (GaussRestore.cpp indexes past its
+ * allocation at the top layer).  The step sizes are stable for unit spacings.  stats3 = iterations, max |G|,
+ * RMS G after a final measurement. */
+int glasma_gauss_restore(glasma_ctx* ctx, int transverse_only, double tol, int maxit, double* stats3);
 /* GaugeTransformation::Reset / SetIdentity (CoulombGaugeFixing.cpp:26-30) */
 int glasma_gauge_reset(glasma_ctx* ctx);
 /* Gauge-fixed copies (GaugeFixedVariables::U/E, GAUGETRANSFORMATION/GaugeTransformation.cpp:8-21) and G

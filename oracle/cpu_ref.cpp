@@ -706,6 +706,74 @@ template <class G> static void Gauss(const Lat<G>& L, typename G::elem* U, doubl
     *grms = std::sqrt(sq / (G::NG * (double)L.N[0] * L.N[1] * L.N[2]));
 }
 
+// GaussLawRestoration::Restore (SC/GAUSSLAWFIXING/GaussRestore.cpp:134-182) and GaussLawRestoration2D
+// (SC/GAUSSLAWFIXING/GaussRestore2D.cpp, variant 1: only E_x, E_y relax): relaxation of E toward the Gauss law with
+// the JpStat, JmStat charges.  An iteration is three UpdateSteps with Gamma = 7/24, 7/48, 7/72 (:152-154); a step
+// computes the violation G of every site (COMPUTE_GAUSS_VIOLATION, SC/CALC/GaussViolation.cpp:13-23), fills G's
+// ghost layers with copies of the edge layers (GaussRestore2D.cpp:65-67), updates
+// E_mu(x) -= Gamma (G(x) - U_mu(x) G(x + mu) U_mu(x)^dag) (:77-113) and synchronises the ghost cells of U and E
+// (GaussRestore2D.cpp:118-119).  The loop stops when the last step's max |G| <= tol or after maxit iterations
+// (reference: 10^-(digits10 - 1) = 1e-14 and 30000); stats = iterations, max |G| and RMS G after a final measurement.
+// GaussRestore.cpp's own array has Nz layers but is indexed with the ghost-shifted Index3D (its top layer and the
+// z + 1 neighbour of the last layer fall outside the allocation) and never synchronises E; both variants here use
+// the 2D file's (Nz + 2)-layer array and synchronisation.
+template <class G> static void GaussRestore(const Lat<G>& L, typename G::elem* U, double* E, double* JpS, double* JmS, int variant, double tol, int maxit, double* stats) {
+    const int NG = G::NG;
+    const long np = L.plane();
+    std::vector<double> Gv((size_t)(L.N[2] + 2) * np * NG, 0.0);
+    L.syncJ(JpS); L.syncJ(JmS); L.syncU(U); L.syncE(E);
+    const double norm = 1.0 / (L.a[0] * L.a[1] * L.a[2]);
+    double gmax = 1.0, gavg = 0.0;
+    auto deviation = [&]() {  // UpdateDeviation (:14-62)
+        double mx = 0.0, sq = 0.0;
+#pragma omp parallel for reduction(+ : sq) reduction(max : mx)
+        for (int z = 0; z < L.N[2]; z++)
+            for (int y = 0; y < L.N[1]; y++)
+                for (int x = 0; x < L.N[0]; x++) {
+                    double Xd[8], Yd[8], Zd[8];
+                    G::InverseAdjointMultiplication(L.Ul(U, x - 1, y, z, 0), L.El(E, x - 1, y, z, 0), Xd);
+                    G::InverseAdjointMultiplication(L.Ul(U, x, y - 1, z, 1), L.El(E, x, y - 1, z, 1), Yd);
+                    G::InverseAdjointMultiplication(L.Ul(U, x, y, z - 1, 2), L.El(E, x, y, z - 1, 2), Zd);
+                    double* g = &Gv[(size_t)NG * L.idx3(x, y, z)];
+                    for (int a = 0; a < NG; a++) {
+                        g[a] = norm * ((L.El(E, x, y, z, 0)[a] - Xd[a]) + (L.El(E, x, y, z, 1)[a] - Yd[a]) + (L.El(E, x, y, z, 2)[a] - Zd[a]) - (L.Jl(JpS, x, y, z)[a] + L.Jl(JmS, x, y, z)[a]) / SQRT2);
+                        mx = std::max(mx, std::fabs(g[a]));
+                        sq += SQR(g[a]);
+                    }
+                }
+        std::memcpy(&Gv[0], &Gv[(size_t)NG * np], NG * np * sizeof(double));
+        std::memcpy(&Gv[(size_t)NG * np * (L.N[2] + 1)], &Gv[(size_t)NG * np * L.N[2]], NG * np * sizeof(double));
+        gmax = mx;
+        gavg = std::sqrt(sq / (NG * (double)np * L.N[2]));
+    };
+    const int nmu = variant == 1 ? 2 : 3;
+    auto update = [&](double gamma) {  // UpdateElectricFields (:72-115)
+#pragma omp parallel for
+        for (int z = 0; z < L.N[2]; z++)
+            for (int y = 0; y < L.N[1]; y++)
+                for (int x = 0; x < L.N[0]; x++) {
+                    const double* g = &Gv[(size_t)NG * L.idx3(x, y, z)];
+                    const int nb[3][3] = {{x + 1, y, z}, {x, y + 1, z}, {x, y, z + 1}};
+                    for (int mu = 0; mu < nmu; mu++) {
+                        double up[8];
+                        G::AdjointMultiplication(L.Ul(U, x, y, z, mu), &Gv[(size_t)NG * L.idx3(nb[mu][0], nb[mu][1], nb[mu][2])], up);
+                        for (int a = 0; a < NG; a++) L.El(E, x, y, z, mu)[a] -= gamma * (g[a] - up[a]);
+                    }
+                }
+        L.syncU(U); L.syncE(E);
+    };
+    deviation();
+    int n = 0;
+    while (n < maxit && gmax > tol) {
+        deviation(); update(7.0 / 24.0);
+        deviation(); update(7.0 / 48.0);
+        deviation(); update(7.0 / 72.0);
+        n++;
+    }
+    deviation();
+    stats[0] = n; stats[1] = gmax; stats[2] = gavg;
+}
+
 // Observables::Unitarity  SC/OBSERVABLES/UnitarityViolation.cpp:13-51
 template <class G> static void Unitarity(const Lat<G>& L, typename G::elem* U, double* umax, double* urms) {
     double mx = 0.0, sq = 0.0;
@@ -1596,6 +1664,11 @@ void ref_coulomb_gauge_algo(const int* N, const double* a, double* U, double* E,
                             int algo, double tol, int maxit, double param, double* stats) {
     Lat<SU3> L = mk3(N, a, 0);
     CoulombGaugeAlgo(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, algo, tol, maxit, param, stats);
+}
+void ref_gauss_restore(const int* N, const double* a, int bc, double* U, double* E, double* JpS, double* JmS, int variant,
+                       double tol, int maxit, double* stats) {
+    Lat<SU3> L = mk3(N, a, bc);
+    GaussRestore<SU3>(L, (cd*)U, E, JpS, JmS, variant, tol, maxit, stats);
 }
 void ref_max_trace_projection(const double* V, double* VT) { MaxTraceProjection((const cd*)V, (cd*)VT); }
 // Gauge transformation of every field with a site-dependent g (with ghosts): U -> g U g^dag(x+mu), E, J -> g E g^dag.

@@ -75,6 +75,8 @@ def lib():
         L.ref_coulomb_gauge_algo.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _dp]
         L.ref_max_trace_projection.argtypes = [_dp, _dp]
+        L.ref_gauss_restore.argtypes = [_ip, _dp, ctypes.c_int] + [_dp] * 4 + [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                                            _dp]
         L.ref_sync_ghosts.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_long]
         L.ref_num_threads.restype = ctypes.c_int
         L.ref_set_threads.argtypes = [ctypes.c_int]
@@ -313,6 +315,14 @@ def coulomb_gauge_algo(f: Fields, a, G: np.ndarray, algo: str, tol: float, maxit
     lib().ref_coulomb_gauge_algo(ivec(f.N), dvec(a), P(f.U), P(f.E), P(G), P(UG), P(EG), ALGORITHMS[algo], tol, maxit,
                                  param, P(stats))
     return UG, EG, stats
+
+
+def gauss_restore(f: Fields, a, variant: int = 0, tol: float = 1e-14, maxit: int = 30000, bc: int = 0):
+    """GaussLawRestoration::Restore (variant 0) / GaussLawRestoration2D (variant 1): E relaxed in place toward the
+    Gauss law with the static charges; returns (iterations, max |G|, RMS G)."""
+    stats = np.zeros(3)
+    lib().ref_gauss_restore(ivec(f.N), dvec(a), bc, P(f.U), P(f.E), P(f.JpS), P(f.JmS), variant, tol, maxit, P(stats))
+    return stats
 
 
 def max_trace_projection(V: np.ndarray) -> np.ndarray:

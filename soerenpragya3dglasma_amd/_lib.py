@@ -76,6 +76,7 @@ def lib():
         "glasma_set_wilson_lines": ([vp, c_char, ctypes.POINTER(MVParams), dp, dp, dp], c_int),
         "glasma_coulomb_gauge": ([vp, c_double, c_int, c_double, dp], c_int),
         "glasma_coulomb_gauge_algorithm": ([vp, c_int, c_double, c_int, c_double, dp], c_int),
+        "glasma_gauss_restore": ([vp, c_int, c_double, c_int, dp], c_int),
         "glasma_gauge_reset": ([vp], c_int),
         "glasma_get_gauge_fixed": ([vp, dp, dp, dp], c_int),
         "glasma_set_gauge_transformation": ([vp, dp], c_int),

@@ -305,6 +305,65 @@ __global__ void k_cg_efield(const double* __restrict__ G, const double* __restri
     }
 }
 
+// ---- Gauss-law restoration (GAUSSLAWFIXING/GaussRestore.cpp, GaussRestore2D.cpp) -------------------------
+// violation G^a(x) = (1/a^3) (sum_mu E_mu(x) - U_mu^dag(x - mu) E_mu(x - mu) U_mu(x - mu) - (JpS + JmS)/sqrt2)
+// (COMPUTE_GAUSS_VIOLATION, CALC/GaussViolation.cpp:13-23) into Gv (8 tiled planes), monitors into part
+__global__ __launch_bounds__(kCgBlock) void k_gr_violation(const double* __restrict__ U, const double* __restrict__ E,
+                                                            const double* __restrict__ J, double* __restrict__ Gv,
+                                                            double* __restrict__ part, Geom g, double norm) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sq = 0.0, mx = 0.0;
+    if (s < g.S) {
+        const int z = (int)(s / g.P);
+        const long s2 = s - (long)z * g.P;
+        const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+        const long sm[3] = {sidx(g, x - 1, y, z), sidx(g, x, y - 1, z), sidx(g, x, y, zdn(g, z))};
+        double acc[8], e[8], d[8];
+        for (int a = 0; a < 8; a++) acc[a] = 0.0;
+        for (int mu = 0; mu < 3; mu++) {
+            M3 V;
+            ldm<long>(U, kUPlanes, 18 * mu, sm[mu], V);
+            ldv<long>(E, kEPlanes, 8 * mu, sm[mu], e);
+            inv_adj_mul(V, e, d);
+            ldv<long>(E, kEPlanes, 8 * mu, s, e);
+            for (int a = 0; a < 8; a++) acc[a] += e[a] - d[a];
+        }
+        double jp[8], jm[8], gv[8];
+        ldv<long>(J, kJPlanes, JPS * 8, s, jp);
+        ldv<long>(J, kJPlanes, JMS * 8, s, jm);
+        for (int a = 0; a < 8; a++) {
+            gv[a] = norm * (acc[a] - (jp[a] + jm[a]) / kSqrt2);
+            mx = fmax(mx, fabs(gv[a]));
+            sq += gv[a] * gv[a];
+        }
+        stv<long>(Gv, 8, 0, s, gv);
+    }
+    block_partials(sq, 0.0, mx, 0.0, part + 4 * (long)blockIdx.x);
+}
+
+// E_mu(x) -= gamma (G(x) - U_mu(x) G(x + mu) U_mu(x)^dag) for mu < nmu (GaussRestore.cpp:72-115); above the last
+// layer G's ghost is a copy of that layer (GaussRestore2D.cpp:65-67)
+__global__ __launch_bounds__(kCgBlock) void k_gr_efield(const double* __restrict__ U, double* __restrict__ E,
+                                                         const double* __restrict__ Gv, Geom g, double gamma, int nmu) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    const long sp[3] = {sidx(g, x + 1, y, z), sidx(g, x, y + 1, z), z + 1 < g.nz ? s + g.P : s};
+    double gx[8], gn[8], up[8], e[8];
+    ldv<long>(Gv, 8, 0, s, gx);
+    for (int mu = 0; mu < nmu; mu++) {
+        M3 V;
+        ldm<long>(U, kUPlanes, 18 * mu, s, V);
+        ldv<long>(Gv, 8, 0, sp[mu], gn);
+        adj_mul(V, gn, up);
+        ldv<long>(E, kEPlanes, 8 * mu, s, e);
+        for (int a = 0; a < 8; a++) e[a] -= gamma * (gx[a] - up[a]);
+        stv<long>(E, kEPlanes, 8 * mu, s, e);
+    }
+}
+
 int ensure_gauge(glasma_ctx* ctx) {
     if (ctx->G) return 0;
     const long Sp = pad_sites(ctx->g.S), Pp = pad_sites(ctx->g.P);
@@ -419,6 +478,55 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
 }
 
 }  // namespace
+
+extern "C" int glasma_gauss_restore(glasma_ctx* ctx, int transverse_only, double tol, int maxit, double* stats3) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    if (maxit < 0 || !(tol >= 0.0)) return set_error(ctx, GLASMA_ERR_ARG, "invalid tolerance or iteration cap");
+    const Geom& g = ctx->g;
+    const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
+    int rc = ensure_red(ctx, 4 * (size_t)nblk + 8);
+    if (!rc) rc = ensure_fftbuf(ctx, 8 * (size_t)pad_sites(g.S));
+    if (rc) return rc;
+    const double norm = 1.0 / (ctx->p.a[0] * ctx->p.a[1] * ctx->p.a[2]);
+    const int nmu = transverse_only ? 2 : 3;
+    double* Gv = ctx->fftbuf;
+    double* part = ctx->red;
+    double* fin = ctx->red + 4 * nblk;
+    const double* U = ctx->U[ctx->cur];
+    const double* J = ctx->J[ctx->cur];
+    auto deviation = [&]() {
+        hipLaunchKernelGGL(k_gr_violation, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, U, ctx->E, J, Gv, part,
+                           g, norm);
+        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, 0.0);
+    };
+    double h[4];
+    auto read = [&]() -> int {
+        GL_HIP(ctx, hipMemcpyAsync(h, fin, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return 0;
+    };
+    deviation();
+    if ((rc = read())) return rc;
+    int n = 0;
+    const double gammas[3] = {7.0 / 24.0, 7.0 / 48.0, 7.0 / 72.0};
+    while (n < maxit && h[2] > tol) {
+        for (int k = 0; k < 3; k++) {
+            deviation();
+            hipLaunchKernelGGL(k_gr_efield, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, U, ctx->E, Gv, g, gammas[k], nmu);
+        }
+        GL_HIP(ctx, hipGetLastError());
+        if ((rc = read())) return rc;  // the violation of the third step, measured before its update
+        n++;
+    }
+    deviation();
+    if ((rc = read())) return rc;
+    if (stats3) {
+        stats3[0] = n;
+        stats3[1] = h[2];
+        stats3[2] = std::sqrt(h[0] / (8.0 * (double)g.S));
+    }
+    return 0;
+}
 
 extern "C" int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, double* stats7) {
     return coulomb_gauge(ctx, GLASMA_GAUGE_FOURIER, tol, maxit, alpha, stats7);

@@ -359,6 +359,14 @@ class Glasma:
                   "glasma_coulomb_gauge_algorithm")
         return stats
 
+    def gauss_restore(self, transverse_only: bool = False, tol: float = 1e-14, maxit: int = 30000) -> np.ndarray:
+        """GaussLawRestoration::Restore (or the 2D variant): relax E toward the Gauss law in place; returns
+        (iterations, max |G|, RMS G)."""
+        stats = np.zeros(3)
+        self._chk(_lib.lib().glasma_gauss_restore(self._h, int(bool(transverse_only)), tol, int(maxit), _p(stats)),
+                  "glasma_gauss_restore")
+        return stats
+
     def gauge_reset(self):
         self._chk(_lib.lib().glasma_gauge_reset(self._h), "glasma_gauge_reset")
 

@@ -81,3 +81,46 @@ def test_argument_errors():
         from soerenpragya3dglasma_amd import _lib
         assert _lib.lib().glasma_coulomb_gauge_algorithm(g._h, 7, 1e-12, 10, 0.0, None) != 0
         assert b"unknown" in _lib.lib().glasma_last_error(g._h)
+
+
+# ------------------------------------------------------------------------------------------------
+# Gauss-law restoration (GAUSSLAWFIXING/GaussRestore.cpp, GaussRestore2D.cpp)
+# ------------------------------------------------------------------------------------------------
+def _neutral_fields(N, seed, bc=0):
+    f = random_fields(N, seed=seed, link_scale=0.4, e_scale=0.3, j_scale=0.2, bc=bc)
+    for J in (f.JpS, f.JmS):
+        J[1:-1] -= J[1:-1].mean(axis=(0, 1, 2))
+    f.sync(bc)
+    return f
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("N,a,bc", [((6, 5, 7), (1.1, 1.0, 1.2), 0), ((8, 4, 6), A1, 1)])
+def test_gauss_restore_iterations_match_oracle(variant, N, a, bc):
+    from soerenpragya3dglasma_amd import Glasma
+    f = _neutral_fields(N, 14, bc)
+    with Glasma(N, a, 0.1, bc) as g:
+        upload(g, f)
+        st = g.gauss_restore(bool(variant), 0.0, 6)
+        E = g.get_fields()["E"]
+    st_o = O.gauss_restore(f, a, variant, 0.0, 6, bc)
+    print(st, st_o)
+    assert st[0] == st_o[0] == 6
+    assert np.allclose(st[1:], st_o[1:], rtol=1e-11, atol=1e-15)
+    assert np.abs(E[1:-1] - f.E[1:-1]).max() < 1e-12
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gauss_restore_converges(variant):
+    from soerenpragya3dglasma_amd import Glasma
+    N = (8, 8, 6)
+    f = _neutral_fields(N, 3)
+    with Glasma(N, A1, 0.1, 0) as g:
+        upload(g, f)
+        _, before, _ = g.gauss()
+        st = g.gauss_restore(bool(variant), 1e-12, 30000)
+        _, after, rms = g.gauss()
+    st_o = O.gauss_restore(f, A1, variant, 1e-12, 30000)
+    print(before, st, st_o)
+    assert before > 0.1 and after <= 1e-12 and st[1] == pytest.approx(after, rel=1e-12, abs=1e-15)
+    assert abs(st[0] - st_o[0]) <= 2

@@ -159,3 +159,72 @@ def test_alternative_algorithms_recover_rotated_copy(algo, param):
     f2.sync(0)
     assert np.allclose(O.bulk(f2, a)[[0, 1, 2, 5]], O.bulk(fc, a)[[0, 1, 2, 5]], rtol=1e-12)
     assert abs(st2[6] - st[6]) < 1e-10 * abs(st[6])
+
+
+# ------------------------------------------------------------------------------------------------
+# Gauss-law restoration (SC/GAUSSLAWFIXING/GaussRestore.cpp, GaussRestore2D.cpp)
+# ------------------------------------------------------------------------------------------------
+def _neutral_fields(N, seed):
+    f = random_fields(N, seed=seed, link_scale=0.4, e_scale=0.3, j_scale=0.2)
+    for J in (f.JpS, f.JmS):  # the initial conditions remove the net static charge before restoring
+        J[1:-1] -= J[1:-1].mean(axis=(0, 1, 2))
+    f.sync(0)
+    return f
+
+
+def _violation_np(f, a):
+    """G^a(x) = (1/a^3) (sum_mu E_mu(x) - U_mu^dag(x-mu) E_mu(x-mu) U_mu(x-mu) - (JpS + JmS)/sqrt2), ghosts as
+    the synchronised fields hold them (open z: edge copies)."""
+    Nx, Ny, Nz = f.N
+    G = np.zeros((Nz, Ny, Nx, 8))
+    for z in range(Nz):
+        for y in range(Ny):
+            for x in range(Nx):
+                acc = np.zeros(8)
+                for mu, d in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+                    xm, ym, zm = x - d[0], y - d[1], z - d[2]
+                    Um = f.U[zm + 1, ym % Ny, xm % Nx, mu].T
+                    rot = Um.conj().T @ herm(f.E[zm + 1, ym % Ny, xm % Nx, mu]) @ Um
+                    acc += f.E[z + 1, y, x, mu] - np.array([2 * np.trace(T[k] @ rot).real for k in range(8)])
+                G[z, y, x] = (acc - (f.JpS[z + 1, y, x] + f.JmS[z + 1, y, x]) / np.sqrt(2)) / np.prod(a)
+    return G
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gauss_restore_one_iteration_against_numpy(variant):
+    N = (3, 4, 3)
+    a = (1.0, 0.9, 0.8)
+    f = _neutral_fields(N, 12)
+    g = f.copy()
+    O.gauss_restore(g, a, variant, 0.0, 1)
+    for gamma in (7 / 24, 7 / 48, 7 / 72):
+        G = _violation_np(f, a)
+        Gg = np.concatenate([G[:1], G, G[-1:]])  # edge-copy ghost layers of the violation
+        for z in range(N[2]):
+            for y in range(N[1]):
+                for x in range(N[0]):
+                    for mu, d in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))[: 2 if variant else 3]):
+                        U = f.U[z + 1, y, x, mu].T
+                        Gn = Gg[z + 1 + d[2], (y + d[1]) % N[1], (x + d[0]) % N[0]]
+                        M = U @ herm(Gn) @ U.conj().T
+                        up = np.array([2 * np.trace(T[k] @ M).real for k in range(8)])
+                        f.E[z + 1, y, x, mu] -= gamma * (G[z, y, x] - up)
+        f.sync(0)
+    assert np.abs(g.E - f.E).max() < 1e-14
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_gauss_restore_reaches_the_gauss_law(variant):
+    """Unit spacings: the reference's step sizes 7/24, 7/48, 7/72 are stable for a = 1 (with a^3 < 1 the
+    normalised violation grows and the relaxation diverges)."""
+    N = (4, 4, 4)
+    a = A1
+    f = _neutral_fields(N, 3)
+    e0 = f.E.copy()
+    before = O.gauss(f, a)[1]
+    st = O.gauss_restore(f, a, variant, 1e-12, 30000)
+    _, gmax, grms = O.gauss(f, a)
+    assert before > 0.1 and st[0] < 30000 and gmax <= 1e-12
+    assert st[1] == pytest.approx(gmax, rel=1e-12, abs=1e-15) and st[2] == pytest.approx(grms, rel=1e-12, abs=1e-15)
+    if variant == 1:
+        assert np.array_equal(f.E[..., 2, :], e0[..., 2, :])  # the transverse variant leaves E_z alone
