@@ -11,7 +11,6 @@ that hold at any size:
 * the per-layer EMT energy profile and the bulk T00 describe the same state (their sums differ only by the
   clover average of the links at t - dt/2, bounded at 1e-3 relative).
 """
-import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
