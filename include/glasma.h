@@ -144,6 +144,32 @@ int glasma_mv_initial_state(glasma_ctx* ctx, int driver, const glasma_mv_params*
 int glasma_set_wilson_lines(glasma_ctx* ctx, char dir, const glasma_mv_params* mv, const double* xi,
                             double* V_out, double* rho_out);
 
+/* ---- physical-charge (TMD / GBW) nuclei (TMDNUCLEUS/*.cpp, SimulationmitPC.cpp:315-370) ----------- */
+/* Parameters of LATTICE/Parameters.cpp:4-48 that the physical-charge path reads. */
+typedef struct glasma_tmd_params {
+    double qs0a;        /* Parameters::Qs0A (lattice units) */
+    double m_eff;       /* Parameters::MEff, IR regulator of the transverse filter (lattice units) */
+    double com_energy;  /* Parameters::CoMEnergy (fm^-1; 500/0.197 in the reference) */
+    double a_fm[3];     /* Parameters::aXInFm, aYInFm, aZInFm */
+    int atomic_number;  /* Parameters::AtomicNumber: 16, 54, 197 or 208 (NuclearData.cpp) */
+    int center_l;       /* z index of the left nucleus' centre (TMDbased.cpp:43: 785) */
+    int center_r;       /* (TMDbased.cpp:48: 1261) */
+} glasma_tmd_params;
+/* The random draws of SimulationmitPC::Run (SimulationmitPC.cpp:326-344) from one MT19937 stream seeded as
+ * glasma_draw_charges: the left nucleus' noise (Nz*Ny*Nx*8 Gauss() in z, y, x, a order, index
+ * a + 8*(x + Nx*(y + Ny*z))), its nucleons (Woods-Saxon rejection sampling with gsl_rng_uniform,
+ * GBWSampling.cpp:21-87; 3*A doubles in fm, centre of mass at the origin), then the right noise and
+ * nucleons.  Host only; GLASMA_ERR_ARG for an atomic number without Woods-Saxon parameters. */
+int glasma_tmd_draw(unsigned long seed, int nx, int ny, int nz, int atomic_number, double* noise_l,
+                    double* noise_r, double* nucleons_l, double* nucleons_r);
+/* SimulationmitPC::Run's initial state: per nucleus ChargeFluctuation (CorrelationFunction.cpp:24-95: the noise
+ * / sqrt(a^3) filtered by sqrt(GammaK) in 3D momentum space), the charge a^3 sqrt(aZInFm T / a_z) CF with T the
+ * nucleus profile (TMDbased.cpp:27-57, GBWSampling.cpp:110-157), the IR filter p^2/(p^2 + m^2), Poisson solve
+ * and Wilson lines (TMDbased.cpp:256-303), then SingleSheet as glasma_mv_initial_state; driver as there.  The
+ * reference's own build of this path does not compile; this follows its source.  Resets fields and time. */
+int glasma_tmd_initial_state(glasma_ctx* ctx, int driver, const glasma_tmd_params* p, const double* noise_l,
+                             const double* noise_r, const double* nucleons_l, const double* nucleons_r);
+
 /* ---- Coulomb gauge fixing (GAUGETRANSFORMATION/CoulombGaugeFixing.cpp:32-88 with
  *      ALGORITHMS/FourierAcceleration.cpp:15-148) ------------------------------------------------- */
 /* Fourier-accelerated steepest descent on the gauge transformation G (kept in the context across

@@ -36,6 +36,7 @@
 #include <cstring>
 #include <vector>
 #include <algorithm>
+#include <limits>
 #include <omp.h>
 
 typedef std::complex<double> cd;
@@ -1493,6 +1494,114 @@ static void CoulombGauge(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, do
 }
 
 // ============================================================================================
+//  Physical-charge (TMD / GBW) initial conditions: SC/TMDNUCLEUS/*.cpp and the initial state of
+//  SC/SimulationmitPC.cpp:315-370.  The reference's path does not compile (SimulationmitPC needs
+//  CollidingObjects, which it does not include); this follows its source as written.
+// ============================================================================================
+struct TmdParams {
+    double qs0a;        // Parameters::Qs0A (SC/LATTICE/Parameters.cpp:8)
+    double meff;        // Parameters::MEff (:25)
+    double com_energy;  // Parameters::CoMEnergy in fm^-1 (:38)
+    double afm[3];      // Parameters::aXInFm, aYInFm, aZInFm (:40-44)
+    int A;              // Parameters::AtomicNumber (:36)
+    int center[2];      // the z index of each nucleus' centre (TMDbased.cpp:43,48: 785 'L', 1261 'R')
+};
+
+// NuclearData::GetRadius / GetSurface (SC/TMDNUCLEUS/NuclearData.cpp:16-87); false for the unlisted A
+static bool WoodsSaxon(int A, double* R, double* d) {
+    switch (A) {
+        case 16: *R = 2.608; *d = 0.5130; return true;
+        case 54: *R = 5.420; *d = 0.57; return true;
+        case 197: *R = 6.38; *d = 0.535; return true;
+        case 208: *R = 6.62; *d = 0.546; return true;
+        default: return false;
+    }
+}
+
+// Nucleus::SetNucleonPositions (SC/TMDNUCLEUS/GBWSampling.cpp:45-87): A positions by GlauberSamplePosition
+// (:21-42; three uniforms in [-4R, 4R), accepted with probability (1 + e^{-R/d}) / (1 + e^{(r-R)/d}) against a
+// fourth uniform), then the centre of mass (the sum of x_n / A) subtracted.  pos[3 n + i], fm.
+static void SampleNucleus(MT& g, int A, double R, double d, double* pos) {
+    for (int n = 0; n < A; n++) {
+        for (;;) {
+            double* x = pos + 3 * n;
+            x[0] = 8.0 * R * (g.uniform() - 0.5);
+            x[1] = 8.0 * R * (g.uniform() - 0.5);
+            x[2] = 8.0 * R * (g.uniform() - 0.5);
+            const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+            const double prob = (1.0 + std::exp(-R / d)) / (1.0 + std::exp((r - R) / d));
+            if (g.uniform() < prob) break;
+        }
+    }
+    double bar[3] = {0, 0, 0};
+    for (int n = 0; n < A; n++)
+        for (int i = 0; i < 3; i++) bar[i] += pos[3 * n + i] / double(A);
+    for (int n = 0; n < A; n++)
+        for (int i = 0; i < 3; i++) pos[3 * n + i] -= bar[i];
+}
+
+// Nucleus::GetProfile (GBWSampling.cpp:118-157): sum over nucleons of gamma exp(-|x - x_n|^2 / 2 BG) /
+// sqrt(2 pi BG) with the z distance gamma z - z_n, BG = 4 (0.197 fm)^2, gamma = CoMEnergy 0.197 / 2 (:3-6, :189)
+static double NucleusProfile(const TmdParams& P, const double* pos, double x, double y, double z) {
+    const double BG = 4.0 * 0.197 * 0.197, gamma = P.com_energy * 0.197 / 2.0;
+    double T = 0.0;
+    for (int n = 0; n < P.A; n++) {
+        const double* c = pos + 3 * n;
+        const double d2 = (x - c[0]) * (x - c[0]) + (y - c[1]) * (y - c[1]) + SQR(gamma * z - c[2]);
+        T += gamma * std::exp(-0.5 * d2 / BG) / std::sqrt(2.0 * PI * BG);
+    }
+    return T;
+}
+
+// ChargeDistributions::GammaK (SC/TMDNUCLEUS/CorrelationFunction.cpp:17-22), Nc = 3
+static double GammaK(const TmdParams& P, const double* a, double x, double kT2) {
+    const double Qs = P.qs0a * std::pow(x, -0.1) * (1 - x) / std::sqrt(a[0] * a[1]);
+    return 4.0 * PI * 3.0 / (3.0 * 3.0 - 1.0) * kT2 * kT2 / (Qs * Qs) * std::exp(-kT2 / (Qs * Qs));
+}
+
+// ChargeDistributions::ChargeFluctuation (CorrelationFunction.cpp:24-95) from given standard-normal draws
+// noise[a + 8 s] (drawn in z, y, x, a order): zeta = noise / sqrt(a^3); 3D FFT; zeta~(k) nf sqrt(GammaK(x_k, kT^2))
+// for x_k = |p_z| / (CoMEnergy / sqrt2) <= 1, else 0, with |p_z| = sqrt((2 - 2 cos(2 pi n_z / N_z)) / aZInFm^2) / sqrt2;
+// inverse FFT; the real part.  CF[a + 8 s], s = x + Nx (y + Ny z).
+static void ChargeFluctuation(const TmdParams& P, const int* N, const double* a, const double* noise, double* CF) {
+    const long V = (long)N[0] * N[1] * N[2];
+    const double acube = a[0] * a[1] * a[2];
+    std::vector<cd> F(8 * V);
+    for (long i = 0; i < 8 * V; i++) F[i] = noise[i] / std::sqrt(acube);
+    int dims[3] = {N[0], N[1], N[2]};
+    fft_nd(F.data(), 3, dims, 8, -1);
+    const double nf = 1.0 / ((double)N[0] * N[1] * N[2]);
+    for (int pz = 0; pz < N[2]; pz++)
+        for (int py = 0; py < N[1]; py++)
+            for (int px = 0; px < N[0]; px++) {
+                const double pl = (1 / SQRT2) * std::sqrt((2.0 - 2.0 * std::cos(2.0 * PI * pz / double(N[2]))) / SQR(P.afm[2]));
+                const double kT2 = (2.0 - 2.0 * std::cos(2.0 * PI * px / double(N[0]))) / SQR(a[0]) + (2.0 - 2.0 * std::cos(2.0 * PI * py / double(N[1]))) / SQR(a[1]);
+                const double x = pl / (P.com_energy / SQRT2);
+                const long s = px + (long)N[0] * (py + (long)N[1] * pz);
+                for (int c = 0; c < 8; c++) F[c + 8 * s] = x <= 1.0 ? nf * F[c + 8 * s] * std::sqrt(GammaK(P, a, x, kT2)) : cd(0.0, 0.0);
+            }
+    fft_nd(F.data(), 3, dims, 8, +1);
+    for (long i = 0; i < 8 * V; i++) CF[i] = F[i].real();
+}
+
+// TmdBasedCollision::SetChargeDensities' charge (SC/TMDNUCLEUS/TMDbased.cpp:27-57): rho = a^3 sqrt(aZInFm
+// T(x, y, z) / a_z) CF with x = (ix - Nx/2) aXInFm, y = (iy - Ny/2) aYInFm, z = (iz - centre) aZInFm.
+// rho_in[a + 8 (s + P)] (ghost layout, ghosts zero).
+static void TmdCharge(const TmdParams& P, const int* N, const double* a, const double* CF, const double* pos, char D, double* rho_in) {
+    const long np = (long)N[0] * N[1];
+    const double acube = a[0] * a[1] * a[2];
+    const int zc = D == 'L' ? P.center[0] : P.center[1];
+#pragma omp parallel for
+    for (int z = 0; z < N[2]; z++)
+        for (int y = 0; y < N[1]; y++)
+            for (int x = 0; x < N[0]; x++) {
+                const double T = NucleusProfile(P, pos, (x - N[0] / 2) * P.afm[0], (y - N[1] / 2) * P.afm[1], (z - zc) * P.afm[2]);
+                const long s = x + (long)N[0] * (y + (long)N[1] * z);
+                for (int c = 0; c < 8; c++) rho_in[c + 8 * (s + np)] = acube * std::sqrt(P.afm[2] * T / a[2]) * CF[c + 8 * s];
+            }
+}
+
+// ============================================================================================
 //  extern "C" surface used by tests/ and bench.py (ctypes).  All field arrays are host arrays in the
 //  reference AoS layout including the two z ghost layers; complex numbers interleaved (re, im).
 // ============================================================================================
@@ -1669,6 +1778,69 @@ void ref_gauss_restore(const int* N, const double* a, int bc, double* U, double*
                        double tol, int maxit, double* stats) {
     Lat<SU3> L = mk3(N, a, bc);
     GaussRestore<SU3>(L, (cd*)U, E, JpS, JmS, variant, tol, maxit, stats);
+}
+// --- physical-charge (TMD) initial conditions ---
+static TmdParams mk_tmd(const double* prm, const int* ip) {
+    TmdParams P;
+    P.qs0a = prm[0]; P.meff = prm[1]; P.com_energy = prm[2];
+    P.afm[0] = prm[3]; P.afm[1] = prm[4]; P.afm[2] = prm[5];
+    P.A = ip[0]; P.center[0] = ip[1]; P.center[1] = ip[2];
+    return P;
+}
+// RandomNumberGenerator draws of SimulationmitPC::Run (SC/SimulationmitPC.cpp:326-344): the left noise
+// (Nz Ny Nx 8 Gauss() in z, y, x, a order), the left nucleus' nucleons, then the right noise and nucleons.
+int ref_tmd_draw(unsigned long seed, const int* N, int A, double* noise_l, double* noise_r, double* pos_l, double* pos_r) {
+    double R, d;
+    if (!WoodsSaxon(A, &R, &d)) return -1;
+    MT g;
+    g.seed(seed);
+    const long n = 8L * N[0] * N[1] * N[2];
+    for (long i = 0; i < n; i++) noise_l[i] = g.gauss();
+    SampleNucleus(g, A, R, d, pos_l);
+    for (long i = 0; i < n; i++) noise_r[i] = g.gauss();
+    SampleNucleus(g, A, R, d, pos_r);
+    return 0;
+}
+void ref_charge_fluctuation(const int* N, const double* a, const double* prm, const int* ip, const double* noise, double* CF) {
+    ChargeFluctuation(mk_tmd(prm, ip), N, a, noise, CF);
+}
+double ref_nucleus_profile(const double* prm, const int* ip, const double* pos, double x, double y, double z) {
+    return NucleusProfile(mk_tmd(prm, ip), pos, x, y, z);
+}
+// SimulationmitPC::Run's initial state (SC/SimulationmitPC.cpp:326-370): for each nucleus ChargeFluctuation,
+// TmdBasedCollision::SetWilsonLines (TMDbased.cpp:256-303: the charge, the IR filter p^2/(p^2 + m^2) without a UV
+// factor, the Poisson solve and the Wilson-line product; the MV machinery with Lambda = infinity), then
+// SingleSheet as in the MV collision.  driver 0 collision, 1 left, 2 right (as ref_mv_initial_state).
+// rho_l / rho_r (may be NULL) receive the charge before the transverse filter, ghost layout.
+void ref_tmd_initial_state(int driver, const int* N, const double* a, int bc, double dt, const double* prm, const int* ip,
+                           const double* noise_l, const double* noise_r, const double* pos_l, const double* pos_r,
+                           double* U, double* UOld, double* E, double* Jp, double* Jm, double* JpS, double* JmS,
+                           double* rho_l, double* rho_r) {
+    Lat<SU3> L = mk3(N, a, bc);
+    TmdParams T = mk_tmd(prm, ip);
+    MVParams P{{N[0], N[1], N[2]}, {a[0], a[1], a[2]}, dt, bc, std::numeric_limits<double>::infinity(), T.meff, 0, 0, 1, 1};
+    long np = L.plane(), nzg = N[2] + 2, V = np * N[2];
+    std::vector<double> CF(8 * V), rin(8 * np * nzg, 0.0), rhoOld(8 * np * nzg, 0.0), rho(8 * np * nzg, 0.0), RhoT1(8 * np * nzg, 0.0);
+    std::vector<cd> Vw(9 * np * nzg);
+    for (int side = 0; side < 2; side++) {
+        const bool right = side == 1;
+        if ((driver == 1 && right) || (driver == 2 && !right)) continue;
+        const char D = right ? 'R' : 'L';
+        ChargeFluctuation(T, N, a, right ? noise_r : noise_l, CF.data());
+        std::fill(rin.begin(), rin.end(), 0.0);
+        TmdCharge(T, N, a, CF.data(), right ? pos_r : pos_l, D, rin.data());
+        double* out = right ? rho_r : rho_l;
+        if (out) std::memcpy(out, rin.data(), rin.size() * sizeof(double));
+        for (long i = 0; i < np * nzg; i++) SU3::Unit(&Vw[9 * i]);
+        std::fill(RhoT1.begin(), RhoT1.end(), 0.0);
+        SetWilsonLinesRho(P, rin.data(), D, rhoOld.data(), rho.data(), Vw.data());
+        const int zlo = driver == 0 ? (right ? N[2] / 2 : 0) : 0;
+        const int zhi = driver == 0 ? (right ? N[2] - 1 : (N[2] - 2) / 2) : N[2] - 1;
+        if (right) SingleSheet(P, rho.data(), zlo, zhi, RhoT1.data(), Vw.data(), 'R', JmS, Jm, (cd*)U, (cd*)UOld, E);
+        else SingleSheet(P, rho.data(), zlo, zhi, RhoT1.data(), Vw.data(), 'L', JpS, Jp, (cd*)U, (cd*)UOld, E);
+    }
+    L.syncJ(JpS); L.syncJ(Jp); L.syncJ(JmS); L.syncJ(Jm);
+    L.syncU((cd*)U); L.syncU((cd*)UOld); L.syncE(E);
 }
 void ref_max_trace_projection(const double* V, double* VT) { MaxTraceProjection((const cd*)V, (cd*)VT); }
 // Gauge transformation of every field with a site-dependent g (with ghosts): U -> g U g^dag(x+mu), E, J -> g E g^dag.

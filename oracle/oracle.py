@@ -75,6 +75,12 @@ def lib():
         L.ref_coulomb_gauge_algo.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _dp]
         L.ref_max_trace_projection.argtypes = [_dp, _dp]
+        L.ref_tmd_draw.argtypes = [ctypes.c_ulong, _ip, ctypes.c_int] + [_dp] * 4
+        L.ref_tmd_draw.restype = ctypes.c_int
+        L.ref_charge_fluctuation.argtypes = [_ip, _dp, _dp, _ip, _dp, _dp]
+        L.ref_nucleus_profile.argtypes = [_dp, _ip, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.ref_nucleus_profile.restype = ctypes.c_double
+        L.ref_tmd_initial_state.argtypes = [ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_double, _dp, _ip] + [_dp] * 13
         L.ref_gauss_restore.argtypes = [_ip, _dp, ctypes.c_int] + [_dp] * 4 + [ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                                             _dp]
         L.ref_sync_ghosts.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _dp, ctypes.c_long]
@@ -323,6 +329,63 @@ def gauss_restore(f: Fields, a, variant: int = 0, tol: float = 1e-14, maxit: int
     stats = np.zeros(3)
     lib().ref_gauss_restore(ivec(f.N), dvec(a), bc, P(f.U), P(f.E), P(f.JpS), P(f.JmS), variant, tol, maxit, P(stats))
     return stats
+
+
+# ------------------------------------------------------------------------------------------------
+# physical-charge (TMD / GBW) initial conditions
+# ------------------------------------------------------------------------------------------------
+class TmdParams:
+    """Parameters of the physical-charge nucleus (SC/LATTICE/Parameters.cpp): qs0a, meff (lattice units),
+    com_energy (fm^-1), afm = (aXInFm, aYInFm, aZInFm), A (atomic number), centers = (left, right) z index."""
+
+    def __init__(self, qs0a, meff, com_energy, afm, A, centers):
+        self.qs0a, self.meff, self.com_energy = float(qs0a), float(meff), float(com_energy)
+        self.afm = tuple(float(v) for v in afm)
+        self.A = int(A)
+        self.centers = tuple(int(c) for c in centers)
+
+    def arrays(self):
+        return (np.array([self.qs0a, self.meff, self.com_energy, *self.afm]),
+                np.array([self.A, *self.centers], np.int32))
+
+
+def tmd_draw(seed: int, N, A: int):
+    """The draws of SimulationmitPC::Run: (noise_l, noise_r) shaped (Nz, Ny, Nx, 8) and nucleon positions
+    (pos_l, pos_r) shaped (A, 3) in fm."""
+    nl = np.zeros((N[2], N[1], N[0], 8))
+    nr = np.zeros_like(nl)
+    pl = np.zeros((A, 3))
+    pr = np.zeros((A, 3))
+    rc = lib().ref_tmd_draw(int(seed), ivec(N), A, P(nl), P(nr), P(pl), P(pr))
+    if rc != 0:
+        raise ValueError(f"no Woods-Saxon parameters for A = {A}")
+    return nl, nr, pl, pr
+
+
+def charge_fluctuation(N, a, prm: TmdParams, noise: np.ndarray) -> np.ndarray:
+    d, i = prm.arrays()
+    out = np.zeros((N[2], N[1], N[0], 8))
+    lib().ref_charge_fluctuation(ivec(N), dvec(a), P(d), i.ctypes.data_as(_ip), P(np.ascontiguousarray(noise)), P(out))
+    return out
+
+
+def nucleus_profile(prm: TmdParams, pos: np.ndarray, x: float, y: float, z: float) -> float:
+    d, i = prm.arrays()
+    return lib().ref_nucleus_profile(P(d), i.ctypes.data_as(_ip), P(np.ascontiguousarray(pos, np.float64)), x, y, z)
+
+
+def tmd_initial_state(driver: int, N, a, dt: float, prm: TmdParams, noise_l, noise_r, pos_l, pos_r, bc: int = 0):
+    """SimulationmitPC::Run's initial state; returns (Fields, rho_l, rho_r) with the charges before the
+    transverse filter (ghost layout (Nz+2, Ny, Nx, 8))."""
+    f = Fields(N)
+    d, i = prm.arrays()
+    rl = np.zeros((N[2] + 2, N[1], N[0], 8))
+    rr = np.zeros_like(rl)
+    lib().ref_tmd_initial_state(driver, ivec(N), dvec(a), bc, dt, P(d), i.ctypes.data_as(_ip),
+                                P(np.ascontiguousarray(noise_l)), P(np.ascontiguousarray(noise_r)),
+                                P(np.ascontiguousarray(pos_l, np.float64)), P(np.ascontiguousarray(pos_r, np.float64)),
+                                P(f.U), P(f.UOld), P(f.E), P(f.Jp), P(f.Jm), P(f.JpS), P(f.JmS), P(rl), P(rr))
+    return f, rl, rr
 
 
 def max_trace_projection(V: np.ndarray) -> np.ndarray:

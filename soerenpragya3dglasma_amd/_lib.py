@@ -32,6 +32,12 @@ class MVParams(ctypes.Structure):
                 ("center_l", ctypes.c_int), ("center_r", ctypes.c_int)]
 
 
+class TmdParams(ctypes.Structure):
+    _fields_ = [("qs0a", ctypes.c_double), ("m_eff", ctypes.c_double), ("com_energy", ctypes.c_double),
+                ("a_fm", ctypes.c_double * 3), ("atomic_number", ctypes.c_int), ("center_l", ctypes.c_int),
+                ("center_r", ctypes.c_int)]
+
+
 def header_functions() -> list[str]:
     """Names of every function declared in include/glasma.h."""
     txt = open(HEADER).read()
@@ -72,6 +78,8 @@ def lib():
         "glasma_current_profile": ([vp, dp], c_int),
         "glasma_transverse_energy_map": ([vp, dp], c_int),
         "glasma_draw_charges": ([c_ulong, c_int, c_int, c_double, c_double, dp, dp], c_int),
+        "glasma_tmd_draw": ([c_ulong, c_int, c_int, c_int, c_int, dp, dp, dp, dp], c_int),
+        "glasma_tmd_initial_state": ([vp, c_int, ctypes.POINTER(TmdParams), dp, dp, dp, dp], c_int),
         "glasma_mv_initial_state": ([vp, c_int, ctypes.POINTER(MVParams), dp, dp], c_int),
         "glasma_set_wilson_lines": ([vp, c_char, ctypes.POINTER(MVParams), dp, dp, dp], c_int),
         "glasma_coulomb_gauge": ([vp, c_double, c_int, c_double, dp], c_int),

@@ -17,6 +17,7 @@
 //   rho   real,    rho[a*S + s]  (s = x + Nx*(y + Ny*z))
 //   V     Wilson lines in the tiled matrix layout (18 planes)
 #include <cmath>
+#include <limits>
 #include <vector>
 
 #include "internal.h"
@@ -338,6 +339,22 @@ int poisson_all(glasma_ctx* ctx, double* F) {
     return 0;
 }
 
+// the per-z part of CollidingObjects::SetWilsonLines for the charge held in F (real parts): smoothing
+// (UV factor exp(-p^2 / 2 Lambda^2), 1 for Lambda = infinity; IR factor p^2 / (p^2 + m^2)), w.rho = the
+// smoothed charge, Poisson solve and the path-ordered Wilson lines w.V
+int wilson_from_charge(glasma_ctx* ctx, Wk& w, double lambda, double meff, bool right) {
+    const Geom& g = ctx->g;
+    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, -1));
+    hipLaunchKernelGGL(k_transverse_filter, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[0], ctx->p.a[1],
+                       lambda, meff, 0);
+    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, +1));
+    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.rho, g);
+    GL_RC(poisson_all(ctx, w.F));
+    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.V, g, right ? 1 : 0);
+    GL_HIP(ctx, hipGetLastError());
+    return 0;
+}
+
 // CollidingObjects::SetWilsonLines for one nucleus: w.rho = smoothed charge, w.V = Wilson lines
 int set_wilson_lines(glasma_ctx* ctx, Wk& w, const glasma_mv_params* mv, const double* xi_host, bool right) {
     const Geom& g = ctx->g;
@@ -346,15 +363,7 @@ int set_wilson_lines(glasma_ctx* ctx, Wk& w, const glasma_mv_params* mv, const d
     const double mult = right ? mv->nucleus_r : mv->nucleus_l;
     hipLaunchKernelGGL(k_charge, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.xi, g, mult * norm,
                        right ? mv->center_r : mv->center_l, right ? mv->sigma_r : mv->sigma_l);
-    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, -1));
-    hipLaunchKernelGGL(k_transverse_filter, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, ctx->p.a[0], ctx->p.a[1],
-                       mv->lambda, mv->m_eff, 0);
-    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, +1));
-    hipLaunchKernelGGL(k_take_real, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.rho, g);
-    GL_RC(poisson_all(ctx, w.F));
-    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.V, g, right ? 1 : 0);
-    GL_HIP(ctx, hipGetLastError());
-    return 0;
+    return wilson_from_charge(ctx, w, mv->lambda, mv->m_eff, right);
 }
 
 // InitialConditions::SingleSheet for one nucleus over [zlo, zhi]; J-stat into plane group wstat, J into w
@@ -594,6 +603,129 @@ __global__ void k_scan_vjp_left(const double* __restrict__ A, const double* __re
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Physical-charge (TMD / GBW) nuclei: SC/TMDNUCLEUS/CorrelationFunction.cpp, GBWSampling.cpp, TMDbased.cpp
+// and the initial state of SC/SimulationmitPC.cpp:326-370.
+// ------------------------------------------------------------------------------------------------
+// F = (noise / sqrt(a^3), 0); noise[a + 8 s] (ChargeFluctuation :29-39)
+__global__ void k_tmd_noise(double* __restrict__ F, const double* __restrict__ noise, Geom g, double sqrt_acube) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        F[2 * i] = noise[a + 8 * s] / sqrt_acube;
+        F[2 * i + 1] = 0.0;
+    }
+}
+
+// ChargeFluctuation's momentum-space factor (:45-79) on the 3D transform: nf sqrt(GammaK(x, kT^2)) for
+// x = |p_z| / (CoMEnergy / sqrt2) <= 1, else 0; |p_z| = sqrt((2 - 2 cos(2 pi n_z / N_z)) / aZInFm^2) / sqrt2;
+// GammaK = 4 pi Nc / (Nc^2 - 1) kT^4 / Qs^2 exp(-kT^2 / Qs^2), Qs = Qs0A x^-0.1 (1 - x) / sqrt(ax ay) (:17-22)
+__global__ void k_tmd_cf_filter(double* __restrict__ F, Geom g, double ax, double ay, double qs0a, double com_energy,
+                                double az_fm) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int pz = (int)(s / g.P);
+    const long s2 = s - (long)pz * g.P;
+    const int px = (int)(s2 % g.nx), py = (int)(s2 / g.nx);
+    const double pl = (1 / kSqrt2) * sqrt((2.0 - 2.0 * cos(2.0 * kPi * pz / (double)g.nz)) / (az_fm * az_fm));
+    const double kT2 = psqr2(g, px, py, ax, ay);
+    const double x = pl / (com_energy / kSqrt2);
+    const double nf = 1.0 / ((double)g.nx * g.ny * g.nz);
+    double f = 0.0;
+    if (x <= 1.0) {
+        const double Qs = qs0a * pow(x, -0.1) * (1 - x) / sqrt(ax * ay);
+        f = sqrt(4.0 * kPi * 3.0 / (3.0 * 3.0 - 1.0) * kT2 * kT2 / (Qs * Qs) * exp(-kT2 / (Qs * Qs)));
+    }
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        if (x <= 1.0) {
+            F[2 * i] = nf * F[2 * i] * f;
+            F[2 * i + 1] = nf * F[2 * i + 1] * f;
+        } else {
+            F[2 * i] = 0.0;
+            F[2 * i + 1] = 0.0;
+        }
+    }
+}
+
+struct TmdCharge {
+    const double* pos;  // nucleon positions, 3 per nucleon (fm)
+    int A;
+    double afm[3];
+    double gamma, acube, az;
+    int zc;
+};
+
+// TmdBasedCollision::SetChargeDensities' charge (TMDbased.cpp:27-57): F = (a^3 sqrt(aZInFm T / a_z) Re F, 0), T the
+// nucleus profile (GBWSampling.cpp:110-157) at x = (ix - Nx/2) aXInFm, y = (iy - Ny/2) aYInFm, z = (iz - centre) aZInFm
+__global__ void k_tmd_charge(double* __restrict__ F, Geom g, TmdCharge C) {
+    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const int z = (int)(s / g.P);
+    const long s2 = s - (long)z * g.P;
+    const int ix = (int)(s2 % g.nx), iy = (int)(s2 / g.nx);
+    const double x = (ix - g.nx / 2) * C.afm[0], y = (iy - g.ny / 2) * C.afm[1], zf = (z - C.zc) * C.afm[2];
+    const double BG = 4.0 * 0.197 * 0.197;
+    double T = 0.0;
+    for (int n = 0; n < C.A; n++) {
+        const double dx = x - C.pos[3 * n], dy = y - C.pos[3 * n + 1], dz = C.gamma * zf - C.pos[3 * n + 2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        T += C.gamma * exp(-0.5 * d2 / BG) / sqrt(2.0 * kPi * BG);
+    }
+    const double c = C.acube * sqrt(C.afm[2] * T / C.az);
+    for (int a = 0; a < 8; a++) {
+        const long i = fidx(g, a, s);
+        F[2 * i] = c * F[2 * i];
+        F[2 * i + 1] = 0.0;
+    }
+}
+
+bool tmd_valid(const glasma_tmd_params* p) {
+    return p && p->qs0a > 0.0 && p->m_eff >= 0.0 && p->com_energy > 0.0 && p->a_fm[0] > 0.0 && p->a_fm[1] > 0.0 &&
+           p->a_fm[2] > 0.0 && p->atomic_number > 0;
+}
+
+// ChargeFluctuation + TmdBasedCollision::SetWilsonLines for one nucleus: w.rho = filtered charge, w.V
+int tmd_wilson_lines(glasma_ctx* ctx, Wk& w, const glasma_tmd_params* p, const double* noise_host,
+                     const double* pos_host, bool right) {
+    const Geom& g = ctx->g;
+    const double* a = ctx->p.a;
+    // the noise goes through w.rho (8 S doubles), which the filter step overwrites later
+    GL_HIP(ctx, hipMemcpyAsync(w.rho, noise_host, 8 * (size_t)g.S * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    double* pos = nullptr;
+    GL_HIP(ctx, hipMalloc(&pos, 3 * (size_t)p->atomic_number * sizeof(double)));
+    hipError_t e = hipMemcpyAsync(pos, pos_host, 3 * (size_t)p->atomic_number * sizeof(double), hipMemcpyHostToDevice,
+                                  ctx->stream);
+    int rc = 0;
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "tmd nucleon positions");
+    if (!rc) {
+        hipLaunchKernelGGL(k_tmd_noise, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.rho, g, std::sqrt(a[0] * a[1] * a[2]));
+        rc = fft_3d(ctx->stream, w.F, g, 8, -1);
+        if (!rc) {
+            hipLaunchKernelGGL(k_tmd_cf_filter, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, a[0], a[1], p->qs0a,
+                               p->com_energy, p->a_fm[2]);
+            rc = fft_3d(ctx->stream, w.F, g, 8, +1);
+        }
+        if (rc) rc = set_error(ctx, rc, "FFT failed (lengths must be 2^a 3^b p <= 2048, prime factors <= 13)");
+    }
+    if (!rc) {
+        TmdCharge C;
+        C.pos = pos;
+        C.A = p->atomic_number;
+        for (int m = 0; m < 3; m++) C.afm[m] = p->a_fm[m];
+        C.gamma = p->com_energy * 0.197 / 2.0;
+        C.acube = a[0] * a[1] * a[2];
+        C.az = a[2];
+        C.zc = right ? p->center_r : p->center_l;
+        hipLaunchKernelGGL(k_tmd_charge, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, C);
+        rc = wilson_from_charge(ctx, w, std::numeric_limits<double>::infinity(), p->m_eff, right);
+    }
+    hipStreamSynchronize(ctx->stream);  // the positions buffer is freed below
+    hipFree(pos);
+    return rc;
+}
+
 bool mv_valid(const glasma_mv_params* mv) {
     return mv && mv->sigma_l > 0.0 && mv->sigma_r > 0.0 && mv->lambda > 0.0 && mv->m_eff >= 0.0;
 }
@@ -742,6 +874,36 @@ extern "C" int glasma_mv_initial_state(glasma_ctx* ctx, int driver, const glasma
     if (rc == 0) {
         hipError_t e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "mv_initial_state");
+    }
+    free_work(w);
+    return rc;
+}
+
+extern "C" int glasma_tmd_initial_state(glasma_ctx* ctx, int driver, const glasma_tmd_params* p, const double* noise_l,
+                                        const double* noise_r, const double* nucleons_l, const double* nucleons_r) {
+    if (!ctx) return GLASMA_ERR_ARG;
+    if (!tmd_valid(p)) return set_error(ctx, GLASMA_ERR_ARG, "invalid physical-charge parameters");
+    if (driver < 0 || driver > 2) return set_error(ctx, GLASMA_ERR_ARG, "driver must be 0, 1 or 2");
+    if ((driver != 2 && (!noise_l || !nucleons_l)) || (driver != 1 && (!noise_r || !nucleons_r)))
+        return set_error(ctx, GLASMA_ERR_ARG, "missing noise or nucleon positions");
+    int rc = reset_fields(ctx);
+    if (rc) return rc;
+    const int nz = ctx->g.nz;
+    Wk w;
+    rc = alloc_work(ctx, w);
+    if (rc == 0 && driver != 2) {
+        const int zhi = driver == 0 ? (nz - 2) / 2 : nz - 1;
+        rc = tmd_wilson_lines(ctx, w, p, noise_l, nucleons_l, false);
+        if (rc == 0) rc = single_sheet(ctx, w, false, 0, zhi, JPS, JP);
+    }
+    if (rc == 0 && driver != 1) {
+        const int zlo = driver == 0 ? nz / 2 : 0;
+        rc = tmd_wilson_lines(ctx, w, p, noise_r, nucleons_r, true);
+        if (rc == 0) rc = single_sheet(ctx, w, true, zlo, nz - 1, JMS, JM);
+    }
+    if (rc == 0) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "tmd_initial_state");
     }
     free_work(w);
     return rc;

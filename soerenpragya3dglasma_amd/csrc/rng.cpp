@@ -39,6 +39,7 @@ public:
         y ^= y >> 18;
         return y;
     }
+    double uniform() { return (double)(*this)() * (1.0 / 4294967296.0); }  // gsl_rng_uniform: [0, 1)
     double uniform_pos() {
         for (;;) {
             const double u = (double)(*this)() * (1.0 / 4294967296.0);
@@ -78,5 +79,53 @@ extern "C" int glasma_draw_charges(unsigned long seed, int nx, int ny, double qs
     const long n = 8L * nx * ny;  // y, x, a order = index a + 8*(x + nx*y)
     for (long i = 0; i < n; i++) xi_l[i] = qs_l * g.normal();
     for (long i = 0; i < n; i++) xi_r[i] = qs_r * g.normal();
+    return 0;
+}
+
+// Woods-Saxon radius and surface thickness (SC/TMDNUCLEUS/NuclearData.cpp:16-87)
+static bool woods_saxon(int A, double* R, double* d) {
+    switch (A) {
+        case 16: *R = 2.608; *d = 0.5130; return true;
+        case 54: *R = 5.420; *d = 0.57; return true;
+        case 197: *R = 6.38; *d = 0.535; return true;
+        case 208: *R = 6.62; *d = 0.546; return true;
+        default: return false;
+    }
+}
+
+// Nucleus::SetNucleonPositions (SC/TMDNUCLEUS/GBWSampling.cpp:21-87): for each nucleon, uniform positions in
+// [-4R, 4R)^3 accepted with probability (1 + e^{-R/d}) / (1 + e^{(r - R)/d}) against a fourth uniform; then the
+// centre of mass, accumulated as the sum of x_n / A, subtracted.
+static void sample_nucleus(glasma::Mt19937& g, int A, double R, double d, double* pos) {
+    for (int n = 0; n < A; n++) {
+        double* x = pos + 3 * n;
+        for (;;) {
+            x[0] = 8.0 * R * (g.uniform() - 0.5);
+            x[1] = 8.0 * R * (g.uniform() - 0.5);
+            x[2] = 8.0 * R * (g.uniform() - 0.5);
+            const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+            const double prob = (1.0 + std::exp(-R / d)) / (1.0 + std::exp((r - R) / d));
+            if (g.uniform() < prob) break;
+        }
+    }
+    double bar[3] = {0.0, 0.0, 0.0};
+    for (int n = 0; n < A; n++)
+        for (int i = 0; i < 3; i++) bar[i] += pos[3 * n + i] / double(A);
+    for (int n = 0; n < A; n++)
+        for (int i = 0; i < 3; i++) pos[3 * n + i] -= bar[i];
+}
+
+extern "C" int glasma_tmd_draw(unsigned long seed, int nx, int ny, int nz, int atomic_number, double* noise_l,
+                               double* noise_r, double* nucleons_l, double* nucleons_r) {
+    double R, d;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !noise_l || !noise_r || !nucleons_l || !nucleons_r ||
+        !woods_saxon(atomic_number, &R, &d))
+        return GLASMA_ERR_ARG;
+    glasma::Mt19937 g(seed);
+    const long n = 8L * nx * ny * nz;
+    for (long i = 0; i < n; i++) noise_l[i] = g.normal();
+    sample_nucleus(g, atomic_number, R, d, nucleons_l);
+    for (long i = 0; i < n; i++) noise_r[i] = g.normal();
+    sample_nucleus(g, atomic_number, R, d, nucleons_r);
     return 0;
 }

@@ -287,6 +287,35 @@ class Glasma:
         self._chk(_lib.lib().glasma_mv_initial_state(self._h, int(driver), ctypes.byref(mv), _p(xl), _p(xr)),
                   "glasma_mv_initial_state")
 
+    @staticmethod
+    def tmd_params(qs0a, m_eff, com_energy, a_fm, atomic_number, center_l, center_r) -> "_lib.TmdParams":
+        """Physical-charge parameters (SC/LATTICE/Parameters.cpp): Qs0A and MEff in lattice units, CoMEnergy in
+        fm^-1, the spacings in fm, the atomic number and the nuclei's z centres."""
+        p = _lib.TmdParams()
+        p.qs0a, p.m_eff, p.com_energy = qs0a, m_eff, com_energy
+        for i in range(3):
+            p.a_fm[i] = a_fm[i]
+        p.atomic_number, p.center_l, p.center_r = int(atomic_number), int(center_l), int(center_r)
+        return p
+
+    @staticmethod
+    def tmd_draw(seed: int, N, atomic_number: int):
+        """SimulationmitPC::Run's draws: (noise_l, noise_r) shaped (Nz, Ny, Nx, 8), nucleons (A, 3) in fm."""
+        Nx, Ny, Nz = N
+        nl = np.zeros((Nz, Ny, Nx, 8))
+        nr = np.zeros_like(nl)
+        pl = np.zeros((atomic_number, 3))
+        pr = np.zeros_like(pl)
+        check(_lib.lib().glasma_tmd_draw(int(seed), Nx, Ny, Nz, int(atomic_number), _p(nl), _p(nr), _p(pl), _p(pr)),
+              None, "glasma_tmd_draw")
+        return nl, nr, pl, pr
+
+    def tmd_initial_state(self, driver: int, p, noise_l, noise_r, nucleons_l, nucleons_r):
+        """SimulationmitPC::Run's initial state (physical-charge nuclei); driver 0 collision, 1 left, 2 right."""
+        arrs = [np.ascontiguousarray(v, np.float64) for v in (noise_l, noise_r, nucleons_l, nucleons_r)]
+        self._chk(_lib.lib().glasma_tmd_initial_state(self._h, int(driver), ctypes.byref(p), *[_p(v) for v in arrs]),
+                  "glasma_tmd_initial_state")
+
     def set_wilson_lines(self, direction: str, mv: MVParams, xi):
         Nx, Ny, Nz = self.N
         V = np.empty((Nz + 2, Ny, Nx, 3, 3), np.complex128)
