@@ -15,7 +15,9 @@
 //   restart   -Loaded 1 -i1 U -i2 E -i3 J -Time t (Simulation.cpp:396-421): the fields from the reference's
 //             text configuration files, the configuration saved again as UDynT/EDynT/JDynT<t>ID<seed>.txt,
 //             then Evolve(100).
-// Project keys: -driver collision|left|right, -QsAL (the charge scale, Parameters::Qs0A_L, which the
+// Project keys: -driver collision|left|right|tmd (tmd: SimulationmitPC.cpp, the collision of physical-charge
+// nuclei, with -A, -centerL, -centerR, -CoMEnergy, -Qs0, -SigmaLOverAz for the Parameters it reads, evolved to
+// t = 100 with configurations every 1500 steps), -QsAL (the charge scale, Parameters::Qs0A_L, which the
 // reference's command line does not reach), -tmax (Evolve's end time: 1100, or 100 after -Loaded),
 // -steps (stop after this many steps), -bc 0|1 (open / periodic z), -tune 0|1 (time the step's tile
 // orders first), -block n (steps per glasma_update call between observable points; at most 20),
@@ -133,14 +135,26 @@ int main(int argc, char** argv) {
     mv.center_r = (int)(1.15 * p.n[2] / 2 - 1);
     const long seed = A.integer("SEED", 0);
     const std::string drv = A.str("driver", "collision");
+    const bool tmd = drv == "tmd";
     const int driver = drv == "left" ? 1 : (drv == "right" ? 2 : 0);
-    if (driver == 0 && drv != "collision") {
-        std::cerr << "#ERROR -driver must be collision, left or right" << std::endl;
+    if (driver == 0 && drv != "collision" && !tmd) {
+        std::cerr << "#ERROR -driver must be collision, left, right or tmd" << std::endl;
         return 2;
     }
+    // physical-charge nuclei (SimulationmitPC.cpp; LATTICE/Parameters.cpp:30-46: Qs0 = 0.5 GeV, Qs0R = 1,
+    // SigmaLOverAz = 512, AtomicNumber = 197, CoMEnergy = 500 / 0.197 fm^-1; TMDbased.cpp:43,48: centres)
+    glasma_tmd_params tp;
+    tp.qs0a = qs0a;
+    tp.m_eff = mv.m_eff;
+    tp.com_energy = A.num("CoMEnergy", 500 / 0.197);
+    tp.a_fm[0] = tp.a_fm[1] = qs0a * 0.197 / A.num("Qs0", 0.5);
+    tp.a_fm[2] = tp.a_fm[0] * 1.0 / (qs0a * A.num("SigmaLOverAz", 512));
+    tp.atomic_number = (int)A.integer("A", 197);
+    tp.center_l = (int)A.integer("centerL", 785);
+    tp.center_r = (int)A.integer("centerR", 1261);
     const bool loaded = A.integer("Loaded", 0) != 0;
     const std::string state_in = A.str("state_in", ""), state_out = A.str("state_out", "");
-    const double tmax = A.num("tmax", loaded ? 100.0 : 1100.0);
+    const double tmax = A.num("tmax", (loaded || tmd) ? 100.0 : 1100.0);
     const long max_steps = A.integer("steps", -1);
     const int block = (int)std::max(1L, std::min(20L, A.integer("block", 20)));
     std::string out = A.str("o", "OUTPUT");
@@ -189,11 +203,22 @@ int main(int argc, char** argv) {
     } else if (!state_in.empty()) {
         if (int rc = glasma_load_state(ctx, state_in.c_str())) die("glasma_load_state", ctx, rc);
     } else {
-        // initial state (Simulation.cpp:337-378)
-        std::vector<double> xl(8L * p.n[0] * p.n[1]), xr(8L * p.n[0] * p.n[1]);
-        if (glasma_draw_charges((unsigned long)seed, p.n[0], p.n[1], qsl, qsl, xl.data(), xr.data()))
-            die("glasma_draw_charges", ctx);
-        if (glasma_mv_initial_state(ctx, driver, &mv, xl.data(), xr.data())) die("glasma_mv_initial_state", ctx);
+        if (tmd) {
+            // SimulationmitPC::Run (SimulationmitPC.cpp:326-370): noise and nucleons of both nuclei from one stream
+            const size_t nn = 8 * (size_t)p.n[0] * p.n[1] * p.n[2];
+            std::vector<double> nl(nn), nr(nn), pl(3 * (size_t)tp.atomic_number), pr(3 * (size_t)tp.atomic_number);
+            if (glasma_tmd_draw((unsigned long)seed, p.n[0], p.n[1], p.n[2], tp.atomic_number, nl.data(), nr.data(),
+                                pl.data(), pr.data()))
+                die("glasma_tmd_draw (no Woods-Saxon parameters for this -A)", ctx);
+            if (glasma_tmd_initial_state(ctx, 0, &tp, nl.data(), nr.data(), pl.data(), pr.data()))
+                die("glasma_tmd_initial_state", ctx);
+        } else {
+            // initial state (Simulation.cpp:337-378)
+            std::vector<double> xl(8L * p.n[0] * p.n[1]), xr(8L * p.n[0] * p.n[1]);
+            if (glasma_draw_charges((unsigned long)seed, p.n[0], p.n[1], qsl, qsl, xl.data(), xr.data()))
+                die("glasma_draw_charges", ctx);
+            if (glasma_mv_initial_state(ctx, driver, &mv, xl.data(), xr.data())) die("glasma_mv_initial_state", ctx);
+        }
     }
     if (A.integer("tune", 0)) {
         if (glasma_tune_step(ctx, 3, nullptr, nullptr, nullptr, nullptr)) die("glasma_tune_step", ctx);
@@ -286,7 +311,20 @@ int main(int argc, char** argv) {
             emt_map();
             profiles();
         }
+        if (tmd && s % 1500 == 0) {  // SimulationmitPC.cpp Evolve: configurations every 1500 steps
+            const std::string u = cat(out, "UDynT", now(ctx), "ID", seed, ".txt"),
+                              e = cat(out, "EDynT", now(ctx), "ID", seed, ".txt"),
+                              j = cat(out, "JDynT", now(ctx), "ID", seed, ".txt");
+            if (int rc = glasma_save_configuration(ctx, u.c_str(), e.c_str(), j.c_str()))
+                die("glasma_save_configuration", ctx, rc);
+        }
         if (s % 1000 == 0) {
+            if (tmd) {  // and the Gauss-law profile with the unitarity check every 1000 steps
+                if (glasma_gauss(ctx, per_z.data(), &gmax, &grms)) die("glasma_gauss", ctx);
+                std::ofstream f(cat(out, "GVDynT", now(ctx), "ID", seed, ".txt"));
+                f.precision(15);
+                for (long z = 0; z < Nz; z++) f << z << "  " << per_z[z] << std::endl;
+            }
             if (glasma_unitarity(ctx, &umax, &urms)) die("glasma_unitarity", ctx);
             std::cerr << "#UNITARITY VIOLATION AT T=" << now(ctx) << " MAX=" << umax << " AVG=" << urms << std::endl;
         }

@@ -54,3 +54,33 @@ def test_tmd_argument_errors():
         p.com_energy = 500 / 0.197
         with pytest.raises(GlasmaError, match="driver"):
             g.tmd_initial_state(3, p, nl, nr, pl, pr)
+
+
+def test_driver_tmd_run(tmp_path):
+    """glasma_solver -driver tmd: the physical-charge collision's initial energy equals the API's, then 100 steps
+    with the collision cadence."""
+    import os
+    import subprocess
+    from soerenpragya3dglasma_amd import Glasma
+    solver = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "soerenpragya3dglasma_amd",
+                          "_lib", "glasma_solver")
+    N, seed = (8, 8, 32), 4
+    out = tmp_path / "out"
+    r = subprocess.run([solver, "-o", str(out), "-Nx", "8", "-Ny", "8", "-Nz", "32", "-a", "1", "-dT", "0.25",
+                        "-SEED", str(seed), "-driver", "tmd", "-A", "16", "-centerL", "8", "-centerR", "23",
+                        "-steps", "100"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    energy = [l.split() for l in open(out / f"EnergyID{seed}.txt").read().splitlines()]
+    assert len(energy) == 6 and [int(e[0]) for e in energy[1:]] == [20, 40, 60, 80, 100]
+    assert os.path.exists(out / f"EMTDynT25ID{seed}.txt")
+    qs = 0.33
+    afm = qs * 0.197 / 0.5
+    p = Glasma.tmd_params(qs, 0.33, 500 / 0.197, (afm, afm, afm / (qs * 512)), 16, 8, 23)
+    nl, nr, pl, pr = Glasma.tmd_draw(seed, N, 16)
+    with Glasma(N, (1.0, 1.0, 1.0), 0.25) as g:
+        g.tmd_initial_state(0, p, nl, nr, pl, pr)
+        T0 = g.bulk()["T00"]
+    assert float(energy[0][1]) == pytest.approx(T0, rel=1e-13)
+    r = subprocess.run([solver, "-o", str(tmp_path / "bad"), "-N", "4", "-Nz", "8", "-driver", "tmd", "-A", "12"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Woods-Saxon" in r.stderr
