@@ -318,40 +318,46 @@ GHD void step_vjp_staples(const StepAdjArgs& A, I s, ADD add) {
 // Mbar Sigma^dag at x; C2 gathers, for each link (y, lam), the staple terms of every site x whose E update
 // reads it.  The sources of a target are the preimages of the forward neighbour map, so the clamped
 // neighbours of open z boundaries contribute exactly as in the scatter form.
-template <class I>
-GHD void step_vjp_sigbar(const StepAdjArgs& A, I s, double* __restrict__ Sb) {
+template <class I, int MU>
+GHD void step_vjp_sigbar_link(const StepAdjArgs& A, I s, double* __restrict__ Sb) {
     const Geom& g = A.g;
     int x, y, z;
     site_coords<I>(g, s, x, y, z);
-    for (int mu = 0; mu < 3; mu++) {
-        double eb[8];
-        ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
-        M3 Mb, Umu, Sig, T, Ub;
-        m_from_retr_cot(A.gc[mu], eb, Mb);
-        ldm<I>(A.U, kUPlanes, 18 * mu, s, Umu);
-        mul_dn(Umu, Mb, T);
-        stm<I>(Sb, kUPlanes, 18 * mu, s, T);
-        m3_zero(Sig);
-        for (int q = 1; q <= 2; q++) {
-            const int nu = (mu + q) % 3;
-            M3 P1, P2, Tq;
-            ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, 0), P1);
-            ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, nu, 1, mu, 0), P2);
-            mul_nd(P1, P2, Tq);
-            ldm<I>(A.U, kUPlanes, 18 * nu, s, P1);
-            acc_mul_nd(A.cE[nu], Tq, P1, Sig);
-            const I sm = nbr<I>(g, x, y, z, nu, -1, mu, 0);
-            ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, -1), P1);
-            ldm<I>(A.U, kUPlanes, 18 * mu, sm, P2);
-            mul_dd(P1, P2, Tq);
-            ldm<I>(A.U, kUPlanes, 18 * nu, sm, P1);
-            acc_mul_nn(A.cE[nu], Tq, P1, Sig);
-        }
-        mul_nd(Mb, Sig, T);
-        ldm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
-        m3_axpy(1.0, T, Ub);
-        stm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
+    constexpr int mu = MU;
+    double eb[8];
+    ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+    M3 Mb, Umu, Sig, T, Ub;
+    m_from_retr_cot(A.gc[mu], eb, Mb);
+    ldm<I>(A.U, kUPlanes, 18 * mu, s, Umu);
+    mul_dn(Umu, Mb, T);
+    stm<I>(Sb, kUPlanes, 18 * mu, s, T);
+    m3_zero(Sig);
+    for (int q = 1; q <= 2; q++) {
+        const int nu = (mu + q) % 3;
+        M3 P1, P2, Tq;
+        ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, 0), P1);
+        ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, nu, 1, mu, 0), P2);
+        mul_nd(P1, P2, Tq);
+        ldm<I>(A.U, kUPlanes, 18 * nu, s, P1);
+        acc_mul_nd(A.cE[nu], Tq, P1, Sig);
+        const I sm = nbr<I>(g, x, y, z, nu, -1, mu, 0);
+        ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, x, y, z, mu, 1, nu, -1), P1);
+        ldm<I>(A.U, kUPlanes, 18 * mu, sm, P2);
+        mul_dd(P1, P2, Tq);
+        ldm<I>(A.U, kUPlanes, 18 * nu, sm, P1);
+        acc_mul_nn(A.cE[nu], Tq, P1, Sig);
     }
+    mul_nd(Mb, Sig, T);
+    ldm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
+    m3_axpy(1.0, T, Ub);
+    stm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
+}
+
+template <class I>
+GHD void step_vjp_sigbar(const StepAdjArgs& A, I s, double* __restrict__ Sb) {
+    step_vjp_sigbar_link<I, 0>(A, s, Sb);
+    step_vjp_sigbar_link<I, 1>(A, s, Sb);
+    step_vjp_sigbar_link<I, 2>(A, s, Sb);
 }
 
 // source z layers of a target layer zy under a z shift dz of the forward neighbour map (up to two with
@@ -372,12 +378,13 @@ GHD int z_sources(const Geom& g, int zy, int dz, int* zs) {
     return n;
 }
 
-template <class I>
-GHD void step_vjp_gather(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
+template <class I, int LAM>
+GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
     const Geom& g = A.g;
     int x, y, z;
     site_coords<I>(g, s, x, y, z);
-    for (int lam = 0; lam < 3; lam++) {
+    constexpr int lam = LAM;
+    {
         M3 acc, Sbar, P, Q, R, T1, T2;
         ldm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
         for (int k = 1; k <= 2; k++) {
@@ -476,6 +483,13 @@ GHD void step_vjp_gather(const StepAdjArgs& A, const double* __restrict__ Sb, I 
         }
         stm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
     }
+}
+
+template <class I>
+GHD void step_vjp_gather(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
+    step_vjp_gather_link<I, 0>(A, Sb, s);
+    step_vjp_gather_link<I, 1>(A, Sb, s);
+    step_vjp_gather_link<I, 2>(A, Sb, s);
 }
 
 // Slab energy 1/2 sum_{|z - zc| < half} sum_mu c_mu (E_mu^2 + 4 Re tr(1 - P_mu)), c_mu = (a_mu/a^3)^2,

@@ -33,13 +33,16 @@ __global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s < A.g.S) step_vjp_down<unsigned>(A, (unsigned)s);
 }
+// passes C1 and C2 run one launch per link direction: a third of the work per thread, fewer live registers
+template <int MU>
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_sigbar(StepAdjArgs A, double* Sb) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < A.g.S) step_vjp_sigbar<unsigned>(A, (unsigned)s, Sb);
+    if (s < A.g.S) step_vjp_sigbar_link<unsigned, MU>(A, (unsigned)s, Sb);
 }
+template <int LAM>
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < A.g.S) step_vjp_gather<unsigned>(A, Sb, (unsigned)s);
+    if (s < A.g.S) step_vjp_gather_link<unsigned, LAM>(A, Sb, (unsigned)s);
 }
 
 // slab energy: per-block partial sums, then one block sums them in a fixed order
@@ -122,8 +125,12 @@ int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A, double* Sb) {
     const long S = ctx->g.S;
     hipLaunchKernelGGL(k_vjp_local, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
     hipLaunchKernelGGL(k_vjp_down, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_vjp_sigbar, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
-    hipLaunchKernelGGL(k_vjp_gather, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
+    hipLaunchKernelGGL(k_vjp_sigbar<0>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
+    hipLaunchKernelGGL(k_vjp_sigbar<1>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
+    hipLaunchKernelGGL(k_vjp_sigbar<2>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
+    hipLaunchKernelGGL(k_vjp_gather<0>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
+    hipLaunchKernelGGL(k_vjp_gather<1>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
+    hipLaunchKernelGGL(k_vjp_gather<2>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
     GL_HIP(ctx, hipGetLastError());
     return 0;
 }
