@@ -113,6 +113,26 @@ def batched_ensemble(nevents: int, steps: int) -> dict:
     return out
 
 
+def gauge_fixing_collision(iters: int) -> dict:
+    """Coulomb gauge fixing of the 64 x 64 x 256 MV collision (charge seeds 1 / 2, width 6, centres 64 apart)
+    after 50 steps, the state the reference's collision workload fixes every 50 steps: ms per iteration."""
+    from soerenpragya3dglasma_amd import Glasma
+    N = (64, 64, 256)
+    xl, _ = Glasma.draw_charges(1, N[0], N[1], 0.5, 0.5)
+    _, xr = Glasma.draw_charges(2, N[0], N[1], 0.5, 0.5)
+    mv = Glasma.mv_params(lam=1.65, meff=0.1, sigma_l=6.0, sigma_r=6.0, center_l=128 - 32, center_r=128 + 32)
+    with Glasma(N, (1.0, 1.0, 1.0), 0.25) as g:
+        g.mv_initial_state(0, mv, xl, xr)
+        g.Update(50)
+        g.coulomb_gauge(tol=0.0, maxit=1, alpha=2.0)
+        t0 = time.perf_counter()
+        st = g.coulomb_gauge(tol=0.0, maxit=iters, alpha=2.0)
+        t1 = time.perf_counter()
+    n = max(1, int(st[0]))
+    return {"iterations": int(st[0]), "ms_per_iteration": 1e3 * (t1 - t0) / n,
+            "iterations_per_s": n / (t1 - t0), "max_deviation": float(st[1])}
+
+
 def adjoint_cost(N=(48, 48, 128), steps: int = 200, ckpt: int = 20) -> dict:
     """Gradient of the slab energy (|z - Nz/2| < 8) after `steps` steps of a left-moving MV sheet with
     respect to its charge density rho(x), against the forward evaluation of the same energy."""
@@ -245,8 +265,10 @@ def main():
         st = g.coulomb_gauge(tol=0.0, maxit=args.gauge_iters, alpha=2.0)
         t2 = time.perf_counter()
         gauge = {"iterations": int(st[0]), "ms_per_iteration": 1e3 * (t2 - t1) / max(1, int(st[0])),
+                 "iterations_per_s": max(1, int(st[0])) / (t2 - t1),
                  "first_call_ms": 1e3 * (t1 - t0), "max_deviation": float(st[1])}
         log(f"[bench] Coulomb gauge fixing: {gauge['ms_per_iteration']:.1f} ms per iteration")
+        gauge["collision_64x64x256"] = gauge_fixing_collision(args.gauge_iters)
     log(f"[bench] {ms_per_step:.3f} ms/step wall, {kern_ms:.3f} ms/kernel, {achieved:.0f} GB/s algorithmic, "
         f"unitarity max {umax:.2e}")
     g.close()
