@@ -70,6 +70,36 @@ def cpu_baseline(sample_n, steps):
     }
 
 
+def reference_case(threads: int) -> dict:
+    """The reference's own CPU-sized case in full: a 32 x 32 x 64 left-moving MV nucleus (g^2 mu 0.5, width 4,
+    m 0.1, charge seed 1, dt = a/4), 100 steps, on the GPU and with the oracle on the host; the energy after
+    100 steps is compared as a parity check."""
+    from oracle import oracle as O
+    from soerenpragya3dglasma_amd import Glasma
+    N, a, dt, steps = (32, 32, 64), (1.0, 1.0, 1.0), 0.25, 100
+    xl, xr = Glasma.draw_charges(1, N[0], N[1], 0.5, 0.5)
+    kw = dict(lam=1.65, meff=0.1, sigma_l=4.0, sigma_r=4.0, center_l=N[2] // 2, center_r=N[2] // 2)
+    with Glasma(N, a, dt) as g:
+        g.mv_initial_state(1, Glasma.mv_params(**kw), xl, xr)
+        g.Update(1)
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.update_async(steps)
+        g.synchronize()
+        gpu_s = time.perf_counter() - t0
+        g.mv_initial_state(1, Glasma.mv_params(**kw), xl, xr)
+        g.Update(steps)
+        t00_gpu = g.bulk()["T00"]
+    f = O.mv_initial_state(1, N, a, dt, xl, xr, **kw)
+    t0 = time.perf_counter()
+    O.update(f, a, dt, steps)
+    cpu_s = time.perf_counter() - t0
+    t00_cpu = O.bulk_T(O.bulk(f, a), a)["T00"]
+    return {"lattice": list(N), "steps": steps, "gpu_ms_per_step": 1e3 * gpu_s / steps,
+            "cpu_ms_per_step": 1e3 * cpu_s / steps, "cpu_cores": threads, "gpu_over_cpu": cpu_s / gpu_s,
+            "t00_rel_diff": abs(t00_gpu - t00_cpu) / abs(t00_cpu)}
+
+
 def batched_ensemble(nevents: int, steps: int) -> dict:
     """The ensemble workload: events drawn with master seed 12345 (numpy RandomState), N_perp uniform in
     {32, 48, 64, 96}, Nz uniform in {128, 192, 256, 384}, g^2 mu uniform in [0.3, 0.8], charge seeds 2k+1
@@ -286,6 +316,7 @@ def main():
         log("[bench] cpu baseline (oracle on the host)")
         try:
             cpu = cpu_baseline(tuple(args.cpu_sample), args.cpu_steps)
+            cpu["reference_case"] = reference_case(cpu["cores"])
         except Exception as e:
             log(f"[bench] cpu baseline failed: {e}")
 
