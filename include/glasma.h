@@ -103,7 +103,8 @@ int glasma_last_step_kernel_ms(const glasma_ctx* ctx, double* ms);
 
 /* ---- observables ------------------------------------------------------------------------------ */
 /* Observables::Bulk::Update (OBSERVABLES/BulkObservables.cpp:172-187): out9 = ESqr[3], BSqr[3],
- * EDotJ[3] (raw sums; T00..SOURCE follow from BulkObservables.cpp:22-62). */
+ * EDotJ[3] (raw sums; T00..SOURCE follow from BulkObservables.cpp:22-62).  GLASMA_ERR_NUMERIC (with the
+ * sums filled in as computed) when a sum is not finite. */
 int glasma_bulk(glasma_ctx* ctx, double* out9);
 /* GaussLaw::UpdateViolation (OBSERVABLES/GaussLawViolation.cpp:16-124): per_z[Nz] (may be NULL),
  * global max |G^a| and RMS. */

@@ -1,5 +1,6 @@
 // libglasma core: context lifetime, field transfer between the reference AoS layout and the device
 // SoA layout, the time loop entry points and the observable reductions.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -720,6 +721,10 @@ int glasma_bulk(glasma_ctx* ctx, double* out9) {
     for (int k = 0; k < 9; k++) out9[k] = 0.0;
     for (int z = 0; z < ctx->g.nz; z++)
         for (int k = 0; k < 9; k++) out9[k] += pz[(size_t)z * 9 + k];
+    for (int k = 0; k < 9; k++)
+        if (!std::isfinite(out9[k]))
+            return set_error(ctx, GLASMA_ERR_NUMERIC, "non-finite fields: the evolution diverged or a step's current "
+                                                      "hand-over timed out (sums returned as computed)");
     return 0;
 }
 

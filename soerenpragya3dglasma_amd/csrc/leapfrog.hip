@@ -81,13 +81,20 @@ __device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y
     acc_mul_nn(cn, T, P1, Sig);
 }
 
-// wait for the J wavefront's hand-over (set unconditionally once its LDS stores are complete; capped)
-__device__ __forceinline__ void wait_ready(StepShared& sh) {
+// wait for the J wavefront's hand-over (set unconditionally once its LDS stores are complete).  The wait is
+// capped so that no wave can spin forever; false if the cap was reached, and the caller then writes NaN
+// currents (glasma_bulk reports non-finite sums as GLASMA_ERR_NUMERIC) instead of silently using stale values.
+__device__ __forceinline__ bool wait_ready(StepShared& sh) {
+    bool ok = false;
     for (int spin = 0; spin < (1 << 24); spin++) {
-        if (__hip_atomic_load(&sh.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+        if (__hip_atomic_load(&sh.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            ok = true;
+            break;
+        }
         __builtin_amdgcn_s_sleep(2);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return ok;
 }
 
 // link_update with the row operands from LDS
@@ -212,7 +219,7 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
         } else if (valid) {
             link_update<I, 2>(A, s, x, y, z, Uh);
         }
-        wait_ready(sh);
+        const bool handed_over = wait_ready(sh);
         if (valid) {
             double jps[8], jms[8], jpsU[8], jmsU[8], jp[8], jm[8];
 #pragma unroll
@@ -224,6 +231,7 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
                 jp[a] = sh.j[4][a][lane];
                 jm[a] = sh.j[5][a][lane];
             }
+            if (!handed_over) jp[0] = jm[0] = __builtin_nan("");
             j_advect_with<I>(A, s, Uh, jps, jms, jpsU, jmsU, jp, jm);
         }
     }

@@ -71,3 +71,18 @@ def test_driver_writes_the_output_files(tmp_path):
         cur = np.loadtxt(out / f"CurrentsDynT{t}ID3.txt")
         assert cur.shape == (18, 5) and np.array_equal(cur[:, 0], np.arange(-1, 17))
         assert np.array_equal(cur[0, 1:], cur[1, 1:]) and cur[:, 1:].max() > 0
+
+
+def test_bulk_reports_non_finite_fields():
+    """glasma_bulk returns GLASMA_ERR_NUMERIC for non-finite sums (a diverged evolution)."""
+    from soerenpragya3dglasma_amd import Glasma, GlasmaError
+    N = (8, 6, 4)
+    f = random_fields(N, seed=40)
+    f.E[2, 3, 4, 1, 5] = np.nan
+    with Glasma(N, (1.0, 1.0, 1.0), 0.1, 0) as g:
+        upload(g, f)
+        with pytest.raises(GlasmaError, match="non-finite"):
+            g.bulk_raw()
+        f.E[2, 3, 4, 1, 5] = 0.0
+        upload(g, f)
+        assert np.all(np.isfinite(g.bulk_raw()))
