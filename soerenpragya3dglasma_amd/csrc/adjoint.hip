@@ -33,16 +33,25 @@ __global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s < A.g.S) step_vjp_down<unsigned>(A, (unsigned)s);
 }
-// passes C1 and C2 run one launch per link direction: a third of the work per thread, fewer live registers
-template <int MU>
+// passes C1 and C2 give each thread one link direction (blockIdx.y): a third of the work per thread, and the
+// three directions of a pass run in one launch
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_sigbar(StepAdjArgs A, double* Sb) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < A.g.S) step_vjp_sigbar_link<unsigned, MU>(A, (unsigned)s, Sb);
+    if (s >= A.g.S) return;
+    switch (blockIdx.y) {
+        case 0: step_vjp_sigbar_link<unsigned, 0>(A, (unsigned)s, Sb); break;
+        case 1: step_vjp_sigbar_link<unsigned, 1>(A, (unsigned)s, Sb); break;
+        default: step_vjp_sigbar_link<unsigned, 2>(A, (unsigned)s, Sb); break;
+    }
 }
-template <int LAM>
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb) {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < A.g.S) step_vjp_gather_link<unsigned, LAM>(A, Sb, (unsigned)s);
+    if (s >= A.g.S) return;
+    switch (blockIdx.y) {
+        case 0: step_vjp_gather_link<unsigned, 0>(A, Sb, (unsigned)s); break;
+        case 1: step_vjp_gather_link<unsigned, 1>(A, Sb, (unsigned)s); break;
+        default: step_vjp_gather_link<unsigned, 2>(A, Sb, (unsigned)s); break;
+    }
 }
 
 // slab energy: per-block partial sums, then one block sums them in a fixed order
@@ -125,12 +134,9 @@ int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A, double* Sb) {
     const long S = ctx->g.S;
     hipLaunchKernelGGL(k_vjp_local, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
     hipLaunchKernelGGL(k_vjp_down, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_vjp_sigbar<0>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
-    hipLaunchKernelGGL(k_vjp_sigbar<1>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
-    hipLaunchKernelGGL(k_vjp_sigbar<2>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, Sb);
-    hipLaunchKernelGGL(k_vjp_gather<0>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
-    hipLaunchKernelGGL(k_vjp_gather<1>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
-    hipLaunchKernelGGL(k_vjp_gather<2>, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
+    const dim3 g3(agrid(S).x, 3);
+    hipLaunchKernelGGL(k_vjp_sigbar, g3, dim3(kAdjBlock), 0, ctx->stream, A, Sb);
+    hipLaunchKernelGGL(k_vjp_gather, g3, dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
     GL_HIP(ctx, hipGetLastError());
     return 0;
 }
