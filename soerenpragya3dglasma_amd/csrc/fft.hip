@@ -64,7 +64,10 @@ __device__ __forceinline__ dbl2 twid(const dbl2* tw, int k, int sign) {
 __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ data, Radices R, long es, long nlines,
                                                             long b1, long s1, long s2, int sign, int lpb, int ld,
                                                             const dbl2* __restrict__ tw) {
-    __shared__ dbl2 buf[2][kFftLds];
+    // two LDS buffers of lpb * ld entries each, sized per launch (at most 2 * kFftLds entries), so that short
+    // lines do not pay for the longest line's LDS and more workgroups fit on a CU
+    extern __shared__ dbl2 lds[];
+    dbl2* const buf[2] = {lds, lds + lpb * ld};
     const int n = R.n;
     const long l0 = (long)blockIdx.x * lpb;
     const int nload = lpb * n;
@@ -203,8 +206,9 @@ int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1
     const int ld = lpb > 1 ? n + 1 : n;
     const long blocks = (nlines + lpb - 1) / lpb;
     if (blocks > 0x7fffffffL) return GLASMA_ERR_ARG;
-    hipLaunchKernelGGL(k_fft_lines, dim3((unsigned)blocks), dim3(kFftThreads), 0, st, data, R, es, nlines, b1, s1, s2,
-                       sign, lpb, ld, tw);
+    const size_t lds = 2 * sizeof(dbl2) * (size_t)lpb * ld;
+    hipLaunchKernelGGL(k_fft_lines, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es, nlines, b1, s1,
+                       s2, sign, lpb, ld, tw);
     return hipGetLastError() == hipSuccess ? 0 : GLASMA_ERR_HIP;
 }
 
