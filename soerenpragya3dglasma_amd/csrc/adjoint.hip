@@ -168,8 +168,9 @@ struct DevBuf {
     }
     double* get(glasma_ctx* ctx, size_t doubles, int* rc) {
         double* q = nullptr;
-        if (hipMalloc(&q, doubles * sizeof(double)) != hipSuccess) {
-            *rc = set_error(ctx, GLASMA_ERR_NOMEM, "device allocation failed in the adjoint");
+        const hipError_t e = hipMalloc(&q, doubles * sizeof(double));
+        if (e != hipSuccess) {
+            *rc = set_error(ctx, hip_code(e), "device allocation failed in the adjoint");
             return nullptr;
         }
         p.push_back(q);

@@ -80,5 +80,6 @@ extern "C" int glasma_su3_batch(int op, long n, double c, const double* in, cons
     if (din) hipFree(din);
     if (din2) hipFree(din2);
     if (dout) hipFree(dout);
+    if (rc) (void)hipGetLastError();  // leave no stale error for the next launch check
     return rc;
 }

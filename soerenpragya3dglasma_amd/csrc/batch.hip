@@ -168,7 +168,7 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
     if (err == hipSuccess) err = hipMalloc(&b->block_site, (size_t)nb * sizeof(unsigned));
     if (err == hipSuccess) err = upload_work_list(b);
     if (err != hipSuccess) {
-        int code = err == hipErrorOutOfMemory ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP;
+        const int code = hip_code(err);
         glasma_batch_destroy(b);
         return code;
     }

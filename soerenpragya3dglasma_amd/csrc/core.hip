@@ -15,9 +15,13 @@ int set_error(glasma_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
     return code;
 }
+int hip_code(hipError_t e) {
+    (void)hipGetLastError();
+    return e == hipErrorOutOfMemory ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP;
+}
 int hip_fail(glasma_ctx* ctx, hipError_t e, const char* what) {
     std::string m = std::string(what) + ": " + hipGetErrorString(e);
-    return set_error(ctx, e == hipErrorOutOfMemory ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP, m);
+    return set_error(ctx, hip_code(e), m);
 }
 int ensure_red(glasma_ctx* ctx, size_t doubles) {
     if (ctx->red_cap >= doubles) return 0;
@@ -574,7 +578,7 @@ int glasma_create(const glasma_params* p, glasma_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&ctx->J[0], (size_t)kJPlanes * S * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->J[1], (size_t)kJPlanes * S * sizeof(double));
     if (e != hipSuccess) {
-        int code = (e == hipErrorOutOfMemory) ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP;
+        const int code = hip_code(e);
         glasma_destroy(ctx);
         return create_fail(code, std::string("glasma_create: ") + hipGetErrorString(e));
     }

@@ -200,7 +200,8 @@ int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1
     Radices R;
     if (factorize(n, R) != 0) return GLASMA_ERR_ARG;
     const dbl2* tw = nullptr;
-    if (twiddles(n, &tw) != hipSuccess) return GLASMA_ERR_HIP;
+    const hipError_t te = twiddles(n, &tw);
+    if (te != hipSuccess) return hip_code(te);
     int lpb = 1;
     while (2 * lpb * (n + 1) <= kFftLds && 2 * lpb <= 64) lpb *= 2;
     const int ld = lpb > 1 ? n + 1 : n;
@@ -243,7 +244,8 @@ extern "C" int glasma_fft_host(double* data, int nd, const int* dims, long batch
     }
     double* dev = nullptr;
     const size_t bytes = sizeof(double) * 2 * (size_t)total;
-    if (hipMalloc(&dev, bytes) != hipSuccess) return GLASMA_ERR_NOMEM;
+    const hipError_t ae = hipMalloc(&dev, bytes);
+    if (ae != hipSuccess) return hip_code(ae);
     int rc = 0;
     if (hipMemcpy(dev, data, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = GLASMA_ERR_HIP;
     long inner = batch;
@@ -255,5 +257,6 @@ extern "C" int glasma_fft_host(double* data, int nd, const int* dims, long batch
     }
     if (rc == 0 && hipMemcpy(data, dev, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = GLASMA_ERR_HIP;
     hipFree(dev);
+    if (rc) (void)hipGetLastError();  // leave no stale error for the next launch check
     return rc;
 }

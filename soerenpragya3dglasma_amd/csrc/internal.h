@@ -68,6 +68,10 @@ namespace glasma {
 
 int set_error(glasma_ctx* ctx, int code, const std::string& msg);
 int hip_fail(glasma_ctx* ctx, hipError_t e, const char* what);
+// Status code for a failed HIP call.  It also clears the thread's last-error state: HIP keeps the error of a failed
+// API call (a hipMalloc past the device's memory, say) until hipGetLastError reads it, and the launch checks of the
+// next entry point would otherwise report that stale error as their own.
+int hip_code(hipError_t e);
 int ensure_red(glasma_ctx* ctx, size_t doubles);
 int ensure_fftbuf(glasma_ctx* ctx, size_t doubles);
 int reset_fields(glasma_ctx* ctx);

@@ -73,7 +73,7 @@ int glasma_tune_step(glasma_ctx* ctx, int reps, int* ncand, int* codes, double* 
     hipFree(sE);
     hipFree(sJ);
     if (e != hipSuccess) {
-        const int code = e == hipErrorOutOfMemory ? GLASMA_ERR_NOMEM : GLASMA_ERR_HIP;
+        const int code = hip_code(e);
         return set_error(ctx, code, std::string("glasma_tune_step: ") + hipGetErrorString(e));
     }
     size_t bi = 0;
