@@ -8,10 +8,14 @@
 // LDS (coalesced: along the line when its elements are contiguous, across lines otherwise), runs a
 // self-sorting Stockham transform in LDS (radix 4, 2, 3 kernels, any other prime by a direct DFT), and
 // writes the lines back in place.  Multi-dimensional transforms are one pass per axis, so each pass
-// reads and writes the array once.  Twiddles come from a table exp(-2 pi i j/n) computed once per length
+// reads and writes the array once.  Strided lines (the y and z passes) are loaded 4 at a time where the
+// device's LDS allows (160 KiB on gfx950), so each row of the load is 64 contiguous bytes, and blocks are dealt
+// to the XCDs so that neighbouring line groups run on one XCD and share the rest of each 128-byte line in its
+// L2.  The twiddle table is copied into LDS with the data where it fits.  Twiddles come from a table exp(-2 pi i j/n) computed once per length
 // in long double and rounded to FP64.
 #include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 #include <cmath>
 
@@ -61,27 +65,86 @@ __device__ __forceinline__ dbl2 twid(const dbl2* tw, int k, int sign) {
     return sign < 0 ? w : dbl2{w.x, -w.y};
 }
 
+// a / d for 0 <= a < 2^20 and 1 <= d < 2^20, with rd = 1/d in FP32: the FP32 product is within 1/8 of a / d,
+// so the truncated estimate is off by at most one, and one remainder test corrects it.  The index arithmetic
+// below divides by run-time lengths once per element; this replaces the integer division sequence.
+__device__ __forceinline__ int qdiv(int a, int d, float rd) {
+    int q = (int)((float)a * rd);
+    const int r = a - q * d;
+    if (r < 0) q--;
+    else if (r >= d) q++;
+    return q;
+}
+
+constexpr int kMaxLinesPerBlock = 64;
+// strided lines per block at most.  8 would make each row of a load one 128-byte cache line, but fewer leave
+// room for more workgroups per CU: at 256^2 x 1024 with 8 components the y pass (length 256) takes 7.6 ms with
+// 8 lines (66 KiB of LDS, two workgroups per CU), 5.4 ms with 4 and 5.7 ms with 2; the z pass (length 1024)
+// 14.3 ms with 4 and 16.6 ms with 2.
+#ifndef GLASMA_FFT_STRIDED_LINES
+#define GLASMA_FFT_STRIDED_LINES 4
+#endif
+constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
+static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
+              "strided lines per block: a power of two dividing the block size");
+
+// TwLds: the block copies the twiddle table into LDS with the data, so the stages read it from LDS instead of
+// waiting on the memory hierarchy once per stage
+template <bool TwLds>
 __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ data, Radices R, long es, long nlines,
                                                             long b1, long s1, long s2, int sign, int lpb, int ld,
-                                                            const dbl2* __restrict__ tw) {
-    // two LDS buffers of lpb * ld entries each, sized per launch (at most 2 * kFftLds entries), so that short
-    // lines do not pay for the longest line's LDS and more workgroups fit on a CU
+                                                            long xcd_per, const dbl2* __restrict__ twg) {
+    // two LDS buffers of lpb * ld entries each, sized per launch, so that short lines do not pay for the longest
+    // line's LDS and more workgroups fit on a CU
     extern __shared__ dbl2 lds[];
     dbl2* const buf[2] = {lds, lds + lpb * ld};
     const int n = R.n;
-    const long l0 = (long)blockIdx.x * lpb;
+    // xcd_per > 0: block b runs on XCD b % 8, which takes line groups [k * xcd_per, (k + 1) * xcd_per) in order;
+    // the grid is padded to 8 * xcd_per blocks and the padding blocks leave at once
+    long grp = blockIdx.x;
+    if (xcd_per > 0) grp = (long)(blockIdx.x % 8) * xcd_per + blockIdx.x / 8;
+    const long l0 = grp * lpb;
+    if (l0 >= nlines) return;
     const int nload = lpb * n;
     dbl2* dv = reinterpret_cast<dbl2*>(data);
-    // load
+    // Element idx of the block is position j of line li.  Contiguous lines are read along the line, strided
+    // ones across the lpb lines (a power of two dividing the block size) so that neighbouring threads read
+    // neighbouring addresses; a thread then always meets the same line, li = threadIdx.x & (lpb - 1), and so
+    // does every thread when lpb = 1.  The first element of that line is found once, in a register; contiguous
+    // lines with lpb > 1 keep a table of line starts after the two buffers.  The 64-bit divisions stay out of
+    // the per-element loops; -1 marks lines past the end.
+    // LDS: two buffers, then the twiddles (TwLds), then the line table
+    dbl2* const twl = lds + 2 * lpb * ld;
+    if (TwLds)
+        for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = twg[k];
+    const dbl2* const tw = TwLds ? twl : twg;
+    const bool table = es == 1 && lpb > 1;
+    long* const ltab = reinterpret_cast<long*>(twl + (TwLds ? n : 0));
+    long tbase = -1;
+    if (table) {
+        for (int li = threadIdx.x; li < lpb; li += blockDim.x) {
+            const long line = l0 + li;
+            ltab[li] = line < nlines ? (line % b1) * s1 + (line / b1) * s2 : -1;
+        }
+        __syncthreads();
+    } else {
+        const long line = l0 + (threadIdx.x & (lpb - 1));
+        tbase = line < nlines ? (line % b1) * s1 + (line / b1) * s2 : -1;
+    }
+    const int lg_lpb = __builtin_ctz(lpb);
+    const float rn = 1.0f / (float)n;
     for (int idx = threadIdx.x; idx < nload; idx += blockDim.x) {
         int li, j;
-        if (es == 1) { li = idx / n; j = idx - li * n; }
-        else { j = idx / lpb; li = idx - j * lpb; }
-        long line = l0 + li;
-        if (line < nlines) {
-            long base = (line % b1) * s1 + (line / b1) * s2;
-            buf[0][li * ld + j] = dv[base + (long)j * es];
+        long base = tbase;
+        if (table) {
+            li = qdiv(idx, n, rn);
+            j = idx - li * n;
+            base = ltab[li];
+        } else {
+            j = idx >> lg_lpb;
+            li = idx & (lpb - 1);
         }
+        if (base >= 0) buf[0][li * ld + j] = dv[base + (long)j * es];
     }
     __syncthreads();
     int cur = 0, Ns = 1;
@@ -91,13 +154,15 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
         const int tstride = n / (Ns * r);
         const dbl2* a = buf[cur];
         dbl2* b = buf[cur ^ 1];
+        const float rnr = 1.0f / (float)nr, rns = 1.0f / (float)Ns;
         for (int it = threadIdx.x; it < lpb * nr; it += blockDim.x) {
-            const int li = it / nr;
+            const int li = qdiv(it, nr, rnr);
             const int j = it - li * nr;
-            const int k = j % Ns;
+            const int jq = qdiv(j, Ns, rns);
+            const int k = j - jq * Ns;
             const dbl2* al = a + li * ld;
             dbl2* bl = b + li * ld;
-            const int od = (j / Ns) * Ns * r + k;
+            const int od = jq * Ns * r + k;
             if (r == 4) {
                 dbl2 v0 = al[j], v1 = al[j + nr], v2 = al[j + 2 * nr], v3 = al[j + 3 * nr];
                 if (k) {
@@ -150,13 +215,16 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
     }
     for (int idx = threadIdx.x; idx < nload; idx += blockDim.x) {
         int li, j;
-        if (es == 1) { li = idx / n; j = idx - li * n; }
-        else { j = idx / lpb; li = idx - j * lpb; }
-        long line = l0 + li;
-        if (line < nlines) {
-            long base = (line % b1) * s1 + (line / b1) * s2;
-            dv[base + (long)j * es] = buf[cur][li * ld + j];
+        long base = tbase;
+        if (table) {
+            li = qdiv(idx, n, rn);
+            j = idx - li * n;
+            base = ltab[li];
+        } else {
+            j = idx >> lg_lpb;
+            li = idx & (lpb - 1);
         }
+        if (base >= 0) dv[base + (long)j * es] = buf[cur][li * ld + j];
     }
 }
 
@@ -192,25 +260,91 @@ hipError_t twiddles(int n, const dbl2** out) {
     return hipSuccess;
 }
 
+// dynamic LDS k_fft_lines may use on the current device, found once per device: the device's per-workgroup
+// limit (160 KiB on gfx950) when the runtime grants the kernel that much, else the 64 KiB every CDNA part allows
+std::map<int, size_t> g_lds;
+
+hipError_t lds_limit(size_t* out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto it = g_lds.find(dev);
+    if (it != g_lds.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    int v = 0;
+    e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    size_t allowed = 65536;
+    if (v > 65536) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess)
+            allowed = (size_t)v;
+        else
+            (void)hipGetLastError();  // not granted: stay at 64 KiB, and leave no stale error
+    }
+    g_lds[dev] = allowed;
+    *out = allowed;
+    return hipSuccess;
+}
+
+thread_local std::string g_fft_err;
+
+int fft_fail(int code, const std::string& what) {
+    g_fft_err = what;
+    return code;
+}
+int fft_fail_hip(hipError_t e, const std::string& what) { return fft_fail(hip_code(e), what + ": " + hipGetErrorString(e)); }
+
 }  // namespace
+
+const char* fft_last_error() { return g_fft_err.c_str(); }
 
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
     if (n <= 1 || nlines <= 0) return 0;
-    if (n > kFftLds) return GLASMA_ERR_ARG;
+    if (n > kFftLds) return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " is above 2048");
     Radices R;
-    if (factorize(n, R) != 0) return GLASMA_ERR_ARG;
+    if (factorize(n, R) != 0)
+        return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " has a prime factor above 13");
     const dbl2* tw = nullptr;
     const hipError_t te = twiddles(n, &tw);
-    if (te != hipSuccess) return hip_code(te);
+    if (te != hipSuccess) return fft_fail_hip(te, "twiddle table");
+    size_t limit = 0;
+    const hipError_t le = lds_limit(&limit);
+    if (le != hipSuccess) return fft_fail_hip(le, "LDS limit query");
     int lpb = 1;
-    while (2 * lpb * (n + 1) <= kFftLds && 2 * lpb <= 64) lpb *= 2;
+    if (es == 1) {
+        while (2 * lpb * (n + 1) <= kFftLds && 2 * lpb <= kMaxLinesPerBlock) lpb *= 2;
+    } else {
+        const size_t cap = limit / (2 * sizeof(dbl2));  // entries per buffer
+        while ((size_t)(2 * lpb) * (n + 1) <= cap && 2 * lpb <= kStridedLines) lpb *= 2;
+    }
     const int ld = lpb > 1 ? n + 1 : n;
-    const long blocks = (nlines + lpb - 1) / lpb;
-    if (blocks > 0x7fffffffL) return GLASMA_ERR_ARG;
-    const size_t lds = 2 * sizeof(dbl2) * (size_t)lpb * ld;
-    hipLaunchKernelGGL(k_fft_lines, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es, nlines, b1, s1,
-                       s2, sign, lpb, ld, tw);
-    return hipGetLastError() == hipSuccess ? 0 : GLASMA_ERR_HIP;
+    const long groups = (nlines + lpb - 1) / lpb;
+    long xcd_per = 0, blocks = groups;
+    if (es != 1 && groups >= 64) {
+        xcd_per = (groups + 7) / 8;
+        blocks = 8 * xcd_per;
+    }
+    if (blocks > 0x7fffffffL) return fft_fail(GLASMA_ERR_ARG, "too many lines");
+    const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld + (es == 1 && lpb > 1 ? sizeof(long) * lpb : 0);
+    const bool twlds = bytes + sizeof(dbl2) * n <= limit;
+    const size_t lds = bytes + (twlds ? sizeof(dbl2) * n : 0);
+    if (twlds)
+        hipLaunchKernelGGL(k_fft_lines<true>, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es, nlines,
+                           b1, s1, s2, sign, lpb, ld, xcd_per, tw);
+    else
+        hipLaunchKernelGGL(k_fft_lines<false>, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es, nlines,
+                           b1, s1, s2, sign, lpb, ld, xcd_per, tw);
+    const hipError_t ke = hipGetLastError();
+    if (ke != hipSuccess)
+        return fft_fail_hip(ke, "k_fft_lines launch (length " + std::to_string(n) + ", " + std::to_string(lpb) +
+                                    " lines per block, " + std::to_string(lds) + " bytes of LDS)");
+    return 0;
 }
 
 // c[(comp*nzl + z)*P + y*nx + x]

@@ -109,6 +109,8 @@ int ic_left_vjp(glasma_ctx* ctx, const glasma_mv_params* mv, const IcTape& tape,
                 const double* Jb, double* rho_bar, double* work);
 
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign);
+// why the calling thread's last failed FFT call failed
+const char* fft_last_error();
 // complex array c[(comp*nz + z)*P + y*nx + x] of `ncomp` components: 1D along z, 2D in (x,y), 3D.
 int fft_z(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);
 int fft_xy(hipStream_t st, double* c, const Geom& g, int ncomp, int nzl, int sign);

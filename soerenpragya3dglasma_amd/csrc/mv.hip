@@ -325,8 +325,8 @@ void free_work(Wk& w) {
 #define GL_FFT(ctx, x)                                                                       \
     do {                                                                                     \
         int _rc = (x);                                                                       \
-        if (_rc) return set_error(ctx, _rc, "FFT failed (lengths must be 2^a 3^b p <= 2048, " \
-                                            "prime factors <= 13)");                         \
+        if (_rc) return set_error(ctx, _rc, std::string("FFT failed (lengths must be 2^a 3^b p <= 2048, " \
+                                                        "prime factors <= 13): ") + fft_last_error()); \
     } while (0)
 
 // Poisson solve of every transverse slice: F (real parts) -> A, imaginary parts of the result dropped
@@ -707,7 +707,10 @@ int tmd_wilson_lines(glasma_ctx* ctx, Wk& w, const glasma_tmd_params* p, const d
                                p->com_energy, p->a_fm[2]);
             rc = fft_3d(ctx->stream, w.F, g, 8, +1);
         }
-        if (rc) rc = set_error(ctx, rc, "FFT failed (lengths must be 2^a 3^b p <= 2048, prime factors <= 13)");
+        if (rc)
+            rc = set_error(ctx, rc,
+                           std::string("FFT failed (lengths must be 2^a 3^b p <= 2048, prime factors <= 13): ") +
+                               fft_last_error());
     }
     if (!rc) {
         TmdCharge C;
