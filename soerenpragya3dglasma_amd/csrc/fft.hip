@@ -25,7 +25,15 @@ namespace glasma {
 
 namespace {
 
-constexpr int kFftLds = 2048;     // complex entries per LDS buffer (two buffers: 64 KiB)
+constexpr int kFftLds = 2048;  // longest line
+// complex entries per LDS buffer for contiguous lines (several short lines per workgroup share it; one longer
+// line takes what it needs).  Smaller buffers leave room for more workgroups per CU: the x pass of length 256
+// at 256^2 x 1024 with 8 components takes 5.3-5.8 ms with 1024 entries (2 lines), 5.8-6.2 ms with 2048 (4) and
+// 8.1-8.4 ms with 4096 (8).
+#ifndef GLASMA_FFT_CONTIG_ENTRIES
+#define GLASMA_FFT_CONTIG_ENTRIES 1024
+#endif
+constexpr int kContigEntries = GLASMA_FFT_CONTIG_ENTRIES;
 constexpr int kFftThreads = 256;
 constexpr int kMaxStages = 24;
 constexpr int kMaxGenericRadix = 16;
@@ -318,7 +326,7 @@ int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1
     if (le != hipSuccess) return fft_fail_hip(le, "LDS limit query");
     int lpb = 1;
     if (es == 1) {
-        while (2 * lpb * (n + 1) <= kFftLds && 2 * lpb <= kMaxLinesPerBlock) lpb *= 2;
+        while (2 * lpb * (n + 1) <= kContigEntries && 2 * lpb <= kMaxLinesPerBlock) lpb *= 2;
     } else {
         const size_t cap = limit / (2 * sizeof(dbl2));  // entries per buffer
         while ((size_t)(2 * lpb) * (n + 1) <= cap && 2 * lpb <= kStridedLines) lpb *= 2;

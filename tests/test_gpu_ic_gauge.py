@@ -33,8 +33,13 @@ def _mv(Nz, sigma=2.0, lam=1.65, meff=0.1, mult_l=1.0, mult_r=1.0):
 # ---------------------------------------------------------------------------------------------------
 # FFT (replaces SC/FFT/FFT1D.cpp, FFT2D.cpp, FFT3D.cpp)
 # ---------------------------------------------------------------------------------------------------
+# Strided axes run 4 lines per workgroup, and with 64 or more line groups the groups are dealt to the XCDs
+# (grid padded to a multiple of 8).  (4, 9, 65): the length-9 axis has 65 groups, so 7 padding blocks, and the
+# 65 = 5 * 13 axis takes the generic radices; (2048,) x 2: strided length 2048, whose twiddle table no longer fits
+# in LDS beside the two line buffers (read from memory instead); (1024,) x 3: fewer lines than one group.
 @pytest.mark.parametrize("dims,batch", [((8,), 1), ((12,), 2), ((6, 10), 3), ((4, 6, 5), 8), ((32, 48, 7), 1),
-                                        ((1024,), 1), ((384,), 2), ((11, 13), 1), ((2048,), 1)])
+                                        ((1024,), 1), ((384,), 2), ((11, 13), 1), ((2048,), 1), ((4, 9, 65), 1),
+                                        ((2048,), 2), ((1024,), 3), ((64, 96, 20), 3)])
 def test_fft_host_matches_numpy(dims, batch):
     from soerenpragya3dglasma_amd import fft_host
     rng = np.random.default_rng(len(dims) * 100 + batch)
