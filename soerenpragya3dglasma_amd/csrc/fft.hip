@@ -13,6 +13,7 @@
 // to the XCDs so that neighbouring line groups run on one XCD and share the rest of each 128-byte line in its
 // L2.  The twiddle table is copied into LDS with the data where it fits.  Twiddles come from a table exp(-2 pi i j/n) computed once per length
 // in long double and rounded to FP64.
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -96,65 +97,11 @@ constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
 static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
               "strided lines per block: a power of two dividing the block size");
 
-// TwLds: the block copies the twiddle table into LDS with the data, so the stages read it from LDS instead of
-// waiting on the memory hierarchy once per stage
-template <bool TwLds>
-__global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ data, Radices R, long es, long nlines,
-                                                            long b1, long s1, long s2, int sign, int lpb, int ld,
-                                                            long xcd_per, const dbl2* __restrict__ twg) {
-    // two LDS buffers of lpb * ld entries each, sized per launch, so that short lines do not pay for the longest
-    // line's LDS and more workgroups fit on a CU
-    extern __shared__ dbl2 lds[];
-    dbl2* const buf[2] = {lds, lds + lpb * ld};
+// The Stockham stages over the lpb lines in buf[0] (line li at buf[0] + li * ld); returns which buffer holds the
+// transform.  Every stage ends with a barrier.
+__device__ __forceinline__ int stockham(dbl2* const buf[2], const Radices& R, int lpb, int ld, int sign,
+                                        const dbl2* tw) {
     const int n = R.n;
-    // xcd_per > 0: block b runs on XCD b % 8, which takes line groups [k * xcd_per, (k + 1) * xcd_per) in order;
-    // the grid is padded to 8 * xcd_per blocks and the padding blocks leave at once
-    long grp = blockIdx.x;
-    if (xcd_per > 0) grp = (long)(blockIdx.x % 8) * xcd_per + blockIdx.x / 8;
-    const long l0 = grp * lpb;
-    if (l0 >= nlines) return;
-    const int nload = lpb * n;
-    dbl2* dv = reinterpret_cast<dbl2*>(data);
-    // Element idx of the block is position j of line li.  Contiguous lines are read along the line, strided
-    // ones across the lpb lines (a power of two dividing the block size) so that neighbouring threads read
-    // neighbouring addresses; a thread then always meets the same line, li = threadIdx.x & (lpb - 1), and so
-    // does every thread when lpb = 1.  The first element of that line is found once, in a register; contiguous
-    // lines with lpb > 1 keep a table of line starts after the two buffers.  The 64-bit divisions stay out of
-    // the per-element loops; -1 marks lines past the end.
-    // LDS: two buffers, then the twiddles (TwLds), then the line table
-    dbl2* const twl = lds + 2 * lpb * ld;
-    if (TwLds)
-        for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = twg[k];
-    const dbl2* const tw = TwLds ? twl : twg;
-    const bool table = es == 1 && lpb > 1;
-    long* const ltab = reinterpret_cast<long*>(twl + (TwLds ? n : 0));
-    long tbase = -1;
-    if (table) {
-        for (int li = threadIdx.x; li < lpb; li += blockDim.x) {
-            const long line = l0 + li;
-            ltab[li] = line < nlines ? (line % b1) * s1 + (line / b1) * s2 : -1;
-        }
-        __syncthreads();
-    } else {
-        const long line = l0 + (threadIdx.x & (lpb - 1));
-        tbase = line < nlines ? (line % b1) * s1 + (line / b1) * s2 : -1;
-    }
-    const int lg_lpb = __builtin_ctz(lpb);
-    const float rn = 1.0f / (float)n;
-    for (int idx = threadIdx.x; idx < nload; idx += blockDim.x) {
-        int li, j;
-        long base = tbase;
-        if (table) {
-            li = qdiv(idx, n, rn);
-            j = idx - li * n;
-            base = ltab[li];
-        } else {
-            j = idx >> lg_lpb;
-            li = idx & (lpb - 1);
-        }
-        if (base >= 0) buf[0][li * ld + j] = dv[base + (long)j * es];
-    }
-    __syncthreads();
     int cur = 0, Ns = 1;
     for (int st = 0; st < R.nst; st++) {
         const int r = R.r[st];
@@ -221,18 +168,111 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
         cur ^= 1;
         Ns *= r;
     }
-    for (int idx = threadIdx.x; idx < nload; idx += blockDim.x) {
-        int li, j;
-        long base = tbase;
-        if (table) {
-            li = qdiv(idx, n, rn);
-            j = idx - li * n;
-            base = ltab[li];
-        } else {
-            j = idx >> lg_lpb;
-            li = idx & (lpb - 1);
+    return cur;
+}
+
+// One launch transforms `nlines` lines of length n in groups of lpb lines.
+// - Packed (es = 1): line l starts at l * n, so a group is one run of lpb * n elements.
+// - Strided: line l starts at (l % b1) * s1 + (l / b1) * s2 and steps by es; a group's lines are read across
+//   (lpb divides the block size, so thread t always meets line t & (lpb - 1), whose start it finds once per
+//   group: the 64-bit divisions stay out of the per-element loops).
+// Each block walks several groups and fetches the next group into registers (MAXE elements per thread) while
+// it transforms and stores the current one, so a CU keeps loads in flight through the LDS stages.  No block
+// waits for another.  TwLds: the twiddle table is copied into LDS once per block.
+template <bool TwLds, int MAXE>
+__global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ data, Radices R, long es, long nlines,
+                                                            long b1, long s1, long s2, int sign, int lpb, int ld,
+                                                            long groups, long xcd_per, const dbl2* __restrict__ twg) {
+    // LDS, sized per launch: two buffers of lpb * ld entries, then the twiddles (TwLds)
+    extern __shared__ dbl2 lds[];
+    dbl2* const buf[2] = {lds, lds + lpb * ld};
+    dbl2* const twl = lds + 2 * lpb * ld;
+    const int n = R.n;
+    // the groups of this block: g, g + gstep, ... below gend.  xcd_per > 0: block b runs on XCD b % 8, which owns
+    // groups [k * xcd_per, (k + 1) * xcd_per); its blocks walk them side by side, so neighbouring groups (which
+    // share cache lines when the lines are strided) are in flight on one XCD together
+    long g, gstep, gend;
+    if (xcd_per > 0) {
+        const long k = blockIdx.x % 8;
+        g = k * xcd_per + blockIdx.x / 8;
+        gstep = gridDim.x / 8;
+        gend = min((k + 1) * xcd_per, groups);
+    } else {
+        g = blockIdx.x;
+        gstep = gridDim.x;
+        gend = groups;
+    }
+    if (g >= gend) return;
+    if (TwLds)
+        for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = twg[k];
+    const dbl2* const tw = TwLds ? twl : twg;
+    dbl2* const dv = reinterpret_cast<dbl2*>(data);
+    const bool packed = es == 1;
+    const int nload = lpb * n;
+    const int lg_lpb = __builtin_ctz(lpb);
+    const int myli = threadIdx.x & (lpb - 1);
+    const float rn = 1.0f / (float)n;
+    // packed: start of group gg; strided: start of this thread's line in group gg, -1 past the last line
+    auto start = [&](long gg) -> long {
+        if (packed) return gg * lpb * (long)n;
+        const long line = gg * lpb + myli;
+        return line < nlines ? (line % b1) * s1 + (line / b1) * s2 : -1;
+    };
+    // packed: number of elements of group gg that exist (the last group may be short)
+    auto valid = [&](long gg) -> long { return packed ? (nlines - gg * lpb) * (long)n : 0; };
+    dbl2 pre[MAXE];  // pre[m] = element threadIdx.x + m * kFftThreads of the group
+    auto fetch = [&](long gg) {
+        const long base = start(gg), lim = valid(gg);
+#pragma unroll
+        for (int m = 0; m < MAXE; m++) {
+            const int idx = threadIdx.x + m * kFftThreads;
+            dbl2 v = {0.0, 0.0};
+            if (idx < nload) {
+                if (packed) {
+                    if (idx < lim) v = dv[base + idx];
+                } else if (base >= 0) {
+                    v = dv[base + (long)(idx >> lg_lpb) * es];
+                }
+            }
+            pre[m] = v;
         }
-        if (base >= 0) dv[base + (long)j * es] = buf[cur][li * ld + j];
+    };
+    fetch(g);
+    while (true) {
+#pragma unroll
+        for (int m = 0; m < MAXE; m++) {
+            const int idx = threadIdx.x + m * kFftThreads;
+            if (idx < nload) {
+                int li = myli, j = idx >> lg_lpb;
+                if (packed) {
+                    li = qdiv(idx, n, rn);
+                    j = idx - li * n;
+                }
+                buf[0][li * ld + j] = pre[m];
+            }
+        }
+        __syncthreads();
+        const long gn = g + gstep;
+        const bool more = gn < gend;
+        if (more) fetch(gn);  // in flight during the stages and the stores below
+        const int cur = stockham(buf, R, lpb, ld, sign, tw);
+        const long base = start(g), lim = valid(g);
+#pragma unroll
+        for (int m = 0; m < MAXE; m++) {
+            const int idx = threadIdx.x + m * kFftThreads;
+            if (idx < nload) {
+                if (packed) {
+                    const int li = qdiv(idx, n, rn);
+                    if (idx < lim) dv[base + idx] = buf[cur][li * ld + idx - li * n];
+                } else if (base >= 0) {
+                    const int j = idx >> lg_lpb;
+                    dv[base + (long)j * es] = buf[cur][myli * ld + j];
+                }
+            }
+        }
+        if (!more) break;
+        __syncthreads();  // every read of this group's result precedes the next group's writes to buf[0]
+        g = gn;
     }
 }
 
@@ -268,36 +308,70 @@ hipError_t twiddles(int n, const dbl2** out) {
     return hipSuccess;
 }
 
-// dynamic LDS k_fft_lines may use on the current device, found once per device: the device's per-workgroup
-// limit (160 KiB on gfx950) when the runtime grants the kernel that much, else the 64 KiB every CDNA part allows
-std::map<int, size_t> g_lds;
+// per device, found once: the dynamic LDS k_fft_lines may use (the device's per-workgroup limit, 160 KiB on
+// gfx950, when the runtime grants the kernel that much, else the 64 KiB every CDNA part allows) and the CU count
+struct FftDevice {
+    size_t lds;
+    int cus;
+};
+std::map<int, FftDevice> g_fft_dev;
 
-hipError_t lds_limit(size_t* out) {
+template <bool TW, int MAXE>
+hipError_t allow_lds(int v) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, v);
+}
+
+hipError_t fft_device(FftDevice* out) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     std::lock_guard<std::mutex> lk(g_tw_mu);
-    auto it = g_lds.find(dev);
-    if (it != g_lds.end()) {
+    auto it = g_fft_dev.find(dev);
+    if (it != g_fft_dev.end()) {
         *out = it->second;
         return hipSuccess;
     }
-    int v = 0;
+    int v = 0, cus = 0;
     e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    size_t allowed = 65536;
+    FftDevice d = {65536, cus > 0 ? cus : 1};
     if (v > 65536) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess &&
-            hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<false>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, v) == hipSuccess)
-            allowed = (size_t)v;
+        if (allow_lds<true, 2>(v) == hipSuccess && allow_lds<true, 4>(v) == hipSuccess &&
+            allow_lds<true, 8>(v) == hipSuccess && allow_lds<true, 16>(v) == hipSuccess &&
+            allow_lds<false, 2>(v) == hipSuccess && allow_lds<false, 4>(v) == hipSuccess &&
+            allow_lds<false, 8>(v) == hipSuccess && allow_lds<false, 16>(v) == hipSuccess)
+            d.lds = (size_t)v;
         else
             (void)hipGetLastError();  // not granted: stay at 64 KiB, and leave no stale error
     }
-    g_lds[dev] = allowed;
-    *out = allowed;
+    g_fft_dev[dev] = d;
+    *out = d;
     return hipSuccess;
+}
+
+// one launch of k_fft_lines<TW, MAXE>.  Where the LDS admits at most two workgroups per CU (long strided lines),
+// the grid is what fits on the device at once and each block walks several groups with the register prefetch:
+// the z pass of length 1024 at 256^2 x 1024 x 8 colours fell from 14.3 to 11.2 ms.  Elsewhere each block takes
+// one group: many resident workgroups already overlap loads with LDS work, and a walking block waits for its own
+// stores before it writes the next group to LDS (the loads are conditional, so the wait counts every memory
+// operation), which made the x and y passes 0.5-0.8 ms slower.  With XCD groups the grid is a multiple of 8.
+template <bool TW, int MAXE>
+hipError_t launch_lines(hipStream_t st, size_t lds, int cus, double* data, const Radices& R, long es, long nlines,
+                        long b1, long s1, long s2, int sign, int lpb, int ld, long groups, long xcd_per,
+                        const dbl2* tw) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE>), kFftThreads, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const long resident = per_cu <= 2 ? (long)cus * per_cu : groups;
+    const long blocks = xcd_per > 0 ? 8 * std::min(xcd_per, (resident + 7) / 8) : std::min(groups, resident);
+    if (blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((k_fft_lines<TW, MAXE>), dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es,
+                       nlines, b1, s1, s2, sign, lpb, ld, groups, xcd_per, tw);
+    return hipGetLastError();
 }
 
 thread_local std::string g_fft_err;
@@ -321,34 +395,42 @@ int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1
     const dbl2* tw = nullptr;
     const hipError_t te = twiddles(n, &tw);
     if (te != hipSuccess) return fft_fail_hip(te, "twiddle table");
-    size_t limit = 0;
-    const hipError_t le = lds_limit(&limit);
-    if (le != hipSuccess) return fft_fail_hip(le, "LDS limit query");
+    // contiguous lines are transformed as one packed run: line l must start at l * n
+    if (es == 1 && !((s1 == n && (b1 >= nlines || s2 == b1 * n)) || (b1 == 1 && s2 == n)))
+        return fft_fail(GLASMA_ERR_ARG, "contiguous lines must lie end to end");
+    FftDevice dev;
+    const hipError_t de = fft_device(&dev);
+    if (de != hipSuccess) return fft_fail_hip(de, "device query");
+    // lines per group; a group is at most 16 elements per thread (the register prefetch)
     int lpb = 1;
     if (es == 1) {
         while (2 * lpb * (n + 1) <= kContigEntries && 2 * lpb <= kMaxLinesPerBlock) lpb *= 2;
     } else {
-        const size_t cap = limit / (2 * sizeof(dbl2));  // entries per buffer
-        while ((size_t)(2 * lpb) * (n + 1) <= cap && 2 * lpb <= kStridedLines) lpb *= 2;
+        const size_t cap = dev.lds / (2 * sizeof(dbl2));  // entries per buffer
+        while ((size_t)(2 * lpb) * (n + 1) <= cap && 2 * lpb <= kStridedLines && 2 * lpb * n <= 16 * kFftThreads)
+            lpb *= 2;
     }
     const int ld = lpb > 1 ? n + 1 : n;
     const long groups = (nlines + lpb - 1) / lpb;
-    long xcd_per = 0, blocks = groups;
-    if (es != 1 && groups >= 64) {
-        xcd_per = (groups + 7) / 8;
-        blocks = 8 * xcd_per;
-    }
-    if (blocks > 0x7fffffffL) return fft_fail(GLASMA_ERR_ARG, "too many lines");
-    const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld + (es == 1 && lpb > 1 ? sizeof(long) * lpb : 0);
-    const bool twlds = bytes + sizeof(dbl2) * n <= limit;
+    const long xcd_per = es != 1 && groups >= 64 ? (groups + 7) / 8 : 0;
+    const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld;
+    if (bytes > dev.lds) return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " exceeds the LDS");
+    const bool twlds = bytes + sizeof(dbl2) * n <= dev.lds;
     const size_t lds = bytes + (twlds ? sizeof(dbl2) * n : 0);
-    if (twlds)
-        hipLaunchKernelGGL(k_fft_lines<true>, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es, nlines,
-                           b1, s1, s2, sign, lpb, ld, xcd_per, tw);
-    else
-        hipLaunchKernelGGL(k_fft_lines<false>, dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es, nlines,
-                           b1, s1, s2, sign, lpb, ld, xcd_per, tw);
-    const hipError_t ke = hipGetLastError();
+    const int need = (lpb * n + kFftThreads - 1) / kFftThreads;  // elements per thread
+    hipError_t ke;
+#define GLASMA_FFT_LAUNCH(TW, M)                                                                                  \
+    ke = launch_lines<TW, M>(st, lds, dev.cus, data, R, es, nlines, b1, s1, s2, sign, lpb, ld, groups, xcd_per, tw)
+    if (need <= 2) {
+        if (twlds) GLASMA_FFT_LAUNCH(true, 2); else GLASMA_FFT_LAUNCH(false, 2);
+    } else if (need <= 4) {
+        if (twlds) GLASMA_FFT_LAUNCH(true, 4); else GLASMA_FFT_LAUNCH(false, 4);
+    } else if (need <= 8) {
+        if (twlds) GLASMA_FFT_LAUNCH(true, 8); else GLASMA_FFT_LAUNCH(false, 8);
+    } else {
+        if (twlds) GLASMA_FFT_LAUNCH(true, 16); else GLASMA_FFT_LAUNCH(false, 16);
+    }
+#undef GLASMA_FFT_LAUNCH
     if (ke != hipSuccess)
         return fft_fail_hip(ke, "k_fft_lines launch (length " + std::to_string(n) + ", " + std::to_string(lpb) +
                                     " lines per block, " + std::to_string(lds) + " bytes of LDS)");

@@ -57,6 +57,21 @@ def test_fft_host_matches_numpy(dims, batch):
     assert np.abs(O.fft_nd(x, dims, batch, -1) - fwd).max() < 1e-12 * np.abs(ref_f).max()
 
 
+@pytest.mark.parametrize("dims,batch", [((16, 1024), 160), ((8, 2048), 96)])
+def test_fft_host_long_strided_lines(dims, batch):
+    """Long strided lines leave room for one workgroup per CU, so each block walks several line groups with the
+    register prefetch: 640 groups of 4 lines of 1024 (twiddles in LDS), and 384 groups of 2 lines of 2048
+    (twiddles read from memory), over 256 resident blocks, so the walks are uneven.  Checked against numpy."""
+    from soerenpragya3dglasma_amd import fft_host
+    rng = np.random.default_rng(sum(dims) + batch)
+    shape = tuple(reversed(dims)) + (batch,)
+    x = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    axes = tuple(range(len(dims)))
+    for sign, ref in ((-1, np.fft.fftn(x, axes=axes)), (+1, np.fft.ifftn(x, axes=axes) * np.prod(dims))):
+        out = fft_host(x, dims, batch, sign)
+        assert np.abs(out - ref).max() < 1e-12 * np.abs(ref).max(), (dims, sign)
+
+
 def test_fft_rejects_unsupported_length():
     from soerenpragya3dglasma_amd import GlasmaError, fft_host
     with pytest.raises(GlasmaError):
