@@ -97,9 +97,12 @@ constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
 static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
               "strided lines per block: a power of two dividing the block size");
 
-// The Stockham stages over the lpb lines in buf[0] (line li at buf[0] + li * ld); returns which buffer holds the
+// The Stockham stages over the lpb lines in buffer 0 (line li at lb + li * ld); buffer c starts at lb + c * bs.
+// Returns which buffer holds the transform.  The buffers are addressed as offsets from the LDS base, not through
+// a run-time indexed pointer array, so the compiler keeps them in the LDS address space (ds_read / ds_write);
+// a generic pointer would give flat instructions, whose waits also wait for every global load in flight.
 // transform.  Every stage ends with a barrier.
-__device__ __forceinline__ int stockham(dbl2* const buf[2], const Radices& R, int lpb, int ld, int sign,
+__device__ __forceinline__ int stockham(dbl2* const lb, int bs, const Radices& R, int lpb, int ld, int sign,
                                         const dbl2* tw) {
     const int n = R.n;
     int cur = 0, Ns = 1;
@@ -107,8 +110,8 @@ __device__ __forceinline__ int stockham(dbl2* const buf[2], const Radices& R, in
         const int r = R.r[st];
         const int nr = n / r;
         const int tstride = n / (Ns * r);
-        const dbl2* a = buf[cur];
-        dbl2* b = buf[cur ^ 1];
+        const dbl2* a = lb + cur * bs;
+        dbl2* b = lb + (cur ^ 1) * bs;
         const float rnr = 1.0f / (float)nr, rns = 1.0f / (float)Ns;
         for (int it = threadIdx.x; it < lpb * nr; it += blockDim.x) {
             const int li = qdiv(it, nr, rnr);
@@ -152,14 +155,15 @@ __device__ __forceinline__ int stockham(dbl2* const buf[2], const Radices& R, in
                 bl[od + Ns] = m + q;
                 bl[od + 2 * Ns] = m - q;
             } else {
-                dbl2 v[kMaxGenericRadix];
-                for (int q = 0; q < r; q++) {
-                    v[q] = al[j + q * nr];
-                    if (k && q) v[q] = cmul(v[q], twid(tw, (k * q * tstride) % n, sign));
-                }
+                // a direct DFT of prime length r <= 13; the inputs are re-read from LDS for each output rather than
+                // held in a run-time indexed array (which would live in scratch memory)
                 for (int p = 0; p < r; p++) {
-                    dbl2 s = v[0];
-                    for (int q = 1; q < r; q++) s += cmul(v[q], twid(tw, ((p * q) % r) * nr, sign));
+                    dbl2 s = al[j];
+                    for (int q = 1; q < r; q++) {
+                        dbl2 v = al[j + q * nr];
+                        if (k) v = cmul(v, twid(tw, (k * q * tstride) % n, sign));
+                        s += cmul(v, twid(tw, ((p * q) % r) * nr, sign));
+                    }
                     bl[od + p * Ns] = s;
                 }
             }
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
                                                             long groups, long xcd_per, const dbl2* __restrict__ twg) {
     // LDS, sized per launch: two buffers of lpb * ld entries, then the twiddles (TwLds)
     extern __shared__ dbl2 lds[];
-    dbl2* const buf[2] = {lds, lds + lpb * ld};
+    const int bs = lpb * ld;  // buffer c starts at lds + c * bs
     dbl2* const twl = lds + 2 * lpb * ld;
     const int n = R.n;
     // the groups of this block: g, g + gstep, ... below gend.  xcd_per > 0: block b runs on XCD b % 8, which owns
@@ -206,6 +210,10 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
     if (TwLds)
         for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = twg[k];
     const dbl2* const tw = TwLds ? twl : twg;
+    // the first entry of buffer 1 takes the writes of elements past the group: nothing reads buffer 1 between
+    // those writes and the first stage, which overwrites it, and the previous group's reads of it end at the
+    // barrier that closes each iteration
+    dbl2* const spare = lds + bs;
     dbl2* const dv = reinterpret_cast<dbl2*>(data);
     const bool packed = es == 1;
     const int nload = lpb * n;
@@ -220,42 +228,49 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
     };
     // packed: number of elements of group gg that exist (the last group may be short)
     auto valid = [&](long gg) -> long { return packed ? (nlines - gg * lpb) * (long)n : 0; };
-    dbl2 pre[MAXE];  // pre[m] = element threadIdx.x + m * kFftThreads of the group
+    // pre[m] = element threadIdx.x + m * kFftThreads of the group, if bit m of ok is set.  The loads are
+    // unconditional (an element that does not exist reads the group's first element, or element 0) and their
+    // values are not touched until the LDS writes, so the compiler can count them: those writes wait for these
+    // loads only, and the stages and stores run while they are in flight.
+    dbl2 pre[MAXE];
+    unsigned ok = 0;
     auto fetch = [&](long gg) {
         const long base = start(gg), lim = valid(gg);
+        ok = 0;
 #pragma unroll
         for (int m = 0; m < MAXE; m++) {
             const int idx = threadIdx.x + m * kFftThreads;
-            dbl2 v = {0.0, 0.0};
-            if (idx < nload) {
-                if (packed) {
-                    if (idx < lim) v = dv[base + idx];
-                } else if (base >= 0) {
-                    v = dv[base + (long)(idx >> lg_lpb) * es];
-                }
+            bool v;
+            long at;
+            if (packed) {
+                v = idx < nload && idx < lim;
+                at = base + (v ? idx : 0);
+            } else {
+                v = idx < nload && base >= 0;
+                at = v ? base + (long)(idx >> lg_lpb) * es : 0;
             }
-            pre[m] = v;
+            pre[m] = dv[at];
+            ok |= (unsigned)v << m;
         }
     };
     fetch(g);
     while (true) {
+        // branch-free, like the loads: an element past the group goes to the spare slot
 #pragma unroll
         for (int m = 0; m < MAXE; m++) {
             const int idx = threadIdx.x + m * kFftThreads;
-            if (idx < nload) {
-                int li = myli, j = idx >> lg_lpb;
-                if (packed) {
-                    li = qdiv(idx, n, rn);
-                    j = idx - li * n;
-                }
-                buf[0][li * ld + j] = pre[m];
-            }
+            const int lq = qdiv(idx, n, rn);
+            const int li = packed ? lq : myli;
+            const int j = packed ? idx - lq * n : idx >> lg_lpb;
+            dbl2* const at = idx < nload ? lds + li * ld + j : spare;
+            *at = (ok >> m) & 1u ? pre[m] : dbl2{0.0, 0.0};
         }
         __syncthreads();
         const long gn = g + gstep;
         const bool more = gn < gend;
         if (more) fetch(gn);  // in flight during the stages and the stores below
-        const int cur = stockham(buf, R, lpb, ld, sign, tw);
+        const int cur = stockham(lds, bs, R, lpb, ld, sign, tw);
+        const dbl2* const res = lds + cur * bs;
         const long base = start(g), lim = valid(g);
 #pragma unroll
         for (int m = 0; m < MAXE; m++) {
@@ -263,15 +278,15 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
             if (idx < nload) {
                 if (packed) {
                     const int li = qdiv(idx, n, rn);
-                    if (idx < lim) dv[base + idx] = buf[cur][li * ld + idx - li * n];
+                    if (idx < lim) dv[base + idx] = res[li * ld + idx - li * n];
                 } else if (base >= 0) {
                     const int j = idx >> lg_lpb;
-                    dv[base + (long)j * es] = buf[cur][myli * ld + j];
+                    dv[base + (long)j * es] = res[myli * ld + j];
                 }
             }
         }
         if (!more) break;
-        __syncthreads();  // every read of this group's result precedes the next group's writes to buf[0]
+        __syncthreads();  // every read of this group's result precedes the next group's writes to buffer 0
         g = gn;
     }
 }
@@ -351,12 +366,16 @@ hipError_t fft_device(FftDevice* out) {
     return hipSuccess;
 }
 
-// one launch of k_fft_lines<TW, MAXE>.  Where the LDS admits at most two workgroups per CU (long strided lines),
-// the grid is what fits on the device at once and each block walks several groups with the register prefetch:
-// the z pass of length 1024 at 256^2 x 1024 x 8 colours fell from 14.3 to 11.2 ms.  Elsewhere each block takes
-// one group: many resident workgroups already overlap loads with LDS work, and a walking block waits for its own
-// stores before it writes the next group to LDS (the loads are conditional, so the wait counts every memory
-// operation), which made the x and y passes 0.5-0.8 ms slower.  With XCD groups the grid is a multiple of 8.
+// One launch of k_fft_lines<TW, MAXE>.  Where the LDS admits at most GLASMA_FFT_WALK_PER_CU workgroups per CU
+// (long strided lines), the grid is what fits on the device at once and each block walks several groups with the
+// register prefetch: the z pass of length 1024 at 256^2 x 1024 x 8 colours fell from 14.3 to 11.2 ms.  Elsewhere
+// each block takes one group, since many resident workgroups already overlap loads with LDS work, and a walking
+// block still waits for its previous group's stores (they are conditional) before it writes the next group to
+// LDS: walking the y pass of length 256 (four workgroups per CU) took 6.9 ms against 4.2 ms.  With XCD groups
+// the grid is a multiple of 8.
+#ifndef GLASMA_FFT_WALK_PER_CU
+#define GLASMA_FFT_WALK_PER_CU 2
+#endif
 template <bool TW, int MAXE>
 hipError_t launch_lines(hipStream_t st, size_t lds, int cus, double* data, const Radices& R, long es, long nlines,
                         long b1, long s1, long s2, int sign, int lpb, int ld, long groups, long xcd_per,
@@ -366,7 +385,7 @@ hipError_t launch_lines(hipStream_t st, size_t lds, int cus, double* data, const
         &per_cu, reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE>), kFftThreads, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
-    const long resident = per_cu <= 2 ? (long)cus * per_cu : groups;
+    const long resident = per_cu <= GLASMA_FFT_WALK_PER_CU ? (long)cus * per_cu : groups;
     const long blocks = xcd_per > 0 ? 8 * std::min(xcd_per, (resident + 7) / 8) : std::min(groups, resident);
     if (blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((k_fft_lines<TW, MAXE>), dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es,
