@@ -100,12 +100,12 @@ def reference_case(threads: int) -> dict:
             "t00_rel_diff": abs(t00_gpu - t00_cpu) / abs(t00_cpu)}
 
 
-def batched_ensemble(nevents: int, steps: int) -> dict:
+def batched_ensemble(nevents: int, steps: int, lattice=None) -> dict:
     """The ensemble workload: events drawn with master seed 12345 (numpy RandomState), N_perp uniform in
-    {32, 48, 64, 96}, Nz uniform in {128, 192, 256, 384}, g^2 mu uniform in [0.3, 0.8], charge seeds 2k+1
-    (left) and 2k+2 (right), MV collision initial states built on the device; `steps` leapfrog steps of
-    every event, batched (one launch per step over all events, replayed from a HIP graph) against serial
-    per-event launches.  Both runs start from the same initial states."""
+    {32, 48, 64, 96}, Nz uniform in {128, 192, 256, 384} (or all on `lattice`), g^2 mu uniform in [0.3, 0.8],
+    charge seeds 2k+1 (left) and 2k+2 (right), MV collision initial states built on the device; `steps`
+    leapfrog steps of every event, batched (one launch per step over all events, replayed from a HIP graph)
+    against serial per-event launches.  Both runs start from the same initial states."""
     from soerenpragya3dglasma_amd import Glasma, GlasmaBatch
     rng = np.random.RandomState(12345)
     specs, qs = [], []
@@ -113,7 +113,7 @@ def batched_ensemble(nevents: int, steps: int) -> dict:
         nperp = int(rng.choice([32, 48, 64, 96]))
         nz = int(rng.choice([128, 192, 256, 384]))
         qs.append(float(rng.uniform(0.3, 0.8)))
-        specs.append(((nperp, nperp, nz), (1.0, 1.0, 1.0), 0.25))
+        specs.append((tuple(lattice) if lattice else (nperp, nperp, nz), (1.0, 1.0, 1.0), 0.25))
     sites = sum(sp[0][0] * sp[0][1] * sp[0][2] for sp in specs)
     log(f"[bench] ensemble: {nevents} events, {sites} sites in total, {steps} steps")
     out = {"events": nevents, "steps": steps, "sites_total": sites}
@@ -306,6 +306,12 @@ def main():
     batched = None
     if not args.no_batch:
         batched = batched_ensemble(args.batch_events, args.batch_steps)
+        # where batching pays: events too small to fill the device alone, whose serial launches are bound by
+        # launch latency (the reference's own CPU-sized lattice, and one of an eighth of its sites)
+        batched["small_events"] = [
+            {"lattice": list(lat), **{k: v for k, v in batched_ensemble(ne, 200, lat).items()
+                                      if k in ("events", "steps", "batched_s", "serial_s", "batched_over_serial")}}
+            for lat, ne in (((32, 32, 64), 512), ((16, 16, 32), 256))]
 
     adjoint = None
     if not args.no_adjoint:
