@@ -128,10 +128,52 @@ __device__ __forceinline__ void local_w(const double* __restrict__ UG, const Geo
     }
 }
 
-// Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52)
-__global__ __launch_bounds__(kCgBlock) void k_cg_dev(const double* __restrict__ UG, double* __restrict__ F,
-                                                      double* __restrict__ part, Geom g, double cx, double cy,
-                                                      double cz, double fm0) {
+// U^G_mu at (x, y, z) from U and G, the same two products in the same order as k_cg_transform (so the values are
+// identical); the z neighbour of the top layer is Gtop
+__device__ __forceinline__ void ug_link(const double* __restrict__ U, const double* __restrict__ G,
+                                        const double* __restrict__ Gtop, const Geom& g, int mu, int x, int y, int z,
+                                        M3& R) {
+    const long s = sidx(g, x, y, z);
+    M3 Gx, Um, Gn, T;
+    ldm<long>(G, kGPlanes, 0, s, Gx);
+    ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+    if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
+    else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
+    else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
+    else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
+    mul_nn(Gx, Um, T);
+    mul_nd(T, Gn, R);
+}
+
+// local_w with the gauge-fixed links formed on the fly from U and G (the Fourier and Jacobi iterations): the
+// iteration then needs no separate link transformation writing U^G and reading it back.
+__device__ __forceinline__ void local_w_fused(const double* __restrict__ U, const double* __restrict__ G,
+                                              const double* __restrict__ Gtop, const Geom& g, int x, int y, int z,
+                                              double cx, double cy, double cz, M3& W) {
+    M3 A, B;
+    const double zf = g.nz > 1 ? 1.0 : 0.0;
+    for (int k = 0; k < 9; k++) { W.re[k] = 0.0; W.im[k] = 0.0; }
+    for (int mu = 0; mu < 3; mu++) {
+        const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
+        ug_link(U, G, Gtop, g, mu, x, y, z, A);
+        if (mu == 0) ug_link(U, G, Gtop, g, 0, x - 1, y, z, B);
+        else if (mu == 1) ug_link(U, G, Gtop, g, 1, x, y - 1, z, B);
+        else if (z > 0) ug_link(U, G, Gtop, g, 2, x, y, z - 1, B);
+        else set_unit(B);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                W.re[mi(i, j)] += c * (A.re[mi(i, j)] + B.re[mi(j, i)]);
+                W.im[mi(i, j)] += c * (A.im[mi(i, j)] - B.im[mi(j, i)]);
+            }
+    }
+}
+
+// Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52), with the monitors,
+// from the ungauged links and G
+__global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restrict__ U, const double* __restrict__ G,
+                                                            const double* __restrict__ Gtop, double* __restrict__ F,
+                                                            double* __restrict__ part, Geom g, double cx, double cy,
+                                                            double cz, double fm0) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0;
     if (s < g.S) {
@@ -139,7 +181,7 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev(const double* __restrict__ 
         const long s2 = s - (long)z * g.P;
         const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
         M3 W;
-        local_w(UG, g, s, x, y, z, cx, cy, cz, W);
+        local_w_fused(U, G, Gtop, g, x, y, z, cx, cy, cz, W);
         double dv[8];
         retr_igen(W, dv);
         for (int a = 0; a < 8; a++) {
@@ -423,7 +465,11 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     double* spart = ctx->red + 8 * nblk;    // Los Alamos step monitors
     double* fin = ctx->red + 16 * nblk;
     double* F = ctx->fftbuf;
-    if ((rc = transform_links(ctx))) return rc;
+    // Los Alamos sweeps read U^G; the Fourier and Jacobi iterations form it on the fly (k_cg_dev_fused) and keep
+    // the reference's lag of the top ghost layer by synchronising Gtop right after each deviation kernel, where
+    // the reference's link transformation at the end of the previous iteration would have
+    const bool fused = algo != GLASMA_GAUGE_LOS_ALAMOS;
+    if (!fused && (rc = transform_links(ctx))) return rc;
     double gmax = 1.0, gavg = 0.0, smax = 0.0, savg = 0.0, fmin = 0.0, fsum = 0.0;
     int n = 0;
     const double V = (double)g.S;
@@ -438,8 +484,9 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, 2 * nblk, fin, fm0);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, spart, 2 * nblk, fin + 4, 0.0);
         } else {
-            hipLaunchKernelGGL(k_cg_dev, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->UG, F, part, g, cx, cy,
-                               cz, fm0);
+            hipLaunchKernelGGL(k_cg_dev_fused, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur],
+                               ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0);
+            hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
             if (algo == GLASMA_GAUGE_FOURIER) {
                 if ((rc = fft_3d(ctx->stream, F, g, 8, -1))) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
@@ -449,7 +496,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g,
                                param);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
-            if ((rc = transform_links(ctx))) return rc;
+            GL_HIP(ctx, hipGetLastError());
         }
         GL_HIP(ctx, hipMemcpyAsync(h, fin, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -461,6 +508,9 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
         savg = std::sqrt(h[4] / (V * 8.0));
         n++;
     }
+    // the gauge-fixed links, transformed with the final G and (as after the reference's last transformation) the
+    // top ghost layer synchronised to it
+    if (fused) hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
     if ((rc = transform_links(ctx))) return rc;
     hipLaunchKernelGGL(k_cg_efield, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->E, ctx->EG, g);
     GL_HIP(ctx, hipGetLastError());
