@@ -97,15 +97,15 @@ constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
 static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
               "strided lines per block: a power of two dividing the block size");
 
-// The Stockham stages over the lpb lines in buffer 0 (line li at lb + li * ld); buffer c starts at lb + c * bs.
-// Returns which buffer holds the transform.  The buffers are addressed as offsets from the LDS base, not through
+// The Stockham stages over the lpb lines in buffer `cur` (line li at lb + cur * bs + li * ld); buffer c starts at
+// lb + c * bs.  Returns which buffer holds the transform.  The buffers are addressed as offsets from the LDS base, not through
 // a run-time indexed pointer array, so the compiler keeps them in the LDS address space (ds_read / ds_write);
 // a generic pointer would give flat instructions, whose waits also wait for every global load in flight.
 // transform.  Every stage ends with a barrier.
-__device__ __forceinline__ int stockham(dbl2* const lb, int bs, const Radices& R, int lpb, int ld, int sign,
-                                        const dbl2* tw) {
+__device__ __forceinline__ int stockham(dbl2* const lb, int bs, int cur, const Radices& R, int lpb, int ld,
+                                        int sign, const dbl2* tw) {
     const int n = R.n;
-    int cur = 0, Ns = 1;
+    int Ns = 1;
     for (int st = 0; st < R.nst; st++) {
         const int r = R.r[st];
         const int nr = n / r;
@@ -175,6 +175,17 @@ __device__ __forceinline__ int stockham(dbl2* const lb, int bs, const Radices& R
     return cur;
 }
 
+// Optional momentum-space filter of a strided z pass (the Poisson step of Fourier-accelerated gauge fixing): the
+// lines are transformed forward, element k_z of the line at transverse site (x, y) is multiplied as
+// v -> nf * v / p^2 when p^2 = sum_mu (2 - 2 cos(2 pi k_mu / N_mu)) / a_mu^2 > 0 (the zero mode passes
+// unscaled), and the lines are transformed back, all in LDS.  The x and y passes of the forward transform run
+// before, those of the inverse after (the passes commute).
+struct LineFilter {
+    int on, nx, ny;
+    long P;  // transverse sites: the line's site is line % P
+    double ax, ay, az, nf;
+};
+
 // One launch transforms `nlines` lines of length n in groups of lpb lines.
 // - Packed (es = 1): line l starts at l * n, so a group is one run of lpb * n elements.
 // - Strided: line l starts at (l % b1) * s1 + (l / b1) * s2 and steps by es; a group's lines are read across
@@ -183,11 +194,13 @@ __device__ __forceinline__ int stockham(dbl2* const lb, int bs, const Radices& R
 // Each block walks several groups and fetches the next group into registers (MAXE elements per thread) while
 // it transforms and stores the current one, so a CU keeps loads in flight through the LDS stages.  No block
 // waits for another.  TwLds: the twiddle table is copied into LDS once per block.
-template <bool TwLds, int MAXE>
+template <bool TwLds, int MAXE, bool Filt>
 __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ data, Radices R, long es, long nlines,
                                                             long b1, long s1, long s2, int sign, int lpb, int ld,
-                                                            long groups, long xcd_per, const dbl2* __restrict__ twg) {
-    // LDS, sized per launch: two buffers of lpb * ld entries, then the twiddles (TwLds)
+                                                            long groups, long xcd_per, LineFilter filt,
+                                                            const dbl2* __restrict__ twg) {
+    // LDS, sized per launch: two buffers of lpb * ld entries, then the twiddles (TwLds), then (Filt) the z term
+    // of p^2 for each k_z
     extern __shared__ dbl2 lds[];
     const int bs = lpb * ld;  // buffer c starts at lds + c * bs
     dbl2* const twl = lds + 2 * lpb * ld;
@@ -210,6 +223,10 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
     if (TwLds)
         for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = twg[k];
     const dbl2* const tw = TwLds ? twl : twg;
+    double* const ztab = reinterpret_cast<double*>(twl + (TwLds ? n : 0));
+    if (Filt)
+        for (int k = threadIdx.x; k < n; k += blockDim.x)
+            ztab[k] = (2.0 - 2.0 * cos(2.0 * kPi * k / (double)n)) / (filt.az * filt.az);
     // the first entry of buffer 1 takes the writes of elements past the group: nothing reads buffer 1 between
     // those writes and the first stage, which overwrites it, and the previous group's reads of it end at the
     // barrier that closes each iteration
@@ -269,7 +286,30 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
         const long gn = g + gstep;
         const bool more = gn < gend;
         if (more) fetch(gn);  // in flight during the stages and the stores below
-        const int cur = stockham(lds, bs, R, lpb, ld, sign, tw);
+        int cur = stockham(lds, bs, 0, R, lpb, ld, sign, tw);
+        if (Filt) {  // strided lines only: this thread's line is myli, its positions idx >> lg_lpb
+            dbl2* const fb = lds + cur * bs;
+            const long t = (g * lpb + myli) % filt.P;
+            const int px = (int)(t % filt.nx), py = (int)(t / filt.nx);
+            const double pxy = (2.0 - 2.0 * cos(2.0 * kPi * px / (double)filt.nx)) / (filt.ax * filt.ax) +
+                               (2.0 - 2.0 * cos(2.0 * kPi * py / (double)filt.ny)) / (filt.ay * filt.ay);
+#pragma unroll
+            for (int m = 0; m < MAXE; m++) {
+                const int idx = threadIdx.x + m * kFftThreads;
+                if (idx < nload) {
+                    const int pz = idx >> lg_lpb;
+                    const double p2 = pxy + ztab[pz];
+                    if (p2 > 0.0) {
+                        dbl2 v = fb[myli * ld + pz];
+                        v.x = filt.nf * v.x / p2;
+                        v.y = filt.nf * v.y / p2;
+                        fb[myli * ld + pz] = v;
+                    }
+                }
+            }
+            __syncthreads();
+            cur = stockham(lds, bs, cur, R, lpb, ld, -sign, tw);
+        }
         const dbl2* const res = lds + cur * bs;
         const long base = start(g), lim = valid(g);
 #pragma unroll
@@ -333,8 +373,12 @@ std::map<int, FftDevice> g_fft_dev;
 
 template <bool TW, int MAXE>
 hipError_t allow_lds(int v) {
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE, false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE, true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    return e;
 }
 
 hipError_t fft_device(FftDevice* out) {
@@ -376,20 +420,20 @@ hipError_t fft_device(FftDevice* out) {
 #ifndef GLASMA_FFT_WALK_PER_CU
 #define GLASMA_FFT_WALK_PER_CU 2
 #endif
-template <bool TW, int MAXE>
+template <bool TW, int MAXE, bool F>
 hipError_t launch_lines(hipStream_t st, size_t lds, int cus, double* data, const Radices& R, long es, long nlines,
                         long b1, long s1, long s2, int sign, int lpb, int ld, long groups, long xcd_per,
-                        const dbl2* tw) {
+                        const LineFilter& filt, const dbl2* tw) {
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE>), kFftThreads, lds);
+        &per_cu, reinterpret_cast<const void*>(&k_fft_lines<TW, MAXE, F>), kFftThreads, lds);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     const long resident = per_cu <= GLASMA_FFT_WALK_PER_CU ? (long)cus * per_cu : groups;
     const long blocks = xcd_per > 0 ? 8 * std::min(xcd_per, (resident + 7) / 8) : std::min(groups, resident);
     if (blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((k_fft_lines<TW, MAXE>), dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es,
-                       nlines, b1, s1, s2, sign, lpb, ld, groups, xcd_per, tw);
+    hipLaunchKernelGGL((k_fft_lines<TW, MAXE, F>), dim3((unsigned)blocks), dim3(kFftThreads), lds, st, data, R, es,
+                       nlines, b1, s1, s2, sign, lpb, ld, groups, xcd_per, filt, tw);
     return hipGetLastError();
 }
 
@@ -405,8 +449,12 @@ int fft_fail_hip(hipError_t e, const std::string& what) { return fft_fail(hip_co
 
 const char* fft_last_error() { return g_fft_err.c_str(); }
 
-int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
+namespace {
+
+int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign,
+                       const LineFilter& filt) {
     if (n <= 1 || nlines <= 0) return 0;
+    if (filt.on && es == 1) return fft_fail(GLASMA_ERR_ARG, "a filtered pass needs strided lines");
     if (n > kFftLds) return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " is above 2048");
     Radices R;
     if (factorize(n, R) != 0)
@@ -432,28 +480,42 @@ int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1
     const int ld = lpb > 1 ? n + 1 : n;
     const long groups = (nlines + lpb - 1) / lpb;
     const long xcd_per = es != 1 && groups >= 64 ? (groups + 7) / 8 : 0;
-    const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld;
+    // two buffers, and with a filter the table of its z term
+    const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld + (filt.on ? sizeof(double) * n : 0);
     if (bytes > dev.lds) return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " exceeds the LDS");
     const bool twlds = bytes + sizeof(dbl2) * n <= dev.lds;
     const size_t lds = bytes + (twlds ? sizeof(dbl2) * n : 0);
     const int need = (lpb * n + kFftThreads - 1) / kFftThreads;  // elements per thread
+    const int maxe = need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : 16;
     hipError_t ke;
-#define GLASMA_FFT_LAUNCH(TW, M)                                                                                  \
-    ke = launch_lines<TW, M>(st, lds, dev.cus, data, R, es, nlines, b1, s1, s2, sign, lpb, ld, groups, xcd_per, tw)
-    if (need <= 2) {
-        if (twlds) GLASMA_FFT_LAUNCH(true, 2); else GLASMA_FFT_LAUNCH(false, 2);
-    } else if (need <= 4) {
-        if (twlds) GLASMA_FFT_LAUNCH(true, 4); else GLASMA_FFT_LAUNCH(false, 4);
-    } else if (need <= 8) {
-        if (twlds) GLASMA_FFT_LAUNCH(true, 8); else GLASMA_FFT_LAUNCH(false, 8);
-    } else {
-        if (twlds) GLASMA_FFT_LAUNCH(true, 16); else GLASMA_FFT_LAUNCH(false, 16);
+#define GLASMA_FFT_LAUNCH(TW, M, F)                                                                                 \
+    ke = launch_lines<TW, M, F>(st, lds, dev.cus, data, R, es, nlines, b1, s1, s2, sign, lpb, ld, groups, xcd_per, \
+                                filt, tw)
+#define GLASMA_FFT_LAUNCH_M(TW, F)                   \
+    switch (maxe) {                                  \
+        case 2: GLASMA_FFT_LAUNCH(TW, 2, F); break;  \
+        case 4: GLASMA_FFT_LAUNCH(TW, 4, F); break;  \
+        case 8: GLASMA_FFT_LAUNCH(TW, 8, F); break;  \
+        default: GLASMA_FFT_LAUNCH(TW, 16, F); break; \
     }
+    if (twlds) {
+        if (filt.on) GLASMA_FFT_LAUNCH_M(true, true) else GLASMA_FFT_LAUNCH_M(true, false)
+    } else {
+        if (filt.on) GLASMA_FFT_LAUNCH_M(false, true) else GLASMA_FFT_LAUNCH_M(false, false)
+    }
+#undef GLASMA_FFT_LAUNCH_M
 #undef GLASMA_FFT_LAUNCH
     if (ke != hipSuccess)
         return fft_fail_hip(ke, "k_fft_lines launch (length " + std::to_string(n) + ", " + std::to_string(lpb) +
                                     " lines per block, " + std::to_string(lds) + " bytes of LDS)");
     return 0;
+}
+
+}  // namespace
+
+int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
+    const LineFilter none = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
+    return fft_lines_filtered(st, data, n, es, nlines, b1, s1, s2, sign, none);
 }
 
 // c[(comp*nzl + z)*P + y*nx + x]
@@ -473,6 +535,16 @@ int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign) {
     int rc = fft_xy(st, c, g, ncomp, g.nz, sign);
     if (rc) return rc;
     return fft_z(st, c, g, ncomp, sign);
+}
+
+int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3]) {
+    if (g.nz < 2) return fft_fail(GLASMA_ERR_ARG, "the fused Poisson filter needs Nz >= 2");
+    const long P = (long)g.nx * g.ny;
+    const LineFilter f = {1, g.nx, g.ny, P, a[0], a[1], a[2], 1.0 / (double)g.S};
+    int rc = fft_xy(st, c, g, ncomp, g.nz, -1);
+    if (!rc) rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
+    if (!rc) rc = fft_xy(st, c, g, ncomp, g.nz, +1);
+    return rc;
 }
 
 }  // namespace glasma

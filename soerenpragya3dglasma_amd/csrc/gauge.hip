@@ -489,9 +489,17 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
             if (algo == GLASMA_GAUGE_FOURIER) {
-                if ((rc = fft_3d(ctx->stream, F, g, 8, -1))) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
-                hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1], a[2]);
-                if ((rc = fft_3d(ctx->stream, F, g, 8, +1))) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
+                if (g.nz >= 2) {
+                    rc = fft_3d_poisson(ctx->stream, F, g, 8, a);
+                } else {
+                    rc = fft_3d(ctx->stream, F, g, 8, -1);
+                    if (!rc) {
+                        hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1],
+                                           a[2]);
+                        rc = fft_3d(ctx->stream, F, g, 8, +1);
+                    }
+                }
+                if (rc) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
             }
             hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g,
                                param);
