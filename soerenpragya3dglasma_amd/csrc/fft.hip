@@ -4,15 +4,20 @@
 // both unnormalised (callers scale by 1/n in momentum space, e.g. SC/INITIALCONDITIONS/SUNCCURRENTS/
 // RevSingleNucleus.cpp:150, SC/GAUGETRANSFORMATION/ALGORITHMS/FourierAcceleration.cpp:62).
 //
-// Design: every transform is a set of independent 1D lines.  One workgroup loads `lpb` whole lines into
-// LDS (coalesced: along the line when its elements are contiguous, across lines otherwise), runs a
-// self-sorting Stockham transform in LDS (radix 4, 2, 3 kernels, any other prime by a direct DFT), and
-// writes the lines back in place.  Multi-dimensional transforms are one pass per axis, so each pass
-// reads and writes the array once.  Strided lines (the y and z passes) are loaded 4 at a time where the
-// device's LDS allows (160 KiB on gfx950), so each row of the load is 64 contiguous bytes, and blocks are dealt
-// to the XCDs so that neighbouring line groups run on one XCD and share the rest of each 128-byte line in its
-// L2.  The twiddle table is copied into LDS with the data where it fits.  Twiddles come from a table exp(-2 pi i j/n) computed once per length
-// in long double and rounded to FP64.
+// Design: every transform is a set of independent 1D lines, transformed in groups.  A workgroup loads a group
+// into LDS (contiguous lines as one packed run; strided lines across the group, so neighbouring threads read
+// neighbouring addresses), runs a self-sorting Stockham transform in LDS (radix 4, 2, 3 kernels, any other
+// prime <= 13 by a direct DFT) and writes the lines back in place.  Multi-dimensional transforms are one pass per
+// axis, so each pass reads and writes the array once.
+// - Strided lines (the y and z passes) go 4 to a group within the device's LDS (160 KiB on gfx950), so each row
+//   of a load is 64 contiguous bytes; their groups are dealt to the XCDs so that neighbouring groups run on one
+//   XCD and share the rest of each 128-byte line in its L2.
+// - Where only one or two workgroups fit per CU, each block walks several groups and prefetches the next one
+//   into registers while it transforms the current one.
+// - The twiddle table exp(-2 pi i j/n) is computed once per length in long double, rounded to FP64, and copied
+//   into LDS with the data where it fits.
+// - fft_3d_poisson fuses the forward z transform, the Poisson filter of gauge fixing and the inverse z transform
+//   into one pass.
 #include <algorithm>
 #include <map>
 #include <mutex>
