@@ -1769,10 +1769,15 @@ void ref_coulomb_gauge(const int* N, const double* a, double* U, double* E, doub
     Lat<SU3> L = mk3(N, a, 0);
     CoulombGauge(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, tol, maxit, alpha, stats);
 }
+// bc: z boundary of the lattice (0 open, 1 periodic); it decides how G's z ghost layers are synchronised
+void ref_coulomb_gauge_algo_bc(const int* N, const double* a, int bc, double* U, double* E, double* G, double* UG,
+                               double* EG, int algo, double tol, int maxit, double param, double* stats) {
+    Lat<SU3> L = mk3(N, a, bc);
+    CoulombGaugeAlgo(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, algo, tol, maxit, param, stats);
+}
 void ref_coulomb_gauge_algo(const int* N, const double* a, double* U, double* E, double* G, double* UG, double* EG,
                             int algo, double tol, int maxit, double param, double* stats) {
-    Lat<SU3> L = mk3(N, a, 0);
-    CoulombGaugeAlgo(L, (cd*)U, E, (cd*)G, (cd*)UG, EG, algo, tol, maxit, param, stats);
+    ref_coulomb_gauge_algo_bc(N, a, 0, U, E, G, UG, EG, algo, tol, maxit, param, stats);
 }
 void ref_gauss_restore(const int* N, const double* a, int bc, double* U, double* E, double* JpS, double* JmS, int variant,
                        double tol, int maxit, double* stats) {

@@ -74,6 +74,8 @@ def lib():
         L.ref_gauge_transform.argtypes = [_ip, _dp, ctypes.c_int, _dp] + [_dp] * 6
         L.ref_coulomb_gauge_algo.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, _dp]
+        L.ref_coulomb_gauge_algo_bc.argtypes = [_ip, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                                ctypes.c_double, ctypes.c_int, ctypes.c_double, _dp]
         L.ref_max_trace_projection.argtypes = [_dp, _dp]
         L.ref_tmd_draw.argtypes = [ctypes.c_ulong, _ip, ctypes.c_int] + [_dp] * 4
         L.ref_tmd_draw.restype = ctypes.c_int
@@ -312,14 +314,15 @@ def coulomb_gauge(f: Fields, a, G: np.ndarray, tol: float, maxit: int, alpha: fl
 ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2}
 
 
-def coulomb_gauge_algo(f: Fields, a, G: np.ndarray, algo: str, tol: float, maxit: int, param: float):
+def coulomb_gauge_algo(f: Fields, a, G: np.ndarray, algo: str, tol: float, maxit: int, param: float, bc: int = 0):
     """SetCoulombGauge with the Fourier-accelerated (param = alpha), Jacobi (param = step size) or Los Alamos
-    (param = over-relaxation probability, 0) update; G is updated in place."""
+    (param = over-relaxation probability, 0) update; G is updated in place.  bc: z boundary (0 open, 1 periodic),
+    which decides how G's z ghost layers are synchronised."""
     UG = Fields(f.N).U
     EG = np.zeros_like(f.E)
     stats = np.zeros(7)
-    lib().ref_coulomb_gauge_algo(ivec(f.N), dvec(a), P(f.U), P(f.E), P(G), P(UG), P(EG), ALGORITHMS[algo], tol, maxit,
-                                 param, P(stats))
+    lib().ref_coulomb_gauge_algo_bc(ivec(f.N), dvec(a), int(bc), P(f.U), P(f.E), P(G), P(UG), P(EG), ALGORITHMS[algo],
+                                    tol, maxit, param, P(stats))
     return UG, EG, stats
 
 

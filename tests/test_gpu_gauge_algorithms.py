@@ -38,6 +38,31 @@ def test_fixed_iterations_match_oracle(algo, param, N, a, iters):
     assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < 1e-11
 
 
+@pytest.mark.parametrize("algo,param", [("fourier", 2.0), ("jacobi", 0.3), ("losalamos", 0.0)])
+def test_periodic_z_matches_oracle(algo, param):
+    """Periodic z: G's top ghost layer is a copy of layer 0 (one transformation behind, as for open z).  The
+    Fourier and Jacobi iterations form the gauge-fixed links inside the deviation kernel and the Fourier one
+    filters in the fused z pass; checked against the oracle run with the same boundary (the open- and
+    periodic-z runs differ by 0.3-0.7).  Tolerances as for open z: 1e-11, and for the Fourier update, whose
+    iterations grow the deviation on this lattice and with it the round-off, 1e-10 (monitors 1e-9 relative)."""
+    from soerenpragya3dglasma_amd import Glasma
+    N, a, iters = (6, 5, 6), (1.0, 0.9, 0.8), 4
+    f = random_fields(N, seed=31, link_scale=0.3, e_scale=0.3)
+    G = _identity_G(N)
+    UG_o, EG_o, st_o = O.coulomb_gauge_algo(f, a, G, algo, 0.0, iters, param, bc=1)
+    with Glasma(N, a, 0.1, 1) as g:
+        upload(g, f)
+        st = g.coulomb_gauge_algorithm(algo, 0.0, iters, param)
+        UG, EG, Gd = g.get_gauge_fixed()
+    print(algo, st, st_o, np.abs(Gd[1:-1] - G[1:-1]).max())
+    tol, rtol = (1e-10, 1e-9) if algo == "fourier" else (1e-11, 1e-10)
+    assert st[0] == st_o[0] == iters
+    assert np.allclose(st[1:], st_o[1:], rtol=rtol, atol=1e-13)
+    assert np.abs(Gd[1:-1] - G[1:-1]).max() < tol
+    assert np.abs(UG[1:-1] - UG_o[1:-1]).max() < tol
+    assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < tol
+
+
 @pytest.mark.parametrize("algo,param,cap", [("jacobi", 0.3, 3000), ("losalamos", 0.0, 3000)])
 def test_transverse_gauge_converges(algo, param, cap):
     from soerenpragya3dglasma_amd import Glasma
