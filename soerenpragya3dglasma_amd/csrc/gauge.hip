@@ -12,6 +12,11 @@
 // U^G_mu(x) = g(x) U_mu(x) g^dag(x + mu).  Monitors (max / RMS of Delta and of the step, min / mean of the
 // functional Re tr w) come from deterministic two-level reductions.
 //
+// The eight real components travel as four complex arrays, Delta^{2k} + i Delta^{2k+1}: the filter is real and
+// even in the momentum, so by linearity the inverse transform returns the two filtered components as its real
+// and imaginary parts.  That is half the transforms of one complex array per component (as the reference runs
+// them), at a round-off-level difference.
+//
 // The reference's two other update algorithms (selected there by #define, CoulombGaugeFixing.cpp:4,55-67) share
 // w, Delta, the monitors and the link transformation:
 //   Jacobi      ALGORITHMS/Jacobi.cpp:13-74      g <- exp(-i alpha Delta t)^dag g at every site, no FFT;
@@ -187,8 +192,11 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restr
         for (int a = 0; a < 8; a++) {
             mx = fmax(mx, fabs(dv[a]));
             sq += dv[a] * dv[a];
-            F[2 * ((long)a * g.S + s)] = dv[a];
-            F[2 * ((long)a * g.S + s) + 1] = 0.0;
+        }
+        // two real components per complex array: Delta^{2k} + i Delta^{2k+1} in array k
+        for (int k = 0; k < 4; k++) {
+            F[2 * ((long)k * g.S + s)] = dv[2 * k];
+            F[2 * ((long)k * g.S + s) + 1] = dv[2 * k + 1];
         }
         const double lf = W.re[0] + W.re[4] + W.re[8];
         fm = fmin(fm, lf);
@@ -290,7 +298,7 @@ __global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay,
                       (2.0 - 2.0 * cos(2.0 * kPi * pz / (double)g.nz)) / (az * az);
     if (!(p2 > 0.0)) return;
     const double nf = 1.0 / (double)g.S;
-    for (int a = 0; a < 8; a++) {
+    for (int a = 0; a < 4; a++) {  // the four packed arrays
         const long i = (long)a * g.S + s;
         F[2 * i] = nf * F[2 * i] / p2;
         F[2 * i + 1] = nf * F[2 * i + 1] / p2;
@@ -304,7 +312,7 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_update(const double* __restrict
     double ssq = 0.0, sm = 0.0;
     if (s < g.S) {
         double ub[8];
-        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)a * g.S + s)];
+        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * g.S + s) + (a & 1)];  // packed as by k_cg_dev_fused
         M3 Wt, Gx, Gn;
         expi(-alpha, ub, Wt);
         const double tr = 3.0 - (Wt.re[0] + Wt.re[4] + Wt.re[8]);
@@ -454,7 +462,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     const Geom& g = ctx->g;
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
     rc = ensure_red(ctx, 16 * (size_t)nblk + 8);
-    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 16 * (size_t)g.S);
+    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)g.S);  // 4 complex per site
     if (rc) return rc;
     const double* a = ctx->p.a;
     // weights (1/a_mu)^2 of w and the functional's starting minimum 6 sum_mu 1/a_mu^2, rounded as the
@@ -490,13 +498,13 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
             if (algo == GLASMA_GAUGE_FOURIER) {
                 if (g.nz >= 2) {
-                    rc = fft_3d_poisson(ctx->stream, F, g, 8, a);
+                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a);
                 } else {
-                    rc = fft_3d(ctx->stream, F, g, 8, -1);
+                    rc = fft_3d(ctx->stream, F, g, 4, -1);
                     if (!rc) {
                         hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1],
                                            a[2]);
-                        rc = fft_3d(ctx->stream, F, g, 8, +1);
+                        rc = fft_3d(ctx->stream, F, g, 4, +1);
                     }
                 }
                 if (rc) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
