@@ -133,38 +133,37 @@ __device__ __forceinline__ void local_w(const double* __restrict__ UG, const Geo
     }
 }
 
-// U^G_mu at (x, y, z) from U and G, the same two products in the same order as k_cg_transform (so the values are
-// identical); the z neighbour of the top layer is Gtop
-__device__ __forceinline__ void ug_link(const double* __restrict__ U, const double* __restrict__ G,
-                                        const double* __restrict__ Gtop, const Geom& g, int mu, int x, int y, int z,
-                                        M3& R) {
-    const long s = sidx(g, x, y, z);
-    M3 Gx, Um, Gn, T;
-    ldm<long>(G, kGPlanes, 0, s, Gx);
-    ldm<long>(U, kUPlanes, 18 * mu, s, Um);
-    if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
-    else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
-    else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
-    else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
-    mul_nn(Gx, Um, T);
-    mul_nd(T, Gn, R);
-}
-
 // local_w with the gauge-fixed links formed on the fly from U and G (the Fourier and Jacobi iterations): the
-// iteration then needs no separate link transformation writing U^G and reading it back.
+// iteration then needs no separate link transformation writing U^G and reading it back.  Each link is formed with
+// the two products of k_cg_transform in the same order, so the values are the same.
 __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, const double* __restrict__ G,
                                               const double* __restrict__ Gtop, const Geom& g, int x, int y, int z,
                                               double cx, double cy, double cz, M3& W) {
-    M3 A, B;
+    M3 A, B, Gx, Um, Gn, T;
     const double zf = g.nz > 1 ? 1.0 : 0.0;
+    const long s = sidx(g, x, y, z);
+    ldm<long>(G, kGPlanes, 0, s, Gx);  // G(x), read once: it starts A and ends every backward link B
     for (int k = 0; k < 9; k++) { W.re[k] = 0.0; W.im[k] = 0.0; }
     for (int mu = 0; mu < 3; mu++) {
         const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
-        ug_link(U, G, Gtop, g, mu, x, y, z, A);
-        if (mu == 0) ug_link(U, G, Gtop, g, 0, x - 1, y, z, B);
-        else if (mu == 1) ug_link(U, G, Gtop, g, 1, x, y - 1, z, B);
-        else if (z > 0) ug_link(U, G, Gtop, g, 2, x, y, z - 1, B);
-        else set_unit(B);
+        // A = U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu)
+        ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+        if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
+        else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
+        else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
+        else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
+        mul_nn(Gx, Um, T);
+        mul_nd(T, Gn, A);
+        // B = U^G_mu(x - mu) = G(x - mu) U_mu(x - mu) G^dag(x); below z = 0 the identity
+        if (mu < 2 || z > 0) {
+            const long sb = mu == 0 ? sidx(g, x - 1, y, z) : (mu == 1 ? sidx(g, x, y - 1, z) : s - g.P);
+            ldm<long>(G, kGPlanes, 0, sb, Gn);
+            ldm<long>(U, kUPlanes, 18 * mu, sb, Um);
+            mul_nn(Gn, Um, T);
+            mul_nd(T, Gx, B);
+        } else {
+            set_unit(B);
+        }
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) {
                 W.re[mi(i, j)] += c * (A.re[mi(i, j)] + B.re[mi(j, i)]);
