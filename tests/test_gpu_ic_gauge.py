@@ -199,6 +199,30 @@ def test_coulomb_gauge_fixed_iterations_match_oracle(N, a, iters):
     assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < 1e-10
 
 
+def test_coulomb_gauge_walking_z_pass_matches_oracle():
+    """At 32 x 32 x 1024 the fused z pass (forward transform, 1/(V p^2), inverse transform) of the Fourier update
+    runs one workgroup per CU and each block walks several line groups (1024 groups of 4 lines over at most 256
+    blocks), which the smaller lattices above do not reach.  Fields from the device's random fill, the same
+    arrays given to the oracle; two iterations, Fourier tolerances."""
+    from soerenpragya3dglasma_amd import Glasma
+    N, a, iters = (32, 32, 1024), A1, 2
+    with Glasma(N, a, 0.1, 0) as g:
+        g.fill_random(seed=41, link_scale=0.3, e_scale=0.3, j_scale=0.0)
+        d = g.get_fields()
+        st = g.coulomb_gauge(0.0, iters, 2.0)
+        UG, EG, Gd = g.get_gauge_fixed()
+    f = O.Fields(N)
+    f.U, f.E, f.Jp, f.Jm, f.JpS, f.JmS = d["U"], d["E"], d["Jp"], d["Jm"], d["JpS"], d["JmS"]
+    G = _identity_G(N)
+    UG_o, EG_o, st_o = O.coulomb_gauge(f, a, G, 0.0, iters, 2.0)
+    print(st, st_o)
+    assert st[0] == st_o[0] == iters
+    assert np.allclose(st[1:], st_o[1:], rtol=1e-9, atol=1e-13)
+    assert np.abs(Gd[1:-1] - G[1:-1]).max() < 1e-10
+    assert np.abs(UG[1:-1] - UG_o[1:-1]).max() < 1e-10
+    assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < 1e-10
+
+
 def test_coulomb_gauge_converges_and_keeps_invariants():
     """Transverse (Nz = 1) Coulomb gauge to 1e-12: iteration count matches the oracle's, the residual is
     below the tolerance, and the gauge-invariant E^2 and plaquette sums are unchanged."""
