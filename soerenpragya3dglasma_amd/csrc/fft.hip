@@ -99,6 +99,18 @@ constexpr int kMaxLinesPerBlock = 64;
 #define GLASMA_FFT_STRIDED_LINES 4
 #endif
 constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
+// the filtered z pass of fft_3d_poisson: lines per group and whether its twiddles go to LDS.  With its two
+// transforms per group the pass gains more from a second workgroup per CU than from 64-byte rows or LDS
+// twiddles: at 256^2 x 1024 (four arrays) 2 lines with twiddles from memory (74 KiB of LDS) took 6.9-7.1 ms,
+// 4 lines with LDS twiddles (155 KiB) 8.2 ms, 4 lines with twiddles from memory 9.3 ms.
+#ifndef GLASMA_FFT_FILTER_LINES
+#define GLASMA_FFT_FILTER_LINES 2
+#endif
+#ifndef GLASMA_FFT_FILTER_TWLDS
+#define GLASMA_FFT_FILTER_TWLDS 0
+#endif
+constexpr int kFilterLines = GLASMA_FFT_FILTER_LINES;
+constexpr bool kFilterTwiddlesInLds = GLASMA_FFT_FILTER_TWLDS != 0;
 static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
               "strided lines per block: a power of two dividing the block size");
 
@@ -479,7 +491,8 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
         while (2 * lpb * (n + 1) <= kContigEntries && 2 * lpb <= kMaxLinesPerBlock) lpb *= 2;
     } else {
         const size_t cap = dev.lds / (2 * sizeof(dbl2));  // entries per buffer
-        while ((size_t)(2 * lpb) * (n + 1) <= cap && 2 * lpb <= kStridedLines && 2 * lpb * n <= 16 * kFftThreads)
+        const int most = filt.on ? kFilterLines : kStridedLines;
+        while ((size_t)(2 * lpb) * (n + 1) <= cap && 2 * lpb <= most && 2 * lpb * n <= 16 * kFftThreads)
             lpb *= 2;
     }
     const int ld = lpb > 1 ? n + 1 : n;
@@ -488,7 +501,7 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
     // two buffers, and with a filter the table of its z term
     const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld + (filt.on ? sizeof(double) * n : 0);
     if (bytes > dev.lds) return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " exceeds the LDS");
-    const bool twlds = bytes + sizeof(dbl2) * n <= dev.lds;
+    const bool twlds = (!filt.on || kFilterTwiddlesInLds) && bytes + sizeof(dbl2) * n <= dev.lds;
     const size_t lds = bytes + (twlds ? sizeof(dbl2) * n : 0);
     const int need = (lpb * n + kFftThreads - 1) / kFftThreads;  // elements per thread
     const int maxe = need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : 16;
