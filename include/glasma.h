@@ -145,7 +145,7 @@ int glasma_mv_initial_state(glasma_ctx* ctx, int driver, const glasma_mv_params*
 int glasma_set_wilson_lines(glasma_ctx* ctx, char dir, const glasma_mv_params* mv, const double* xi,
                             double* V_out, double* rho_out);
 
-/* ---- physical-charge (TMD / GBW) nuclei (TMDNUCLEUS/*.cpp, SimulationmitPC.cpp:315-370) ----------- */
+/* ---- physical-charge (TMD / GBW) nuclei (the TMDNUCLEUS sources, SimulationmitPC.cpp:315-370) ----- */
 /* Parameters of LATTICE/Parameters.cpp:4-48 that the physical-charge path reads. */
 typedef struct glasma_tmd_params {
     double qs0a;        /* Parameters::Qs0A (lattice units) */

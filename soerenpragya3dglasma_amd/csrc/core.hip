@@ -176,7 +176,7 @@ int reset_fields(glasma_ctx* ctx) {
 // written to red[(z*bpp + b)*K + k] and summed afterwards in a fixed order: results do not depend on
 // scheduling and are reproducible bit for bit.
 template <int K>
-__device__ __forceinline__ void block_store(double (&v)[K], double* out, bool is_max_mask_bit0 = false) {
+__device__ __forceinline__ void block_store(double (&v)[K], double* out) {
     __shared__ double sh[4][K];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -274,7 +274,6 @@ __global__ void __launch_bounds__(256) k_gauss(ObsArgs A) {
     long s;
     if (site_of(A, x, y, z, s)) {
         const Geom& g = A.g;
-        const long S = g.S;
         const long sxm = sidx(g, x - 1, y, z), sym = sidx(g, x, y - 1, z), szm = sidx(g, x, y, zdn(g, z));
         const double norm = 1.0 / (A.a[0] * A.a[1] * A.a[2]);
         M3 V;
@@ -622,7 +621,6 @@ int glasma_memory_bytes(const glasma_ctx* ctx, size_t* bytes) {
 int glasma_set_fields(glasma_ctx* ctx, const double* U, const double* E, const double* Jp, const double* Jm,
                       const double* JpStat, const double* JmStat) {
     if (!ctx) return GLASMA_ERR_ARG;
-    const long S = ctx->g.S;
     int rc;
     if (U && (rc = upload_aos(ctx, U, ctx->U[ctx->cur], kUPlanes))) return rc;
     if (E && (rc = upload_aos(ctx, E, ctx->E, kEPlanes))) return rc;
@@ -634,7 +632,6 @@ int glasma_set_fields(glasma_ctx* ctx, const double* U, const double* E, const d
 
 int glasma_get_fields(glasma_ctx* ctx, double* U, double* E, double* Jp, double* Jm, double* JpStat, double* JmStat) {
     if (!ctx) return GLASMA_ERR_ARG;
-    const long S = ctx->g.S;
     int rc;
     if (U && (rc = download_aos(ctx, ctx->U[ctx->cur], U, kUPlanes))) return rc;
     if (E && (rc = download_aos(ctx, ctx->E, E, kEPlanes))) return rc;
