@@ -105,6 +105,87 @@ def test_jacobi_one_iteration_against_numpy(N):
     assert abs(st[3] - smax) < 1e-14 and abs(st[5] - fmin) < 1e-13 and abs(st[6] - fsum / V) < 1e-13
 
 
+def _fourier_iterations_np(f, a, G0, alpha, niter, bc):
+    """SetCoulombGauge with FourierAccelerationAlgorithm (SC/GAUGETRANSFORMATION/CoulombGaugeFixing.cpp:32-88,
+    ALGORITHMS/FourierAcceleration.cpp:15-148) in numpy, written from the reference rather than from the oracle:
+    * CopyFields copies the interior only, so the gauge-fixed links' z = -1 ghost layer stays the identity
+      (GaugeTransformation.cpp:84-89), and every transform writes the interior only (GaugeTransformation.cpp:111-124);
+    * a transform reads G(x + mu) with G's ghost layers as they are and synchronises them afterwards
+      (GaugeTransformation.cpp:124), so inside the loop the top ghost lags one update;
+    * Delta^a = Re tr(i t^a w), w = sum_mu a_mu^-2 (U^G_mu(x) + U^G_mu(x - mu)^dag), the z term only for Nz > 1
+      (CoulombGaugeDeviation.cpp:23-41);
+    * forward FFT, Delta~/(V p^2) for p^2 > 0 with periodic momenta in all three directions, the zero mode
+      untouched, unnormalised inverse FFT (FourierAcceleration.cpp:50-79);
+    * g <- DU(exp(-i alpha Delta' t), g) = exp(-i alpha Delta' t)^dag g (FourierAcceleration.cpp:104-112).
+    Returns the final gauge-fixed links [z+1, y, x, mu, i, j], G [z+1, y, x, i, j] and the last iteration's
+    monitors (iterations, max / avg deviation, max / avg step, min / mean functional)."""
+    Nx, Ny, Nz = f.N
+    V = Nx * Ny * Nz
+    U = np.swapaxes(f.U, -1, -2)
+    g = np.swapaxes(G0, -1, -2).copy()
+    dag = lambda M: np.conj(np.swapaxes(M, -1, -2))  # noqa: E731
+    eye = np.broadcast_to(np.eye(3, dtype=complex), (Nz + 2, Ny, Nx, 3, 3, 3)).copy()
+
+    def sync(h):
+        if bc == 0:
+            h[0], h[-1] = h[1], h[-2]
+        else:
+            h[0], h[-1] = h[-2], h[1]
+
+    def transform():
+        UG = eye.copy()
+        gi = g[1:-1]
+        nb = (np.roll(gi, -1, axis=2), np.roll(gi, -1, axis=1), g[2:])
+        for mu in range(3):
+            UG[1:-1, :, :, mu] = gi @ U[1:-1, :, :, mu] @ dag(nb[mu])
+        sync(g)
+        return UG
+
+    c = [1 / a[0] ** 2, 1 / a[1] ** 2, (1 / a[2] ** 2) if Nz > 1 else 0.0]
+    n = [np.arange(Nz)[:, None, None], np.arange(Ny)[None, :, None], np.arange(Nx)[None, None, :]]
+    p2 = ((2 - 2 * np.cos(2 * np.pi * n[2] / Nx)) / a[0] ** 2 + (2 - 2 * np.cos(2 * np.pi * n[1] / Ny)) / a[1] ** 2 +
+          (2 - 2 * np.cos(2 * np.pi * n[0] / Nz)) / a[2] ** 2)
+    UG = transform()
+    stats = None
+    for it in range(niter):
+        down = (np.roll(UG[1:-1, :, :, 0], 1, axis=2), np.roll(UG[1:-1, :, :, 1], 1, axis=1), UG[:-2, :, :, 2])
+        w = sum(c[mu] * (UG[1:-1, :, :, mu] + dag(down[mu])) for mu in range(3))
+        dev = np.stack([np.trace(1j * T[k] @ w, axis1=-2, axis2=-1).real for k in range(8)], axis=-1)
+        fun = np.trace(w, axis1=-2, axis2=-1).real
+        F = np.fft.fftn(dev, axes=(0, 1, 2))
+        F[p2 > 0] /= (V * p2[p2 > 0])[:, None]
+        upd = (np.fft.ifftn(F, axes=(0, 1, 2)) * V).real
+        Wt = np.array([sl.expm(-1j * alpha * herm(u)) for u in upd.reshape(-1, 8)]).reshape(Nz, Ny, Nx, 3, 3)
+        step = np.abs(np.trace(np.eye(3) - Wt, axis1=-2, axis2=-1))
+        g[1:-1] = dag(Wt) @ g[1:-1]
+        UG = transform()
+        stats = np.array([it + 1, np.abs(dev).max(), np.sqrt((dev ** 2).sum() / (8 * V)), step.max(),
+                          np.sqrt((step ** 2).sum() / (8 * V)), fun.min(), fun.sum() / V])
+    return transform(), g, stats
+
+
+@pytest.mark.parametrize("N,bc", [((4, 3, 5), 0), ((3, 4, 4), 1), ((5, 4, 1), 0)])
+def test_fourier_iterations_against_numpy_nz_above_one(N, bc):
+    """Two iterations from a random G at Nz > 1, open and periodic z (and Nz = 1, where w has no z term): G, the
+    gauge-fixed links and the second iteration's monitors against the numpy restatement.  Two iterations, so that
+    the second one sees the links transformed with the lagging top ghost of G.  Each of these slips in the
+    restatement moves G by O(1): dropping the zero mode, exp(-i alpha Delta' t) instead of its adjoint, a fresh
+    top ghost, the gauge-fixed links' bottom ghost taken from U, the z term of w kept at Nz = 1."""
+    a = (1.0, 0.9, 0.8)
+    f = random_fields(N, seed=17, link_scale=0.3)
+    f.sync(bc)
+    rng = np.random.default_rng(18)
+    G = np.zeros((N[2] + 2, N[1], N[0], 3, 3), complex)
+    G[1:-1] = np.swapaxes(random_su3(rng, (N[2], N[1], N[0]), 0.5), -1, -2)
+    G[0], G[-1] = (G[1], G[-2]) if bc == 0 else (G[-2], G[1])
+    UGn, gn, stn = _fourier_iterations_np(f, a, G, 2.0, 2, bc)
+    G_or = G.copy()
+    UG, EG, st = O.coulomb_gauge_algo(f.copy(), a, G_or, "fourier", 0.0, 2, 2.0, bc=bc)
+    assert np.abs(np.swapaxes(G_or[1:-1], -1, -2) - gn[1:-1]).max() < 1e-12
+    assert np.abs(np.swapaxes(UG[1:-1], -1, -2) - UGn[1:-1]).max() < 1e-12
+    assert np.abs(st - stn).max() < 1e-12 * max(1.0, np.abs(stn).max()), (st, stn)
+
+
 def test_los_alamos_functional_increases_and_converges():
     """Nz = 1 (no z links in w): with even Nx, Ny the checkerboard is consistent and every site update raises
     its local term Re tr(h w), so the functional cannot fall."""

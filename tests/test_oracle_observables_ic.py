@@ -114,6 +114,86 @@ def test_emt_uniform_electric_field():
     assert np.allclose(emt, np.tile(ref, (N[2], 1)), rtol=1e-13, atol=1e-15)
 
 
+def _emt_brute_force(f, a):
+    """EnergyMomentumTensor::CreateMap (SC/OBSERVABLES/EnergyMomentumTensor.cpp:92-170) in numpy, one site at a
+    time, written from the reference's macros rather than from the oracle:
+    * B: the four plaquettes around the site in each plane, on UOld, averaged (COMPUTE_ELEMENTARY /
+      NEIGHBORING / EXTENDED_PLAQUETTES, SC/CALC/Plaquettes.cpp:83-97; AvgMatrix, SC/SUNc/AdvancedOperations.cpp:35);
+      B_mu^a = 2 Re tr(i t^a avg), B_mu^2 = 4 Re tr(1 - avg) (SC/CALC/AvgFieldStrength.cpp:68-82);
+    * E: the vector is the mean of E_mu(x) and U_mu(x-mu)^dag E_mu(x-mu) U_mu(x-mu) with the CURRENT links (the
+      macro's U is CreateMap's U), E^2 the mean of the two squares (SC/CALC/AvgFieldStrength.cpp:86-105);
+    * T^{mu nu} with q_mu = a_mu / a^3, per-site terms divided by Nx Ny and summed per z."""
+    Nx, Ny, Nz = f.N
+    N = f.N
+    q = np.array(a) / np.prod(a)
+    D = lambda M: M.conj().T  # noqa: E731
+    out = np.zeros((Nz, 10))
+    for z in range(Nz):
+        for y in range(Ny):
+            for x in range(Nx):
+                V = lambda dx, dy, dz, mu: mat(f.UOld, z + dz, y + dy, x + dx, mu, N)  # noqa: E731
+                W = lambda dx, dy, dz, mu: mat(f.U, z + dz, y + dy, x + dx, mu, N)  # noqa: E731
+                Ev = lambda dx, dy, dz, mu: f.E[z + dz + 1, (y + dy) % Ny, (x + dx) % Nx, mu]  # noqa: E731
+                # plane (mu, nu): x-y, y-z, z-x; UUDD = V1 V2 V3^dag V4^dag, UDDU = V1 V2^dag V3^dag V4, ...
+                Uxy = V(0, 0, 0, 0) @ V(1, 0, 0, 1) @ D(V(0, 1, 0, 0)) @ D(V(0, 0, 0, 1))
+                Uyz = V(0, 0, 0, 1) @ V(0, 1, 0, 2) @ D(V(0, 0, 1, 1)) @ D(V(0, 0, 0, 2))
+                Uzx = V(0, 0, 0, 2) @ V(0, 0, 1, 0) @ D(V(1, 0, 0, 2)) @ D(V(0, 0, 0, 0))
+                UMxy = V(0, 0, 0, 1) @ D(V(-1, 1, 0, 0)) @ D(V(-1, 0, 0, 1)) @ V(-1, 0, 0, 0)
+                UMyz = V(0, 0, 0, 2) @ D(V(0, -1, 1, 1)) @ D(V(0, -1, 0, 2)) @ V(0, -1, 0, 1)
+                UMzx = V(0, 0, 0, 0) @ D(V(1, 0, -1, 2)) @ D(V(0, 0, -1, 0)) @ V(0, 0, -1, 2)
+                UxMy = D(V(0, -1, 0, 1)) @ V(0, -1, 0, 0) @ V(1, -1, 0, 1) @ D(V(0, 0, 0, 0))
+                UyMz = D(V(0, 0, -1, 2)) @ V(0, 0, -1, 1) @ V(0, 1, -1, 2) @ D(V(0, 0, 0, 1))
+                UzMx = D(V(-1, 0, 0, 0)) @ V(-1, 0, 0, 2) @ V(-1, 0, 1, 0) @ D(V(0, 0, 0, 2))
+                UMxMy = D(V(-1, 0, 0, 0)) @ D(V(-1, -1, 0, 1)) @ V(-1, -1, 0, 0) @ V(0, -1, 0, 1)
+                UMyMz = D(V(0, -1, 0, 1)) @ D(V(0, -1, -1, 2)) @ V(0, -1, -1, 1) @ V(0, 0, -1, 2)
+                UMzMx = D(V(0, 0, -1, 2)) @ D(V(-1, 0, -1, 0)) @ V(-1, 0, -1, 2) @ V(-1, 0, 0, 0)
+                avg = [0.25 * (Uyz + UMyz + UyMz + UMyMz), 0.25 * (Uzx + UMzx + UzMx + UMzMx),
+                       0.25 * (Uxy + UMxy + UxMy + UMxMy)]
+                B = [np.array([2.0 * np.trace(1j * T[c] @ M).real for c in range(8)]) for M in avg]
+                B2 = [4.0 * (3.0 - np.trace(M).real) for M in avg]
+                Evec, E2 = [], []
+                for mu, d in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+                    Um = W(-d[0], -d[1], -d[2], mu)
+                    em = Ev(-d[0], -d[1], -d[2], mu)
+                    rot = D(Um) @ herm(em) @ Um
+                    down = np.array([2.0 * np.trace(T[c] @ rot).real for c in range(8)])
+                    Evec.append(0.5 * (Ev(0, 0, 0, mu) + down))
+                    E2.append(0.5 * ((Ev(0, 0, 0, mu) ** 2).sum() + (em ** 2).sum()))
+                e2 = [q[m] ** 2 * E2[m] for m in range(3)]
+                b2 = [q[m] ** 2 * B2[m] for m in range(3)]
+                t = [0.5 * (sum(e2) + sum(b2)),
+                     0.5 * ((e2[1] + e2[2] + b2[1] + b2[2]) - (e2[0] + b2[0])),
+                     0.5 * ((e2[0] + e2[2] + b2[0] + b2[2]) - (e2[1] + b2[1])),
+                     0.5 * ((e2[0] + e2[1] + b2[0] + b2[1]) - (e2[2] + b2[2])),
+                     q[1] * q[2] * Evec[1] @ B[2] - q[2] * q[1] * Evec[2] @ B[1],
+                     q[2] * q[0] * Evec[2] @ B[0] - q[0] * q[2] * Evec[0] @ B[2],
+                     q[0] * q[1] * Evec[0] @ B[1] - q[1] * q[0] * Evec[1] @ B[0],
+                     q[0] * q[1] * Evec[0] @ Evec[1] + q[0] * q[1] * B[0] @ B[1],
+                     q[1] * q[2] * Evec[1] @ Evec[2] + q[1] * q[2] * B[1] @ B[2],
+                     q[2] * q[0] * Evec[2] @ Evec[0] + q[2] * q[0] * B[2] @ B[0]]
+                out[z] += np.array(t) / (Nx * Ny)
+    return out
+
+
+@pytest.mark.parametrize("N,a,bc", [((3, 4, 5), (1.0, 0.9, 0.7), 0), ((4, 3, 4), (1.0, 1.0, 1.0), 1),
+                                    ((3, 3, 4), (0.8, 1.0, 0.5), 0)])
+def test_emt_brute_force(N, a, bc):
+    """All ten components per layer against the brute force, with UOld independent of U (so the clovers on UOld,
+    the transport with the current U and the layer each clover belongs to are all exercised), open and periodic
+    z, anisotropic spacings and Nx != Ny.  Each of these slips in the restatement moves a component by 0.7 % or
+    more of the largest: clovers on U, transport with UOld, the sign of T0X, one plaquette of the z-x clover a
+    layer off, E(x - z) read from the same layer."""
+    f = random_fields(N, seed=21, bc=bc)
+    old = random_fields(N, seed=22, bc=bc)
+    f.UOld[:] = old.U
+    f.sync(bc)
+    ref = _emt_brute_force(f, a)
+    got = O.emt(f, a, bc=bc)
+    scale = np.abs(ref).max()
+    assert np.abs(ref[:, 4:]).max() > 1e-2 * scale  # the off-diagonal components are not trivially small
+    assert np.abs(got - ref).max() < 1e-13 * scale, np.abs(got - ref).max(axis=0) / scale
+
+
 # ------------------------------------------------------------------------------------------------
 # MV initial conditions
 # ------------------------------------------------------------------------------------------------
