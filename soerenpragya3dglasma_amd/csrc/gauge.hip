@@ -413,14 +413,28 @@ __global__ __launch_bounds__(kCgBlock) void k_gr_efield(const double* __restrict
     }
 }
 
+// G and its top ghost layer are published in the context together, after both allocations and the
+// identity fills were issued, so a refused allocation leaves the context as it was and the call can be
+// retried.
 int ensure_gauge(glasma_ctx* ctx) {
-    if (ctx->G) return 0;
+    if (ctx->G && ctx->Gtop) return 0;
     const long Sp = pad_sites(ctx->g.S), Pp = pad_sites(ctx->g.P);
-    GL_HIP(ctx, hipMalloc(&ctx->G, (size_t)kGPlanes * Sp * sizeof(double)));
-    GL_HIP(ctx, hipMalloc(&ctx->Gtop, (size_t)kGPlanes * Pp * sizeof(double)));
-    hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->g.S);
-    hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.P), dim3(kCgBlock), 0, ctx->stream, ctx->Gtop, ctx->g.P);
-    GL_HIP(ctx, hipGetLastError());
+    double* G = nullptr;
+    double* Gtop = nullptr;
+    hipError_t e = hipMalloc(&G, (size_t)kGPlanes * Sp * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&Gtop, (size_t)kGPlanes * Pp * sizeof(double));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.S), dim3(kCgBlock), 0, ctx->stream, G, ctx->g.S);
+        hipLaunchKernelGGL(k_cg_identity, grid1(ctx->g.P), dim3(kCgBlock), 0, ctx->stream, Gtop, ctx->g.P);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        if (G) hipFree(G);
+        if (Gtop) hipFree(Gtop);
+        return hip_fail(ctx, e, "ensure_gauge");
+    }
+    ctx->G = G;
+    ctx->Gtop = Gtop;
     return 0;
 }
 
