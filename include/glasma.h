@@ -208,6 +208,13 @@ int glasma_set_gauge_transformation(glasma_ctx* ctx, const double* G);
  * sign -1 forward, +1 backward.  Runs on the current device; for tests and tools. */
 int glasma_fft_host(double* data, int nd, const int* dims, long batch, int sign);
 
+/* ---- bitwise state checksum (no counterpart in the reference; for run-to-run identity checks) ------- */
+/* out4 = checksums of the current links U, UOld, E and the four currents, each the wrapping 64-bit sum over
+ * every physical site and component of SplitMix64(bits ^ key(component, site)).  Two contexts on the same
+ * lattice hold bit-identical fields exactly when (up to a 2^-64 collision) their checksums agree; the value
+ * depends only on the field values, not on launch order or tile order. */
+int glasma_checksum(glasma_ctx* ctx, uint64_t* out4);
+
 /* ---- synthetic input (no counterpart in the reference; for benchmarks and tests) ---------------- */
 /* Fills the current links with exp(i link_scale alpha^a t^a), alpha_a uniform in [-1,1), and E, J+,
  * J-, J+stat, J-stat with uniform values in [-scale, scale), from a counter-based hash of (seed, site). */

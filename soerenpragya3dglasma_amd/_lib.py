@@ -74,6 +74,7 @@ def lib():
         "glasma_gauss": ([vp, dp, dp, dp], c_int),
         "glasma_unitarity": ([vp, dp, dp], c_int),
         "glasma_emt": ([vp, dp], c_int),
+        "glasma_checksum": ([vp, ctypes.POINTER(ctypes.c_uint64)], c_int),
         "glasma_field_profile": ([vp, dp], c_int),
         "glasma_current_profile": ([vp, dp], c_int),
         "glasma_transverse_energy_map": ([vp, dp], c_int),

@@ -1,5 +1,6 @@
 // libglasma core: context lifetime, field transfer between the reference AoS layout and the device
 // SoA layout, the time loop entry points and the observable reductions.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -500,6 +501,44 @@ __global__ void k_finalize(const double* __restrict__ red, double* __restrict__ 
     out[i] = acc;
 }
 
+// ------------------------------------------------------------------------------------------------
+// bitwise state checksum: sum over the physical sites' words of mix(bits ^ index-dependent key), wrapping
+// 64-bit addition.  Integer addition is associative and commutative, so the result does not depend on the
+// order of the per-block atomics; a change of any bit of any word changes the sum unless two 64-bit mixes
+// collide.  The word index is the (plane, site) position, so permutations are detected too.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {  // SplitMix64 finaliser (Steele et al.)
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) k_checksum(const double* __restrict__ f, int NP, long S,
+                                                  unsigned long long* __restrict__ out) {
+    const long n = (long)NP * pad_sites(S);
+    unsigned long long acc = 0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        // inverse of lidx: tile, plane pair, site in tile, member of the pair
+        const long tile = i / ((long)NP * kTile);
+        const int r = (int)(i - tile * NP * kTile);
+        const long s = tile * kTile + ((r & 127) >> 1);
+        if (s >= S) continue;
+        const int p = 2 * (r >> 7) + (r & 1);
+        const unsigned long long key = (unsigned long long)(s * NP + p) * 0x9e3779b97f4a7c15ull;
+        acc += mix64((unsigned long long)__double_as_longlong(f[i]) ^ key);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ unsigned long long sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, sh[0] + sh[1] + sh[2] + sh[3]);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 static int run_obs(glasma_ctx* ctx, void (*kern)(ObsArgs), int K, unsigned maxmask, std::vector<double>& per_z,
                    bool use_old) {
     const Geom& g = ctx->g;
@@ -757,6 +796,27 @@ int glasma_unitarity(glasma_ctx* ctx, double* umax, double* urms) {
     }
     if (umax) *umax = mx;
     if (urms) *urms = sqrt(sq / (3.0 * (double)ctx->g.S));
+    return 0;
+}
+
+int glasma_checksum(glasma_ctx* ctx, uint64_t* out4) {
+    if (!ctx || !out4) return GLASMA_ERR_ARG;
+    int rc = ensure_red(ctx, 4);
+    if (rc) return rc;
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(ctx->red);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ctx->stream, acc, 4);
+    const double* fields[4] = {ctx->U[ctx->cur], ctx->UOld, ctx->E, ctx->J[ctx->cur]};
+    const int np[4] = {kUPlanes, kUPlanes, kEPlanes, kJPlanes};
+    for (int k = 0; k < 4; k++) {
+        const long n = (long)np[k] * pad_sites(ctx->g.S);
+        const unsigned nb = (unsigned)std::min<long>(4096, (n + 255) / 256);
+        hipLaunchKernelGGL(k_checksum, dim3(nb), dim3(256), 0, ctx->stream, fields[k], np[k], ctx->g.S, acc + k);
+    }
+    GL_HIP(ctx, hipGetLastError());
+    unsigned long long h[4];
+    GL_HIP(ctx, hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; k++) out4[k] = h[k];
     return 0;
 }
 

@@ -240,6 +240,12 @@ class Glasma:
         self._chk(_lib.lib().glasma_unitarity(self._h, ctypes.byref(mx), ctypes.byref(rms)), "glasma_unitarity")
         return mx.value, rms.value
 
+    def checksum(self) -> tuple:
+        """Bitwise checksums (U, UOld, E, currents) of the device state, see glasma_checksum."""
+        out = (ctypes.c_uint64 * 4)()
+        self._chk(_lib.lib().glasma_checksum(self._h, out), "glasma_checksum")
+        return tuple(int(v) for v in out)
+
     def emt_profile(self) -> np.ndarray:
         out = np.zeros((self.N[2], 10))
         self._chk(_lib.lib().glasma_emt(self._h, _p(out)), "glasma_emt")
