@@ -106,6 +106,14 @@ int glasma_last_step_kernel_ms(const glasma_ctx* ctx, double* ms);
  * EDotJ[3] (raw sums; T00..SOURCE follow from BulkObservables.cpp:22-62).  GLASMA_ERR_NUMERIC (with the
  * sums filled in as computed) when a sum is not finite. */
 int glasma_bulk(glasma_ctx* ctx, double* out9);
+/* No counterpart in the reference: glasma_bulk for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same sums, bit for bit, from the same
+ * kernels; the block partials go to a fixed 8 MiB device array of the library (one per device, held until the
+ * process ends) instead of the context's reduction buffer, and no allocation is made on the success path.  The
+ * partials of one call take Nz * (ceil(Nx*Ny/256) + 1) * 9 + 9 doubles; a lattice needing more than 2^20 (e.g.
+ * 256 x 256 x 512) is refused with GLASMA_ERR_ARG.  Calls that overlap on one device (from several threads or
+ * contexts) share the array and race; the caller serialises them. */
+int glasma_bulk_noalloc(glasma_ctx* ctx, double* out9);
 /* GaussLaw::UpdateViolation (OBSERVABLES/GaussLawViolation.cpp:16-124): per_z[Nz] (may be NULL),
  * global max |G^a| and RMS. */
 int glasma_gauss(glasma_ctx* ctx, double* per_z, double* gmax, double* grms);

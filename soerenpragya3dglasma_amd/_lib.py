@@ -71,6 +71,7 @@ def lib():
         "glasma_synchronize": ([vp], c_int),
         "glasma_last_step_kernel_ms": ([vp, dp], c_int),
         "glasma_bulk": ([vp, dp], c_int),
+        "glasma_bulk_noalloc": ([vp, dp], c_int),
         "glasma_gauss": ([vp, dp, dp, dp], c_int),
         "glasma_unitarity": ([vp, dp, dp], c_int),
         "glasma_emt": ([vp, dp], c_int),

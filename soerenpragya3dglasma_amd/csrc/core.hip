@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -539,6 +540,46 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Fixed device scratch of glasma_bulk_noalloc: the block partials, the per-layer sums and the 9 totals.
+// Every device the library runs on holds its own copy of the array, at its own address, until the process
+// ends.  Host code never passes the array's name to a kernel (on the host it only stands for the symbol):
+// it uses the device address that hipGetSymbolAddress returns, looked up once per device with that device
+// current.
+// ------------------------------------------------------------------------------------------------
+constexpr long kScratchDoubles = 1L << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(glasma_ctx* ctx, double** out) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return set_error(ctx, GLASMA_ERR_ARG, "glasma_bulk_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[ctx->device]) {
+        int cur = 0;
+        GL_HIP(ctx, hipGetDevice(&cur));
+        if (cur != ctx->device) GL_HIP(ctx, hipSetDevice(ctx->device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != ctx->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipGetSymbolAddress(scratch)");
+        g_scratch[ctx->device] = static_cast<double*>(p);
+    }
+    *out = g_scratch[ctx->device];
+    return 0;
+}
+
+// out[k] = the per-layer sums fin[z*K + k] added in increasing z, the order of glasma_bulk's host loop
+__global__ void k_sum_layers(const double* __restrict__ fin, double* __restrict__ out, int nz, int K) {
+    const int k = threadIdx.x;
+    if (k >= K) return;
+    double acc = 0.0;
+    for (int z = 0; z < nz; z++) acc += fin[(long)z * K + k];
+    out[k] = acc;
+}
+
 static int run_obs(glasma_ctx* ctx, void (*kern)(ObsArgs), int K, unsigned maxmask, std::vector<double>& per_z,
                    bool use_old) {
     const Geom& g = ctx->g;
@@ -761,6 +802,45 @@ int glasma_bulk(glasma_ctx* ctx, double* out9) {
     for (int k = 0; k < 9; k++) out9[k] = 0.0;
     for (int z = 0; z < ctx->g.nz; z++)
         for (int k = 0; k < 9; k++) out9[k] += pz[(size_t)z * 9 + k];
+    for (int k = 0; k < 9; k++)
+        if (!std::isfinite(out9[k]))
+            return set_error(ctx, GLASMA_ERR_NUMERIC, "non-finite fields: the evolution diverged or a step's current "
+                                                      "hand-over timed out (sums returned as computed)");
+    return 0;
+}
+
+int glasma_bulk_noalloc(glasma_ctx* ctx, double* out9) {
+    if (!ctx || !out9) return GLASMA_ERR_ARG;
+    const Geom& g = ctx->g;
+    const int bpp = (int)((g.P + 255) / 256);
+    const long need = (long)g.nz * bpp * 9 + (long)g.nz * 9 + 9;
+    if (need > kScratchDoubles) {
+        char msg[200];
+        std::snprintf(msg, sizeof(msg), "glasma_bulk_noalloc: this lattice needs %ld doubles of scratch, more than the "
+                      "fixed %ld (use glasma_bulk)", need, kScratchDoubles);
+        return set_error(ctx, GLASMA_ERR_ARG, msg);
+    }
+    double* ptr = nullptr;
+    int rc = scratch_ptr(ctx, &ptr);
+    if (rc) return rc;
+    // the kernels write every entry they later read (all blocks store their partials), so nothing is cleared
+    ObsArgs A;
+    A.U = ctx->U[ctx->cur];
+    A.UOld = nullptr;
+    A.E = ctx->E;
+    A.J = ctx->J[ctx->cur];
+    A.red = ptr;
+    A.g = g;
+    A.bpp = bpp;
+    for (int m = 0; m < 3; m++) A.a[m] = ctx->p.a[m];
+    double* fin = ptr + (long)g.nz * bpp * 9;
+    double* tot = fin + (long)g.nz * 9;
+    hipLaunchKernelGGL(k_bulk, dim3(bpp, g.nz), dim3(256), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_finalize, dim3((g.nz * 9 + 255) / 256), dim3(256), 0, ctx->stream, ptr, fin, g.nz, bpp, 9, 0u);
+    hipLaunchKernelGGL(k_sum_layers, dim3(1), dim3(64), 0, ctx->stream, fin, tot, g.nz, 9);
+    GL_HIP(ctx, hipGetLastError());
+    GL_HIP(ctx, hipMemcpyAsync(out9, tot, 9 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int k = 0; k < 9; k++)
         if (!std::isfinite(out9[k]))
             return set_error(ctx, GLASMA_ERR_NUMERIC, "non-finite fields: the evolution diverged or a step's current "
