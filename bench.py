@@ -44,6 +44,18 @@ def measured_traffic():
     return None
 
 
+def host_cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sample_n, steps):
     """The oracle (CPU restatement, OpenMP) on a bounded sample of the same workload."""
     from oracle import oracle as O
@@ -64,6 +76,8 @@ def cpu_baseline(sample_n, steps):
         "value": sites * steps / dt,
         "unit": "site-updates/s",
         "cores": threads,
+        "cpu_model": host_cpu_model(),
+        "logical_cpus": os.cpu_count(),
         "kind": "port",
         "sample": f"oracle (C++/OpenMP restatement of Dynamics::SUNc::Update) on a {sample_n[0]}x{sample_n[1]}x"
                   f"{sample_n[2]} slab, {steps} timed steps after 1 warm-up ({dt:.1f} s)",

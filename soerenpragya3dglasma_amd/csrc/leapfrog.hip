@@ -11,9 +11,11 @@
 // HBM traffic per site-update: read U (432 B) + E (192 B) + J (256 B), write the same: 1760 B, the
 // compulsory minimum.  The neighbour link matrices are re-read through L2 / MALL, so the order in
 // which the 8 XCDs (each with its own 4 MB L2) visit the lattice matters: workgroups are dealt to the
-// XCDs round-robin, and with the default column-band order (TileMap mode 2 in site.h) each XCD walks
-// its bands of 4 x 4 tiles through z, so the y and z neighbours of a tile are read by the same XCD
-// shortly before or after it (L2 hits); HBM reads come to 1.25x the compulsory bytes.
+// XCDs round-robin, and with the column-band orders (TileMap mode 2 in site.h) each XCD walks its bands
+// through z, so the y and z neighbours of a tile are read by the same XCD shortly before or after it (L2
+// hits).  Measured at 256 x 256 x 1024 with the tuned 1 x 16 bands and three waves per SIMD, reads come to
+// 1.40x the compulsory bytes (the y-halo rows of a band belong to another XCD, and the tiles in flight
+// outgrow an XCD's 4 MB L2); writes are exactly compulsory.
 //
 // GLASMA_STEP_WAVES (2 or 3: register budget of 256 or 168 per lane), GLASMA_STEP_FOUR=2 (plain form only) and
 // GLASMA_STEP_MAP / _BW / _BH select variants for measurement sweeps; the defaults are the fastest

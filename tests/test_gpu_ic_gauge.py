@@ -201,9 +201,10 @@ def test_coulomb_gauge_fixed_iterations_match_oracle(N, a, iters):
 
 def test_coulomb_gauge_walking_z_pass_matches_oracle():
     """At 32 x 32 x 1024 the fused z pass (forward transform, 1/(V p^2), inverse transform) of the Fourier update
-    runs one workgroup per CU and each block walks several line groups (1024 groups of 4 lines over at most 256
-    blocks), which the smaller lattices above do not reach.  Fields from the device's random fill, the same
-    arrays given to the oracle; two iterations, Fourier tolerances."""
+    runs two workgroups per CU (74 KiB of LDS each) and each block walks several line groups: the four packed
+    deviation arrays give 4096 lines of length 1024, in 2048 groups of 2 lines over at most 512 blocks, four
+    groups per block, which the smaller lattices above do not reach.  Fields from the device's random fill, the
+    same arrays given to the oracle; two iterations, Fourier tolerances."""
     from soerenpragya3dglasma_amd import Glasma
     N, a, iters = (32, 32, 1024), A1, 2
     with Glasma(N, a, 0.1, 0) as g:
