@@ -191,7 +191,15 @@ int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, d
  * (ALGORITHMS/Jacobi.cpp:13-74, param = step size, 0.3 in the reference) or GLASMA_GAUGE_LOS_ALAMOS
  * (ALGORITHMS/LosAlamos.cpp:13-87, even then odd checkerboard sites, MaxTraceProjection of w; param = the
  * over-relaxation probability, which must be 0, the reference's value). */
-enum { GLASMA_GAUGE_FOURIER = 0, GLASMA_GAUGE_JACOBI = 1, GLASMA_GAUGE_LOS_ALAMOS = 2 };
+/* GLASMA_GAUGE_FOURIER_CONSISTENT (param = alpha, 2): no counterpart in the reference, whose Fourier iteration cannot
+ * converge for Nz > 1 (periodic z momenta despite open z, the gauge-fixed links' z = -1 ghost left at the identity,
+ * the zero mode passed through unscaled).  w takes only the links that exist (open z: none below layer 0 and none
+ * from layer Nz - 1 to its ghost copy; periodic z: the wrapped links, with the current G at both ends), the Poisson
+ * step inverts the Laplacian of that graph (open z: Neumann, as the periodic transform of the lattice doubled by its
+ * mirror image in z; periodic z: periodic), and the zero mode is dropped; it reaches tolerances of 1e-10 and below.
+ * Every algorithm returns 0 at the iteration cap as the reference does; when the cap is reached above a tolerance
+ * tol > 0, glasma_last_error says so. */
+enum { GLASMA_GAUGE_FOURIER = 0, GLASMA_GAUGE_JACOBI = 1, GLASMA_GAUGE_LOS_ALAMOS = 2, GLASMA_GAUGE_FOURIER_CONSISTENT = 3 };
 int glasma_coulomb_gauge_algorithm(glasma_ctx* ctx, int algorithm, double tol, int maxit, double param,
                                    double* stats7);
 /* Gauss-law restoration (GAUSSLAWFIXING/GaussRestore.cpp:134-182; transverse_only != 0:

@@ -1352,13 +1352,22 @@ static void MaxTraceProjection(const cd* V, cd* VT) {
 //   algo 2  LosAlamosAlgorithm (ALGORITHMS/LosAlamos.cpp:13-87): even then odd checkerboard sites ((x+y+z)
 //           % 2), g <- MaxTraceProjection(w)^dag g, the links re-transformed after each half sweep; the
 //           monitors accumulate over both halves.  Over-relaxation probability 0 (the reference's value).
+//   algo 3  the Fourier update made consistent with the lattice's z boundary (no counterpart in the reference,
+//           whose iteration cannot converge for Nz > 1): w takes only the links that exist (open z: no link
+//           below layer 0 and none from layer Nz - 1 to its ghost copy; periodic z: the wrapped links), the links
+//           are transformed with G's ghost layers synchronised first, the Poisson step inverts the Laplacian of
+//           the same graph (open z: Neumann, done as the periodic transform of the lattice doubled by the mirror
+//           image z -> 2 Nz - 1 - z, whose symmetric modes are the cosine modes with p_z^2 = 2 - 2 cos(pi k /
+//           Nz)), and the zero mode is dropped.  param = alpha (2 is the Newton step of the abelian limit).
 // Monitors (CoulombGaugeDeviation.cpp:43-70): max / RMS of Delta, max / RMS of |tr(1 - wTilde)|, min / mean
 // of Re tr w.  The Jacobi sweep's unreduced MaxStepSize (a data race in the reference) is taken as the max.
 static void CoulombGaugeAlgo(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG, double* EG, int algo, double tol, int maxit, double param, double* stats) {
     int Nx = L.N[0], Ny = L.N[1], Nz = L.N[2];
     long np = L.plane(), V = np * Nz;
     auto syncG = [&]() { L.sync(Gt, 9L); };
+    const bool cons = algo == 3;
     auto transform_links = [&]() {  // GaugeTransformLinks(UOld=U, UNew=UG, G)
+        if (cons) syncG();
 #pragma omp parallel for
         for (int z = 0; z < Nz; z++)
             for (int y = 0; y < Ny; y++)
@@ -1370,6 +1379,7 @@ static void CoulombGaugeAlgo(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG
                     SU3::UU(g, L.Ul(U, x, y, z, 2), t); SU3::UD(t, Gt + 9 * L.idx3(x, y, z + 1), L.Ul(UG, x, y, z, 2));
                 }
         syncG();
+        if (cons) L.syncU(UG);  // the wrapped links of a periodic lattice are read from the ghost layer
     };
     // GET_LOCAL_GAUGE_VIOLATION (CoulombGaugeDeviation.cpp:23-41): w and Delta^a = Re tr(i t^a w)
     auto local_w = [&](int x, int y, int z, cd* w, double* dv) {
@@ -1381,8 +1391,12 @@ static void CoulombGaugeAlgo(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG
         const cd* u1 = L.Ul(UG, x, y, z, 1);
         const cd* u2 = L.Ul(UG, x, y, z, 2);
         double zf = (Nz > 1) ? 1.0 : 0.0;
+        // the consistent update: only links that exist (open z: none below layer 0, none from the top layer)
+        const double zu = !cons ? zf : (Nz > 1 && (L.bc == 1 || z + 1 < Nz) ? 1.0 : 0.0);
+        const double zd = !cons ? zf : (Nz > 1 && (L.bc == 1 || z > 0) ? 1.0 : 0.0);
         for (int k = 0; k < 9; k++)
-            w[k] = SQR(1.0 / L.a[0]) * (u0[k] + d0[k]) + SQR(1.0 / L.a[1]) * (u1[k] + d1[k]) + zf * SQR(1.0 / L.a[2]) * (u2[k] + d2[k]);
+            w[k] = SQR(1.0 / L.a[0]) * (u0[k] + d0[k]) + SQR(1.0 / L.a[1]) * (u1[k] + d1[k]) +
+                   SQR(1.0 / L.a[2]) * (zu * u2[k] + zd * d2[k]);
         SU3::ReTrIGenU(w, dv);
     };
     // CopyFields: only the interior (the ghosts of UG keep their identity / previous contents)
@@ -1441,6 +1455,31 @@ static void CoulombGaugeAlgo(const Lat<SU3>& L, cd* U, double* E, cd* Gt, cd* UG
                         long s = x + (long)Nx * (y + (long)Ny * z);
                         for (int a = 0; a < 8; a++) F[a + 8 * s] = dv[a];
                     }
+            if (algo == 3) {
+                // open z with Nz > 1: the lattice doubled by its mirror image in z; else the lattice itself
+                const bool mirror = L.bc == 0 && Nz > 1;
+                const int Nz2 = mirror ? 2 * Nz : Nz;
+                const long V2 = np * Nz2;
+                std::vector<cd> D(8 * V2);
+                for (int z = 0; z < Nz2; z++) {
+                    const int zs = z < Nz ? z : 2 * Nz - 1 - z;
+                    std::memcpy(D.data() + 8 * np * z, F.data() + 8 * np * zs, 8 * np * sizeof(cd));
+                }
+                int dims2[3] = {Nx, Ny, Nz2};
+                fft_nd(D.data(), 3, dims2, 8, -1);
+                const double nf = 1.0 / (double)V2;
+                for (int pz = 0; pz < Nz2; pz++)
+                    for (int py = 0; py < Ny; py++)
+                        for (int px = 0; px < Nx; px++) {
+                            const double p2 = (2.0 - 2.0 * std::cos(2.0 * PI * px / (double)Nx)) / SQR(L.a[0]) +
+                                              (2.0 - 2.0 * std::cos(2.0 * PI * py / (double)Ny)) / SQR(L.a[1]) +
+                                              (Nz > 1 ? (2.0 - 2.0 * std::cos(2.0 * PI * pz / (double)Nz2)) / SQR(L.a[2]) : 0.0);
+                            const long s = px + (long)Nx * (py + (long)Ny * pz);
+                            for (int a = 0; a < 8; a++) D[a + 8 * s] = p2 > 0.0 ? nf * D[a + 8 * s] / p2 : cd(0.0);
+                        }
+                fft_nd(D.data(), 3, dims2, 8, +1);
+                std::memcpy(F.data(), D.data(), 8 * V * sizeof(cd));
+            }
             if (algo == 0) {
                 fft_nd(F.data(), 3, dims, 8, -1);
                 double nf = 1.0 / ((double)V);

@@ -311,7 +311,7 @@ def coulomb_gauge(f: Fields, a, G: np.ndarray, tol: float, maxit: int, alpha: fl
     return UG, EG, stats
 
 
-ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2}
+ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2, "consistent": 3}
 
 
 def coulomb_gauge_algo(f: Fields, a, G: np.ndarray, algo: str, tol: float, maxit: int, param: float, bc: int = 0):

@@ -201,6 +201,7 @@ struct LineFilter {
     int on, nx, ny;
     long P;  // transverse sites: the line's site is line % P
     double ax, ay, az, nf;
+    int drop_zero;  // the zero mode set to 0 (else passed through unscaled)
 };
 
 // One launch transforms `nlines` lines of length n in groups of lpb lines.
@@ -321,6 +322,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
                         v.x = filt.nf * v.x / p2;
                         v.y = filt.nf * v.y / p2;
                         fb[myli * ld + pz] = v;
+                    } else if (filt.drop_zero) {
+                        fb[myli * ld + pz] = dbl2{0.0, 0.0};
                     }
                 }
             }
@@ -532,7 +535,7 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
 }  // namespace
 
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
-    const LineFilter none = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0};
+    const LineFilter none = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0};
     return fft_lines_filtered(st, data, n, es, nlines, b1, s1, s2, sign, none);
 }
 
@@ -555,10 +558,10 @@ int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign) {
     return fft_z(st, c, g, ncomp, sign);
 }
 
-int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3]) {
+int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3], bool drop_zero) {
     if (g.nz < 2) return fft_fail(GLASMA_ERR_ARG, "the fused Poisson filter needs Nz >= 2");
     const long P = (long)g.nx * g.ny;
-    const LineFilter f = {1, g.nx, g.ny, P, a[0], a[1], a[2], 1.0 / (double)g.S};
+    const LineFilter f = {1, g.nx, g.ny, P, a[0], a[1], a[2], 1.0 / (double)g.S, drop_zero ? 1 : 0};
     int rc = fft_xy(st, c, g, ncomp, g.nz, -1);
     if (!rc) rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
     if (!rc) rc = fft_xy(st, c, g, ncomp, g.nz, +1);

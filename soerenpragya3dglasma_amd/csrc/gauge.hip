@@ -30,6 +30,7 @@
 // after each link transformation, so a transformation sees the top layer of the previous iteration.  Here
 // that layer is the separate array Gtop (18 planes over one transverse plane).
 #include <cmath>
+#include <cstdio>
 #include <vector>
 
 #include "internal.h"
@@ -136,6 +137,10 @@ __device__ __forceinline__ void local_w(const double* __restrict__ UG, const Geo
 // local_w with the gauge-fixed links formed on the fly from U and G (the Fourier and Jacobi iterations): the
 // iteration then needs no separate link transformation writing U^G and reading it back.  Each link is formed with
 // the two products of k_cg_transform in the same order, so the values are the same.
+// CONS: the boundary-consistent update (no counterpart in the reference): only links that exist enter w (open z:
+// none from the top layer to its ghost copy and none below layer 0; periodic z: the wrapped links, with the
+// current G on both ends instead of the reference's lagging ghost layer and identity below layer 0).
+template <bool CONS>
 __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, const double* __restrict__ G,
                                               const double* __restrict__ Gtop, const Geom& g, int x, int y, int z,
                                               double cx, double cy, double cz, M3& W) {
@@ -147,20 +152,31 @@ __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, cons
     for (int mu = 0; mu < 3; mu++) {
         const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
         // A = U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu)
-        ldm<long>(U, kUPlanes, 18 * mu, s, Um);
-        if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
-        else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
-        else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
-        else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
-        mul_nn(Gx, Um, T);
-        mul_nd(T, Gn, A);
-        // B = U^G_mu(x - mu) = G(x - mu) U_mu(x - mu) G^dag(x); below z = 0 the identity
-        if (mu < 2 || z > 0) {
-            const long sb = mu == 0 ? sidx(g, x - 1, y, z) : (mu == 1 ? sidx(g, x, y - 1, z) : s - g.P);
+        const bool has_up = !CONS || mu < 2 || (g.nz > 1 && (g.bc || z + 1 < g.nz));
+        if (has_up) {
+            ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+            if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
+            else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
+            else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
+            else if (CONS) ldm<long>(G, kGPlanes, 0, s - (long)z * g.P, Gn);  // periodic: layer 0 now
+            else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
+            mul_nn(Gx, Um, T);
+            mul_nd(T, Gn, A);
+        } else {
+            for (int k = 0; k < 9; k++) { A.re[k] = 0.0; A.im[k] = 0.0; }
+        }
+        // B = U^G_mu(x - mu) = G(x - mu) U_mu(x - mu) G^dag(x); below z = 0 the identity (CONS: none for open z,
+        // the top layer's link for periodic z)
+        const bool has_dn = mu < 2 || z > 0 || (CONS && g.bc && g.nz > 1);
+        if (has_dn) {
+            const long sb = mu == 0 ? sidx(g, x - 1, y, z)
+                                    : (mu == 1 ? sidx(g, x, y - 1, z) : (z > 0 ? s - g.P : s + (long)(g.nz - 1) * g.P));
             ldm<long>(G, kGPlanes, 0, sb, Gn);
             ldm<long>(U, kUPlanes, 18 * mu, sb, Um);
             mul_nn(Gn, Um, T);
             mul_nd(T, Gx, B);
+        } else if (CONS) {
+            for (int k = 0; k < 9; k++) { B.re[k] = 0.0; B.im[k] = 0.0; }
         } else {
             set_unit(B);
         }
@@ -173,11 +189,14 @@ __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, cons
 }
 
 // Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52), with the monitors,
-// from the ungauged links and G
+// from the ungauged links and G.  The four packed arrays are cs complex entries apart; with mirror, layer z is also
+// written to layer 2 Nz - 1 - z of the arrays (the lattice doubled by its mirror image, for the open-z Poisson step
+// of the consistent update).
+template <bool CONS>
 __global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restrict__ U, const double* __restrict__ G,
                                                             const double* __restrict__ Gtop, double* __restrict__ F,
                                                             double* __restrict__ part, Geom g, double cx, double cy,
-                                                            double cz, double fm0) {
+                                                            double cz, double fm0, long cs, int mirror) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0;
     if (s < g.S) {
@@ -185,7 +204,7 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restr
         const long s2 = s - (long)z * g.P;
         const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
         M3 W;
-        local_w_fused(U, G, Gtop, g, x, y, z, cx, cy, cz, W);
+        local_w_fused<CONS>(U, G, Gtop, g, x, y, z, cx, cy, cz, W);
         double dv[8];
         retr_igen(W, dv);
         for (int a = 0; a < 8; a++) {
@@ -194,8 +213,15 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restr
         }
         // two real components per complex array: Delta^{2k} + i Delta^{2k+1} in array k
         for (int k = 0; k < 4; k++) {
-            F[2 * ((long)k * g.S + s)] = dv[2 * k];
-            F[2 * ((long)k * g.S + s) + 1] = dv[2 * k + 1];
+            F[2 * ((long)k * cs + s)] = dv[2 * k];
+            F[2 * ((long)k * cs + s) + 1] = dv[2 * k + 1];
+        }
+        if (mirror) {
+            const long sm = s2 + (long)(2 * g.nz - 1 - z) * g.P;
+            for (int k = 0; k < 4; k++) {
+                F[2 * ((long)k * cs + sm)] = dv[2 * k];
+                F[2 * ((long)k * cs + sm) + 1] = dv[2 * k + 1];
+            }
         }
         const double lf = W.re[0] + W.re[4] + W.re[8];
         fm = fmin(fm, lf);
@@ -285,8 +311,9 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_losalamos(const double* __restr
     block_partials(ssq, 0.0, sm, 0.0, spart + 4 * (long)blockIdx.x);
 }
 
-// Fourier acceleration: F *= 1/(V p^2) for p^2 > 0 (FourierAcceleration.cpp:55-75)
-__global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay, double az) {
+// Fourier acceleration: F *= 1/(V p^2) for p^2 > 0 (FourierAcceleration.cpp:55-75); the zero mode is passed
+// through, or set to 0 with drop_zero (the consistent update)
+__global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay, double az, int drop_zero) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.S) return;
     const int pz = (int)(s / g.P);
@@ -295,7 +322,15 @@ __global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay,
     const double p2 = (2.0 - 2.0 * cos(2.0 * kPi * px / (double)g.nx)) / (ax * ax) +
                       (2.0 - 2.0 * cos(2.0 * kPi * py / (double)g.ny)) / (ay * ay) +
                       (2.0 - 2.0 * cos(2.0 * kPi * pz / (double)g.nz)) / (az * az);
-    if (!(p2 > 0.0)) return;
+    if (!(p2 > 0.0)) {
+        if (drop_zero)
+            for (int a = 0; a < 4; a++) {
+                const long i = (long)a * g.S + s;
+                F[2 * i] = 0.0;
+                F[2 * i + 1] = 0.0;
+            }
+        return;
+    }
     const double nf = 1.0 / (double)g.S;
     for (int a = 0; a < 4; a++) {  // the four packed arrays
         const long i = (long)a * g.S + s;
@@ -306,12 +341,12 @@ __global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay,
 
 // g <- exp(-i alpha Delta t)^dag g and the step size |3 - tr exp(...)| (FourierAcceleration.cpp:80-140)
 __global__ __launch_bounds__(kCgBlock) void k_cg_update(const double* __restrict__ F, double* __restrict__ G,
-                                                         double* __restrict__ part, Geom g, double alpha) {
+                                                         double* __restrict__ part, Geom g, double alpha, long cs) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double ssq = 0.0, sm = 0.0;
     if (s < g.S) {
         double ub[8];
-        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * g.S + s) + (a & 1)];  // packed as by k_cg_dev_fused
+        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * cs + s) + (a & 1)];  // packed as by k_cg_dev_fused
         M3 Wt, Gx, Gn;
         expi(-alpha, ub, Wt);
         const double tr = 3.0 - (Wt.re[0] + Wt.re[4] + Wt.re[8]);
@@ -463,7 +498,7 @@ namespace {
 int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param, double* stats7) {
     if (!ctx) return GLASMA_ERR_ARG;
     if (maxit < 0 || !(tol >= 0.0)) return set_error(ctx, GLASMA_ERR_ARG, "invalid tolerance or iteration cap");
-    if (algo < GLASMA_GAUGE_FOURIER || algo > GLASMA_GAUGE_LOS_ALAMOS)
+    if (algo < GLASMA_GAUGE_FOURIER || algo > GLASMA_GAUGE_FOURIER_CONSISTENT)
         return set_error(ctx, GLASMA_ERR_ARG, "unknown gauge-fixing algorithm");
     if (algo == GLASMA_GAUGE_LOS_ALAMOS && param != 0.0)
         return set_error(ctx, GLASMA_ERR_ARG,
@@ -474,8 +509,13 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     if (rc) return rc;
     const Geom& g = ctx->g;
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
+    const bool cons = algo == GLASMA_GAUGE_FOURIER_CONSISTENT;
+    // the consistent update on an open lattice with Nz > 1 inverts the Neumann Laplacian in z as the periodic one of
+    // the lattice doubled by its mirror image: the four packed arrays then hold 2 Nz layers
+    const bool mirror = cons && g.bc == 0 && g.nz > 1;
+    const long cs = mirror ? 2 * g.S : g.S;  // complex entries between the packed arrays
     rc = ensure_red(ctx, 16 * (size_t)nblk + 8);
-    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)g.S);  // 4 complex per site
+    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)cs);  // 4 complex per site
     if (rc) return rc;
     const double* a = ctx->p.a;
     // weights (1/a_mu)^2 of w and the functional's starting minimum 6 sum_mu 1/a_mu^2, rounded as the
@@ -505,10 +545,35 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, 2 * nblk, fin, fm0);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, spart, 2 * nblk, fin + 4, 0.0);
         } else {
-            hipLaunchKernelGGL(k_cg_dev_fused, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur],
-                               ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0);
-            hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
+            if (cons) {
+                hipLaunchKernelGGL(k_cg_dev_fused<true>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,
+                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, cs, mirror ? 1 : 0);
+            } else {
+                hipLaunchKernelGGL(k_cg_dev_fused<false>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,
+                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, cs, 0);
+                hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
+            }
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
+            if (cons) {
+                // the Laplacian of the links that exist; the zero mode (zero by construction) dropped
+                if (mirror) {
+                    Geom gd = g;
+                    gd.nz = 2 * g.nz;
+                    gd.S = 2 * g.S;
+                    gd.bc = 1;
+                    rc = fft_3d_poisson(ctx->stream, F, gd, 4, a, true);
+                } else if (g.nz >= 2) {
+                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a, true);
+                } else {
+                    rc = fft_3d(ctx->stream, F, g, 4, -1);
+                    if (!rc) {
+                        hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1],
+                                           a[2], 1);
+                        rc = fft_3d(ctx->stream, F, g, 4, +1);
+                    }
+                }
+                if (rc) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
+            }
             if (algo == GLASMA_GAUGE_FOURIER) {
                 if (g.nz >= 2) {
                     rc = fft_3d_poisson(ctx->stream, F, g, 4, a);
@@ -516,14 +581,14 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
                     rc = fft_3d(ctx->stream, F, g, 4, -1);
                     if (!rc) {
                         hipLaunchKernelGGL(k_cg_scale, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, F, g, a[0], a[1],
-                                           a[2]);
+                                           a[2], 0);
                         rc = fft_3d(ctx->stream, F, g, 4, +1);
                     }
                 }
                 if (rc) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
             }
             hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g,
-                               param);
+                               param, cs);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
             GL_HIP(ctx, hipGetLastError());
         }
@@ -544,6 +609,14 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     hipLaunchKernelGGL(k_cg_efield, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->E, ctx->EG, g);
     GL_HIP(ctx, hipGetLastError());
     GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // the iteration cap reached above the tolerance: success as the reference's SetCoulombGauge returns, with a
+    // message in glasma_last_error for callers that check (the reference's update cannot converge for Nz > 1)
+    if (n >= maxit && gmax > tol && tol > 0.0) {
+        char msg[200];
+        std::snprintf(msg, sizeof(msg), "gauge fixing stopped at the iteration cap %d with max|Delta| = %.3e above the "
+                      "tolerance %.3e", maxit, gmax, tol);
+        ctx->err = msg;
+    }
     if (stats7) {
         stats7[0] = n;
         stats7[1] = gmax;

@@ -116,8 +116,9 @@ int fft_z(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);
 int fft_xy(hipStream_t st, double* c, const Geom& g, int ncomp, int nzl, int sign);
 int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);
 // forward 3D transform, c~ -> c~ / (V p^2) for p^2 = sum_mu (2 - 2 cos(2 pi k_mu / N_mu)) / a_mu^2 > 0 (the zero
-// mode unscaled), inverse 3D transform; the z transforms and the filter run in one pass (Nz >= 2)
-int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3]);
+// mode unscaled, or set to 0 with drop_zero), inverse 3D transform; the z transforms and the filter run in one pass
+// (Nz >= 2)
+int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3], bool drop_zero = false);
 
 }  // namespace glasma
 

@@ -378,21 +378,30 @@ class Glasma:
     def coulomb_gauge(self, tol: float = 1e-15, maxit: int = 1000, alpha: float = 2.0) -> np.ndarray:
         stats = np.zeros(7)
         self._chk(_lib.lib().glasma_coulomb_gauge(self._h, tol, int(maxit), alpha, _p(stats)), "glasma_coulomb_gauge")
+        self._warn_cap(stats, tol, maxit)
         return stats
 
-    GAUGE_ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2}
+    GAUGE_ALGORITHMS = {"fourier": 0, "jacobi": 1, "losalamos": 2, "consistent": 3}
 
     def coulomb_gauge_algorithm(self, algorithm: str, tol: float = 1e-15, maxit: int = 1000,
                                 param: float = None) -> np.ndarray:
         """Coulomb gauge fixing with the reference's "fourier" (param = alpha, 2), "jacobi" (param = step size,
-        0.3) or "losalamos" (param = over-relaxation probability, 0) update."""
+        0.3) or "losalamos" (param = over-relaxation probability, 0) update, or the boundary-consistent Fourier
+        update "consistent" (param = alpha, 2; no counterpart in the reference), which converges for Nz > 1.
+        A RuntimeWarning is issued when the iteration cap is reached above a tolerance tol > 0."""
         algo = self.GAUGE_ALGORITHMS[algorithm]
         if param is None:
-            param = (2.0, 0.3, 0.0)[algo]
+            param = (2.0, 0.3, 0.0, 2.0)[algo]
         stats = np.zeros(7)
         self._chk(_lib.lib().glasma_coulomb_gauge_algorithm(self._h, algo, tol, int(maxit), float(param), _p(stats)),
                   "glasma_coulomb_gauge_algorithm")
+        self._warn_cap(stats, tol, maxit)
         return stats
+
+    def _warn_cap(self, stats, tol, maxit):
+        if tol > 0 and stats[0] >= maxit and stats[1] > tol:
+            import warnings
+            warnings.warn(_lib.lib().glasma_last_error(self._h).decode(), RuntimeWarning, stacklevel=3)
 
     def gauss_restore(self, transverse_only: bool = False, tol: float = 1e-14, maxit: int = 30000) -> np.ndarray:
         """GaussLawRestoration::Restore (or the 2D variant): relax E toward the Gauss law in place; returns

@@ -186,6 +186,119 @@ def test_fourier_iterations_against_numpy_nz_above_one(N, bc):
     assert np.abs(st - stn).max() < 1e-12 * max(1.0, np.abs(stn).max()), (st, stn)
 
 
+def _consistent_iteration_np(f, a, G0, alpha, bc):
+    """One iteration of the boundary-consistent Fourier update in numpy, with its Poisson step written independently
+    of the oracle's mirror-doubled transform: open z uses the orthonormal DCT-II / DCT-III (scipy.fft) along z,
+    whose basis cos(pi k (z + 1/2) / Nz) diagonalises the Neumann Laplacian of the open chain with eigenvalues
+    2 - 2 cos(pi k / Nz); periodic z uses the FFT.  w takes only existing links, G's ghost layers are synchronised
+    before the links are transformed, the zero mode is dropped, g <- exp(-i alpha Delta' t)^dag g.
+    Returns G after the iteration [z+1, y, x, i, j] and the monitors (max / avg deviation, max / avg step,
+    min / mean functional)."""
+    import scipy.fft as sf
+    Nx, Ny, Nz = f.N
+    V = Nx * Ny * Nz
+    U = np.swapaxes(f.U, -1, -2)[1:-1]
+    g = np.swapaxes(G0, -1, -2).copy()
+    g[0], g[-1] = (g[1], g[-2]) if bc == 0 else (g[-2], g[1])
+    dag = lambda M: np.conj(np.swapaxes(M, -1, -2))  # noqa: E731
+    gi = g[1:-1]
+    nb = (np.roll(gi, -1, axis=2), np.roll(gi, -1, axis=1), g[2:])
+    UG = [gi @ U[:, :, :, mu] @ dag(nb[mu]) for mu in range(3)]
+    c = [1 / a[0] ** 2, 1 / a[1] ** 2, 1 / a[2] ** 2]
+    w = c[0] * (UG[0] + dag(np.roll(UG[0], 1, axis=2))) + c[1] * (UG[1] + dag(np.roll(UG[1], 1, axis=1)))
+    if Nz > 1:
+        up = UG[2].copy()
+        dn = dag(np.roll(UG[2], 1, axis=0))
+        if bc == 0:
+            up[-1] = 0.0  # no link from the top layer
+            dn[0] = 0.0   # no link below layer 0
+        w = w + c[2] * (up + dn)
+    dev = np.stack([np.trace(1j * T[k] @ w, axis1=-2, axis2=-1).real for k in range(8)], axis=-1)
+    fun = np.trace(w, axis1=-2, axis2=-1).real
+    kx = 2 * np.pi * np.arange(Nx) / Nx
+    ky = 2 * np.pi * np.arange(Ny) / Ny
+    pt = (2 - 2 * np.cos(kx))[None, None, :] / a[0] ** 2 + (2 - 2 * np.cos(ky))[None, :, None] / a[1] ** 2
+    if bc == 0:
+        kz = np.pi * np.arange(Nz) / Nz
+        H = sf.fft2(sf.dct(dev, type=2, norm="ortho", axis=0), axes=(1, 2))
+    else:
+        kz = 2 * np.pi * np.arange(Nz) / Nz
+        H = sf.fftn(dev, axes=(0, 1, 2))
+    p2 = pt + ((2 - 2 * np.cos(kz)) / a[2] ** 2 if Nz > 1 else 0.0 * kz)[:, None, None]
+    H[p2 > 0] /= p2[p2 > 0][:, None]
+    H[p2 == 0] = 0.0
+    if bc == 0:
+        upd = sf.idct(sf.ifft2(H, axes=(1, 2)).real, type=2, norm="ortho", axis=0)
+    else:
+        upd = sf.ifftn(H, axes=(0, 1, 2)).real
+    Wt = np.array([sl.expm(-1j * alpha * herm(u)) for u in upd.reshape(-1, 8)]).reshape(Nz, Ny, Nx, 3, 3)
+    step = np.abs(np.trace(np.eye(3) - Wt, axis1=-2, axis2=-1))
+    g[1:-1] = dag(Wt) @ g[1:-1]
+    mon = np.array([np.abs(dev).max(), np.sqrt((dev ** 2).sum() / (8 * V)), step.max(),
+                    np.sqrt((step ** 2).sum() / (8 * V)), fun.min(), fun.sum() / V])
+    return g, mon
+
+
+@pytest.mark.parametrize("N,bc", [((4, 3, 5), 0), ((3, 4, 4), 1), ((5, 4, 1), 0)])
+def test_consistent_fourier_iteration_against_numpy(N, bc):
+    """One iteration of the boundary-consistent update (algorithm "consistent") from a random G against the numpy
+    restatement above, whose open-z Poisson step is a DCT rather than the oracle's mirror-doubled FFT."""
+    a = (1.0, 0.9, 0.8)
+    f = random_fields(N, seed=23, link_scale=0.3)
+    f.sync(bc)
+    rng = np.random.default_rng(24)
+    G = np.zeros((N[2] + 2, N[1], N[0], 3, 3), complex)
+    G[1:-1] = np.swapaxes(random_su3(rng, (N[2], N[1], N[0]), 0.5), -1, -2)
+    gn, mon = _consistent_iteration_np(f, a, G, 2.0, bc)
+    G_or = G.copy()
+    _, _, st = O.coulomb_gauge_algo(f.copy(), a, G_or, "consistent", 0.0, 1, 2.0, bc=bc)
+    assert st[0] == 1
+    assert np.abs(np.swapaxes(G_or[1:-1], -1, -2) - gn[1:-1]).max() < 1e-12
+    assert np.abs(st[1:] - mon).max() < 1e-12 * max(1.0, np.abs(mon).max()), (st, mon)
+
+
+@pytest.mark.parametrize("N,bc", [((6, 5, 4), 0), ((5, 4, 6), 1)])
+def test_consistent_fourier_converges_and_recovers_the_orbit(N, bc):
+    """At Nz > 1, where the reference's iteration stalls, the boundary-consistent update reaches max|Delta| below
+    1e-12; a random gauge rotation of the fixed links is undone up to one global rotation: G2(x) g_r(x) G1(x)^dag is
+    the same matrix at every site (the Coulomb gauge is unique up to global rotations near the solution)."""
+    a = (1.0, 0.9, 0.8)
+    f = random_fields(N, seed=25, link_scale=0.3)
+    f.sync(bc)
+    G1 = _identity_G(N)
+    UG1, EG1, s1 = O.coulomb_gauge_algo(f.copy(), a, G1, "consistent", 1e-12, 200, 2.0, bc=bc)
+    assert s1[1] <= 1e-12 and s1[0] < 200, s1
+    rng = np.random.default_rng(26)
+    gr = random_su3(rng, (N[2], N[1], N[0]), 0.3)  # [z, y, x, i, j]
+    grg = np.zeros((N[2] + 2, N[1], N[0], 3, 3), complex)
+    grg[1:-1] = gr
+    grg[0], grg[-1] = (grg[1], grg[-2]) if bc == 0 else (grg[-2], grg[1])
+    U = np.swapaxes(f.U, -1, -2)
+    Ur = U.copy()
+    nb = (np.roll(grg[1:-1], -1, axis=2), np.roll(grg[1:-1], -1, axis=1), grg[2:])
+    for mu in range(3):
+        Ur[1:-1, :, :, mu] = grg[1:-1] @ U[1:-1, :, :, mu] @ np.conj(np.swapaxes(nb[mu], -1, -2))
+    fr = f.copy()
+    fr.U[:] = np.swapaxes(Ur, -1, -2)
+    fr.sync(bc)
+    G2 = _identity_G(N)
+    UG2, EG2, s2 = O.coulomb_gauge_algo(fr, a, G2, "consistent", 1e-12, 200, 2.0, bc=bc)
+    assert s2[1] <= 1e-12, s2
+    g1 = np.swapaxes(G1[1:-1], -1, -2)
+    g2 = np.swapaxes(G2[1:-1], -1, -2)
+    M = g2 @ gr @ np.conj(np.swapaxes(g1, -1, -2))
+    assert np.abs(M - M[0, 0, 0]).max() < 1e-9, np.abs(M - M[0, 0, 0]).max()
+    assert abs(s1[6] - s2[6]) < 1e-10 * abs(s1[6])
+
+
+def test_reference_fourier_stalls_where_the_consistent_one_converges():
+    """The same open-z lattice: the reference's update keeps max|Delta| of order one after 200 iterations."""
+    N, a = (6, 5, 4), (1.0, 0.9, 0.8)
+    f = random_fields(N, seed=25, link_scale=0.3)
+    _, _, s0 = O.coulomb_gauge_algo(f.copy(), a, _identity_G(N), "fourier", 1e-12, 200, 2.0)
+    assert s0[0] == 200 and s0[1] > 1e-3
+
+
 def test_los_alamos_functional_increases_and_converges():
     """Nz = 1 (no z links in w): with even Nx, Ny the checkerboard is consistent and every site update raises
     its local term Re tr(h w), so the functional cannot fall."""
