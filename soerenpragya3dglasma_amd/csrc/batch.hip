@@ -22,7 +22,8 @@ struct glasma_batch {
     unsigned* block_site = nullptr;
     long nblocks = 0;
     std::vector<long> t0;  // first block of each event
-    long nrow = 0;  // the first nrow blocks belong to events whose tiles are whole rows (row-staged form)
+    long nrow = 0;  // the first nrow blocks: events of row_form 1 (row-staged, whole rows or one segment)
+    long nseg = 0;  // the next nseg blocks: row_form 2 (row-staged, two row segments); the rest the plain form
     std::map<int, hipGraphExec_t> graphs;  // by step count
     double last_ms = 0.0;
     std::string err;
@@ -65,7 +66,7 @@ int graph_for(glasma_batch* b, int nsteps, hipGraphExec_t* out) {
     for (int i = 0; i < nsteps && rc == 0; i++) {
         const int ph = i & 1, old = (i == nsteps - 1) ? 1 : 0;
         rc = launch_step_batch(b->stream, b->table + (2 * ph + old) * n, b->block_event, b->block_site, b->nrow,
-                               b->nblocks - b->nrow);
+                               b->nseg, b->nblocks - b->nrow - b->nseg);
     }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -146,20 +147,23 @@ int glasma_batch_create(const glasma_params* p, int nevents, glasma_batch** out)
         }
         b->ev.push_back(c);
     }
-    // events with whole rows per tile first (the row-staged form), then the rest
+    // events of row_form 1 first, then row_form 2 (both the row-staged form), then the plain form
     std::vector<long> t0(nevents);
-    long nb = 0, nrow = 0;
-    for (int pass = 0; pass < 2; pass++)
+    long nb = 0, nrow = 0, nseg = 0;
+    const int order[3] = {1, 2, 0};
+    for (int pass = 0; pass < 3; pass++) {
         for (int e = 0; e < nevents; e++) {
-            const bool row = row_tiles(b->ev[e]->g);
-            if (row != (pass == 0)) continue;
+            if (row_form(b->ev[e]->g) != order[pass]) continue;
             t0[e] = nb;
             nb += pad_sites(b->ev[e]->g.S) / kTile;
-            if (row) nrow = nb;
         }
+        if (pass == 0) nrow = nb;
+        if (pass == 1) nseg = nb - nrow;
+    }
     b->t0 = t0;
     b->nblocks = nb;
     b->nrow = nrow;
+    b->nseg = nseg;
     hipError_t err = hipStreamCreate(&b->stream);
     if (err == hipSuccess) err = hipEventCreate(&b->e0);
     if (err == hipSuccess) err = hipEventCreate(&b->e1);

@@ -88,9 +88,10 @@ __global__ void k_set_identity(double* __restrict__ U, long S, int nmat);
 int launch_step(glasma_ctx* ctx, bool write_old);
 int launch_step_args(glasma_ctx* ctx, const StepArgs& A);  // explicit buffers; ctx->cur is left alone
 StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old);
-// ensemble step over nrow row-form blocks followed by nplain plain-form blocks
+// ensemble step over nrow row-form blocks (whole rows or one segment), nseg two-segment row-form blocks and
+// nplain plain-form blocks, in that order
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
-                      long nrow, long nplain);
+                      long nrow, long nseg, long nplain);
 TileMap default_tile_map(const Geom& g);
 TileMap tile_map(const Geom& g, int mode, int bw, int bh);  // the requested order, or the next simpler one
 TileMap ctx_tile_map(const glasma_ctx* ctx);                // the context's tuned order, or the default

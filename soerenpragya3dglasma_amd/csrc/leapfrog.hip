@@ -32,19 +32,69 @@ static int env_int(const char* name, int dflt) {
 }
 
 
-// The links of a tile's sites in LDS, slot 1 + lane, when the tile is made of whole rows: nx a multiple of
-// 64 (one row; slot 0 then holds x0-1 and slot 65 x0+64, x periodic) or a divisor of 64 (64 / nx rows,
-// whose x neighbours all lie in the tile).
-constexpr int kRow = kTile + 2;
+// The links of a tile's sites in LDS, slot 1 + lane, when row_tiles holds: nx a divisor of 64 (64 / nx whole
+// rows, whose x neighbours all lie in the tile), or the tile is one row segment or two (the end of row r and the
+// start of row r + 1).  The x neighbours past a segment's ends (x periodic within each row) sit in halo slots:
+// slot 0 left of lane 0, slot 65 right of lane 63, and with two segments slot 66 right of the first segment's
+// last lane (x = nx - 1 -> 0 of row r) and slot 67 left of the second segment's first lane (x = 0 -> nx - 1 of
+// row r + 1).
+constexpr int kRow = kTile + 2;     // one segment or whole rows: halo slots 0 and 65
+constexpr int kRowSeg = kTile + 4;  // two segments: halo slots 0, 65, 66, 67
 
-// LDS of one block of the four-wavefront step
+// The LDS slots of the x + 1 and x - 1 neighbours of this lane's site, and how many halo slots the tile uses (0;
+// 2: slots 0, 65; 4: slots 0, 65, 66, 67).
+struct RowSlots {
+    int sp, sm, nhalo;
+};
+// x0: the x of the tile's first site (lane 0's x)
+__device__ __forceinline__ RowSlots row_slots(const Geom& g, int x0, int lane, int x) {
+    RowSlots r;
+    const int nx = g.nx;
+    if (kTile % nx == 0) {  // whole rows in the tile
+        r.sp = lane + 1 + (x + 1 == nx ? 1 - nx : 1);
+        r.sm = lane + 1 + (x == 0 ? nx - 1 : -1);
+        r.nhalo = 0;
+        return r;
+    }
+    const int k = nx - x0;  // the tile's sites in its first row
+    if (k >= kTile) {                      // one segment
+        r.sp = lane + 2;
+        r.sm = lane;
+        r.nhalo = 2;
+        return r;
+    }
+    r.sp = lane == k - 1 ? kTile + 2 : (lane == kTile - 1 ? kTile + 1 : lane + 2);
+    r.sm = lane == 0 ? 0 : (lane == k ? kTile + 3 : lane);
+    r.nhalo = 4;
+    return r;
+}
+// the site held by halo slot h (0: slot 0, 1: slot 65, 2: slot 66, 3: slot 67) of the tile starting at `first`,
+// whose first site has x = x0
+__device__ __forceinline__ long halo_site(const Geom& g, long first, int x0, int h) {
+    const int nx = g.nx;
+    const long r1 = first - x0;  // start of the tile's first row
+    if (h == 0) return r1 + (x0 == 0 ? nx - 1 : x0 - 1);
+    const int k = nx - x0;
+    if (h == 1) {
+        if (k >= kTile) return r1 + (x0 + kTile == nx ? 0 : x0 + kTile);
+        const int x2 = kTile - k;  // right of lane 63, at x = kTile - k - 1 of row r + 1
+        return r1 + nx + (x2 == nx ? 0 : x2);
+    }
+    return h == 2 ? r1 : r1 + 2L * nx - 1;
+}
+
+// LDS of one block of the four-wavefront step (RL slots per row: kRow or kRowSeg).  The row length is a template
+// parameter so that the whole-row form keeps the 66-slot layout, whose LDS offsets let it fit 168 registers
+// without spills (68 slots cost two spilled doubles).
+template <int RL>
 struct StepShared {
-    double row[3][18][kRow];  // U_mu on the tile's row, plane pairs as in ldm (re, im of element k)
-    double j[6][8][kTile];    // J+stat'(x), J-stat'(x), J+stat'(x+z), J-stat'(x+z), J+(x), J-(x)
+    double row[3][18][RL];  // U_mu on the tile's row(s) and halo sites, plane pairs as in ldm (re, im of element k)
+    double j[6][8][kTile];  // J+stat'(x), J-stat'(x), J+stat'(x+z), J-stat'(x+z), J+(x), J-(x)
     int ready;
 };
 
-__device__ __forceinline__ void ld_row(const double (*r)[kRow], int slot, M3& M) {
+template <int RL>
+__device__ __forceinline__ void ld_row(const double (*r)[RL], int slot, M3& M) {
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         M.re[k] = r[2 * k][slot];
@@ -56,9 +106,9 @@ __device__ __forceinline__ void ld_row(const double (*r)[kRow], int slot, M3& M)
 // the tile's own rows -- U_nu(x), U_nu(x+x) for MU = x, U_mu(x+-x) and U_x(x-x) for NU = x -- read from
 // the LDS copy instead of global memory (slots: own 1 + lane, x+1 at sp, x-1 at sm).  Same products in the
 // same order as link_update.
-template <class I, int MU, int NU>
+template <class I, int MU, int NU, int RL>
 __device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y, int z, int lane, int sp, int sm,
-                                            const double (*row)[18][kRow], M3& Sig) {
+                                            const double (*row)[18][RL], M3& Sig) {
     const Geom& g = A.g;
     const double cn = A.cE[NU];
     M3 P1, P2, T;
@@ -86,7 +136,8 @@ __device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y
 // wait for the J wavefront's hand-over (set unconditionally once its LDS stores are complete).  The wait is
 // capped so that no wave can spin forever; false if the cap was reached, and the caller then writes NaN
 // currents (glasma_bulk reports non-finite sums as GLASMA_ERR_NUMERIC) instead of silently using stale values.
-__device__ __forceinline__ bool wait_ready(StepShared& sh) {
+template <class SH>
+__device__ __forceinline__ bool wait_ready(SH& sh) {
     bool ok = false;
     for (int spin = 0; spin < (1 << 24); spin++) {
         if (__hip_atomic_load(&sh.ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
@@ -100,18 +151,18 @@ __device__ __forceinline__ bool wait_ready(StepShared& sh) {
 }
 
 // link_update with the row operands from LDS
-template <class I, int MU>
+template <class I, int MU, int RL>
 __device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, int y, int z, int lane, int sp,
-                                                int sm, const double (*row)[18][kRow], M3& Uh) {
+                                                int sm, const double (*row)[18][RL], M3& Uh) {
     M3 Sig;
 #pragma unroll
     for (int k = 0; k < 9; k++) { Sig.re[k] = 0.0; Sig.im[k] = 0.0; }
     // scheduling barriers keep the compiler from hoisting the next staple pair's loads into this one:
     // that fits the form into 168 registers without spills (three waves per SIMD, 31.3 -> 30.1 ms at
     // 256 x 256 x 1024), the waves hiding the latency instead
-    staples_row<I, MU, (MU + 1) % 3>(A, s, x, y, z, lane, sp, sm, row, Sig);
+    staples_row<I, MU, (MU + 1) % 3, RL>(A, s, x, y, z, lane, sp, sm, row, Sig);
     __builtin_amdgcn_sched_barrier(0);
-    staples_row<I, MU, (MU + 2) % 3>(A, s, x, y, z, lane, sp, sm, row, Sig);
+    staples_row<I, MU, (MU + 2) % 3, RL>(A, s, x, y, z, lane, sp, sm, row, Sig);
     __builtin_amdgcn_sched_barrier(0);
     M3 Uo;
     ld_row(row[MU], lane + 1, Uo);
@@ -125,26 +176,37 @@ __device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, i
 // The hand-over is a workgroup-scope flag: wave 3 sets it unconditionally once its stores to LDS are
 // complete, so wave 2's wait always ends; the wait is capped all the same.
 //
-// ROW (row_tiles: a tile is whole rows): waves 0-2 first copy U_w of the tile (and, for one-row tiles, the
-// two halo sites) into LDS, and the 18 of 41 link operands per site that lie on those rows are read from
+// ROW (row_tiles: whole rows, or one or two row segments): waves 0-2 first copy U_w of the tile and of its halo
+// sites (row_slots) into LDS, and the 18 of 41 link operands per site that lie on those rows are read from
 // there.
 // This takes about a third of the vector-memory accesses off the L1, whose pipeline is what the
 // kernel waits on.
-template <class I, bool ROW>
-__device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepShared& sh) {
+template <class I, bool ROW, bool SEG, int RL = SEG ? kRowSeg : kRow>
+__device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepShared<RL>& sh) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long sl = first + lane;
     const bool valid = sl < A.g.S;
     const I s = (I)(valid ? sl : 0);
     int x, y, z;
     site_coords<I>(A.g, s, x, y, z);
-    // LDS slots of x+1 and x-1: the next / previous lane, or the halo slots at the ends of a one-row tile;
-    // in a tile of several short rows, the lane of the periodic neighbour in the same row
+    // LDS slots of x+1 and x-1: the next / previous lane, or halo slots; in a tile of several short rows, the
+    // lane of the periodic neighbour in the same row; with two row segments (SEG), row_slots
+    int sp, sm, nhalo, x0 = 0;
     const bool one_row = A.g.nx % kTile == 0;
-    const int sp = one_row ? lane + 2 : lane + 1 + (x + 1 == A.g.nx ? 1 - A.g.nx : 1);
-    const int sm = one_row ? lane : lane + 1 + (x == 0 ? A.g.nx - 1 : -1);
+    if (SEG) {
+        x0 = __builtin_amdgcn_readfirstlane(x);  // lane 0's x: every lane of the wave is active here
+        const RowSlots rs = row_slots(A.g, x0, lane, x);
+        sp = rs.sp;
+        sm = rs.sm;
+        nhalo = rs.nhalo;
+    } else {
+        sp = one_row ? lane + 2 : lane + 1 + (x + 1 == A.g.nx ? 1 - A.g.nx : 1);
+        sm = one_row ? lane : lane + 1 + (x == 0 ? A.g.nx - 1 : -1);
+        nhalo = one_row ? 2 : 0;
+    }
     if (ROW && wave < 3) {
-        // lanes past the end of the lattice (last tile only) copy site 0; no valid lane reads their slots
+        // lanes past the end of the lattice (last tile only, whole-row tiles only) copy site 0; no valid lane
+        // reads their slots
         M3 M;
         ldm<I>(A.U, kUPlanes, 18 * wave, s, M);
 #pragma unroll
@@ -152,11 +214,16 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
             sh.row[wave][2 * k][lane + 1] = M.re[k];
             sh.row[wave][2 * k + 1][lane + 1] = M.im[k];
         }
-        if (one_row && lane < 2) {
-            const int x0 = x - lane;  // the tile's first x
-            const int xh = lane == 0 ? xwrap(x0 - 1, A.g.nx) : xwrap(x0 + kTile, A.g.nx);
-            ldm<I>(A.U, kUPlanes, 18 * wave, (I)(s - (I)x + (I)xh), M);
-            const int slot = lane == 0 ? 0 : kRow - 1;
+        if (lane < nhalo) {
+            long hs;
+            if (SEG) {
+                hs = halo_site(A.g, first, x0, lane);
+            } else {
+                const int xt = x - lane;  // the tile's first x
+                hs = (long)s - x + (lane == 0 ? xwrap(xt - 1, A.g.nx) : xwrap(xt + kTile, A.g.nx));
+            }
+            ldm<I>(A.U, kUPlanes, 18 * wave, (I)hs, M);
+            const int slot = lane == 0 ? 0 : kTile + lane;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 sh.row[wave][2 * k][slot] = M.re[k];
@@ -169,14 +236,14 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
     if (wave == 0) {
         M3 Uh;
         if (ROW) {
-            if (valid) link_update_row<I, 0>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
+            if (valid) link_update_row<I, 0, RL>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
         } else if (valid) {
             link_update<I, 0>(A, s, x, y, z, Uh);
         }
     } else if (wave == 1) {
         M3 Uh;
         if (ROW) {
-            if (valid) link_update_row<I, 1>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
+            if (valid) link_update_row<I, 1, RL>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
         } else if (valid) {
             link_update<I, 1>(A, s, x, y, z, Uh);
         }
@@ -217,7 +284,7 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
     } else {
         M3 Uh;
         if (ROW) {
-            if (valid) link_update_row<I, 2>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
+            if (valid) link_update_row<I, 2, RL>(A, s, x, y, z, lane, sp, sm, sh.row, Uh);
         } else if (valid) {
             link_update<I, 2>(A, s, x, y, z, Uh);
         }
@@ -241,37 +308,41 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
 
 // Four-wavefront variant (step_tile4); two such blocks fill the 8 wavefront slots of a CU at two waves
 // per SIMD.
-template <class I, int W, bool ROW>
+template <class I, int W, bool ROW, bool SEG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))) k_step4(StepArgs A, TileMap tm) {
-    __shared__ StepShared sh;
-    step_tile4<I, ROW>(A, tile_site(A.g, tm, blockIdx.x), sh);
+    __shared__ StepShared<SEG ? kRowSeg : kRow> sh;
+    step_tile4<I, ROW, SEG>(A, tile_site(A.g, tm, blockIdx.x), sh);
 }
 
 // Ensemble step: one launch per form advances every event of a batch.  Block b works on the tile starting
 // at site block_site[b] of event e = block_event[b] (each event's blocks in the order tile_site gives it
 // alone); the event's arguments come from a device table (uniform per block, so they are read with scalar
-// loads).  The per-site arithmetic is that of k_step4.  The blocks of events whose tiles are whole rows
-// (row_tiles) come first and run the row-staged form at three waves per SIMD, the rest the plain form at
-// two, as each runs alone.
-template <int W, bool ROW>
+// loads).  The per-site arithmetic is that of k_step4.  The blocks come in three groups by row_form: whole rows
+// or one row segment per tile, then two segments (both the row-staged form at three waves per SIMD), then the
+// plain form at two, as each runs alone.
+template <int W, bool ROW, bool SEG>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 k_step_batch(const StepArgs* __restrict__ table, const int* __restrict__ block_event,
              const unsigned* __restrict__ block_site) {
-    __shared__ StepShared sh;
+    __shared__ StepShared<SEG ? kRowSeg : kRow> sh;
     const StepArgs A = table[block_event[blockIdx.x]];
-    step_tile4<unsigned int, ROW>(A, (long)block_site[blockIdx.x], sh);
+    step_tile4<unsigned int, ROW, SEG>(A, (long)block_site[blockIdx.x], sh);
 }
 
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
-                      long nrow, long nplain) {
-    if (nrow < 0 || nplain < 0 || nrow + nplain <= 0 || nrow > 0x7fffffffL || nplain > 0x7fffffffL)
+                      long nrow, long nseg, long nplain) {
+    if (nrow < 0 || nseg < 0 || nplain < 0 || nrow + nseg + nplain <= 0 || nrow > 0x7fffffffL ||
+        nseg > 0x7fffffffL || nplain > 0x7fffffffL)
         return GLASMA_ERR_ARG;
     if (nrow > 0)
-        hipLaunchKernelGGL((k_step_batch<3, true>), dim3((unsigned)nrow), dim3(256), 0, st, table, block_event,
+        hipLaunchKernelGGL((k_step_batch<3, true, false>), dim3((unsigned)nrow), dim3(256), 0, st, table, block_event,
                            block_site);
-    if (nplain > 0)
-        hipLaunchKernelGGL((k_step_batch<2, false>), dim3((unsigned)nplain), dim3(256), 0, st, table,
+    if (nseg > 0)
+        hipLaunchKernelGGL((k_step_batch<3, true, true>), dim3((unsigned)nseg), dim3(256), 0, st, table,
                            block_event + nrow, block_site + nrow);
+    if (nplain > 0)
+        hipLaunchKernelGGL((k_step_batch<2, false, false>), dim3((unsigned)nplain), dim3(256), 0, st, table,
+                           block_event + nrow + nseg, block_site + nrow + nseg);
     return hipGetLastError() == hipSuccess ? 0 : GLASMA_ERR_HIP;
 }
 
@@ -295,12 +366,15 @@ static void launch(int waves, int four, const TileMap& tm, const StepArgs& A, hi
     // the row-staged form needs whole rows per tile (GLASMA_STEP_FOUR=2 forces the plain form); three
     // waves per SIMD (168 registers) measured faster for it (30.1 against 31.8 ms at 256 x 256 x 1024),
     // the plain form spills at three and runs at two
-    if (row_tiles(A.g) && four != 2) {
-        if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true>), grid, dim3(256), 0, st, A, tm);
-        else hipLaunchKernelGGL((k_step4<I, 3, true>), grid, dim3(256), 0, st, A, tm);
+    const int form = four == 2 ? 0 : row_form(A.g);
+    if (form == 2) {
+        hipLaunchKernelGGL((k_step4<I, 3, true, true>), grid, dim3(256), 0, st, A, tm);
+    } else if (form == 1) {
+        if (waves == 2) hipLaunchKernelGGL((k_step4<I, 2, true, false>), grid, dim3(256), 0, st, A, tm);
+        else hipLaunchKernelGGL((k_step4<I, 3, true, false>), grid, dim3(256), 0, st, A, tm);
     } else {
-        if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, false>), grid, dim3(256), 0, st, A, tm);
-        else hipLaunchKernelGGL((k_step4<I, 2, false>), grid, dim3(256), 0, st, A, tm);
+        if (waves == 3) hipLaunchKernelGGL((k_step4<I, 3, false, false>), grid, dim3(256), 0, st, A, tm);
+        else hipLaunchKernelGGL((k_step4<I, 2, false, false>), grid, dim3(256), 0, st, A, tm);
     }
 }
 

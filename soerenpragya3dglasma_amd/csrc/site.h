@@ -235,9 +235,21 @@ GHD void plaq_mat(const double* __restrict__ U, int kind, int m1, I s1, int m2, 
 //           are in flight at once share their z neighbours in L2.
 constexpr int kXcds = 8;
 
-// whether a geometry's 64-site tiles are made of whole rows: nx a multiple of 64 (one row per tile) or a
-// divisor of 64; the step then takes its row operands from an LDS copy of the tile
-GHD bool row_tiles(const Geom& g) { return g.nx % kTile == 0 || kTile % g.nx == 0; }
+// whether the step can take its row operands from an LDS copy of the tile: nx a multiple of 64 (a tile is part
+// of one row), a divisor of 64 (whole rows), or a tile spans at most two row segments (every tile start x0 =
+// 64 t mod nx, a multiple of gcd(64, nx), has x0 + 64 <= 2 nx, i.e. nx >= 64 - gcd(64, nx); nx = 48 and 96
+// among others) with no partial last tile
+GHD int gcd64(int n) {
+    int g = kTile;
+    while (n % g) g >>= 1;
+    return g;
+}
+// 0: the plain form; 1: the row-staged form with whole rows or one row segment per tile; 2: two segments
+GHD int row_form(const Geom& g) {
+    if (g.nx % kTile == 0 || kTile % g.nx == 0) return 1;
+    return g.nx >= kTile - gcd64(g.nx) && g.S % kTile == 0 ? 2 : 0;
+}
+GHD bool row_tiles(const Geom& g) { return row_form(g) != 0; }
 struct TileMap {
     int mode, bpb;
     int bw, bh, nbx, tpbp;
