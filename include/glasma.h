@@ -287,6 +287,11 @@ int glasma_load_state(glasma_ctx* ctx, const char* file);
 #define GLASMA_TUNE_MAX 16
 int glasma_tune_step(glasma_ctx* ctx, int reps, int* ncand, int* codes, double* ms, int* best);
 int glasma_tile_order(const glasma_ctx* ctx, int* code, int* tuned);
+/* The bandwidth ceiling of the step's compulsory traffic on this device: a copy kernel with the step's tiled
+ * layout and grid (one 64-site tile per block) reads the current U, E and currents once and writes them once to
+ * scratch (880 + 880 bytes per site), `reps` timed launches after one warm-up; ms = average per launch.  The
+ * fields are unchanged.  Needs 880 B per site of scratch device memory while it runs. */
+int glasma_copy_ceiling(glasma_ctx* ctx, int reps, double* ms);
 int glasma_batch_tune(glasma_batch* b, int reps);
 
 /* ---- reverse-mode derivatives (no counterpart in the reference, which has no gradient code) ------------ */

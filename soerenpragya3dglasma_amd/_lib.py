@@ -107,6 +107,7 @@ def lib():
         "glasma_load_state": ([vp, ctypes.c_char_p], c_int),
         "glasma_tune_step": ([vp, c_int, ip, ip, dp, ip], c_int),
         "glasma_tile_order": ([vp, ip, ip], c_int),
+        "glasma_copy_ceiling": ([vp, c_int, dp], c_int),
         "glasma_batch_tune": ([vp, c_int], c_int),
         "glasma_slab_energy": ([vp, c_int, c_int, dp], c_int),
         "glasma_step_vjp": ([vp] + [dp] * 6, c_int),

@@ -36,6 +36,47 @@ std::vector<TileMap> candidates(const Geom& g) {
 
 int code_of(const TileMap& m) { return m.mode * 10000 + m.bw * 100 + m.bh; }
 
+// The step's compulsory traffic alone: one 64-site tile per block of 256 threads, each block reading the tile's
+// U (27 plane pairs), E (12) and currents (16) once with non-temporal loads, all issued before the first store,
+// and writing them once to the destination arrays with non-temporal stores (880 + 880 bytes per site, the same
+// tiled layout and the same grid as the step).
+typedef double cp2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cp2 cp_ld(const cp2* p) {
+    return __builtin_nontemporal_load((const __attribute__((address_space(1))) cp2*)p);
+}
+__device__ __forceinline__ void cp_st(cp2* p, cp2 v) {
+    __builtin_nontemporal_store(v, (__attribute__((address_space(1))) cp2*)p);
+}
+constexpr int kUPairs = kUPlanes / 2 * kTile, kEPairs = kEPlanes / 2 * kTile, kJPairs = kJPlanes / 2 * kTile;
+__global__ void __launch_bounds__(256) k_tile_copy(const cp2* __restrict__ U, const cp2* __restrict__ E,
+                                                   const cp2* __restrict__ J, cp2* __restrict__ Uo,
+                                                   cp2* __restrict__ Eo, cp2* __restrict__ Jo) {
+    static_assert(kUPairs == 6 * 256 + 192 && kEPairs == 3 * 256 && kJPairs == 4 * 256, "tile sizes");
+    const long t = blockIdx.x;
+    const int i = threadIdx.x;
+    const cp2* u = U + t * kUPairs;
+    const cp2* e = E + t * kEPairs;
+    const cp2* j = J + t * kJPairs;
+    cp2 r[14];
+#pragma unroll
+    for (int k = 0; k < 6; k++) r[k] = cp_ld(u + k * 256 + i);
+    r[6] = i < 192 ? cp_ld(u + 1536 + i) : cp2{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[7 + k] = cp_ld(e + k * 256 + i);
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[10 + k] = cp_ld(j + k * 256 + i);
+    cp2* uo = Uo + t * kUPairs;
+    cp2* eo = Eo + t * kEPairs;
+    cp2* jo = Jo + t * kJPairs;
+#pragma unroll
+    for (int k = 0; k < 6; k++) cp_st(uo + k * 256 + i, r[k]);
+    if (i < 192) cp_st(uo + 1536 + i, r[6]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) cp_st(eo + k * 256 + i, r[7 + k]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) cp_st(jo + k * 256 + i, r[10 + k]);
+}
+
 }  // namespace
 
 extern "C" {
@@ -87,6 +128,40 @@ int glasma_tune_step(glasma_ctx* ctx, int reps, int* ncand, int* codes, double* 
         if (ms) ms[i] = t[i];
     }
     if (best) *best = (int)bi;
+    return 0;
+}
+
+int glasma_copy_ceiling(glasma_ctx* ctx, int reps, double* ms) {
+    if (!ctx || reps <= 0 || !ms) return GLASMA_ERR_ARG;
+    const size_t Sp = pad_sites(ctx->g.S);
+    const long tiles = (long)(Sp / kTile);
+    if (tiles > 0x7fffffffL) return set_error(ctx, GLASMA_ERR_ARG, "glasma_copy_ceiling: too many tiles");
+    double *sU = nullptr, *sE = nullptr, *sJ = nullptr;
+    hipError_t e = hipMalloc(&sU, (size_t)kUPlanes * Sp * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&sE, (size_t)kEPlanes * Sp * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&sJ, (size_t)kJPlanes * Sp * sizeof(double));
+    const cp2* U = reinterpret_cast<const cp2*>(ctx->U[ctx->cur]);
+    const cp2* E = reinterpret_cast<const cp2*>(ctx->E);
+    const cp2* J = reinterpret_cast<const cp2*>(ctx->J[ctx->cur]);
+    float f = 0.f;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_tile_copy, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, U, E, J,
+                           reinterpret_cast<cp2*>(sU), reinterpret_cast<cp2*>(sE), reinterpret_cast<cp2*>(sJ));
+        e = hipEventRecord(ctx->ev0, ctx->stream);
+    }
+    for (int r = 0; r < reps && e == hipSuccess; r++)
+        hipLaunchKernelGGL(k_tile_copy, dim3((unsigned)tiles), dim3(256), 0, ctx->stream, U, E, J,
+                           reinterpret_cast<cp2*>(sU), reinterpret_cast<cp2*>(sE), reinterpret_cast<cp2*>(sJ));
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(sU);
+    hipFree(sE);
+    hipFree(sJ);
+    if (e != hipSuccess) return set_error(ctx, hip_code(e), std::string("glasma_copy_ceiling: ") + hipGetErrorString(e));
+    *ms = f / reps;
     return 0;
 }
 

@@ -212,6 +212,13 @@ class Glasma:
         cand = {self._order(int(c)): float(t) for c, t in zip(codes[:n.value], ms[:n.value])}
         return {"candidates_ms": cand, "chosen": self._order(int(codes[best.value]))}
 
+    def copy_ceiling(self, reps: int = 5) -> float:
+        """ms per launch of a copy of the step's compulsory traffic (880 + 880 B per site, the step's tile layout
+        and grid): the bandwidth ceiling the step kernel is measured against (glasma_copy_ceiling)."""
+        ms = ctypes.c_double()
+        self._chk(_lib.lib().glasma_copy_ceiling(self._h, int(reps), ctypes.byref(ms)), "glasma_copy_ceiling")
+        return ms.value
+
     def tile_order(self) -> tuple:
         """(order in use, whether it was tuned)."""
         code, tuned = ctypes.c_int(), ctypes.c_int()

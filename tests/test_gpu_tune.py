@@ -57,3 +57,15 @@ def test_batch_tune_keeps_results():
             ref = g.get_fields()
         for k in KEYS:
             assert np.array_equal(d[k], ref[k]), (sp, k)
+
+
+def test_copy_ceiling_leaves_fields_unchanged():
+    """glasma_copy_ceiling times a copy of the step's compulsory traffic into scratch; the fields stay as they were."""
+    from soerenpragya3dglasma_amd import Glasma
+    N = (64, 8, 6)
+    with Glasma(N, (1.0, 1.0, 1.0), 0.25) as g:
+        upload(g, random_fields(N, seed=9))
+        before = g.checksum()
+        ms = g.copy_ceiling(3)
+        assert ms > 0.0
+        assert g.checksum() == before
