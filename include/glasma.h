@@ -195,8 +195,9 @@ int glasma_coulomb_gauge(glasma_ctx* ctx, double tol, int maxit, double alpha, d
  * converge for Nz > 1 (periodic z momenta despite open z, the gauge-fixed links' z = -1 ghost left at the identity,
  * the zero mode passed through unscaled).  w takes only the links that exist (open z: none below layer 0 and none
  * from layer Nz - 1 to its ghost copy; periodic z: the wrapped links, with the current G at both ends), the Poisson
- * step inverts the Laplacian of that graph (open z: Neumann, as the periodic transform of the lattice doubled by its
- * mirror image in z; periodic z: periodic), and the zero mode is dropped; it reaches tolerances of 1e-10 and below.
+ * step inverts the Laplacian of that graph (open z: Neumann, a cosine transform in z, which equals the periodic
+ * transform of the lattice doubled by its mirror image in z; periodic z: periodic), and the zero mode is dropped; it
+ * reaches tolerances of 1e-10 and below.
  * Every algorithm returns 0 at the iteration cap as the reference does; when the cap is reached above a tolerance
  * tol > 0, glasma_last_error says so. */
 enum { GLASMA_GAUGE_FOURIER = 0, GLASMA_GAUGE_JACOBI = 1, GLASMA_GAUGE_LOS_ALAMOS = 2, GLASMA_GAUGE_FOURIER_CONSISTENT = 3 };

@@ -197,12 +197,23 @@ __device__ __forceinline__ int stockham(dbl2* const lb, int bs, int cur, const R
 // v -> nf * v / p^2 when p^2 = sum_mu (2 - 2 cos(2 pi k_mu / N_mu)) / a_mu^2 > 0 (the zero mode passes
 // unscaled), and the lines are transformed back, all in LDS.  The x and y passes of the forward transform run
 // before, those of the inverse after (the passes commute).
+// Neumann (open z, the links that exist only): the z transform is the DCT-II of the line, computed as a complex FFT of
+// length n of the line in even-odd order (Makhoul, IEEE Trans. ASSP 28 (1980) 27-34; element j sits at position
+// nperm(j)), with p_z^2 = (2 - 2 cos(pi k_z / n)) / a_z^2 and the inverse a DCT-III: the result is the periodic
+// Poisson step of the line doubled by its mirror image, restricted to the line.  The quarter-wave table wq holds
+// exp(-i pi k / (2 n)), k < n.
 struct LineFilter {
     int on, nx, ny;
     long P;  // transverse sites: the line's site is line % P
     double ax, ay, az, nf;
     int drop_zero;  // the zero mode set to 0 (else passed through unscaled)
+    int neumann;
+    const dbl2* wq;
 };
+
+// position of element j of a line of length n in even-odd order: the even elements ascending, then the odd ones
+// descending
+__device__ __forceinline__ int nperm(int j, int n) { return (j & 1) ? n - ((j + 1) >> 1) : (j >> 1); }
 
 // One launch transforms `nlines` lines of length n in groups of lpb lines.
 // - Packed (es = 1): line l starts at l * n, so a group is one run of lpb * n elements.
@@ -244,7 +255,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
     double* const ztab = reinterpret_cast<double*>(twl + (TwLds ? n : 0));
     if (Filt)
         for (int k = threadIdx.x; k < n; k += blockDim.x)
-            ztab[k] = (2.0 - 2.0 * cos(2.0 * kPi * k / (double)n)) / (filt.az * filt.az);
+            ztab[k] = (2.0 - 2.0 * cos((filt.neumann ? 1.0 : 2.0) * kPi * k / (double)n)) / (filt.az * filt.az);
     // the first entry of buffer 1 takes the writes of elements past the group: nothing reads buffer 1 between
     // those writes and the first stage, which overwrites it, and the previous group's reads of it end at the
     // barrier that closes each iteration
@@ -296,7 +307,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
             const int idx = threadIdx.x + m * kFftThreads;
             const int lq = qdiv(idx, n, rn);
             const int li = packed ? lq : myli;
-            const int j = packed ? idx - lq * n : idx >> lg_lpb;
+            const int j0 = packed ? idx - lq * n : idx >> lg_lpb;
+            const int j = Filt && filt.neumann ? nperm(j0, n) : j0;
             dbl2* const at = idx < nload ? lds + li * ld + j : spare;
             *at = (ok >> m) & 1u ? pre[m] : dbl2{0.0, 0.0};
         }
@@ -311,19 +323,42 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
             const int px = (int)(t % filt.nx), py = (int)(t / filt.nx);
             const double pxy = (2.0 - 2.0 * cos(2.0 * kPi * px / (double)filt.nx)) / (filt.ax * filt.ax) +
                                (2.0 - 2.0 * cos(2.0 * kPi * py / (double)filt.ny)) / (filt.ay * filt.ay);
+            if (filt.neumann) {
+                // V (the FFT of the even-odd line) -> X_k = (w_k V_k + conj(w_k) V_{n-k}) / 2, the DCT-II; Y = h X with
+                // h = nf / p^2 (0 at p^2 = 0); Q_k = conj(w_k) (Y_k - i Y_{n-k}), Y_n = 0, whose inverse FFT read
+                // in even-odd order is the DCT-III of Y.  One thread takes the pair (k, n - k).
+                dbl2* const row = fb + myli * ld;
+                for (int k = threadIdx.x >> lg_lpb; 2 * k <= n; k += kFftThreads >> lg_lpb) {
+                    const int kr = k == 0 ? 0 : n - k;
+                    const dbl2 vk = row[k], vr = row[kr];
+                    const dbl2 wk = filt.wq[k], wr = filt.wq[kr];
+                    const dbl2 cwk = {wk.x, -wk.y}, cwr = {wr.x, -wr.y};
+                    const double pk = pxy + ztab[k], pr = pxy + ztab[kr];
+                    const double hk = pk > 0.0 ? filt.nf / pk : 0.0, hr = pr > 0.0 ? filt.nf / pr : 0.0;
+                    const dbl2 yk = (0.5 * hk) * (cmul(wk, vk) + cmul(cwk, vr));
+                    const dbl2 yr = (0.5 * hr) * (cmul(wr, vr) + cmul(cwr, vk));
+                    if (k == 0) {
+                        row[0] = yk;
+                    } else {
+                        row[k] = cmul(cwk, dbl2{yk.x + yr.y, yk.y - yr.x});
+                        row[kr] = cmul(cwr, dbl2{yr.x + yk.y, yr.y - yk.x});
+                    }
+                }
+            } else {
 #pragma unroll
-            for (int m = 0; m < MAXE; m++) {
-                const int idx = threadIdx.x + m * kFftThreads;
-                if (idx < nload) {
-                    const int pz = idx >> lg_lpb;
-                    const double p2 = pxy + ztab[pz];
-                    if (p2 > 0.0) {
-                        dbl2 v = fb[myli * ld + pz];
-                        v.x = filt.nf * v.x / p2;
-                        v.y = filt.nf * v.y / p2;
-                        fb[myli * ld + pz] = v;
-                    } else if (filt.drop_zero) {
-                        fb[myli * ld + pz] = dbl2{0.0, 0.0};
+                for (int m = 0; m < MAXE; m++) {
+                    const int idx = threadIdx.x + m * kFftThreads;
+                    if (idx < nload) {
+                        const int pz = idx >> lg_lpb;
+                        const double p2 = pxy + ztab[pz];
+                        if (p2 > 0.0) {
+                            dbl2 v = fb[myli * ld + pz];
+                            v.x = filt.nf * v.x / p2;
+                            v.y = filt.nf * v.y / p2;
+                            fb[myli * ld + pz] = v;
+                        } else if (filt.drop_zero) {
+                            fb[myli * ld + pz] = dbl2{0.0, 0.0};
+                        }
                     }
                 }
             }
@@ -341,7 +376,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
                     if (idx < lim) dv[base + idx] = res[li * ld + idx - li * n];
                 } else if (base >= 0) {
                     const int j = idx >> lg_lpb;
-                    dv[base + (long)j * es] = res[myli * ld + j];
+                    dv[base + (long)j * es] = res[myli * ld + (Filt && filt.neumann ? nperm(j, n) : j)];
                 }
             }
         }
@@ -379,6 +414,41 @@ hipError_t twiddles(int n, const dbl2** out) {
     e = hipMemcpy(d, h.data(), sizeof(dbl2) * (size_t)n, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     g_tw[key] = d;
+    *out = d;
+    return hipSuccess;
+}
+
+// quarter-wave tables exp(-i pi j / (2 n)), j < n, of the Neumann filter, one per (device, n); kept for the process
+// lifetime
+std::map<std::pair<int, int>, dbl2*> g_wq;
+
+hipError_t quarter_wave(int n, const dbl2** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(dev, n);
+    auto it = g_wq.find(key);
+    if (it != g_wq.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    std::vector<double> h(2 * (size_t)n);
+    const long double hp = 1.570796326794896619231321691639751442L;  // pi / 2
+    for (int j = 0; j < n; j++) {
+        long double ang = hp * (long double)j / (long double)n;
+        h[2 * j] = (double)cosl(ang);
+        h[2 * j + 1] = (double)(-sinl(ang));
+    }
+    dbl2* d = nullptr;
+    e = hipMalloc(&d, sizeof(dbl2) * (size_t)n);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(d, h.data(), sizeof(dbl2) * (size_t)n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return e;
+    }
+    g_wq[key] = d;
     *out = d;
     return hipSuccess;
 }
@@ -535,7 +605,7 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
 }  // namespace
 
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
-    const LineFilter none = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0};
+    const LineFilter none = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, nullptr};
     return fft_lines_filtered(st, data, n, es, nlines, b1, s1, s2, sign, none);
 }
 
@@ -558,10 +628,18 @@ int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign) {
     return fft_z(st, c, g, ncomp, sign);
 }
 
-int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3], bool drop_zero) {
+int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3], bool drop_zero,
+                   bool neumann) {
     if (g.nz < 2) return fft_fail(GLASMA_ERR_ARG, "the fused Poisson filter needs Nz >= 2");
+    if (neumann && !drop_zero) return fft_fail(GLASMA_ERR_ARG, "the Neumann filter drops the zero mode");
     const long P = (long)g.nx * g.ny;
-    const LineFilter f = {1, g.nx, g.ny, P, a[0], a[1], a[2], 1.0 / (double)g.S, drop_zero ? 1 : 0};
+    const dbl2* wq = nullptr;
+    if (neumann) {
+        const hipError_t we = quarter_wave(g.nz, &wq);
+        if (we != hipSuccess) return fft_fail_hip(we, "quarter-wave table");
+    }
+    const LineFilter f = {1, g.nx, g.ny, P, a[0], a[1], a[2], 1.0 / (double)g.S, drop_zero ? 1 : 0,
+                          neumann ? 1 : 0, wq};
     int rc = fft_xy(st, c, g, ncomp, g.nz, -1);
     if (!rc) rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
     if (!rc) rc = fft_xy(st, c, g, ncomp, g.nz, +1);

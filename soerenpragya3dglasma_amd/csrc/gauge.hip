@@ -189,14 +189,12 @@ __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, cons
 }
 
 // Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52), with the monitors,
-// from the ungauged links and G.  The four packed arrays are cs complex entries apart; with mirror, layer z is also
-// written to layer 2 Nz - 1 - z of the arrays (the lattice doubled by its mirror image, for the open-z Poisson step
-// of the consistent update).
+// from the ungauged links and G.
 template <bool CONS>
 __global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restrict__ U, const double* __restrict__ G,
                                                             const double* __restrict__ Gtop, double* __restrict__ F,
                                                             double* __restrict__ part, Geom g, double cx, double cy,
-                                                            double cz, double fm0, long cs, int mirror) {
+                                                            double cz, double fm0) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0;
     if (s < g.S) {
@@ -213,15 +211,8 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restr
         }
         // two real components per complex array: Delta^{2k} + i Delta^{2k+1} in array k
         for (int k = 0; k < 4; k++) {
-            F[2 * ((long)k * cs + s)] = dv[2 * k];
-            F[2 * ((long)k * cs + s) + 1] = dv[2 * k + 1];
-        }
-        if (mirror) {
-            const long sm = s2 + (long)(2 * g.nz - 1 - z) * g.P;
-            for (int k = 0; k < 4; k++) {
-                F[2 * ((long)k * cs + sm)] = dv[2 * k];
-                F[2 * ((long)k * cs + sm) + 1] = dv[2 * k + 1];
-            }
+            F[2 * ((long)k * g.S + s)] = dv[2 * k];
+            F[2 * ((long)k * g.S + s) + 1] = dv[2 * k + 1];
         }
         const double lf = W.re[0] + W.re[4] + W.re[8];
         fm = fmin(fm, lf);
@@ -341,12 +332,12 @@ __global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay,
 
 // g <- exp(-i alpha Delta t)^dag g and the step size |3 - tr exp(...)| (FourierAcceleration.cpp:80-140)
 __global__ __launch_bounds__(kCgBlock) void k_cg_update(const double* __restrict__ F, double* __restrict__ G,
-                                                         double* __restrict__ part, Geom g, double alpha, long cs) {
+                                                         double* __restrict__ part, Geom g, double alpha) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double ssq = 0.0, sm = 0.0;
     if (s < g.S) {
         double ub[8];
-        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * cs + s) + (a & 1)];  // packed as by k_cg_dev_fused
+        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * g.S + s) + (a & 1)];  // packed as by k_cg_dev_fused
         M3 Wt, Gx, Gn;
         expi(-alpha, ub, Wt);
         const double tr = 3.0 - (Wt.re[0] + Wt.re[4] + Wt.re[8]);
@@ -510,12 +501,8 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     const Geom& g = ctx->g;
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
     const bool cons = algo == GLASMA_GAUGE_FOURIER_CONSISTENT;
-    // the consistent update on an open lattice with Nz > 1 inverts the Neumann Laplacian in z as the periodic one of
-    // the lattice doubled by its mirror image: the four packed arrays then hold 2 Nz layers
-    const bool mirror = cons && g.bc == 0 && g.nz > 1;
-    const long cs = mirror ? 2 * g.S : g.S;  // complex entries between the packed arrays
     rc = ensure_red(ctx, 16 * (size_t)nblk + 8);
-    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)cs);  // 4 complex per site
+    if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)g.S);  // 4 complex per site
     if (rc) return rc;
     const double* a = ctx->p.a;
     // weights (1/a_mu)^2 of w and the functional's starting minimum 6 sum_mu 1/a_mu^2, rounded as the
@@ -547,23 +534,18 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
         } else {
             if (cons) {
                 hipLaunchKernelGGL(k_cg_dev_fused<true>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,
-                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, cs, mirror ? 1 : 0);
+                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0);
             } else {
                 hipLaunchKernelGGL(k_cg_dev_fused<false>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,
-                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, cs, 0);
+                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0);
                 hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
             }
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
             if (cons) {
-                // the Laplacian of the links that exist; the zero mode (zero by construction) dropped
-                if (mirror) {
-                    Geom gd = g;
-                    gd.nz = 2 * g.nz;
-                    gd.S = 2 * g.S;
-                    gd.bc = 1;
-                    rc = fft_3d_poisson(ctx->stream, F, gd, 4, a, true);
-                } else if (g.nz >= 2) {
-                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a, true);
+                // the Laplacian of the links that exist (open z: the Neumann Laplacian, a cosine transform in z); the
+                // zero mode (zero by construction) dropped
+                if (g.nz >= 2) {
+                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a, true, g.bc == 0);
                 } else {
                     rc = fft_3d(ctx->stream, F, g, 4, -1);
                     if (!rc) {
@@ -588,7 +570,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
                 if (rc) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
             }
             hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g,
-                               param, cs);
+                               param);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
             GL_HIP(ctx, hipGetLastError());
         }

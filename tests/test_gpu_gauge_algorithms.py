@@ -151,11 +151,13 @@ def test_gauss_restore_converges(variant):
     assert abs(st[0] - st_o[0]) <= 2
 
 
-@pytest.mark.parametrize("N,bc", [((6, 5, 6), 0), ((5, 6, 4), 1), ((6, 4, 1), 0), ((32, 32, 1024), 0)])
+@pytest.mark.parametrize("N,bc", [((6, 5, 6), 0), ((5, 6, 7), 0), ((4, 6, 9), 0), ((5, 6, 4), 1), ((6, 4, 1), 0),
+                                  ((32, 32, 1024), 0)])
 def test_consistent_fourier_matches_oracle(N, bc):
     """The boundary-consistent Fourier update (no counterpart in the reference) against the oracle's: open z (the
-    Poisson step on the mirror-doubled lattice, at 32 x 32 x 1024 with z lines of length 2048 in the fused pass),
-    periodic z and Nz = 1, four iterations from the identity; the same tolerances as the reference's update."""
+    device's Poisson step is a cosine transform in z, a length-Nz FFT of the line in even-odd order, even and odd
+    Nz; the oracle's is the periodic transform of the mirror-doubled lattice), periodic z and Nz = 1, four
+    iterations from the identity; the same tolerances as the reference's update."""
     from soerenpragya3dglasma_amd import Glasma
     a, iters = (1.0, 0.9, 0.8), 4
     f = random_fields(N, seed=33, link_scale=0.3, e_scale=0.3)
