@@ -25,6 +25,7 @@
 #include <vector>
 #include <cmath>
 
+#include "fft_lds.h"
 #include "internal.h"
 
 namespace glasma {
@@ -41,55 +42,6 @@ constexpr int kFftLds = 2048;  // longest line
 #endif
 constexpr int kContigEntries = GLASMA_FFT_CONTIG_ENTRIES;
 constexpr int kFftThreads = 256;
-constexpr int kMaxStages = 24;
-constexpr int kMaxGenericRadix = 16;
-
-struct Radices {
-    int n, nst;
-    int r[kMaxStages];
-};
-
-int factorize(int n, Radices& R) {
-    R.n = n;
-    R.nst = 0;
-    int m = n;
-    for (int f : {4, 2, 3}) {
-        while (m % f == 0) {
-            if (R.nst == kMaxStages) return -1;
-            R.r[R.nst++] = f;
-            m /= f;
-        }
-    }
-    for (int p = 5; m > 1; p += 2) {
-        while (m % p == 0) {
-            if (p > kMaxGenericRadix || R.nst == kMaxStages) return -1;
-            R.r[R.nst++] = p;
-            m /= p;
-        }
-    }
-    return 0;
-}
-
-typedef double dbl2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ dbl2 cmul(dbl2 a, dbl2 b) { return dbl2{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-// twiddle exp(sign * 2 pi i k / n) from the forward table
-__device__ __forceinline__ dbl2 twid(const dbl2* tw, int k, int sign) {
-    dbl2 w = tw[k];
-    return sign < 0 ? w : dbl2{w.x, -w.y};
-}
-
-// a / d for 0 <= a < 2^20 and 1 <= d < 2^20, with rd = 1/d in FP32: the FP32 product is within 1/8 of a / d,
-// so the truncated estimate is off by at most one, and one remainder test corrects it.  The index arithmetic
-// below divides by run-time lengths once per element; this replaces the integer division sequence.
-__device__ __forceinline__ int qdiv(int a, int d, float rd) {
-    int q = (int)((float)a * rd);
-    const int r = a - q * d;
-    if (r < 0) q--;
-    else if (r >= d) q++;
-    return q;
-}
-
 constexpr int kMaxLinesPerBlock = 64;
 // strided lines per block at most.  8 would make each row of a load one 128-byte cache line, but fewer leave
 // room for more workgroups per CU: at 256^2 x 1024 with 8 components the y pass (length 256) takes 7.6 ms with
@@ -99,98 +51,17 @@ constexpr int kMaxLinesPerBlock = 64;
 #define GLASMA_FFT_STRIDED_LINES 4
 #endif
 constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
-// the filtered z pass of fft_3d_poisson: lines per group and whether its twiddles go to LDS.  With its two
-// transforms per group the pass gains more from a second workgroup per CU than from 64-byte rows or LDS
-// twiddles: at 256^2 x 1024 (four arrays) 2 lines with twiddles from memory (74 KiB of LDS) took 6.9-7.1 ms,
-// 4 lines with LDS twiddles (155 KiB) 8.2 ms, 4 lines with twiddles from memory 9.3 ms.
+// the filtered z pass of fft_3d_poisson: lines per group.  With its two transforms per group the pass gains more
+// from a second workgroup per CU than from 64-byte rows: at 256^2 x 1024 (four arrays) 2 lines with twiddles from
+// memory (74 KiB of LDS) took 6.9-7.1 ms, 4 lines with the full twiddle table in LDS (155 KiB) 8.2 ms, 4 lines with
+// twiddles from memory 9.3 ms.  Its twiddles now come from the two-level table (twid_split) in LDS, which fits
+// beside two lines at two workgroups per CU.
 #ifndef GLASMA_FFT_FILTER_LINES
 #define GLASMA_FFT_FILTER_LINES 2
 #endif
-#ifndef GLASMA_FFT_FILTER_TWLDS
-#define GLASMA_FFT_FILTER_TWLDS 0
-#endif
 constexpr int kFilterLines = GLASMA_FFT_FILTER_LINES;
-constexpr bool kFilterTwiddlesInLds = GLASMA_FFT_FILTER_TWLDS != 0;
 static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
               "strided lines per block: a power of two dividing the block size");
-
-// The Stockham stages over the lpb lines in buffer `cur` (line li at lb + cur * bs + li * ld); buffer c starts at
-// lb + c * bs.  Returns which buffer holds the transform.  The buffers are addressed as offsets from the LDS base, not through
-// a run-time indexed pointer array, so the compiler keeps them in the LDS address space (ds_read / ds_write);
-// a generic pointer would give flat instructions, whose waits also wait for every global load in flight.
-// transform.  Every stage ends with a barrier.
-__device__ __forceinline__ int stockham(dbl2* const lb, int bs, int cur, const Radices& R, int lpb, int ld,
-                                        int sign, const dbl2* tw) {
-    const int n = R.n;
-    int Ns = 1;
-    for (int st = 0; st < R.nst; st++) {
-        const int r = R.r[st];
-        const int nr = n / r;
-        const int tstride = n / (Ns * r);
-        const dbl2* a = lb + cur * bs;
-        dbl2* b = lb + (cur ^ 1) * bs;
-        const float rnr = 1.0f / (float)nr, rns = 1.0f / (float)Ns;
-        for (int it = threadIdx.x; it < lpb * nr; it += blockDim.x) {
-            const int li = qdiv(it, nr, rnr);
-            const int j = it - li * nr;
-            const int jq = qdiv(j, Ns, rns);
-            const int k = j - jq * Ns;
-            const dbl2* al = a + li * ld;
-            dbl2* bl = b + li * ld;
-            const int od = jq * Ns * r + k;
-            if (r == 4) {
-                dbl2 v0 = al[j], v1 = al[j + nr], v2 = al[j + 2 * nr], v3 = al[j + 3 * nr];
-                if (k) {
-                    v1 = cmul(v1, twid(tw, k * tstride, sign));
-                    v2 = cmul(v2, twid(tw, 2 * k * tstride, sign));
-                    v3 = cmul(v3, twid(tw, 3 * k * tstride, sign));
-                }
-                dbl2 t0 = v0 + v2, t1 = v0 - v2, t2 = v1 + v3, d = v1 - v3;
-                // t3 = (sign * i) * d
-                dbl2 t3 = sign < 0 ? dbl2{d.y, -d.x} : dbl2{-d.y, d.x};
-                bl[od] = t0 + t2;
-                bl[od + Ns] = t1 + t3;
-                bl[od + 2 * Ns] = t0 - t2;
-                bl[od + 3 * Ns] = t1 - t3;
-            } else if (r == 2) {
-                dbl2 v0 = al[j], v1 = al[j + nr];
-                if (k) v1 = cmul(v1, twid(tw, k * tstride, sign));
-                bl[od] = v0 + v1;
-                bl[od + Ns] = v0 - v1;
-            } else if (r == 3) {
-                dbl2 v0 = al[j], v1 = al[j + nr], v2 = al[j + 2 * nr];
-                if (k) {
-                    v1 = cmul(v1, twid(tw, k * tstride, sign));
-                    v2 = cmul(v2, twid(tw, 2 * k * tstride, sign));
-                }
-                const double h = 0.86602540378443864676;  // sqrt(3)/2
-                dbl2 t = v1 + v2, d = v1 - v2;
-                dbl2 m = v0 - 0.5 * t;
-                // (sign * i * h) * d
-                dbl2 q = sign < 0 ? dbl2{h * d.y, -h * d.x} : dbl2{-h * d.y, h * d.x};
-                bl[od] = v0 + t;
-                bl[od + Ns] = m + q;
-                bl[od + 2 * Ns] = m - q;
-            } else {
-                // a direct DFT of prime length r <= 13; the inputs are re-read from LDS for each output rather than
-                // held in a run-time indexed array (which would live in scratch memory)
-                for (int p = 0; p < r; p++) {
-                    dbl2 s = al[j];
-                    for (int q = 1; q < r; q++) {
-                        dbl2 v = al[j + q * nr];
-                        if (k) v = cmul(v, twid(tw, (k * q * tstride) % n, sign));
-                        s += cmul(v, twid(tw, ((p * q) % r) * nr, sign));
-                    }
-                    bl[od + p * Ns] = s;
-                }
-            }
-        }
-        __syncthreads();
-        cur ^= 1;
-        Ns *= r;
-    }
-    return cur;
-}
 
 // Optional momentum-space filter of a strided z pass (the Poisson step of Fourier-accelerated gauge fixing): the
 // lines are transformed forward, element k_z of the line at transverse site (x, y) is multiplied as
@@ -228,8 +99,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
                                                             long b1, long s1, long s2, int sign, int lpb, int ld,
                                                             long groups, long xcd_per, LineFilter filt,
                                                             const dbl2* __restrict__ twg) {
-    // LDS, sized per launch: two buffers of lpb * ld entries, then the twiddles (TwLds), then (Filt) the z term
-    // of p^2 for each k_z
+    // LDS, sized per launch: two buffers of lpb * ld entries, then the twiddles (TwLds: the full table; Filt: the
+    // two-level one), then (Filt) the z term of p^2 for each k_z
     extern __shared__ dbl2 lds[];
     const int bs = lpb * ld;  // buffer c starts at lds + c * bs
     dbl2* const twl = lds + 2 * lpb * ld;
@@ -249,10 +120,11 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
         gend = groups;
     }
     if (g >= gend) return;
-    if (TwLds)
-        for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = twg[k];
-    const dbl2* const tw = TwLds ? twl : twg;
-    double* const ztab = reinterpret_cast<double*>(twl + (TwLds ? n : 0));
+    // Filt: twg is the two-level table (split_entries(n) entries), always copied
+    const int ntw = Filt ? split_entries(n) : (TwLds ? n : 0);
+    for (int k = threadIdx.x; k < ntw; k += blockDim.x) twl[k] = twg[k];
+    const dbl2* const tw = (TwLds || Filt) ? twl : twg;
+    double* const ztab = reinterpret_cast<double*>(twl + ntw);
     if (Filt)
         for (int k = threadIdx.x; k < n; k += blockDim.x)
             ztab[k] = (2.0 - 2.0 * cos((filt.neumann ? 1.0 : 2.0) * kPi * k / (double)n)) / (filt.az * filt.az);
@@ -316,7 +188,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
         const long gn = g + gstep;
         const bool more = gn < gend;
         if (more) fetch(gn);  // in flight during the stages and the stores below
-        int cur = stockham(lds, bs, 0, R, lpb, ld, sign, tw);
+        int cur = stockham<Filt>(lds, bs, 0, R, lpb, ld, sign, tw);
         if (Filt) {  // strided lines only: this thread's line is myli, its positions idx >> lg_lpb
             dbl2* const fb = lds + cur * bs;
             const long t = (g * lpb + myli) % filt.P;
@@ -363,7 +235,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_lines(double* __restrict__ 
                 }
             }
             __syncthreads();
-            cur = stockham(lds, bs, cur, R, lpb, ld, -sign, tw);
+            cur = stockham<Filt>(lds, bs, cur, R, lpb, ld, -sign, tw);
         }
         const dbl2* const res = lds + cur * bs;
         const long base = start(g), lim = valid(g);
@@ -414,6 +286,41 @@ hipError_t twiddles(int n, const dbl2** out) {
     e = hipMemcpy(d, h.data(), sizeof(dbl2) * (size_t)n, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
     g_tw[key] = d;
+    *out = d;
+    return hipSuccess;
+}
+
+// two-level twiddle tables (twid_split), one per (device, n); kept for the process lifetime
+std::map<std::pair<int, int>, dbl2*> g_tws;
+
+hipError_t twiddles_split(int n, const dbl2** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    auto key = std::make_pair(dev, n);
+    auto it = g_tws.find(key);
+    if (it != g_tws.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    const int ns = split_entries(n);
+    std::vector<double> h(2 * (size_t)ns);
+    const long double tp = 6.283185307179586476925286766559005768L;
+    for (int j = 0; j < ns; j++) {
+        const long double ang = tp * (long double)(j < kTwSplit ? j : kTwSplit * (j - kTwSplit)) / (long double)n;
+        h[2 * j] = (double)cosl(ang);
+        h[2 * j + 1] = (double)(-sinl(ang));
+    }
+    dbl2* d = nullptr;
+    e = hipMalloc(&d, sizeof(dbl2) * (size_t)ns);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(d, h.data(), sizeof(dbl2) * (size_t)ns, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(d);
+        return e;
+    }
+    g_tws[key] = d;
     *out = d;
     return hipSuccess;
 }
@@ -539,6 +446,8 @@ int fft_fail_hip(hipError_t e, const std::string& what) { return fft_fail(hip_co
 
 const char* fft_last_error() { return g_fft_err.c_str(); }
 
+hipError_t fft_twiddles(int n, const dbl2** out) { return twiddles(n, out); }
+
 namespace {
 
 int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign,
@@ -550,7 +459,7 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
     if (factorize(n, R) != 0)
         return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " has a prime factor above 13");
     const dbl2* tw = nullptr;
-    const hipError_t te = twiddles(n, &tw);
+    const hipError_t te = filt.on ? twiddles_split(n, &tw) : twiddles(n, &tw);
     if (te != hipSuccess) return fft_fail_hip(te, "twiddle table");
     // contiguous lines are transformed as one packed run: line l must start at l * n
     if (es == 1 && !((s1 == n && (b1 >= nlines || s2 == b1 * n)) || (b1 == 1 && s2 == n)))
@@ -571,10 +480,11 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
     const int ld = lpb > 1 ? n + 1 : n;
     const long groups = (nlines + lpb - 1) / lpb;
     const long xcd_per = es != 1 && groups >= 64 ? (groups + 7) / 8 : 0;
-    // two buffers, and with a filter the table of its z term
-    const size_t bytes = 2 * sizeof(dbl2) * (size_t)lpb * ld + (filt.on ? sizeof(double) * n : 0);
+    // two buffers, and with a filter the two-level twiddle table and the table of its z term
+    const size_t bytes =
+        2 * sizeof(dbl2) * (size_t)lpb * ld + (filt.on ? sizeof(dbl2) * split_entries(n) + sizeof(double) * n : 0);
     if (bytes > dev.lds) return fft_fail(GLASMA_ERR_ARG, "length " + std::to_string(n) + " exceeds the LDS");
-    const bool twlds = (!filt.on || kFilterTwiddlesInLds) && bytes + sizeof(dbl2) * n <= dev.lds;
+    const bool twlds = !filt.on && bytes + sizeof(dbl2) * n <= dev.lds;
     const size_t lds = bytes + (twlds ? sizeof(dbl2) * n : 0);
     const int need = (lpb * n + kFftThreads - 1) / kFftThreads;  // elements per thread
     const int maxe = need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : 16;
@@ -610,11 +520,18 @@ int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1
 }
 
 // c[(comp*nzl + z)*P + y*nx + x]
+namespace {
+int fft_x(hipStream_t st, double* c, const Geom& g, int ncomp, int nzl, int sign) {
+    return fft_lines(st, c, g.nx, 1, (long)g.ny * nzl * ncomp, (long)g.ny * nzl * ncomp, g.nx, 0, sign);
+}
+int fft_y(hipStream_t st, double* c, const Geom& g, int ncomp, int nzl, int sign) {
+    return fft_lines(st, c, g.ny, g.nx, (long)g.nx * nzl * ncomp, g.nx, 1, (long)g.nx * g.ny, sign);
+}
+}  // namespace
+
 int fft_xy(hipStream_t st, double* c, const Geom& g, int ncomp, int nzl, int sign) {
-    const long P = (long)g.nx * g.ny;
-    int rc = fft_lines(st, c, g.nx, 1, (long)g.ny * nzl * ncomp, (long)g.ny * nzl * ncomp, g.nx, 0, sign);
-    if (rc) return rc;
-    return fft_lines(st, c, g.ny, g.nx, (long)g.nx * nzl * ncomp, g.nx, 1, P, sign);
+    const int rc = fft_x(st, c, g, ncomp, nzl, sign);
+    return rc ? rc : fft_y(st, c, g, ncomp, nzl, sign);
 }
 
 int fft_z(hipStream_t st, double* c, const Geom& g, int ncomp, int sign) {
@@ -629,7 +546,7 @@ int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign) {
 }
 
 int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3], bool drop_zero,
-                   bool neumann) {
+                   bool neumann, bool fwd_x, bool inv_x) {
     if (g.nz < 2) return fft_fail(GLASMA_ERR_ARG, "the fused Poisson filter needs Nz >= 2");
     if (neumann && !drop_zero) return fft_fail(GLASMA_ERR_ARG, "the Neumann filter drops the zero mode");
     const long P = (long)g.nx * g.ny;
@@ -640,9 +557,11 @@ int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const do
     }
     const LineFilter f = {1, g.nx, g.ny, P, a[0], a[1], a[2], 1.0 / (double)g.S, drop_zero ? 1 : 0,
                           neumann ? 1 : 0, wq};
-    int rc = fft_xy(st, c, g, ncomp, g.nz, -1);
+    int rc = fwd_x ? fft_x(st, c, g, ncomp, g.nz, -1) : 0;
+    if (!rc) rc = fft_y(st, c, g, ncomp, g.nz, -1);
     if (!rc) rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
-    if (!rc) rc = fft_xy(st, c, g, ncomp, g.nz, +1);
+    if (!rc) rc = fft_y(st, c, g, ncomp, g.nz, +1);
+    if (!rc && inv_x) rc = fft_x(st, c, g, ncomp, g.nz, +1);
     return rc;
 }
 

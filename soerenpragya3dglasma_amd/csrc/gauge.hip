@@ -31,8 +31,10 @@
 // that layer is the separate array Gtop (18 planes over one transverse plane).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
+#include "fft_lds.h"
 #include "internal.h"
 
 namespace glasma {
@@ -42,10 +44,49 @@ constexpr int kCgBlock = 256;
 
 inline dim3 grid1(long n) { return dim3((unsigned)((n + kCgBlock - 1) / kCgBlock)); }
 
+// Block order of the site kernels that read neighbours (k_cg_dev_fused, k_cg_transform, k_cg_losalamos).  A block
+// takes 256 consecutive sites of one z-plane (a unit).  With U = P / 256 units per plane divisible by 8 h, block b
+// runs on XCD b % 8 (the hardware deals workgroups to the eight XCDs in turn), which owns units [k U/8, (k+1) U/8)
+// of every plane and walks them in chunks of h units, z fastest: a site's y neighbours lie in its own unit or the
+// one before or after it on the same XCD, its z neighbours h units back or ahead, so both are still in that XCD's
+// L2 (at 256 x 256 x 1024, h = 16 units of links and G are 2.4 MB against 4 MB of L2).  In flat order a unit's y
+// neighbours run on other XCDs and its z neighbours a whole plane (37 MB) apart: the deviation kernel then read
+// 2.5 times its compulsory bytes.  Otherwise (P not a multiple of 256 sites) blocks take units in flat order.
+struct CgOrder {
+    long upx;  // units per plane per XCD; 0: flat order
+    int h;     // units per chunk
+};
+
+CgOrder cg_order(const Geom& g) {
+    static const int want = [] {
+        const char* e = std::getenv("GLASMA_CG_CHUNK");
+        const int v = e ? std::atoi(e) : 16;
+        return v >= 1 ? v : 16;
+    }();
+    if (g.P % kCgBlock) return {0, 0};
+    const long U = g.P / kCgBlock;
+    int h = want;
+    while (h > 1 && U % (8L * h)) h /= 2;
+    if (U % (8L * h)) return {0, 0};
+    return {U / 8, h};
+}
+
+// the first site of this block's unit plus threadIdx.x (callers check s < S for the flat order's last block)
+__device__ __forceinline__ long cg_site(const Geom& g, const CgOrder& o) {
+    if (o.upx == 0) return (long)blockIdx.x * kCgBlock + threadIdx.x;
+    const long b = blockIdx.x;
+    const long k = b & 7, i = b >> 3;
+    const long per_chunk = (long)g.nz * o.h;
+    const long chunk = i / per_chunk, rem = i - chunk * per_chunk;
+    const long z = rem / o.h, j = rem - z * o.h;
+    return z * g.P + (k * o.upx + chunk * o.h + j) * kCgBlock + threadIdx.x;
+}
+
 // U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu); the z neighbour of the top layer is Gtop
-__global__ void k_cg_transform(const double* __restrict__ U, const double* __restrict__ G,
-                               const double* __restrict__ Gtop, double* __restrict__ UG, Geom g) {
-    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kCgBlock) void k_cg_transform(const double* __restrict__ U, const double* __restrict__ G,
+                                                            const double* __restrict__ Gtop, double* __restrict__ UG,
+                                                            Geom g, CgOrder o) {
+    const long s = cg_site(g, o);
     if (s >= g.S) return;
     const int z = (int)(s / g.P);
     const long s2 = s - (long)z * g.P;
@@ -134,38 +175,66 @@ __device__ __forceinline__ void local_w(const double* __restrict__ UG, const Geo
     }
 }
 
+// The ten linear functions of w that the iteration reads (retr_igen's eight Re tr(i t^a w) and Re tr w) as
+// accumulators: q[0..5] = Im(w01 + w10), Re(w01 - w10), Im(w02 + w20), Re(w02 - w20), Im(w12 + w21), Re(w12 - w21);
+// q[6..8] = Im w00, Im w11, Im w22; q[9] = Re tr w.  q += c X Y^dag, element by element.
+__device__ __forceinline__ void acc_proj_nd(double c, const M3& X, const M3& Y, double* q) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            double r = 0.0, m = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                r = fma(X.re[mi(i, k)], Y.re[mi(j, k)], r);
+                r = fma(X.im[mi(i, k)], Y.im[mi(j, k)], r);
+                m = fma(X.im[mi(i, k)], Y.re[mi(j, k)], m);
+                m = fma(-X.re[mi(i, k)], Y.im[mi(j, k)], m);
+            }
+            if (i == j) {
+                q[6 + i] = fma(c, m, q[6 + i]);
+                q[9] = fma(c, r, q[9]);
+            } else {
+                const int p = i + j - 1;  // (0,1) -> 0, (0,2) -> 1, (1,2) -> 2
+                q[2 * p] = fma(c, m, q[2 * p]);
+                q[2 * p + 1] = fma(i < j ? c : -c, r, q[2 * p + 1]);
+            }
+        }
+}
+
 // local_w with the gauge-fixed links formed on the fly from U and G (the Fourier and Jacobi iterations): the
-// iteration then needs no separate link transformation writing U^G and reading it back.  Each link is formed with
-// the two products of k_cg_transform in the same order, so the values are the same.
+// iteration then needs no separate link transformation writing U^G and reading it back.  Each term goes straight into
+// the accumulators of acc_proj_nd: c (G(x) U_mu(x)) G(x + mu)^dag and c G(x) (G(x - mu) U_mu(x - mu))^dag, so that
+// neither the links nor w are held whole and four matrices stay live instead of seven (the deviation kernel had two
+// waves per SIMD, now three).  The sums differ from local_w's grouping c (A + B^dag) at round-off.
 // CONS: the boundary-consistent update (no counterpart in the reference): only links that exist enter w (open z:
 // none from the top layer to its ghost copy and none below layer 0; periodic z: the wrapped links, with the
 // current G on both ends instead of the reference's lagging ghost layer and identity below layer 0).
 template <bool CONS>
 __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, const double* __restrict__ G,
                                               const double* __restrict__ Gtop, const Geom& g, int x, int y, int z,
-                                              double cx, double cy, double cz, M3& W) {
-    M3 A, B, Gx, Um, Gn, T;
+                                              double cx, double cy, double cz, double* q) {
+    M3 Gx, Um, Gn, T;
     const double zf = g.nz > 1 ? 1.0 : 0.0;
     const long s = sidx(g, x, y, z);
-    ldm<long>(G, kGPlanes, 0, s, Gx);  // G(x), read once: it starts A and ends every backward link B
-    for (int k = 0; k < 9; k++) { W.re[k] = 0.0; W.im[k] = 0.0; }
+    ldm<long>(G, kGPlanes, 0, s, Gx);  // G(x), read once: it starts every forward term and ends every backward one
+    for (int k = 0; k < 10; k++) q[k] = 0.0;
+#pragma unroll 1
     for (int mu = 0; mu < 3; mu++) {
         const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
-        // A = U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu)
+        // U^G_mu(x) = (G(x) U_mu(x)) G^dag(x + mu)
         const bool has_up = !CONS || mu < 2 || (g.nz > 1 && (g.bc || z + 1 < g.nz));
         if (has_up) {
             ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+            mul_nn(Gx, Um, T);
             if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
             else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
             else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
             else if (CONS) ldm<long>(G, kGPlanes, 0, s - (long)z * g.P, Gn);  // periodic: layer 0 now
             else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
-            mul_nn(Gx, Um, T);
-            mul_nd(T, Gn, A);
-        } else {
-            for (int k = 0; k < 9; k++) { A.re[k] = 0.0; A.im[k] = 0.0; }
+            acc_proj_nd(c, T, Gn, q);
         }
-        // B = U^G_mu(x - mu) = G(x - mu) U_mu(x - mu) G^dag(x); below z = 0 the identity (CONS: none for open z,
+        // U^G_mu(x - mu)^dag = G(x) (G(x - mu) U_mu(x - mu))^dag; below z = 0 the identity (CONS: none for open z,
         // the top layer's link for periodic z)
         const bool has_dn = mu < 2 || z > 0 || (CONS && g.bc && g.nz > 1);
         if (has_dn) {
@@ -174,49 +243,72 @@ __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, cons
             ldm<long>(G, kGPlanes, 0, sb, Gn);
             ldm<long>(U, kUPlanes, 18 * mu, sb, Um);
             mul_nn(Gn, Um, T);
-            mul_nd(T, Gx, B);
-        } else if (CONS) {
-            for (int k = 0; k < 9; k++) { B.re[k] = 0.0; B.im[k] = 0.0; }
-        } else {
-            set_unit(B);
+            acc_proj_nd(c, Gx, T, q);
+        } else if (!CONS) {
+            q[9] += 3.0 * c;
         }
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                W.re[mi(i, j)] += c * (A.re[mi(i, j)] + B.re[mi(j, i)]);
-                W.im[mi(i, j)] += c * (A.im[mi(i, j)] - B.im[mi(j, i)]);
-            }
     }
 }
 
+// Delta^a = Re tr(i t^a w) (retr_igen) from the accumulators of acc_proj_nd
+__device__ __forceinline__ void delta_from_proj(const double* q, double* dv) {
+    dv[0] = -0.5 * q[0];
+    dv[1] = -0.5 * q[1];
+    dv[2] = -0.5 * (q[6] - q[7]);
+    dv[3] = -0.5 * q[2];
+    dv[4] = -0.5 * q[3];
+    dv[5] = -0.5 * q[4];
+    dv[6] = -0.5 * q[5];
+    dv[7] = -0.5 * (q[6] + q[7] - 2.0 * q[8]) * kInvSqrt3;
+}
+
+// XF: the forward x transform of the Fourier step fused into the deviation kernel and the inverse one into the update.
+// Each block then holds 256 / Nx whole x lines of each of the four packed arrays (Nx divides 256, S a multiple of
+// 256), so it transforms them in LDS with the Stockham stages of the transform passes and the arrays make two fewer
+// trips through HBM per iteration.
+constexpr int kXfEntries = 4 * kCgBlock;  // complex entries of one LDS buffer: the block's lines of the four arrays
+
 // Delta and the functional (CoulombGaugeDeviation.cpp:23-41, FourierAcceleration.cpp:20-52), with the monitors,
-// from the ungauged links and G.
-template <bool CONS>
-__global__ __launch_bounds__(kCgBlock) void k_cg_dev_fused(const double* __restrict__ U, const double* __restrict__ G,
+// from the ungauged links and G.  XF: the four arrays leave x-transformed (R: the factors of Nx, tw: its twiddles).
+template <bool CONS, bool XF>
+__global__ __launch_bounds__(kCgBlock, 3) void k_cg_dev_fused(const double* __restrict__ U, const double* __restrict__ G,
                                                             const double* __restrict__ Gtop, double* __restrict__ F,
                                                             double* __restrict__ part, Geom g, double cx, double cy,
-                                                            double cz, double fm0) {
-    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                            double cz, double fm0, CgOrder o, Radices R,
+                                                            const dbl2* __restrict__ tw) {
+    const long s = cg_site(g, o);
+    __shared__ dbl2 xb[XF ? 2 * kXfEntries : 1];
     double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0;
     if (s < g.S) {
         const int z = (int)(s / g.P);
         const long s2 = s - (long)z * g.P;
         const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
-        M3 W;
-        local_w_fused<CONS>(U, G, Gtop, g, x, y, z, cx, cy, cz, W);
+        double q[10];
+        local_w_fused<CONS>(U, G, Gtop, g, x, y, z, cx, cy, cz, q);
         double dv[8];
-        retr_igen(W, dv);
+        delta_from_proj(q, dv);
         for (int a = 0; a < 8; a++) {
             mx = fmax(mx, fabs(dv[a]));
             sq += dv[a] * dv[a];
         }
         // two real components per complex array: Delta^{2k} + i Delta^{2k+1} in array k
-        for (int k = 0; k < 4; k++) {
-            F[2 * ((long)k * g.S + s)] = dv[2 * k];
-            F[2 * ((long)k * g.S + s) + 1] = dv[2 * k + 1];
+        if (XF) {
+            for (int k = 0; k < 4; k++) xb[k * kCgBlock + threadIdx.x] = dbl2{dv[2 * k], dv[2 * k + 1]};
+        } else {
+            for (int k = 0; k < 4; k++) {
+                F[2 * ((long)k * g.S + s)] = dv[2 * k];
+                F[2 * ((long)k * g.S + s) + 1] = dv[2 * k + 1];
+            }
         }
-        const double lf = W.re[0] + W.re[4] + W.re[8];
+        const double lf = q[9];
         fm = fmin(fm, lf);
         fs = lf;
+    }
+    if (XF) {  // every block is whole: all threads hold a site
+        __syncthreads();
+        const int cur = stockham(xb, kXfEntries, 0, R, kXfEntries / R.n, R.n, -1, tw);
+        dbl2* const Fv = reinterpret_cast<dbl2*>(F);
+        for (int k = 0; k < 4; k++) Fv[(long)k * g.S + s] = xb[cur * kXfEntries + k * kCgBlock + threadIdx.x];
     }
     block_partials(sq, fs, mx, fm, part + 4 * (long)blockIdx.x);
 }
@@ -269,8 +361,8 @@ __device__ void max_trace_projection(const M3& V, M3& Vt) {
 __global__ __launch_bounds__(kCgBlock) void k_cg_losalamos(const double* __restrict__ UG, double* __restrict__ G,
                                                             double* __restrict__ dpart, double* __restrict__ spart,
                                                             Geom g, double cx, double cy, double cz, double fm0,
-                                                            int parity) {
-    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                            int parity, CgOrder o) {
+    const long s = cg_site(g, o);
     double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0, ssq = 0.0, sm = 0.0;
     if (s < g.S) {
         const int z = (int)(s / g.P);
@@ -330,14 +422,30 @@ __global__ void k_cg_scale(double* __restrict__ F, Geom g, double ax, double ay,
     }
 }
 
-// g <- exp(-i alpha Delta t)^dag g and the step size |3 - tr exp(...)| (FourierAcceleration.cpp:80-140)
+// g <- exp(-i alpha Delta t)^dag g and the step size |3 - tr exp(...)| (FourierAcceleration.cpp:80-140).  XF: the
+// arrays arrive without their inverse x transform, which the block runs on its lines first.
+template <bool XF>
 __global__ __launch_bounds__(kCgBlock) void k_cg_update(const double* __restrict__ F, double* __restrict__ G,
-                                                         double* __restrict__ part, Geom g, double alpha) {
-    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                         double* __restrict__ part, Geom g, double alpha, Radices R,
+                                                         const dbl2* __restrict__ tw) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ dbl2 xb[XF ? 2 * kXfEntries : 1];
     double ssq = 0.0, sm = 0.0;
+    double ub[8];
+    if (XF) {  // every block is whole
+        const dbl2* const Fv = reinterpret_cast<const dbl2*>(F);
+        for (int k = 0; k < 4; k++) xb[k * kCgBlock + threadIdx.x] = Fv[(long)k * g.S + s];
+        __syncthreads();
+        const int cur = stockham(xb, kXfEntries, 0, R, kXfEntries / R.n, R.n, +1, tw);
+        for (int k = 0; k < 4; k++) {
+            const dbl2 v = xb[cur * kXfEntries + k * kCgBlock + threadIdx.x];
+            ub[2 * k] = v.x;
+            ub[2 * k + 1] = v.y;
+        }
+    }
     if (s < g.S) {
-        double ub[8];
-        for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * g.S + s) + (a & 1)];  // packed as by k_cg_dev_fused
+        if (!XF)
+            for (int a = 0; a < 8; a++) ub[a] = F[2 * ((long)(a >> 1) * g.S + s) + (a & 1)];  // as k_cg_dev_fused packs
         M3 Wt, Gx, Gn;
         expi(-alpha, ub, Wt);
         const double tr = 3.0 - (Wt.re[0] + Wt.re[4] + Wt.re[8]);
@@ -473,7 +581,7 @@ int ensure_gauge_fixed(glasma_ctx* ctx) {
 int transform_links(glasma_ctx* ctx) {
     const Geom& g = ctx->g;
     hipLaunchKernelGGL(k_cg_transform, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], ctx->G, ctx->Gtop,
-                       ctx->UG, g);
+                       ctx->UG, g, cg_order(g));
     hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
     GL_HIP(ctx, hipGetLastError());
     return 0;
@@ -501,6 +609,13 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     const Geom& g = ctx->g;
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
     const bool cons = algo == GLASMA_GAUGE_FOURIER_CONSISTENT;
+    const CgOrder ord = cg_order(g);
+    // the x transforms fused into the deviation and update kernels (the Fourier steps with Nz >= 2)
+    Radices R{};
+    const dbl2* tw = nullptr;
+    const bool xf = (algo == GLASMA_GAUGE_FOURIER || cons) && g.nz >= 2 && g.S % kCgBlock == 0 &&
+                    kCgBlock % g.nx == 0 && factorize(g.nx, R) == 0 && fft_twiddles(g.nx, &tw) == hipSuccess;
+    (void)hipGetLastError();
     rc = ensure_red(ctx, 16 * (size_t)nblk + 8);
     if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)g.S);  // 4 complex per site
     if (rc) return rc;
@@ -526,26 +641,31 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
         if (algo == GLASMA_GAUGE_LOS_ALAMOS) {
             for (int parity = 0; parity < 2; parity++) {
                 hipLaunchKernelGGL(k_cg_losalamos, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->UG, ctx->G,
-                                   part + 4 * parity * nblk, spart + 4 * parity * nblk, g, cx, cy, cz, fm0, parity);
+                                   part + 4 * parity * nblk, spart + 4 * parity * nblk, g, cx, cy, cz, fm0, parity,
+                                   ord);
                 if ((rc = transform_links(ctx))) return rc;
             }
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, 2 * nblk, fin, fm0);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, spart, 2 * nblk, fin + 4, 0.0);
         } else {
+#define GLASMA_CG_DEV(C, X)                                                                                    \
+    hipLaunchKernelGGL((k_cg_dev_fused<C, X>), dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], \
+                       ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, ord, R, tw)
             if (cons) {
-                hipLaunchKernelGGL(k_cg_dev_fused<true>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,
-                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0);
+                if (xf) GLASMA_CG_DEV(true, true);
+                else GLASMA_CG_DEV(true, false);
             } else {
-                hipLaunchKernelGGL(k_cg_dev_fused<false>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,
-                                   ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0);
+                if (xf) GLASMA_CG_DEV(false, true);
+                else GLASMA_CG_DEV(false, false);
                 hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
             }
+#undef GLASMA_CG_DEV
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
             if (cons) {
                 // the Laplacian of the links that exist (open z: the Neumann Laplacian, a cosine transform in z); the
                 // zero mode (zero by construction) dropped
                 if (g.nz >= 2) {
-                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a, true, g.bc == 0);
+                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a, true, g.bc == 0, !xf, !xf);
                 } else {
                     rc = fft_3d(ctx->stream, F, g, 4, -1);
                     if (!rc) {
@@ -558,7 +678,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             }
             if (algo == GLASMA_GAUGE_FOURIER) {
                 if (g.nz >= 2) {
-                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a);
+                    rc = fft_3d_poisson(ctx->stream, F, g, 4, a, false, false, !xf, !xf);
                 } else {
                     rc = fft_3d(ctx->stream, F, g, 4, -1);
                     if (!rc) {
@@ -569,8 +689,12 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
                 }
                 if (rc) return set_error(ctx, rc, std::string("gauge fixing FFT failed: ") + fft_last_error());
             }
-            hipLaunchKernelGGL(k_cg_update, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G, part, g,
-                               param);
+            if (xf)
+                hipLaunchKernelGGL(k_cg_update<true>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G,
+                                   part, g, param, R, tw);
+            else
+                hipLaunchKernelGGL(k_cg_update<false>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G,
+                                   part, g, param, R, tw);
             hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
             GL_HIP(ctx, hipGetLastError());
         }

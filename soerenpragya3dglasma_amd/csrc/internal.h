@@ -119,9 +119,10 @@ int fft_3d(hipStream_t st, double* c, const Geom& g, int ncomp, int sign);
 // forward 3D transform, c~ -> c~ / (V p^2) for p^2 = sum_mu (2 - 2 cos(2 pi k_mu / N_mu)) / a_mu^2 > 0 (the zero
 // mode unscaled, or set to 0 with drop_zero), inverse 3D transform; the z transforms and the filter run in one pass
 // (Nz >= 2).  neumann (with drop_zero): in z the cosine transform, p_z^2 = (2 - 2 cos(pi k_z / N_z)) / a_z^2, which
-// is the periodic step of the lattice doubled by its mirror image in z, restricted to the lattice.
+// is the periodic step of the lattice doubled by its mirror image in z, restricted to the lattice.  fwd_x / inv_x
+// false: the caller runs the forward / inverse x transform itself (fused into the kernels either side).
 int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const double a[3], bool drop_zero = false,
-                   bool neumann = false);
+                   bool neumann = false, bool fwd_x = true, bool inv_x = true);
 
 }  // namespace glasma
 
