@@ -29,6 +29,7 @@
 // written, so U^G(z = -1) is the identity; the gauge transformation's top ghost layer is synchronised
 // after each link transformation, so a transformation sees the top layer of the previous iteration.  Here
 // that layer is the separate array Gtop (18 planes over one transverse plane).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -82,9 +83,12 @@ __device__ __forceinline__ long cg_site(const Geom& g, const CgOrder& o) {
     return z * g.P + (k * o.upx + chunk * o.h + j) * kCgBlock + threadIdx.x;
 }
 
-// U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu); the z neighbour of the top layer is Gtop
+// U^G_mu(x) = G(x) U_mu(x) G^dag(x + mu); the z neighbour of the top layer is Gtop.  EF (the end of a gauge-fixing
+// call): also E^G_mu(x) = G(x) E_mu(x) G^dag(x), with G(x) read once for both.
+template <bool EF>
 __global__ __launch_bounds__(kCgBlock) void k_cg_transform(const double* __restrict__ U, const double* __restrict__ G,
                                                             const double* __restrict__ Gtop, double* __restrict__ UG,
+                                                            const double* __restrict__ E, double* __restrict__ EG,
                                                             Geom g, CgOrder o) {
     const long s = cg_site(g, o);
     if (s >= g.S) return;
@@ -103,6 +107,13 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_transform(const double* __restr
         mul_nd(T, Gn, R);
         stm<long>(UG, kUPlanes, 18 * mu, s, R);
     }
+    if (EF)
+        for (int mu = 0; mu < 3; mu++) {
+            double e[8], f[8];
+            ldv<long>(E, kEPlanes, 8 * mu, s, e);
+            adj_mul(Gx, e, f);
+            stv<long>(EG, kEPlanes, 8 * mu, s, f);
+        }
 }
 
 // ghost synchronisation of G's top layer: open -> copy of layer Nz-1, periodic -> copy of layer 0
@@ -460,32 +471,33 @@ __global__ __launch_bounds__(kCgBlock) void k_cg_update(const double* __restrict
     block_partials(ssq, 0.0, sm, 0.0, part + 4 * (long)blockIdx.x);
 }
 
-// fixed-order second level: one block of 256 threads, strided per-thread sums then the block tree
-__global__ __launch_bounds__(kCgBlock) void k_cg_reduce(const double* __restrict__ part, long nblk,
+// fixed-order reduction of block partials: block i takes partials [i per, (i + 1) per), strided per-thread sums then the
+// block tree, into out + 4 i
+__global__ __launch_bounds__(kCgBlock) void k_cg_reduce(const double* __restrict__ part, long nblk, long per,
                                                          double* __restrict__ out, double fm0) {
     double s0 = 0, s1 = 0, m0 = 0, m1 = fm0;
-    for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const long end = min(nblk, (long)(blockIdx.x + 1) * per);
+    for (long b = (long)blockIdx.x * per + threadIdx.x; b < end; b += blockDim.x) {
         s0 += part[4 * b];
         s1 += part[4 * b + 1];
         m0 = fmax(m0, part[4 * b + 2]);
         m1 = fmin(m1, part[4 * b + 3]);
     }
-    block_partials(s0, s1, m0, m1, out);
+    block_partials(s0, s1, m0, m1, out + 4 * (long)blockIdx.x);
 }
 
-// E^G_mu(x) = G(x) E_mu(x) G^dag(x)
-__global__ void k_cg_efield(const double* __restrict__ G, const double* __restrict__ E, double* __restrict__ EG,
-                            Geom g) {
-    long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= g.S) return;
-    M3 Gx;
-    ldm<long>(G, kGPlanes, 0, s, Gx);
-    for (int mu = 0; mu < 3; mu++) {
-        double e[8], o[8];
-        ldv<long>(E, kEPlanes, 8 * mu, s, e);
-        adj_mul(Gx, e, o);
-        stv<long>(EG, kEPlanes, 8 * mu, s, o);
+// the four monitors of nblk block partials into out[0..3]: one block up to 4096 partials, else up to 256 blocks of
+// 1024 or more into tmp (4 * 256 doubles) and one block over those (a single block over 262144 partials took 0.25 ms)
+constexpr long kReduceTmp = 4 * 256;
+void reduce_partials(hipStream_t st, const double* part, long nblk, double* out, double fm0, double* tmp) {
+    if (nblk <= 4096) {
+        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, st, part, nblk, nblk, out, fm0);
+        return;
     }
+    const long nb2 = std::min(256L, (nblk + 1023) / 1024);
+    const long per = (nblk + nb2 - 1) / nb2;
+    hipLaunchKernelGGL(k_cg_reduce, dim3((unsigned)nb2), dim3(kCgBlock), 0, st, part, nblk, per, tmp, fm0);
+    hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, st, tmp, nb2, nb2, out, fm0);
 }
 
 // ---- Gauss-law restoration (GAUSSLAWFIXING/GaussRestore.cpp, GaussRestore2D.cpp) -------------------------
@@ -578,10 +590,15 @@ int ensure_gauge_fixed(glasma_ctx* ctx) {
     return 0;
 }
 
-int transform_links(glasma_ctx* ctx) {
+// U^G from U and G (with_e: E^G from E as well), then G's top ghost layer synchronised
+int transform_links(glasma_ctx* ctx, bool with_e = false) {
     const Geom& g = ctx->g;
-    hipLaunchKernelGGL(k_cg_transform, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], ctx->G, ctx->Gtop,
-                       ctx->UG, g, cg_order(g));
+    if (with_e)
+        hipLaunchKernelGGL(k_cg_transform<true>, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], ctx->G,
+                           ctx->Gtop, ctx->UG, ctx->E, ctx->EG, g, cg_order(g));
+    else
+        hipLaunchKernelGGL(k_cg_transform<false>, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], ctx->G,
+                           ctx->Gtop, ctx->UG, nullptr, nullptr, g, cg_order(g));
     hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
     GL_HIP(ctx, hipGetLastError());
     return 0;
@@ -616,7 +633,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     const bool xf = (algo == GLASMA_GAUGE_FOURIER || cons) && g.nz >= 2 && g.S % kCgBlock == 0 &&
                     kCgBlock % g.nx == 0 && factorize(g.nx, R) == 0 && fft_twiddles(g.nx, &tw) == hipSuccess;
     (void)hipGetLastError();
-    rc = ensure_red(ctx, 16 * (size_t)nblk + 8);
+    rc = ensure_red(ctx, 16 * (size_t)nblk + 8 + kReduceTmp);
     if (!rc && algo != GLASMA_GAUGE_LOS_ALAMOS) rc = ensure_fftbuf(ctx, 8 * (size_t)g.S);  // 4 complex per site
     if (rc) return rc;
     const double* a = ctx->p.a;
@@ -627,6 +644,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     double* part = ctx->red;                // 2 * nblk blocks of 4 (Los Alamos: both half sweeps)
     double* spart = ctx->red + 8 * nblk;    // Los Alamos step monitors
     double* fin = ctx->red + 16 * nblk;
+    double* tmp = fin + 8;
     double* F = ctx->fftbuf;
     // Los Alamos sweeps read U^G; the Fourier and Jacobi iterations form it on the fly (k_cg_dev_fused) and keep
     // the reference's lag of the top ghost layer by synchronising Gtop right after each deviation kernel, where
@@ -645,8 +663,8 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
                                    ord);
                 if ((rc = transform_links(ctx))) return rc;
             }
-            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, 2 * nblk, fin, fm0);
-            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, spart, 2 * nblk, fin + 4, 0.0);
+            reduce_partials(ctx->stream, part, 2 * nblk, fin, fm0, tmp);
+            reduce_partials(ctx->stream, spart, 2 * nblk, fin + 4, 0.0, tmp);
         } else {
 #define GLASMA_CG_DEV(C, X)                                                                                    \
     hipLaunchKernelGGL((k_cg_dev_fused<C, X>), dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], \
@@ -660,7 +678,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
                 hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
             }
 #undef GLASMA_CG_DEV
-            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, fm0);
+            reduce_partials(ctx->stream, part, nblk, fin, fm0, tmp);
             if (cons) {
                 // the Laplacian of the links that exist (open z: the Neumann Laplacian, a cosine transform in z); the
                 // zero mode (zero by construction) dropped
@@ -695,7 +713,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             else
                 hipLaunchKernelGGL(k_cg_update<false>, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, F, ctx->G,
                                    part, g, param, R, tw);
-            hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin + 4, 0.0);
+            reduce_partials(ctx->stream, part, nblk, fin + 4, 0.0, tmp);
             GL_HIP(ctx, hipGetLastError());
         }
         GL_HIP(ctx, hipMemcpyAsync(h, fin, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -711,9 +729,7 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     // the gauge-fixed links, transformed with the final G and (as after the reference's last transformation) the
     // top ghost layer synchronised to it
     if (fused) hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
-    if ((rc = transform_links(ctx))) return rc;
-    hipLaunchKernelGGL(k_cg_efield, grid1(g.S), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->E, ctx->EG, g);
-    GL_HIP(ctx, hipGetLastError());
+    if ((rc = transform_links(ctx, true))) return rc;
     GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     // the iteration cap reached above the tolerance: success as the reference's SetCoulombGauge returns, with a
     // message in glasma_last_error for callers that check (the reference's update cannot converge for Nz > 1)
@@ -742,7 +758,7 @@ extern "C" int glasma_gauss_restore(glasma_ctx* ctx, int transverse_only, double
     if (maxit < 0 || !(tol >= 0.0)) return set_error(ctx, GLASMA_ERR_ARG, "invalid tolerance or iteration cap");
     const Geom& g = ctx->g;
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
-    int rc = ensure_red(ctx, 4 * (size_t)nblk + 8);
+    int rc = ensure_red(ctx, 4 * (size_t)nblk + 8 + kReduceTmp);
     if (!rc) rc = ensure_fftbuf(ctx, 8 * (size_t)pad_sites(g.S));
     if (rc) return rc;
     const double norm = 1.0 / (ctx->p.a[0] * ctx->p.a[1] * ctx->p.a[2]);
@@ -755,7 +771,7 @@ extern "C" int glasma_gauss_restore(glasma_ctx* ctx, int transverse_only, double
     auto deviation = [&]() {
         hipLaunchKernelGGL(k_gr_violation, dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, U, ctx->E, J, Gv, part,
                            g, norm);
-        hipLaunchKernelGGL(k_cg_reduce, dim3(1), dim3(kCgBlock), 0, ctx->stream, part, nblk, fin, 0.0);
+        reduce_partials(ctx->stream, part, nblk, fin, 0.0, fin + 8);
     };
     double h[4];
     auto read = [&]() -> int {
