@@ -60,6 +60,11 @@ constexpr int kStridedLines = GLASMA_FFT_STRIDED_LINES;
 #define GLASMA_FFT_FILTER_LINES 2
 #endif
 constexpr int kFilterLines = GLASMA_FFT_FILTER_LINES;
+// the radix-16 filtered z pass (k_zpoisson) where it applies; 0 keeps the generic pass everywhere
+#ifndef GLASMA_FFT_ZPOISSON
+#define GLASMA_FFT_ZPOISSON 1
+#endif
+constexpr bool kZPoisson = GLASMA_FFT_ZPOISSON != 0;
 static_assert(kStridedLines >= 1 && kStridedLines <= 64 && (kStridedLines & (kStridedLines - 1)) == 0,
               "strided lines per block: a power of two dividing the block size");
 
@@ -450,6 +455,393 @@ hipError_t fft_twiddles(int n, const dbl2** out) { return twiddles(n, out); }
 
 namespace {
 
+// ---- the filtered z pass for Nz = 256 R3, R3 in {1, 2, 4} -------------------------------------------------------
+// One group is 4096 entries: L = 16 / R3 neighbouring z lines (64-byte rows at Nz = 1024), in place in one LDS
+// buffer, so that two workgroups fit per CU.  The transforms are Stockham stages of radix 16, 16 and R3 whose
+// butterflies run in registers (one radix-16 butterfly per thread), three stages per transform where the generic
+// pass takes five radix-4 ones: the first forward stage works on the registers the loads filled, the last inverse
+// stage stores from registers, the stages between read all their inputs, wait, and write all their outputs.  The
+// plain filter is applied as the last forward stage writes; the Neumann filter (even-odd lines) runs its pair step
+// between the transforms.
+
+// DFT of length R in registers, natural order in and out: V_k = sum_m v_m w^(m k), w = exp(sign 2 pi i / R)
+__device__ __forceinline__ dbl2 mul_si(dbl2 a, int sign) { return sign < 0 ? dbl2{a.y, -a.x} : dbl2{-a.y, a.x}; }
+__device__ __forceinline__ void dft2(dbl2& a, dbl2& b) {
+    const dbl2 t = a - b;
+    a = a + b;
+    b = t;
+}
+__device__ __forceinline__ void dft4(dbl2& v0, dbl2& v1, dbl2& v2, dbl2& v3, int sign) {
+    const dbl2 t0 = v0 + v2, t1 = v0 - v2, t2 = v1 + v3, t3 = mul_si(v1 - v3, sign);
+    v0 = t0 + t2;
+    v1 = t1 + t3;
+    v2 = t0 - t2;
+    v3 = t1 - t3;
+}
+// exp(sign 2 pi i e / 16), e < 16: cos and sin of e pi / 8
+__device__ __forceinline__ dbl2 w16(int e, int sign) {
+    constexpr double c1 = 0.92387953251128675613, c2 = 0.70710678118654752440, c3 = 0.38268343236508977173;
+    const double cs[16] = {1.0, c1, c2, c3, 0.0, -c3, -c2, -c1, -1.0, -c1, -c2, -c3, 0.0, c3, c2, c1};
+    const double sn = cs[(e + 12) & 15];  // sin(x) = cos(x - pi / 2)
+    return dbl2{cs[e & 15], sign < 0 ? -sn : sn};
+}
+template <int R>
+__device__ __forceinline__ void dft(dbl2* v, int sign);
+template <>
+__device__ __forceinline__ void dft<1>(dbl2*, int) {}
+template <>
+__device__ __forceinline__ void dft<2>(dbl2* v, int) { dft2(v[0], v[1]); }
+template <>
+__device__ __forceinline__ void dft<4>(dbl2* v, int sign) { dft4(v[0], v[1], v[2], v[3], sign); }
+// R = 16 as 4 x 4: m = q + 4 m2, k = k1 + 4 k2; inner DFT_4 over m2, twiddle w16^(q k1), outer DFT_4 over q.  The
+// outputs stay where the outer transforms leave them: V_k in v[oidx<16>(k)] = v[4 (k % 4) + k / 4].
+template <>
+__device__ __forceinline__ void dft<16>(dbl2* v, int sign) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) dft4(v[q], v[q + 4], v[q + 8], v[q + 12], sign);  // a[q][k1] now at v[q + 4 k1]
+#pragma unroll
+    for (int q = 1; q < 4; q++)
+#pragma unroll
+        for (int k1 = 1; k1 < 4; k1++) v[q + 4 * k1] = cmul(v[q + 4 * k1], w16(q * k1, sign));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3], sign);
+}
+// register of output k after dft<R>
+template <int R>
+__device__ __forceinline__ constexpr int oidx(int k) {
+    return R == 16 ? 4 * (k & 3) + (k >> 2) : k;
+}
+
+// v[m] *= w^(m K) for m = 1 .. R - 1, w = exp(sign 2 pi i / n), from the table entries of K, 2K, 4K and 8K (each
+// power at most three products from the table)
+template <int R>
+__device__ __forceinline__ void twiddle_run(dbl2* v, int K, int sign, const dbl2* tw) {
+    const dbl2 w1 = twid_split(tw, K, sign);
+    if (R == 2) {
+        v[1] = cmul(v[1], w1);
+        return;
+    }
+    const dbl2 w2 = twid_split(tw, 2 * K, sign);
+    const dbl2 w3 = cmul(w2, w1);
+    v[1] = cmul(v[1], w1);
+    v[2] = cmul(v[2], w2);
+    v[3] = cmul(v[3], w3);
+    if (R == 4) return;
+    const dbl2 w4 = twid_split(tw, 4 * K, sign);
+    v[4] = cmul(v[4], w4);
+    v[5] = cmul(v[5], cmul(w4, w1));
+    v[6] = cmul(v[6], cmul(w4, w2));
+    v[7] = cmul(v[7], cmul(w4, w3));
+    if (R == 8) return;
+    const dbl2 w8 = twid_split(tw, 8 * K, sign);
+    v[8] = cmul(v[8], w8);
+    v[9] = cmul(v[9], cmul(w8, w1));
+    v[10] = cmul(v[10], cmul(w8, w2));
+    v[11] = cmul(v[11], cmul(w8, w3));
+    const dbl2 w12 = cmul(w8, w4);
+    v[12] = cmul(v[12], w12);
+    v[13] = cmul(v[13], cmul(w12, w1));
+    v[14] = cmul(v[14], cmul(w12, w2));
+    v[15] = cmul(v[15], cmul(w12, w3));
+}
+
+// padded LDS position of entry e of a line: one spare entry per 16, so the stride-16 writes of the first stage fall
+// on distinct banks
+__device__ __forceinline__ int zp_pidx(int e) { return e + (e >> 4); }
+
+// One Stockham stage of radix R over the group in LDS (line li at lds + li * ld): butterfly j < n / R of a line takes
+// entries j + m n / R, twiddles them by w^(m k n / (Ns R)), k = j % Ns, transforms them and puts output m at
+// (j - k) R + k + m Ns.  Butterflies b = threadIdx.x + 256 i, line b / (n / R).  Reads all, waits, writes all, waits.
+// n / R, and Ns unless it is 1, are multiples of 16, so zp_pidx of the m-th entry is that of the first plus m times
+// a constant: the offsets fold into the LDS instructions.
+template <int R, int n, int Ns>
+__device__ __forceinline__ void zp_stage(dbl2* lds, int ld, int sign, const dbl2* tw) {
+    constexpr int nr = n / R;
+    constexpr int B = 4096 / R / 256;  // butterflies per thread
+    static_assert(nr % 16 == 0 && (Ns == 1 || Ns % 16 == 0), "strides of the padded layout");
+    constexpr int rs = nr + nr / 16;                  // padded read stride
+    constexpr int ws = Ns == 1 ? 1 : Ns + Ns / 16;    // padded write stride (Ns = 1: od is a multiple of R = 16)
+    dbl2 v[B][R];
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        const int b = threadIdx.x + 256 * i, li = b / nr, j = b - li * nr;
+        const dbl2* const src = lds + li * ld + zp_pidx(j);
+#pragma unroll
+        for (int m = 0; m < R; m++) v[i][m] = src[m * rs];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        const int b = threadIdx.x + 256 * i, li = b / nr, j = b - li * nr;
+        const int k = j % Ns;
+        if (Ns > 1 && k) twiddle_run<R>(v[i], k * (n / (Ns * R)), sign, tw);
+        dft<R>(v[i], sign);
+        dbl2* const dst = lds + li * ld + zp_pidx((j - k) * R + k);
+#pragma unroll
+        for (int m = 0; m < R; m++) dst[m * ws] = v[i][oidx<R>(m)];
+    }
+    __syncthreads();
+}
+
+// the last stage of a transform (Ns = n / R, so butterfly j leaves its outputs at j + m Ns in natural order):
+// butterflies b = threadIdx.x + 256 i with line b % L and j = b / L, the layout of the loads and stores; the
+// outputs stay in registers, output m of butterfly i in v[i][oidx<R>(m)]
+template <int R, int n, int L>
+__device__ __forceinline__ void zp_last(const dbl2* lds, int ld, int sign, const dbl2* tw, dbl2 (*v)[R]) {
+    constexpr int nr = n / R;
+    constexpr int B = 4096 / R / 256;
+    static_assert(nr % 16 == 0, "strides of the padded layout");
+    constexpr int rs = nr + nr / 16;
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        const int b = threadIdx.x + 256 * i, li = b % L, j = b / L;
+        const dbl2* const src = lds + li * ld + zp_pidx(j);
+#pragma unroll
+        for (int m = 0; m < R; m++) v[i][m] = src[m * rs];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < B; i++) {
+        const int j = (threadIdx.x + 256 * i) / L;  // k = j
+        if (j && R > 1) twiddle_run<R>(v[i], j, sign, tw);
+        dft<R>(v[i], sign);
+    }
+}
+
+// the entry of the natural line held at position e of the even-odd line (the inverse of nperm)
+__device__ __forceinline__ int zp_iperm(int e, int n) { return 2 * e < n ? 2 * e : 2 * (n - 1 - e) + 1; }
+
+template <int R3, bool NEU>
+__global__ __launch_bounds__(256) void k_zpoisson(dbl2* __restrict__ dv, long P, long groups, long xcd_per,
+                                                  LineFilter filt, const dbl2* __restrict__ twg) {
+    constexpr int n = 256 * R3;
+    constexpr int L = 16 / R3;
+    constexpr int nr1 = n / 16;
+    constexpr int B3 = 16 / R3;  // last-stage butterflies per thread (radix R3)
+    extern __shared__ dbl2 lds[];
+    const int ld = n + n / 16 + 4;  // padded line length: lines L apart start 4 banks apart modulo 64
+    dbl2* const tw = lds + L * ld;
+    double* const ztab = reinterpret_cast<double*>(tw + split_entries(n));
+    double* const pxyt = ztab + n;  // the transverse term of p^2 of each line of the group
+    long g, gstep, gend;
+    if (xcd_per > 0) {
+        const long kx = blockIdx.x % 8;
+        g = kx * xcd_per + blockIdx.x / 8;
+        gstep = gridDim.x / 8;
+        gend = min((kx + 1) * xcd_per, groups);
+    } else {
+        g = blockIdx.x;
+        gstep = gridDim.x;
+        gend = groups;
+    }
+    if (g >= gend) return;
+    for (int k = threadIdx.x; k < split_entries(n); k += 256) tw[k] = twg[k];
+    for (int k = threadIdx.x; k < n; k += 256)
+        ztab[k] = (2.0 - 2.0 * cos((NEU ? 1.0 : 2.0) * kPi * k / (double)n)) / (filt.az * filt.az);
+    // the register stages: line t % L, butterfly t / L of the radix-16 stage
+    const int li0 = threadIdx.x % L, j0 = threadIdx.x / L;
+    auto line_base = [&](long gg, int li) -> long {
+        const long l = gg * L + li;
+        return (l / P) * P * n + l % P;
+    };
+    auto set_pxy = [&](long gg) {  // before the first barrier of the group; read after it
+        if (threadIdx.x < L) {
+            const long t = (gg * L + threadIdx.x) % P;
+            const int px = (int)(t % filt.nx), py = (int)(t / filt.nx);
+            pxyt[threadIdx.x] = (2.0 - 2.0 * cos(2.0 * kPi * px / (double)filt.nx)) / (filt.ax * filt.ax) +
+                                (2.0 - 2.0 * cos(2.0 * kPi * py / (double)filt.ny)) / (filt.ay * filt.ay);
+        }
+    };
+    dbl2 pre[16];
+    auto fetch = [&](long gg) {
+        const long base = line_base(gg, li0);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+            const int e = j0 + m * nr1;
+            pre[m] = dv[base + (long)(NEU ? zp_iperm(e, n) : e) * P];
+        }
+    };
+    fetch(g);
+    __syncthreads();  // the tables
+    while (true) {
+        // forward stage 1 (radix 16, Ns = 1) on the loaded registers
+        set_pxy(g);
+        dft<16>(pre, -1);
+        {
+            dbl2* const dst = lds + li0 * ld + 17 * j0;  // zp_pidx(16 j0 + m) = 17 j0 + m
+#pragma unroll
+            for (int m = 0; m < 16; m++) dst[m] = pre[oidx<16>(m)];
+        }
+        const long gn = g + gstep;
+        const bool more = gn < gend;
+        if (more) fetch(gn);  // in flight during the stages and the stores below
+        __syncthreads();
+        // forward stage 2 (radix 16, Ns = 16), then the last one (radix R3, Ns = 256) with the plain filter
+        if (R3 > 1) {
+            zp_stage<16, n, 16>(lds, ld, -1, tw);
+            dbl2 v[B3][R3];
+            zp_last<R3, n, L>(lds, ld, -1, tw, v);
+#pragma unroll
+            for (int i = 0; i < B3; i++) {
+                const int b = threadIdx.x + 256 * i, li = b % L, j = b / L;
+                const double pxy = pxyt[li];
+                dbl2* const dst = lds + li * ld + zp_pidx(j);
+#pragma unroll
+                for (int m = 0; m < R3; m++) {
+                    const int e = j + 256 * m;
+                    dbl2 w = v[i][oidx<R3>(m)];
+                    if (!NEU) {
+                        const double p2 = pxy + ztab[e];
+                        w = (p2 > 0.0 ? filt.nf / p2 : (filt.drop_zero ? 0.0 : 1.0)) * w;
+                    }
+                    dst[272 * m] = w;  // zp_pidx(j + 256 m)
+                }
+            }
+        } else {
+            dbl2 v[1][16];
+            zp_last<16, n, L>(lds, ld, -1, tw, v);
+            const int li = threadIdx.x % L, j = threadIdx.x / L;
+            const double pxy = pxyt[li];
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int e = j + 16 * m;
+                dbl2 w = v[0][oidx<16>(m)];
+                if (!NEU) {
+                    const double p2 = pxy + ztab[e];
+                    w = (p2 > 0.0 ? filt.nf / p2 : (filt.drop_zero ? 0.0 : 1.0)) * w;
+                }
+                lds[li * ld + j + 17 * m] = w;  // zp_pidx(j + 16 m), j < 16
+            }
+        }
+        __syncthreads();
+        if (NEU) {
+            // the pair step of the cosine transform (see the generic pass): X_k = (w_k V_k + conj(w_k) V_{n-k}) / 2,
+            // Y = h X, Q_k = conj(w_k) (Y_k - i Y_{n-k})
+            constexpr int np = n / 2 + 1;
+            for (int it = threadIdx.x; it < L * np; it += 256) {
+                const int li = it / np, k = it - li * np;
+                const double pxy = pxyt[li];
+                dbl2* const row = lds + li * ld;
+                const int kr = k == 0 ? 0 : n - k;
+                const dbl2 vk = row[zp_pidx(k)], vr = row[zp_pidx(kr)];
+                const dbl2 wk = filt.wq[k], wr = filt.wq[kr];
+                const dbl2 cwk = {wk.x, -wk.y}, cwr = {wr.x, -wr.y};
+                const double pk = pxy + ztab[k], pr = pxy + ztab[kr];
+                const double hk = pk > 0.0 ? filt.nf / pk : 0.0, hr = pr > 0.0 ? filt.nf / pr : 0.0;
+                const dbl2 yk = (0.5 * hk) * (cmul(wk, vk) + cmul(cwk, vr));
+                const dbl2 yr = (0.5 * hr) * (cmul(wr, vr) + cmul(cwr, vk));
+                if (k == 0) {
+                    row[0] = yk;
+                } else {
+                    row[zp_pidx(k)] = cmul(cwk, dbl2{yk.x + yr.y, yk.y - yr.x});
+                    row[zp_pidx(kr)] = cmul(cwr, dbl2{yr.x + yk.y, yr.y - yk.x});
+                }
+            }
+            __syncthreads();
+        }
+        // inverse: stage 1 (radix 16, Ns = 1), stage 2 (radix 16, Ns = 16), the last stage from LDS to memory
+        zp_stage<16, n, 1>(lds, ld, +1, tw);
+        if (R3 > 1) {
+            zp_stage<16, n, 16>(lds, ld, +1, tw);
+            dbl2 v[B3][R3];
+            zp_last<R3, n, L>(lds, ld, +1, tw, v);
+#pragma unroll
+            for (int i = 0; i < B3; i++) {
+                const int b = threadIdx.x + 256 * i, li = b % L, j = b / L;
+                const long base = line_base(g, li);
+#pragma unroll
+                for (int m = 0; m < R3; m++) {
+                    const int e = j + 256 * m;
+                    dv[base + (long)(NEU ? zp_iperm(e, n) : e) * P] = v[i][oidx<R3>(m)];
+                }
+            }
+        } else {
+            dbl2 v[1][16];
+            zp_last<16, n, L>(lds, ld, +1, tw, v);
+            const int li = threadIdx.x % L, j = threadIdx.x / L;
+            const long base = line_base(g, li);
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+                const int e = j + 16 * m;
+                dv[base + (long)(NEU ? zp_iperm(e, n) : e) * P] = v[0][oidx<16>(m)];
+            }
+        }
+        if (!more) break;
+        __syncthreads();  // every read of this group's LDS precedes the next group's writes
+        g = gn;
+    }
+}
+
+// the dynamic LDS limit of the k_zpoisson instantiations raised to the device's, once per device
+std::map<int, bool> g_zp_allowed;
+template <int R3, bool NEU>
+hipError_t zp_allow_one(int v) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zpoisson<R3, NEU>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, v);
+}
+hipError_t zp_allow(size_t lds_max) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_tw_mu);
+    if (g_zp_allowed.count(dev)) return hipSuccess;
+    const int v = (int)lds_max;
+    for (hipError_t r : {zp_allow_one<1, false>(v), zp_allow_one<1, true>(v), zp_allow_one<2, false>(v),
+                         zp_allow_one<2, true>(v), zp_allow_one<4, false>(v), zp_allow_one<4, true>(v)})
+        if (r != hipSuccess) return r;
+    g_zp_allowed[dev] = true;
+    return hipSuccess;
+}
+
+template <int R3, bool NEU>
+hipError_t launch_zpoisson(hipStream_t st, size_t lds, int cus, double* c, long P, long groups,
+                           const LineFilter& f, const dbl2* tw) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&k_zpoisson<R3, NEU>), 256, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const long xcd_per = groups >= 64 ? (groups + 7) / 8 : 0;
+    const long resident = (long)cus * per_cu;
+    const long blocks = xcd_per > 0 ? 8 * std::min(xcd_per, (resident + 7) / 8) : std::min(groups, resident);
+    hipLaunchKernelGGL((k_zpoisson<R3, NEU>), dim3((unsigned)blocks), dim3(256), lds, st, reinterpret_cast<dbl2*>(c), P,
+                       groups, xcd_per, f, tw);
+    return hipGetLastError();
+}
+
+// the filtered z pass by k_zpoisson where it applies (Nz in {256, 512, 1024}, P a multiple of 16 / (Nz / 256));
+// returns 1 where it does not
+int zpoisson(hipStream_t st, double* c, const Geom& g, int ncomp, const LineFilter& f) {
+    const int n = g.nz;
+    if (n != 256 && n != 512 && n != 1024) return 1;
+    const int L = 4096 / n;
+    const long P = (long)g.nx * g.ny;
+    if (P % L) return 1;
+    FftDevice dev;
+    const hipError_t de = fft_device(&dev);
+    if (de != hipSuccess) return fft_fail_hip(de, "device query");
+    const size_t lds = sizeof(dbl2) * ((size_t)L * (n + n / 16 + 4) + split_entries(n)) + sizeof(double) * (n + L);
+    if (lds > dev.lds) return 1;
+    const hipError_t ae = zp_allow(dev.lds);
+    if (ae != hipSuccess) return fft_fail_hip(ae, "k_zpoisson LDS limit");
+    const dbl2* tw = nullptr;
+    const hipError_t te = twiddles_split(n, &tw);
+    if (te != hipSuccess) return fft_fail_hip(te, "twiddle table");
+    const long groups = P * ncomp / L;
+    hipError_t ke;
+    const bool neu = f.neumann != 0;
+    if (n == 256) ke = neu ? launch_zpoisson<1, true>(st, lds, dev.cus, c, P, groups, f, tw)
+                           : launch_zpoisson<1, false>(st, lds, dev.cus, c, P, groups, f, tw);
+    else if (n == 512) ke = neu ? launch_zpoisson<2, true>(st, lds, dev.cus, c, P, groups, f, tw)
+                                : launch_zpoisson<2, false>(st, lds, dev.cus, c, P, groups, f, tw);
+    else ke = neu ? launch_zpoisson<4, true>(st, lds, dev.cus, c, P, groups, f, tw)
+                  : launch_zpoisson<4, false>(st, lds, dev.cus, c, P, groups, f, tw);
+    if (ke != hipSuccess)
+        return fft_fail_hip(ke, "k_zpoisson launch (length " + std::to_string(n) + ", " + std::to_string(lds) +
+                                    " bytes of LDS)");
+    return 0;
+}
+
 int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign,
                        const LineFilter& filt) {
     if (n <= 1 || nlines <= 0) return 0;
@@ -559,7 +951,12 @@ int fft_3d_poisson(hipStream_t st, double* c, const Geom& g, int ncomp, const do
                           neumann ? 1 : 0, wq};
     int rc = fwd_x ? fft_x(st, c, g, ncomp, g.nz, -1) : 0;
     if (!rc) rc = fft_y(st, c, g, ncomp, g.nz, -1);
-    if (!rc) rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
+    if (!rc && kZPoisson) {
+        rc = zpoisson(st, c, g, ncomp, f);
+        if (rc == 1) rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
+    } else if (!rc) {
+        rc = fft_lines_filtered(st, c, g.nz, P, P * ncomp, P, 1, P * g.nz, -1, f);
+    }
     if (!rc) rc = fft_y(st, c, g, ncomp, g.nz, +1);
     if (!rc && inv_x) rc = fft_x(st, c, g, ncomp, g.nz, +1);
     return rc;

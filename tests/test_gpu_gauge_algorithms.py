@@ -152,7 +152,7 @@ def test_gauss_restore_converges(variant):
 
 
 @pytest.mark.parametrize("N,bc", [((6, 5, 6), 0), ((5, 6, 7), 0), ((4, 6, 9), 0), ((5, 6, 4), 1), ((6, 4, 1), 0),
-                                  ((32, 32, 1024), 0)])
+                                  ((32, 32, 1024), 0), ((16, 8, 512), 0), ((16, 16, 256), 0), ((8, 8, 256), 1)])
 def test_consistent_fourier_matches_oracle(N, bc):
     """The boundary-consistent Fourier update (no counterpart in the reference) against the oracle's: open z (the
     device's Poisson step is a cosine transform in z, a length-Nz FFT of the line in even-odd order, even and odd

@@ -199,14 +199,16 @@ def test_coulomb_gauge_fixed_iterations_match_oracle(N, a, iters):
     assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < 1e-10
 
 
-def test_coulomb_gauge_walking_z_pass_matches_oracle():
-    """At 32 x 32 x 1024 the fused z pass (forward transform, 1/(V p^2), inverse transform) of the Fourier update
-    runs two workgroups per CU (74 KiB of LDS each) and each block walks several line groups: the four packed
-    deviation arrays give 4096 lines of length 1024, in 2048 groups of 2 lines over at most 512 blocks, four
-    groups per block, which the smaller lattices above do not reach.  Fields from the device's random fill, the
-    same arrays given to the oracle; two iterations, Fourier tolerances."""
+@pytest.mark.parametrize("N", [(32, 32, 1024), (16, 16, 512), (16, 8, 256), (6, 5, 256), (4, 6, 384)])
+def test_coulomb_gauge_walking_z_pass_matches_oracle(N):
+    """The fused z pass (forward transform, 1/(V p^2), inverse transform) of the Fourier update on long z lines:
+    Nz = 1024, 512 and 256 with the transverse plane a multiple of 4096 / Nz sites take the radix-16 pass
+    (4096 entries per group, Stockham stages of radix 16, 16 and Nz / 256 in registers), whose blocks walk several
+    groups; 6 x 5 x 256 and Nz = 384 take the generic pass.  At 32 x 32 x 1024 the x transforms are fused into the
+    deviation and update kernels (Nx divides 256).  Fields from the device's random fill, the same arrays given to
+    the oracle; two iterations, Fourier tolerances."""
     from soerenpragya3dglasma_amd import Glasma
-    N, a, iters = (32, 32, 1024), A1, 2
+    a, iters = A1, 2
     with Glasma(N, a, 0.1, 0) as g:
         g.fill_random(seed=41, link_scale=0.3, e_scale=0.3, j_scale=0.0)
         d = g.get_fields()
