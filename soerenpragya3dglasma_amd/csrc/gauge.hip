@@ -221,44 +221,52 @@ __device__ __forceinline__ void acc_proj_nd(double c, const M3& X, const M3& Y, 
 // CONS: the boundary-consistent update (no counterpart in the reference): only links that exist enter w (open z:
 // none from the top layer to its ghost copy and none below layer 0; periodic z: the wrapped links, with the
 // current G on both ends instead of the reference's lagging ghost layer and identity below layer 0).
+// the direction-mu terms of w at site s = (x, y, z), G(x) in Gx: forward c (G(x) U_mu(x)) G(x + mu)^dag and backward
+// c G(x) (G(x - mu) U_mu(x - mu))^dag
+template <bool CONS>
+__device__ __forceinline__ void w_terms(const double* __restrict__ U, const double* __restrict__ G,
+                                        const double* __restrict__ Gtop, const Geom& g, int mu, long s, int x, int y,
+                                        int z, double c, const M3& Gx, double* q) {
+    M3 Um, Gn, T;
+    // U^G_mu(x) = (G(x) U_mu(x)) G^dag(x + mu)
+    const bool has_up = !CONS || mu < 2 || (g.nz > 1 && (g.bc || z + 1 < g.nz));
+    if (has_up) {
+        ldm<long>(U, kUPlanes, 18 * mu, s, Um);
+        mul_nn(Gx, Um, T);
+        if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
+        else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
+        else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
+        else if (CONS) ldm<long>(G, kGPlanes, 0, s - (long)z * g.P, Gn);  // periodic: layer 0 now
+        else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
+        acc_proj_nd(c, T, Gn, q);
+    }
+    // U^G_mu(x - mu)^dag = G(x) (G(x - mu) U_mu(x - mu))^dag; below z = 0 the identity (CONS: none for open z,
+    // the top layer's link for periodic z)
+    const bool has_dn = mu < 2 || z > 0 || (CONS && g.bc && g.nz > 1);
+    if (has_dn) {
+        const long sb = mu == 0 ? sidx(g, x - 1, y, z)
+                                : (mu == 1 ? sidx(g, x, y - 1, z) : (z > 0 ? s - g.P : s + (long)(g.nz - 1) * g.P));
+        ldm<long>(G, kGPlanes, 0, sb, Gn);
+        ldm<long>(U, kUPlanes, 18 * mu, sb, Um);
+        mul_nn(Gn, Um, T);
+        acc_proj_nd(c, Gx, T, q);
+    } else if (!CONS) {
+        q[9] += 3.0 * c;
+    }
+}
+
 template <bool CONS>
 __device__ __forceinline__ void local_w_fused(const double* __restrict__ U, const double* __restrict__ G,
                                               const double* __restrict__ Gtop, const Geom& g, int x, int y, int z,
                                               double cx, double cy, double cz, double* q) {
-    M3 Gx, Um, Gn, T;
+    M3 Gx;
     const double zf = g.nz > 1 ? 1.0 : 0.0;
     const long s = sidx(g, x, y, z);
     ldm<long>(G, kGPlanes, 0, s, Gx);  // G(x), read once: it starts every forward term and ends every backward one
     for (int k = 0; k < 10; k++) q[k] = 0.0;
 #pragma unroll 1
-    for (int mu = 0; mu < 3; mu++) {
-        const double c = mu == 0 ? cx : (mu == 1 ? cy : zf * cz);
-        // U^G_mu(x) = (G(x) U_mu(x)) G^dag(x + mu)
-        const bool has_up = !CONS || mu < 2 || (g.nz > 1 && (g.bc || z + 1 < g.nz));
-        if (has_up) {
-            ldm<long>(U, kUPlanes, 18 * mu, s, Um);
-            mul_nn(Gx, Um, T);
-            if (mu == 0) ldm<long>(G, kGPlanes, 0, sidx(g, x + 1, y, z), Gn);
-            else if (mu == 1) ldm<long>(G, kGPlanes, 0, sidx(g, x, y + 1, z), Gn);
-            else if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gn);
-            else if (CONS) ldm<long>(G, kGPlanes, 0, s - (long)z * g.P, Gn);  // periodic: layer 0 now
-            else ldm<long>(Gtop, kGPlanes, 0, s - (long)z * g.P, Gn);
-            acc_proj_nd(c, T, Gn, q);
-        }
-        // U^G_mu(x - mu)^dag = G(x) (G(x - mu) U_mu(x - mu))^dag; below z = 0 the identity (CONS: none for open z,
-        // the top layer's link for periodic z)
-        const bool has_dn = mu < 2 || z > 0 || (CONS && g.bc && g.nz > 1);
-        if (has_dn) {
-            const long sb = mu == 0 ? sidx(g, x - 1, y, z)
-                                    : (mu == 1 ? sidx(g, x, y - 1, z) : (z > 0 ? s - g.P : s + (long)(g.nz - 1) * g.P));
-            ldm<long>(G, kGPlanes, 0, sb, Gn);
-            ldm<long>(U, kUPlanes, 18 * mu, sb, Um);
-            mul_nn(Gn, Um, T);
-            acc_proj_nd(c, Gx, T, q);
-        } else if (!CONS) {
-            q[9] += 3.0 * c;
-        }
-    }
+    for (int mu = 0; mu < 3; mu++)
+        w_terms<CONS>(U, G, Gtop, g, mu, s, x, y, z, mu == 0 ? cx : (mu == 1 ? cy : zf * cz), Gx, q);
 }
 
 // Delta^a = Re tr(i t^a w) (retr_igen) from the accumulators of acc_proj_nd
@@ -322,6 +330,130 @@ __global__ __launch_bounds__(kCgBlock, 3) void k_cg_dev_fused(const double* __re
         for (int k = 0; k < 4; k++) Fv[(long)k * g.S + s] = xb[cur * kXfEntries + k * kCgBlock + threadIdx.x];
     }
     block_partials(sq, fs, mx, fm, part + 4 * (long)blockIdx.x);
+}
+
+// The deviation kernel walking z (P a multiple of 256): block b takes unit u (sites [256 u, 256 u + 256) of a plane)
+// in the z chunk [z0, z0 + zc) and walks it plane by plane.  The backward z term at z is the adjoint of the forward z
+// term at z - 1, A = U^G_z(z - 1), and the accumulators of A^dag are those of A with the eight anti-Hermitian ones
+// negated and Re tr unchanged: the walk carries those ten numbers from one plane to the next instead of reading
+// G(z - 1) and U_z(z - 1) and forming the link again, and G(z + 1), read for the forward term, is G(x) of the next
+// plane.  That is three matrices read per site less than k_cg_dev_fused, among them the two whose planes lie furthest
+// away.  With XCD order (U = P / 256 units a multiple of 8), XCD k (= b % 8) owns units [k U/8, (k+1) U/8) of every
+// chunk, so the y neighbour units run on the same XCD.  Two waves per SIMD (at three the kernel spills and was slower
+// than k_cg_dev_fused): at 256 x 256 x 1024 the Fourier iteration took 25.2 ms with chunks of 32 planes, 25.6 ms
+// with 16, against 27.1 ms with k_cg_dev_fused.
+struct CgWalk {
+    int zc;   // planes per chunk
+    int xcd;  // XCD order
+};
+
+template <bool CONS, bool XF>
+__global__ __launch_bounds__(kCgBlock, 2) void k_cg_dev_walk(const double* __restrict__ U, const double* __restrict__ G,
+                                                           const double* __restrict__ Gtop, double* __restrict__ F,
+                                                           double* __restrict__ part, Geom g, double cx, double cy,
+                                                           double cz, double fm0, CgWalk o, Radices R,
+                                                           const dbl2* __restrict__ tw) {
+    __shared__ dbl2 xb[XF ? 2 * kXfEntries : 1];
+    const long nu = g.P / kCgBlock;
+    long unit, zcid;
+    if (o.xcd) {
+        const long b = blockIdx.x, upx = nu / 8;
+        const long i = b >> 3;
+        zcid = i / upx;
+        unit = (b & 7) * upx + (i - zcid * upx);
+    } else {
+        zcid = blockIdx.x / nu;
+        unit = blockIdx.x - zcid * nu;
+    }
+    const int z0 = (int)zcid * o.zc, z1 = min(z0 + o.zc, g.nz);
+    const long s2 = unit * kCgBlock + threadIdx.x;
+    const int x = (int)(s2 % g.nx), y = (int)(s2 / g.nx);
+    const double czf = g.nz > 1 ? cz : 0.0;
+    double sq = 0.0, fs = 0.0, mx = 0.0, fm = fm0;
+    M3 Gx;
+    ldm<long>(G, kGPlanes, 0, s2 + (long)z0 * g.P, Gx);
+    double pz[10];  // accumulators of the forward z term of the previous plane
+#pragma unroll 1
+    for (int z = z0; z < z1; z++) {
+        const long s = s2 + (long)z * g.P;
+        double q[10];
+        for (int k = 0; k < 10; k++) q[k] = 0.0;
+        w_terms<CONS>(U, G, Gtop, g, 0, s, x, y, z, cx, Gx, q);
+        w_terms<CONS>(U, G, Gtop, g, 1, s, x, y, z, cy, Gx, q);
+        // backward z: carried within the chunk (z > z0 >= 0, so the link exists), formed at its first plane
+        if (z > z0) {
+            for (int k = 0; k < 9; k++) q[k] -= pz[k];
+            q[9] += pz[9];
+        } else {
+            const bool has_dn = z > 0 || (CONS && g.bc && g.nz > 1);
+            if (has_dn) {
+                M3 Gn, Um, T;
+                const long sb = z > 0 ? s - g.P : s + (long)(g.nz - 1) * g.P;
+                ldm<long>(G, kGPlanes, 0, sb, Gn);
+                ldm<long>(U, kUPlanes, 36, sb, Um);
+                mul_nn(Gn, Um, T);
+                acc_proj_nd(czf, Gx, T, q);
+            } else if (!CONS) {
+                q[9] += 3.0 * czf;
+            }
+        }
+        // forward z, last: G(z + 1) becomes G(x) of the next plane
+        const bool has_up = !CONS || (g.nz > 1 && (g.bc || z + 1 < g.nz));
+        if (has_up) {
+            M3 Um, T, Gu;
+            ldm<long>(U, kUPlanes, 36, s, Um);
+            mul_nn(Gx, Um, T);
+            if (z + 1 < g.nz) ldm<long>(G, kGPlanes, 0, s + g.P, Gu);
+            else if (CONS) ldm<long>(G, kGPlanes, 0, s2, Gu);  // periodic: layer 0 now
+            else ldm<long>(Gtop, kGPlanes, 0, s2, Gu);
+            for (int k = 0; k < 10; k++) pz[k] = 0.0;
+            acc_proj_nd(czf, T, Gu, pz);
+            for (int k = 0; k < 10; k++) q[k] += pz[k];
+            Gx = Gu;  // read again from G only where the next plane is another chunk's
+        }
+        double dv[8];
+        delta_from_proj(q, dv);
+        for (int a = 0; a < 8; a++) {
+            mx = fmax(mx, fabs(dv[a]));
+            sq += dv[a] * dv[a];
+        }
+        const double lf = q[9];
+        fm = fmin(fm, lf);
+        fs += lf;
+        if (XF) {
+            for (int k = 0; k < 4; k++) xb[k * kCgBlock + threadIdx.x] = dbl2{dv[2 * k], dv[2 * k + 1]};
+            __syncthreads();
+            const int cur = stockham(xb, kXfEntries, 0, R, kXfEntries / R.n, R.n, -1, tw);
+            dbl2* const Fv = reinterpret_cast<dbl2*>(F);
+            for (int k = 0; k < 4; k++) Fv[(long)k * g.S + s] = xb[cur * kXfEntries + k * kCgBlock + threadIdx.x];
+            __syncthreads();  // the reads of this plane's result precede the next plane's writes
+        } else {
+            for (int k = 0; k < 4; k++) {
+                F[2 * ((long)k * g.S + s)] = dv[2 * k];
+                F[2 * ((long)k * g.S + s) + 1] = dv[2 * k + 1];
+            }
+        }
+    }
+    block_partials(sq, fs, mx, fm, part + 4 * (long)blockIdx.x);
+}
+
+// the walk where P is a multiple of 256 and chunks of 32 or 16 planes still give 2048 blocks (eight per CU), else
+// k_cg_dev_fused (at 64 x 64 x 256, 1024 blocks of 16 planes, the walk took 0.62 ms per iteration against 0.56);
+// GLASMA_CG_WALK = planes per chunk overrides, 0 turns the walk off
+CgWalk cg_walk(const Geom& g) {
+    const char* e = std::getenv("GLASMA_CG_WALK");  // read per call, so that tests can set it
+    const int forced = e ? std::atoi(e) : -1;
+    if (forced == 0 || g.P % kCgBlock) return {0, 0};
+    const long nu = g.P / kCgBlock;
+    int zc = forced > 0 ? forced : 0;
+    if (!zc)
+        for (int c : {32, 16})
+            if (nu * ((g.nz + c - 1) / c) >= 2048) {
+                zc = c;
+                break;
+            }
+    if (!zc) return {0, 0};
+    return {zc, nu % 8 == 0 ? 1 : 0};
 }
 
 // rows P and Q of X <- A X, A the identity outside the SU(2) block [[a, b], [c, d]] of rows / columns P, Q
@@ -627,6 +759,8 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
     const long nblk = (g.S + kCgBlock - 1) / kCgBlock;
     const bool cons = algo == GLASMA_GAUGE_FOURIER_CONSISTENT;
     const CgOrder ord = cg_order(g);
+    const CgWalk walk = algo == GLASMA_GAUGE_LOS_ALAMOS ? CgWalk{0, 0} : cg_walk(g);
+    const long nwalk = walk.zc ? g.P / kCgBlock * ((g.nz + walk.zc - 1) / walk.zc) : 0;
     // the x transforms fused into the deviation and update kernels (the Fourier steps with Nz >= 2)
     Radices R{};
     const dbl2* tw = nullptr;
@@ -666,19 +800,29 @@ int coulomb_gauge(glasma_ctx* ctx, int algo, double tol, int maxit, double param
             reduce_partials(ctx->stream, part, 2 * nblk, fin, fm0, tmp);
             reduce_partials(ctx->stream, spart, 2 * nblk, fin + 4, 0.0, tmp);
         } else {
-#define GLASMA_CG_DEV(C, X)                                                                                    \
-    hipLaunchKernelGGL((k_cg_dev_fused<C, X>), dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream, ctx->U[ctx->cur], \
-                       ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, ord, R, tw)
+#define GLASMA_CG_DEV(C, X)                                                                                     \
+    if (walk.zc)                                                                                                 \
+        hipLaunchKernelGGL((k_cg_dev_walk<C, X>), dim3((unsigned)nwalk), dim3(kCgBlock), 0, ctx->stream,            \
+                           ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, walk, R, tw);         \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_cg_dev_fused<C, X>), dim3((unsigned)nblk), dim3(kCgBlock), 0, ctx->stream,           \
+                           ctx->U[ctx->cur], ctx->G, ctx->Gtop, F, part, g, cx, cy, cz, fm0, ord, R, tw)
             if (cons) {
-                if (xf) GLASMA_CG_DEV(true, true);
-                else GLASMA_CG_DEV(true, false);
+                if (xf) {
+                    GLASMA_CG_DEV(true, true);
+                } else {
+                    GLASMA_CG_DEV(true, false);
+                }
             } else {
-                if (xf) GLASMA_CG_DEV(false, true);
-                else GLASMA_CG_DEV(false, false);
+                if (xf) {
+                    GLASMA_CG_DEV(false, true);
+                } else {
+                    GLASMA_CG_DEV(false, false);
+                }
                 hipLaunchKernelGGL(k_cg_sync_top, grid1(g.P), dim3(kCgBlock), 0, ctx->stream, ctx->G, ctx->Gtop, g);
             }
 #undef GLASMA_CG_DEV
-            reduce_partials(ctx->stream, part, nblk, fin, fm0, tmp);
+            reduce_partials(ctx->stream, part, walk.zc ? nwalk : nblk, fin, fm0, tmp);
             if (cons) {
                 // the Laplacian of the links that exist (open z: the Neumann Laplacian, a cosine transform in z); the
                 // zero mode (zero by construction) dropped

@@ -199,17 +199,24 @@ def test_coulomb_gauge_fixed_iterations_match_oracle(N, a, iters):
     assert np.abs(EG[1:-1] - EG_o[1:-1]).max() < 1e-10
 
 
-@pytest.mark.parametrize("N", [(32, 32, 1024), (16, 16, 512), (16, 8, 256), (6, 5, 256), (4, 6, 384)])
-def test_coulomb_gauge_walking_z_pass_matches_oracle(N):
-    """The fused z pass (forward transform, 1/(V p^2), inverse transform) of the Fourier update on long z lines:
-    Nz = 1024, 512 and 256 with the transverse plane a multiple of 4096 / Nz sites take the radix-16 pass
-    (4096 entries per group, Stockham stages of radix 16, 16 and Nz / 256 in registers), whose blocks walk several
-    groups; 6 x 5 x 256 and Nz = 384 take the generic pass.  At 32 x 32 x 1024 the x transforms are fused into the
-    deviation and update kernels (Nx divides 256).  Fields from the device's random fill, the same arrays given to
-    the oracle; two iterations, Fourier tolerances."""
+@pytest.mark.parametrize("N,bc,walk", [((32, 32, 1024), 0, "16"), ((32, 32, 1024), 0, "0"), ((16, 16, 512), 0, "32"),
+                                       ((16, 8, 256), 0, ""), ((6, 5, 256), 0, ""), ((4, 6, 384), 0, ""),
+                                       ((16, 16, 40), 0, "16"), ((16, 16, 40), 1, "16"), ((16, 16, 40), 1, "0")])
+def test_coulomb_gauge_walking_z_pass_matches_oracle(N, bc, walk, monkeypatch):
+    """The Fourier update on long z lines and on the kernels that walk z.  The fused z pass (forward transform,
+    1/(V p^2), inverse transform): Nz = 1024, 512 and 256 with the transverse plane a multiple of 4096 / Nz sites take
+    the radix-16 pass (4096 entries per group, Stockham stages of radix 16, 16 and Nz / 256 in registers), whose
+    blocks walk several groups; 6 x 5 x 256, Nz = 384 and Nz = 40 take the generic pass.  Transverse planes of a
+    multiple of 256 sites take the deviation kernel that walks z in chunks of 16 planes carrying the z term from
+    plane to plane (Nz = 40: a chunk of 8 last; open and periodic z), forced here through GLASMA_CG_WALK (planes
+    per chunk; 0 turns the walk off, as it is by default below 2048 blocks); the others the one that forms every link.
+    Nx dividing 256: the x transforms fused into the deviation and update kernels.  Fields from the device's
+    random fill, the same arrays given to the oracle; two iterations, Fourier tolerances."""
     from soerenpragya3dglasma_amd import Glasma
+    if walk:
+        monkeypatch.setenv("GLASMA_CG_WALK", walk)
     a, iters = A1, 2
-    with Glasma(N, a, 0.1, 0) as g:
+    with Glasma(N, a, 0.1, bc) as g:
         g.fill_random(seed=41, link_scale=0.3, e_scale=0.3, j_scale=0.0)
         d = g.get_fields()
         st = g.coulomb_gauge(0.0, iters, 2.0)
@@ -217,7 +224,7 @@ def test_coulomb_gauge_walking_z_pass_matches_oracle(N):
     f = O.Fields(N)
     f.U, f.E, f.Jp, f.Jm, f.JpS, f.JmS = d["U"], d["E"], d["Jp"], d["Jm"], d["JpS"], d["JmS"]
     G = _identity_G(N)
-    UG_o, EG_o, st_o = O.coulomb_gauge(f, a, G, 0.0, iters, 2.0)
+    UG_o, EG_o, st_o = O.coulomb_gauge_algo(f, a, G, "fourier", 0.0, iters, 2.0, bc=bc)
     print(st, st_o)
     assert st[0] == st_o[0] == iters
     assert np.allclose(st[1:], st_o[1:], rtol=1e-9, atol=1e-13)
