@@ -772,12 +772,94 @@ __global__ __launch_bounds__(256) void k_zpoisson(dbl2* __restrict__ dv, long P,
     }
 }
 
+// ---- strided lines of length n = 256 R3 (R3 in {1, 2, 4}), one transform -----------------------------------------
+// The scheme of k_zpoisson without the filter: groups of L = 16 / R3 lines whose starts are neighbours (line l
+// starts at (l % b1) s1 + (l / b1) s2, s1 = 1, L dividing b1; rows of 16 L bytes), the first stage on the loaded
+// registers, the last one stored from registers.  At 256 x 256 x 1024 the y pass of the four packed arrays
+// (length 256, stride 256) is one of these.
+template <int R3>
+__global__ __launch_bounds__(256) void k_lines16(dbl2* __restrict__ dv, long es, long b1, long s2, long groups,
+                                                 long xcd_per, int sign, const dbl2* __restrict__ twg) {
+    constexpr int n = 256 * R3;
+    constexpr int L = 16 / R3;
+    constexpr int nr1 = n / 16;
+    constexpr int B3 = 16 / R3;
+    extern __shared__ dbl2 lds[];
+    const int ld = n + n / 16 + 4;
+    dbl2* const tw = lds + L * ld;
+    long g, gstep, gend;
+    if (xcd_per > 0) {
+        const long kx = blockIdx.x % 8;
+        g = kx * xcd_per + blockIdx.x / 8;
+        gstep = gridDim.x / 8;
+        gend = min((kx + 1) * xcd_per, groups);
+    } else {
+        g = blockIdx.x;
+        gstep = gridDim.x;
+        gend = groups;
+    }
+    if (g >= gend) return;
+    for (int k = threadIdx.x; k < split_entries(n); k += 256) tw[k] = twg[k];
+    const int li0 = threadIdx.x % L, j0 = threadIdx.x / L;
+    auto line_start = [&](long gg, int li) -> long {
+        const long l = gg * L + li;
+        return l % b1 + (l / b1) * s2;
+    };
+    dbl2 pre[16];
+    auto fetch = [&](long gg) {
+        const dbl2* const src = dv + line_start(gg, li0) + (long)j0 * es;
+#pragma unroll
+        for (int m = 0; m < 16; m++) pre[m] = src[(long)m * nr1 * es];
+    };
+    fetch(g);
+    __syncthreads();  // the twiddles
+    while (true) {
+        dft<16>(pre, sign);
+        {
+            dbl2* const dst = lds + li0 * ld + 17 * j0;  // zp_pidx(16 j0 + m) = 17 j0 + m
+#pragma unroll
+            for (int m = 0; m < 16; m++) dst[m] = pre[oidx<16>(m)];
+        }
+        const long gn = g + gstep;
+        const bool more = gn < gend;
+        if (more) fetch(gn);
+        __syncthreads();
+        if (R3 > 1) {
+            zp_stage<16, n, 16>(lds, ld, sign, tw);
+            dbl2 v[B3][R3];
+            zp_last<R3, n, L>(lds, ld, sign, tw, v);
+#pragma unroll
+            for (int i = 0; i < B3; i++) {
+                const int b = threadIdx.x + 256 * i, li = b % L, j = b / L;
+                dbl2* const dst = dv + line_start(g, li) + (long)j * es;
+#pragma unroll
+                for (int m = 0; m < R3; m++) dst[(long)256 * m * es] = v[i][oidx<R3>(m)];
+            }
+        } else {
+            dbl2 v[1][16];
+            zp_last<16, n, L>(lds, ld, sign, tw, v);
+            const int li = threadIdx.x % L, j = threadIdx.x / L;
+            dbl2* const dst = dv + line_start(g, li) + (long)j * es;
+#pragma unroll
+            for (int m = 0; m < 16; m++) dst[(long)16 * m * es] = v[0][oidx<16>(m)];
+        }
+        if (!more) break;
+        __syncthreads();
+        g = gn;
+    }
+}
+
 // the dynamic LDS limit of the k_zpoisson instantiations raised to the device's, once per device
 std::map<int, bool> g_zp_allowed;
 template <int R3, bool NEU>
 hipError_t zp_allow_one(int v) {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zpoisson<R3, NEU>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, v);
+}
+template <int R3>
+hipError_t l16_allow_one(int v) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lines16<R3>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               v);
 }
 hipError_t zp_allow(size_t lds_max) {
     int dev = 0;
@@ -787,7 +869,8 @@ hipError_t zp_allow(size_t lds_max) {
     if (g_zp_allowed.count(dev)) return hipSuccess;
     const int v = (int)lds_max;
     for (hipError_t r : {zp_allow_one<1, false>(v), zp_allow_one<1, true>(v), zp_allow_one<2, false>(v),
-                         zp_allow_one<2, true>(v), zp_allow_one<4, false>(v), zp_allow_one<4, true>(v)})
+                         zp_allow_one<2, true>(v), zp_allow_one<4, false>(v), zp_allow_one<4, true>(v),
+                         l16_allow_one<1>(v), l16_allow_one<2>(v), l16_allow_one<4>(v)})
         if (r != hipSuccess) return r;
     g_zp_allowed[dev] = true;
     return hipSuccess;
@@ -838,6 +921,49 @@ int zpoisson(hipStream_t st, double* c, const Geom& g, int ncomp, const LineFilt
                   : launch_zpoisson<4, false>(st, lds, dev.cus, c, P, groups, f, tw);
     if (ke != hipSuccess)
         return fft_fail_hip(ke, "k_zpoisson launch (length " + std::to_string(n) + ", " + std::to_string(lds) +
+                                    " bytes of LDS)");
+    return 0;
+}
+
+template <int R3>
+hipError_t launch_lines16(hipStream_t st, size_t lds, int cus, double* c, long es, long b1, long s2, long groups,
+                          int sign, const dbl2* tw) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(&k_lines16<R3>), 256, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    const long xcd_per = groups >= 64 ? (groups + 7) / 8 : 0;
+    const long resident = (long)cus * per_cu;
+    const long blocks = xcd_per > 0 ? 8 * std::min(xcd_per, (resident + 7) / 8) : std::min(groups, resident);
+    hipLaunchKernelGGL((k_lines16<R3>), dim3((unsigned)blocks), dim3(256), lds, st, reinterpret_cast<dbl2*>(c), es, b1,
+                       s2, groups, xcd_per, sign, tw);
+    return hipGetLastError();
+}
+
+// strided lines of length 256, 512 or 1024 by k_lines16 where their starts come in runs of 4096 / n neighbours
+// (s1 = 1, b1 a multiple of the run); returns 1 where it does not apply
+int lines16(hipStream_t st, double* c, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
+    if (!kZPoisson || (n != 256 && n != 512 && n != 1024) || es == 1 || s1 != 1) return 1;
+    const int L = 4096 / n;
+    if (b1 % L || nlines % L || (b1 < nlines && nlines % b1)) return 1;
+    FftDevice dev;
+    const hipError_t de = fft_device(&dev);
+    if (de != hipSuccess) return fft_fail_hip(de, "device query");
+    const size_t lds = sizeof(dbl2) * ((size_t)L * (n + n / 16 + 4) + split_entries(n));
+    if (lds > dev.lds) return 1;
+    const hipError_t ae = zp_allow(dev.lds);
+    if (ae != hipSuccess) return fft_fail_hip(ae, "k_lines16 LDS limit");
+    const dbl2* tw = nullptr;
+    const hipError_t te = twiddles_split(n, &tw);
+    if (te != hipSuccess) return fft_fail_hip(te, "twiddle table");
+    const long groups = nlines / L;
+    hipError_t ke;
+    if (n == 256) ke = launch_lines16<1>(st, lds, dev.cus, c, es, b1, s2, groups, sign, tw);
+    else if (n == 512) ke = launch_lines16<2>(st, lds, dev.cus, c, es, b1, s2, groups, sign, tw);
+    else ke = launch_lines16<4>(st, lds, dev.cus, c, es, b1, s2, groups, sign, tw);
+    if (ke != hipSuccess)
+        return fft_fail_hip(ke, "k_lines16 launch (length " + std::to_string(n) + ", " + std::to_string(lds) +
                                     " bytes of LDS)");
     return 0;
 }
@@ -907,6 +1033,10 @@ int fft_lines_filtered(hipStream_t st, double* data, int n, long es, long nlines
 }  // namespace
 
 int fft_lines(hipStream_t st, double* data, int n, long es, long nlines, long b1, long s1, long s2, int sign) {
+    if (nlines > 0) {
+        const int rc = lines16(st, data, n, es, nlines, b1, s1, s2, sign);
+        if (rc != 1) return rc;
+    }
     const LineFilter none = {0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, nullptr};
     return fft_lines_filtered(st, data, n, es, nlines, b1, s1, s2, sign, none);
 }
