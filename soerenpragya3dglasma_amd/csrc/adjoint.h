@@ -385,7 +385,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
     site_coords<I>(g, s, x, y, z);
     constexpr int lam = LAM;
     {
-        M3 acc, Sbar, P, Q, R, T1, T2;
+        M3 acc, Sbar, P, Q, R, T1;
         ldm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
         for (int k = 1; k <= 2; k++) {
             const int mu = (lam + k) % 3;  // staples of E_mu in which U_lam is a nu = lam link
@@ -395,8 +395,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
             ldm<I>(A.U, kUPlanes, 18 * lam, nbr<I>(g, x, y, z, mu, 1, lam, 0), P);
             ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, lam, 1, mu, 0), Q);
             mul_dn(Sbar, P, T1);
-            mul_nd(T1, Q, T2);
-            m3_axpy(c, T2, acc);
+            acc_mul_nd(c, T1, Q, acc);
             int zs[2];
             // forward P: y = x + mu; Pbar = Sb R Q, R = U_lam(x), Q = U_mu(x+lam)
             {
@@ -409,8 +408,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
                     ldm<I>(A.U, kUPlanes, 18 * lam, sx, R);
                     ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, xx, yy, zz, lam, 1, mu, 0), Q);
                     mul_nn(Sbar, R, T1);
-                    mul_nn(T1, Q, T2);
-                    m3_axpy(c, T2, acc);
+                    acc_mul_nn(c, T1, Q, acc);
                 }
             }
             // backward P: y = x + mu - lam; Pbar = Q^dag R Sb^dag, Q = U_mu(x-lam), R = U_lam(x-lam)
@@ -426,8 +424,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
                     ldm<I>(A.U, kUPlanes, 18 * mu, sm, Q);
                     ldm<I>(A.U, kUPlanes, 18 * lam, sm, R);
                     mul_dn(Q, R, T1);
-                    mul_nd(T1, Sbar, T2);
-                    m3_axpy(c, T2, acc);
+                    acc_mul_nd(c, T1, Sbar, acc);
                 }
             }
             // backward R: y = x - lam; Rbar = Q P Sb, Q = U_mu(x-lam), P = U_lam(x+mu-lam)
@@ -441,8 +438,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
                     ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, xx, yy, zz, lam, -1, mu, 0), Q);
                     ldm<I>(A.U, kUPlanes, 18 * lam, nbr<I>(g, xx, yy, zz, mu, 1, lam, -1), P);
                     mul_nn(Q, P, T1);
-                    mul_nn(T1, Sbar, T2);
-                    m3_axpy(c, T2, acc);
+                    acc_mul_nn(c, T1, Sbar, acc);
                 }
             }
         }
@@ -461,8 +457,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
                     ldm<I>(A.U, kUPlanes, 18 * nu, sx, R);
                     ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, lam, 1, nu, 0), P);
                     mul_dd(R, Sbar, T1);
-                    mul_nn(T1, P, T2);
-                    m3_axpy(c, T2, acc);
+                    acc_mul_nn(c, T1, P, acc);
                 }
             }
             // backward Q: y = x - nu; Qbar = R Sb^dag P^dag, R = U_nu(x-nu), P = U_nu(x+lam-nu)
@@ -476,8 +471,7 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
                     ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, nu, -1, lam, 0), R);
                     ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, lam, 1, nu, -1), P);
                     mul_nd(R, Sbar, T1);
-                    mul_nd(T1, P, T2);
-                    m3_axpy(c, T2, acc);
+                    acc_mul_nd(c, T1, P, acc);
                 }
             }
         }
