@@ -14,6 +14,25 @@ namespace {
 constexpr int kAdjBlock = 256;
 inline dim3 agrid(long n) { return dim3((unsigned)((n + kAdjBlock - 1) / kAdjBlock)); }
 
+// Block order of the backward passes.  nd directions per site block (1, or 3 for the staple passes, which give each
+// thread one link direction); block b runs on XCD b % 8.  With the site blocks a multiple of 8, XCD k takes site
+// blocks [k nsb/8, (k+1) nsb/8) and, for each, its nd directions in turn: a site's y and z neighbours (a row and a
+// plane away) are then read by the same XCD at about the same time, and the three directions of a pass share them in
+// its L2.  In dispatch order (all sites of direction 0, then 1, then 2) the staple gather read 7 times its
+// compulsory bytes from beyond L2.  Otherwise the directions still alternate, without the XCD ranges.
+__device__ __forceinline__ void adj_block(long nsb, int nd, long& sb, int& dir) {
+    const long b = blockIdx.x;
+    if (nsb % 8 == 0) {
+        const long i = b >> 3;
+        dir = (int)(i % nd);
+        sb = (b & 7) * (nsb >> 3) + i / nd;
+    } else {
+        dir = (int)(b % nd);
+        sb = b / nd;
+    }
+}
+inline dim3 adj_grid(long n, int nd) { return dim3((unsigned)(nd * ((n + kAdjBlock - 1) / kAdjBlock))); }
+
 struct AtomicAddU {
     double* Ub;
     __device__ void operator()(int mu, unsigned s, const M3& M) const {
@@ -25,29 +44,41 @@ struct AtomicAddU {
     }
 };
 
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_local(StepAdjArgs A) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_local(StepAdjArgs A, long nsb) {
+    long sb;
+    int dir;
+    adj_block(nsb, 1, sb, dir);
+    const long s = sb * kAdjBlock + threadIdx.x;
     if (s < A.g.S) step_vjp_local<unsigned>(A, (unsigned)s);
 }
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A, long nsb) {
+    long sb;
+    int dir;
+    adj_block(nsb, 1, sb, dir);
+    const long s = sb * kAdjBlock + threadIdx.x;
     if (s < A.g.S) step_vjp_down<unsigned>(A, (unsigned)s);
 }
-// passes C1 and C2 give each thread one link direction (blockIdx.y): a third of the work per thread, and the
-// three directions of a pass run in one launch
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_sigbar(StepAdjArgs A, double* Sb) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// passes C1 and C2 give each thread one link direction: a third of the work per thread, and the three directions
+// of a pass run in one launch
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_sigbar(StepAdjArgs A, double* Sb, long nsb) {
+    long sb;
+    int dir;
+    adj_block(nsb, 3, sb, dir);
+    const long s = sb * kAdjBlock + threadIdx.x;
     if (s >= A.g.S) return;
-    switch (blockIdx.y) {
+    switch (dir) {
         case 0: step_vjp_sigbar_link<unsigned, 0>(A, (unsigned)s, Sb); break;
         case 1: step_vjp_sigbar_link<unsigned, 1>(A, (unsigned)s, Sb); break;
         default: step_vjp_sigbar_link<unsigned, 2>(A, (unsigned)s, Sb); break;
     }
 }
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb) {
-    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb, long nsb) {
+    long sb;
+    int dir;
+    adj_block(nsb, 3, sb, dir);
+    const long s = sb * kAdjBlock + threadIdx.x;
     if (s >= A.g.S) return;
-    switch (blockIdx.y) {
+    switch (dir) {
         case 0: step_vjp_gather_link<unsigned, 0>(A, Sb, (unsigned)s); break;
         case 1: step_vjp_gather_link<unsigned, 1>(A, Sb, (unsigned)s); break;
         default: step_vjp_gather_link<unsigned, 2>(A, Sb, (unsigned)s); break;
@@ -132,11 +163,11 @@ StepAdjArgs adj_args(const glasma_ctx* ctx) {
 // passes A, B, C1, C2 of one backward step; Sb: 54-plane scratch for the staple cotangents
 int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A, double* Sb) {
     const long S = ctx->g.S;
-    hipLaunchKernelGGL(k_vjp_local, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
-    hipLaunchKernelGGL(k_vjp_down, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A);
-    const dim3 g3(agrid(S).x, 3);
-    hipLaunchKernelGGL(k_vjp_sigbar, g3, dim3(kAdjBlock), 0, ctx->stream, A, Sb);
-    hipLaunchKernelGGL(k_vjp_gather, g3, dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb);
+    const long nsb = (S + kAdjBlock - 1) / kAdjBlock;
+    hipLaunchKernelGGL(k_vjp_local, adj_grid(S, 1), dim3(kAdjBlock), 0, ctx->stream, A, nsb);
+    hipLaunchKernelGGL(k_vjp_down, adj_grid(S, 1), dim3(kAdjBlock), 0, ctx->stream, A, nsb);
+    hipLaunchKernelGGL(k_vjp_sigbar, adj_grid(S, 3), dim3(kAdjBlock), 0, ctx->stream, A, Sb, nsb);
+    hipLaunchKernelGGL(k_vjp_gather, adj_grid(S, 3), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb, nsb);
     GL_HIP(ctx, hipGetLastError());
     return 0;
 }
