@@ -200,7 +200,10 @@ def adjoint_cost(N=(48, 48, 128), steps: int = 200, ckpt: int = 20) -> dict:
     rho = prof[:, None, None, None] * xl[None]
     mv = Glasma.mv_params(lam=1.65, meff=0.1, sigma_l=4.0, sigma_r=4.0, center_l=center, center_r=center)
     with Glasma(N, (1.0, 1.0, 1.0), 0.25) as g:
-        g.slab_energy_from_charge(mv, rho, 2, N[2] // 2, 8)  # warm-up
+        # warm-up calls of both at the timed sizes: the context keeps their work arrays, as a caller that repeats
+        # them (a training loop) sees
+        g.slab_energy_from_charge(mv, rho, 2, N[2] // 2, 8)
+        g.slab_energy_gradient(mv, rho, steps, ckpt, N[2] // 2, 8)
         t0 = time.perf_counter()
         e = g.slab_energy_from_charge(mv, rho, steps, N[2] // 2, 8)
         t_fwd = time.perf_counter() - t0

@@ -313,7 +313,9 @@ int glasma_slab_energy_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, 
                                    int half, double* energy);
 /* The same energy and its gradient with respect to rho (grad_rho: same layout, ghosts zero), by
  * reverse-mode differentiation of the steps (forward states checkpointed every ckpt steps, each segment
- * recomputed once) and of the initial conditions.  The context holds the initial state afterwards. */
+ * recomputed once) and of the initial conditions.  The context holds the initial state afterwards.  Both calls keep
+ * their device work arrays in the context (counted by glasma_memory_bytes, freed by glasma_destroy), so that
+ * repeated calls of one size allocate nothing. */
 int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int ckpt,
                                 int zc, int half, double* energy, double* grad_rho);
 

@@ -191,21 +191,28 @@ int load_state(glasma_ctx* ctx, const State& s) {
     return 0;
 }
 
-// device memory of one call, freed on scope exit
+// device work memory of one call from the context's pool: the i-th request of a call takes the pool's i-th array when
+// it is large enough, else that array is replaced.  A call of the same sizes as the previous one allocates nothing
+// (each gradient call at 48 x 48 x 128 otherwise made about a hundred allocations of 4.6 GB in all, 45 ms).
 struct DevBuf {
-    std::vector<double*> p;
-    ~DevBuf() {
-        for (double* q : p) hipFree(q);
-    }
+    size_t next = 0;
     double* get(glasma_ctx* ctx, size_t doubles, int* rc) {
+        auto& pool = ctx->wpool;
+        if (next < pool.size() && pool[next].first >= doubles) return pool[next++].second;
         double* q = nullptr;
         const hipError_t e = hipMalloc(&q, doubles * sizeof(double));
         if (e != hipSuccess) {
             *rc = set_error(ctx, hip_code(e), "device allocation failed in the adjoint");
             return nullptr;
         }
-        p.push_back(q);
-        return q;
+        if (next < pool.size()) {
+            (void)hipStreamSynchronize(ctx->stream);  // the old array may still be read by queued work
+            hipFree(pool[next].second);
+            pool[next] = {doubles, q};
+        } else {
+            pool.push_back({doubles, q});
+        }
+        return pool[next++].second;
     }
 };
 

@@ -679,6 +679,8 @@ int glasma_destroy(glasma_ctx* ctx) {
                       ctx->UG, ctx->EG, ctx->fftbuf};
     for (double* b : bufs)
         if (b) hipFree(b);
+    for (auto& w : ctx->wpool)
+        if (w.second) hipFree(w.second);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -694,6 +696,7 @@ int glasma_memory_bytes(const glasma_ctx* ctx, size_t* bytes) {
     if (ctx->G) b += (18 * S + 18 * ctx->g.P) * sizeof(double);
     if (ctx->UG) b += kUPlanes * S * sizeof(double);
     if (ctx->EG) b += kEPlanes * S * sizeof(double);
+    for (const auto& w : ctx->wpool) b += w.first * sizeof(double);
     *bytes = b;
     return 0;
 }

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/glasma.h"
 #include "site.h"
@@ -61,6 +63,9 @@ struct glasma_ctx {
     // block -> tile order of the step, when glasma_tune_step has chosen one (else the default order)
     glasma::TileMap tm{};
     bool tm_tuned = false;
+    // work arrays of the slab-energy and gradient calls (doubles, pointer), handed out in request order and kept for
+    // the next call, so that repeated calls of one size allocate nothing; freed by glasma_destroy
+    std::vector<std::pair<size_t, double*>> wpool;
     std::string err;
 };
 
