@@ -312,8 +312,8 @@ int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double
 int glasma_slab_energy_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int zc,
                                    int half, double* energy);
 /* The same energy and its gradient with respect to rho (grad_rho: same layout, ghosts zero), by
- * reverse-mode differentiation of the steps (forward states checkpointed every ckpt steps, each segment
- * recomputed once) and of the initial conditions.  The context holds the initial state afterwards.  Both calls keep
+ * reverse-mode differentiation of the steps (forward states checkpointed every ckpt steps, each segment but the
+ * last recomputed once) and of the initial conditions.  The context holds the initial state afterwards.  Both calls keep
  * their device work arrays in the context (counted by glasma_memory_bytes, freed by glasma_destroy), so that
  * repeated calls of one size allocate nothing. */
 int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int ckpt,

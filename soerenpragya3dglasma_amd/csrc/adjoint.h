@@ -12,7 +12,10 @@
 //   B (gather from x+z): the J+-(x-z), U_z(x-z) terms of the J-stat' update;
 //   C (staples): the staple terms of the E update, either scattered into the links of neighbouring sites
 //     (step_vjp_staples, with an accumulation functor) or, without atomics, as C1 (step_vjp_sigbar) +
-//     C2 (step_vjp_gather), which the device uses.
+//     C2 (step_vjp_gather).
+// The device runs two passes: A with B folded in and C1's Sigbar store (step_vjp_local<I, true>), then C2 with C1's
+// own-link term (step_vjp_gather<I, true>); the host build also runs the separate passes and the scatter form, which
+// the tests compare with it.
 // The exponential's adjoint: with A = i cU E' t, dexp_A[H] = e^A sum_k (-ad_A)^k H/(k+1)!, so
 // Zbar = sum_k ad_A^k (X^dag Xbar)/(k+1)! and Ebar^a += cU Re tr(i t^a Zbar^dag).
 #pragma once
@@ -123,6 +126,7 @@ struct StepAdjArgs {
     double* __restrict__ Eb;         // in: cotangent of E(t+dt); out: cotangent of E(t) (in place)
     const double* __restrict__ Jb1;  // cotangent of J(t+dt)
     double* __restrict__ Jb;         // cotangent of J(t)
+    double* __restrict__ Sb = nullptr;  // Sigbar_mu = U_mu^dag Mbar_mu (written by the fused pass A)
     Geom g;
     double gc[3], cE[3], cU[3], dtz;
 };
@@ -140,49 +144,74 @@ GHD I down_site(const Geom& g, I s, int z) {
     return (I)((long)s + g.P * (long)(zm - z));
 }
 
-// Pass A at site s.
+// Cotangents of J+-stat'(s) (layer z): their own J+- terms, and the J advection of the site that reads J+-stat'(s)
+// as "x+z" (the site below, and s itself on the open top layer).  jb1p, jb1m: the cotangents of J+-(s) at t+dt.
 template <class I>
+GHD void jstat_cot(const StepAdjArgs& A, I s, int z, const double* jb1p, const double* jb1m, double* jsbp,
+                   double* jsbm) {
+    const Geom& g = A.g;
+    const double k = A.dtz;
+    double t8[8];
+    ldv<I>(A.Jb1, kJPlanes, JPS * 8, s, jsbp);
+    ldv<I>(A.Jb1, kJPlanes, JMS * 8, s, jsbm);
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+        jsbp[a] += k * jb1p[a];
+        jsbm[a] -= k * jb1m[a];
+    }
+    const int zm = zdn(g, z);
+    if (zm != z) {
+        const I sb = (I)((long)s + g.P * (long)(zm - z));
+        if (up_site<I>(g, sb, zm) == s) {
+            M3 Uhb;
+            double jb[8];
+            ldm<I>(A.Uh, kUPlanes, 36, sb, Uhb);
+            ldv<I>(A.Jb1, kJPlanes, JP * 8, sb, jb);
+            inv_adj_mul(Uhb, jb, t8);
+            for (int a = 0; a < 8; a++) jsbp[a] -= k * t8[a];
+            ldv<I>(A.Jb1, kJPlanes, JM * 8, sb, jb);
+            inv_adj_mul(Uhb, jb, t8);
+            for (int a = 0; a < 8; a++) jsbm[a] += k * t8[a];
+        }
+    }
+    if (up_site<I>(g, s, z) == s) {
+        M3 Uhs;
+        ldm<I>(A.Uh, kUPlanes, 36, s, Uhs);
+        inv_adj_mul(Uhs, jb1p, t8);
+        for (int a = 0; a < 8; a++) jsbp[a] -= k * t8[a];
+        inv_adj_mul(Uhs, jb1m, t8);
+        for (int a = 0; a < 8; a++) jsbm[a] += k * t8[a];
+    }
+}
+
+// Pass B's terms at site s, added to the cotangents of J+-(s) and U_z(s): s is read as "x-z" by the site above (its
+// J+-stat' cotangents jsb_up) and by itself on the open bottom layer (jsb_own).
+GHD void down_terms(const StepAdjArgs& A, const M3& Uz, const double* jp, const double* jm, const double* jsbp,
+                    const double* jsbm, double* jbp, double* jbm, M3& Ubz) {
+    const double k = A.dtz;
+    double t8[8];
+    adj_mul(Uz, jsbp, t8);
+    for (int a = 0; a < 8; a++) jbp[a] += k * t8[a];
+    acc_lvr(k, jp, Uz, jsbp, Ubz);
+    adj_mul(Uz, jsbm, t8);
+    for (int a = 0; a < 8; a++) jbm[a] -= k * t8[a];
+    acc_lvr(-k, jm, Uz, jsbm, Ubz);
+}
+
+// Pass A at site s.  FUSED (the device form): pass B's terms at s are added here, with the J+-stat' cotangents of
+// the site above formed again from the inputs (jstat_cot), and Sigbar_mu(s) = U_mu^dag Mbar_mu is stored to A.Sb for
+// the staple gather.  The sums are those of the separate passes, in the same order.
+template <class I, bool FUSED = false>
 GHD void step_vjp_local(const StepAdjArgs& A, I s) {
     const Geom& g = A.g;
     int x, y, z;
     site_coords<I>(g, s, x, y, z);
     const double k = A.dtz;
-    double jbp[8], jbm[8], jsbp[8], jsbm[8], t8[8];
+    double jbp[8], jbm[8], jsbp[8], jsbm[8];
     ldv<I>(A.Jb1, kJPlanes, JP * 8, s, jbp);
     ldv<I>(A.Jb1, kJPlanes, JM * 8, s, jbm);
-    ldv<I>(A.Jb1, kJPlanes, JPS * 8, s, jsbp);
-    ldv<I>(A.Jb1, kJPlanes, JMS * 8, s, jsbm);
-#pragma unroll
-    for (int a = 0; a < 8; a++) {
-        jsbp[a] += k * jbp[a];
-        jsbm[a] -= k * jbm[a];
-    }
     // J+-stat'(s) is read as "x+z" by the site below (and by s itself on the open top layer)
-    {
-        const int zm = zdn(g, z);
-        if (zm != z) {
-            const I sb = (I)((long)s + g.P * (long)(zm - z));
-            if (up_site<I>(g, sb, zm) == s) {
-                M3 Uhb;
-                double jb[8];
-                ldm<I>(A.Uh, kUPlanes, 36, sb, Uhb);
-                ldv<I>(A.Jb1, kJPlanes, JP * 8, sb, jb);
-                inv_adj_mul(Uhb, jb, t8);
-                for (int a = 0; a < 8; a++) jsbp[a] -= k * t8[a];
-                ldv<I>(A.Jb1, kJPlanes, JM * 8, sb, jb);
-                inv_adj_mul(Uhb, jb, t8);
-                for (int a = 0; a < 8; a++) jsbm[a] += k * t8[a];
-            }
-        }
-        if (up_site<I>(g, s, z) == s) {
-            M3 Uhs;
-            ldm<I>(A.Uh, kUPlanes, 36, s, Uhs);
-            inv_adj_mul(Uhs, jbp, t8);
-            for (int a = 0; a < 8; a++) jsbp[a] -= k * t8[a];
-            inv_adj_mul(Uhs, jbm, t8);
-            for (int a = 0; a < 8; a++) jsbm[a] += k * t8[a];
-        }
-    }
+    jstat_cot<I>(A, s, z, jbp, jbm, jsbp, jsbm);
     // U_h,z cotangent from the J advection at s
     M3 Uhbz;
     m3_zero(Uhbz);
@@ -198,6 +227,7 @@ GHD void step_vjp_local(const StepAdjArgs& A, I s) {
     }
     // the three links: U' = X U_h, U_h = X U, X = exp(i cU E' t)
     double ebz[8];
+    M3 Ubz;  // FUSED: the cotangent of U_z(s), stored after pass B's terms
     for (int mu = 0; mu < 3; mu++) {
         M3 Ub1, Uh, U0, X, Xb, Uhbar, T;
         double e1[8], eb[8];
@@ -213,17 +243,68 @@ GHD void step_vjp_local(const StepAdjArgs& A, I s) {
         mul_nd(Uhbar, U0, T);
         m3_axpy(1.0, T, Xb);
         mul_dn(X, Uhbar, T);
-        stm<I>(A.Ub, kUPlanes, 18 * mu, s, T);
+        if (FUSED && mu == 2) Ubz = T;
+        else stm<I>(A.Ub, kUPlanes, 18 * mu, s, T);
         dexp_adjoint(A.cU[mu], e1, X, Xb, eb);
         stv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+        if (FUSED) {  // U_mu(s) read again rather than kept live through the exponential's adjoint
+            M3 Mb;
+            m_from_retr_cot(A.gc[mu], eb, Mb);
+            ldm<I>(A.U, kUPlanes, 18 * mu, s, U0);
+            mul_dn(U0, Mb, T);
+            stm<I>(A.Sb, kUPlanes, 18 * mu, s, T);
+        }
         if (mu == 2)
             for (int a = 0; a < 8; a++) ebz[a] = eb[a];
+        if (FUSED) sched_fence();
     }
     const double ks = k / kSqrt2;
 #pragma unroll
     for (int a = 0; a < 8; a++) {
         jbp[a] += -k * jsbp[a] - ks * ebz[a];
         jbm[a] += k * jsbm[a] + ks * ebz[a];
+    }
+    if (FUSED) {
+        // pass B: s is read as "x-z" by the site above su (have_up) and by itself on the open bottom layer
+        // (have_own); the J terms first, then the U_z terms, each in pass B's order
+        const int zp = zup(g, z);
+        const I su = (I)((long)s + g.P * (long)(zp - z));
+        const bool have_up = zp != z && down_site<I>(g, su, zp) == s;
+        const bool have_own = down_site<I>(g, s, z) == s;
+        double usp[8], usm[8], t8[8];
+        if (have_up) {
+            double b1p[8], b1m[8];
+            ldv<I>(A.Jb1, kJPlanes, JP * 8, su, b1p);
+            ldv<I>(A.Jb1, kJPlanes, JM * 8, su, b1m);
+            jstat_cot<I>(A, su, zp, b1p, b1m, usp, usm);
+        }
+        sched_fence();
+        M3 Uz;
+        ldm<I>(A.U, kUPlanes, 36, s, Uz);
+        for (int src = 0; src < 2; src++) {
+            if (src == 0 ? !have_up : !have_own) continue;
+            const double* qp = src == 0 ? usp : jsbp;
+            const double* qm = src == 0 ? usm : jsbm;
+            adj_mul(Uz, qp, t8);
+            for (int a = 0; a < 8; a++) jbp[a] += k * t8[a];
+            adj_mul(Uz, qm, t8);
+            for (int a = 0; a < 8; a++) jbm[a] -= k * t8[a];
+        }
+        stv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
+        stv<I>(A.Jb, kJPlanes, JM * 8, s, jbm);
+        sched_fence();
+        double jp[8], jm[8];
+        ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
+        ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
+        for (int src = 0; src < 2; src++) {
+            if (src == 0 ? !have_up : !have_own) continue;
+            acc_lvr(k, jp, Uz, src == 0 ? usp : jsbp, Ubz);
+            acc_lvr(-k, jm, Uz, src == 0 ? usm : jsbm, Ubz);
+        }
+        stm<I>(A.Ub, kUPlanes, 36, s, Ubz);
+        stv<I>(A.Jb, kJPlanes, JPS * 8, s, jsbp);
+        stv<I>(A.Jb, kJPlanes, JMS * 8, s, jsbm);
+        return;
     }
     stv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
     stv<I>(A.Jb, kJPlanes, JM * 8, s, jbm);
@@ -237,11 +318,10 @@ GHD void step_vjp_down(const StepAdjArgs& A, I s) {
     const Geom& g = A.g;
     int x, y, z;
     site_coords<I>(g, s, x, y, z);
-    const double k = A.dtz;
     M3 Uz, Ubz;
     ldm<I>(A.U, kUPlanes, 36, s, Uz);
     ldm<I>(A.Ub, kUPlanes, 36, s, Ubz);
-    double jp[8], jm[8], jbp[8], jbm[8], t8[8];
+    double jp[8], jm[8], jbp[8], jbm[8];
     ldv<I>(A.J, kJPlanes, JP * 8, s, jp);
     ldv<I>(A.J, kJPlanes, JM * 8, s, jm);
     ldv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
@@ -257,15 +337,10 @@ GHD void step_vjp_down(const StepAdjArgs& A, I s) {
             if (down_site<I>(g, s, z) != s) continue;
             sx = s;
         }
-        double jsb[8];
-        ldv<I>(A.Jb, kJPlanes, JPS * 8, sx, jsb);
-        adj_mul(Uz, jsb, t8);
-        for (int a = 0; a < 8; a++) jbp[a] += k * t8[a];
-        acc_lvr(k, jp, Uz, jsb, Ubz);
-        ldv<I>(A.Jb, kJPlanes, JMS * 8, sx, jsb);
-        adj_mul(Uz, jsb, t8);
-        for (int a = 0; a < 8; a++) jbm[a] -= k * t8[a];
-        acc_lvr(-k, jm, Uz, jsb, Ubz);
+        double jsbp[8], jsbm[8];
+        ldv<I>(A.Jb, kJPlanes, JPS * 8, sx, jsbp);
+        ldv<I>(A.Jb, kJPlanes, JMS * 8, sx, jsbm);
+        down_terms(A, Uz, jp, jm, jsbp, jsbm, jbp, jbm, Ubz);
     }
     stv<I>(A.Jb, kJPlanes, JP * 8, s, jbp);
     stv<I>(A.Jb, kJPlanes, JM * 8, s, jbm);
@@ -344,19 +419,15 @@ GHD void step_vjp_staples(const StepAdjArgs& A, I s, ADD add) {
 // Mbar Sigma^dag at x; C2 gathers, for each link (y, lam), the staple terms of every site x whose E update
 // reads it.  The sources of a target are the preimages of the forward neighbour map, so the clamped
 // neighbours of open z boundaries contribute exactly as in the scatter form.
+// The own-link term of link MU at s, T = Mbar_mu Sigma_mu^dag (Sigma_mu: the staple sum of the forward step).
 template <class I, int MU>
-GHD void step_vjp_sigbar_link(const StepAdjArgs& A, I s, double* __restrict__ Sb) {
+GHD void own_link_term(const StepAdjArgs& A, I s, int x, int y, int z, M3& T) {
     const Geom& g = A.g;
-    int x, y, z;
-    site_coords<I>(g, s, x, y, z);
     constexpr int mu = MU;
     double eb[8];
     ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
-    M3 Mb, Umu, Sig, T, Ub;
+    M3 Mb, Sig;
     m_from_retr_cot(A.gc[mu], eb, Mb);
-    ldm<I>(A.U, kUPlanes, 18 * mu, s, Umu);
-    mul_dn(Umu, Mb, T);
-    stm<I>(Sb, kUPlanes, 18 * mu, s, T);
     m3_zero(Sig);
     for (int q = 1; q <= 2; q++) {
         const int nu = (mu + q) % 3;
@@ -374,6 +445,21 @@ GHD void step_vjp_sigbar_link(const StepAdjArgs& A, I s, double* __restrict__ Sb
         acc_mul_nn(A.cE[nu], Tq, P1, Sig);
     }
     mul_nd(Mb, Sig, T);
+}
+
+template <class I, int MU>
+GHD void step_vjp_sigbar_link(const StepAdjArgs& A, I s, double* __restrict__ Sb) {
+    int x, y, z;
+    site_coords<I>(A.g, s, x, y, z);
+    constexpr int mu = MU;
+    double eb[8];
+    ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+    M3 Mb, Umu, T, Ub;
+    m_from_retr_cot(A.gc[mu], eb, Mb);
+    ldm<I>(A.U, kUPlanes, 18 * mu, s, Umu);
+    mul_dn(Umu, Mb, T);
+    stm<I>(Sb, kUPlanes, 18 * mu, s, T);
+    own_link_term<I, MU>(A, s, x, y, z, T);
     ldm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
     m3_axpy(1.0, T, Ub);
     stm<I>(A.Ub, kUPlanes, 18 * mu, s, Ub);
@@ -404,7 +490,9 @@ GHD int z_sources(const Geom& g, int zy, int dz, int* zs) {
     return n;
 }
 
-template <class I, int LAM>
+// OWN (the device form, after the fused pass A): the own-link term of C1 (own_link_term) is added first, as C1
+// would have added it to Ub.
+template <class I, int LAM, bool OWN = false>
 GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
     const Geom& g = A.g;
     int x, y, z;
@@ -413,6 +501,10 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
     {
         M3 acc, Sbar, P, Q, R, T1;
         ldm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
+        if (OWN) {
+            own_link_term<I, LAM>(A, s, x, y, z, T1);
+            m3_axpy(1.0, T1, acc);
+        }
         for (int k = 1; k <= 2; k++) {
             const int mu = (lam + k) % 3;  // staples of E_mu in which U_lam is a nu = lam link
             const double c = A.cE[lam];
@@ -505,11 +597,11 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
     }
 }
 
-template <class I>
+template <class I, bool OWN = false>
 GHD void step_vjp_gather(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
-    step_vjp_gather_link<I, 0>(A, Sb, s);
-    step_vjp_gather_link<I, 1>(A, Sb, s);
-    step_vjp_gather_link<I, 2>(A, Sb, s);
+    step_vjp_gather_link<I, 0, OWN>(A, Sb, s);
+    step_vjp_gather_link<I, 1, OWN>(A, Sb, s);
+    step_vjp_gather_link<I, 2, OWN>(A, Sb, s);
 }
 
 // Slab energy 1/2 sum_{|z - zc| < half} sum_mu c_mu (E_mu^2 + 4 Re tr(1 - P_mu)), c_mu = (a_mu/a^3)^2,

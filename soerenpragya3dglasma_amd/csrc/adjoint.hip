@@ -1,7 +1,8 @@
 // Reverse-mode derivatives on the device (no counterpart in the reference, which has no gradient code):
-// the leapfrog step (passes A, B, C1, C2 of adjoint.h per step, no atomics), the slab energy, and the whole map
-// charge density -> left-moving MV sheet -> nsteps leapfrog steps -> slab energy, with the forward run
-// checkpointed every ckpt steps and each segment recomputed once during the backward sweep.
+// the leapfrog step (two passes per step, no atomics: A with B folded in, then the staple gather), the slab
+// energy, and the whole map charge density -> left-moving MV sheet -> nsteps leapfrog steps -> slab energy, with
+// the forward run checkpointed every ckpt steps and each segment but the last recomputed once during the backward
+// sweep.
 #include <cstring>
 #include <vector>
 
@@ -44,34 +45,16 @@ struct AtomicAddU {
     }
 };
 
+// pass A with pass B's terms and the Sigbar store folded in (step_vjp_local<I, true>)
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_local(StepAdjArgs A, long nsb) {
     long sb;
     int dir;
     adj_block(nsb, 1, sb, dir);
     const long s = sb * kAdjBlock + threadIdx.x;
-    if (s < A.g.S) step_vjp_local<unsigned>(A, (unsigned)s);
+    if (s < A.g.S) step_vjp_local<unsigned, true>(A, (unsigned)s);
 }
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_down(StepAdjArgs A, long nsb) {
-    long sb;
-    int dir;
-    adj_block(nsb, 1, sb, dir);
-    const long s = sb * kAdjBlock + threadIdx.x;
-    if (s < A.g.S) step_vjp_down<unsigned>(A, (unsigned)s);
-}
-// passes C1 and C2 give each thread one link direction: a third of the work per thread, and the three directions
-// of a pass run in one launch
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_sigbar(StepAdjArgs A, double* Sb, long nsb) {
-    long sb;
-    int dir;
-    adj_block(nsb, 3, sb, dir);
-    const long s = sb * kAdjBlock + threadIdx.x;
-    if (s >= A.g.S) return;
-    switch (dir) {
-        case 0: step_vjp_sigbar_link<unsigned, 0>(A, (unsigned)s, Sb); break;
-        case 1: step_vjp_sigbar_link<unsigned, 1>(A, (unsigned)s, Sb); break;
-        default: step_vjp_sigbar_link<unsigned, 2>(A, (unsigned)s, Sb); break;
-    }
-}
+// the staple pass gives each thread one link direction (a third of the work per thread; the three directions run
+// in one launch): the own-link term, then the twelve staple terms gathered from the Sigbar of the neighbours
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb, long nsb) {
     long sb;
     int dir;
@@ -79,9 +62,9 @@ __global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const d
     const long s = sb * kAdjBlock + threadIdx.x;
     if (s >= A.g.S) return;
     switch (dir) {
-        case 0: step_vjp_gather_link<unsigned, 0>(A, Sb, (unsigned)s); break;
-        case 1: step_vjp_gather_link<unsigned, 1>(A, Sb, (unsigned)s); break;
-        default: step_vjp_gather_link<unsigned, 2>(A, Sb, (unsigned)s); break;
+        case 0: step_vjp_gather_link<unsigned, 0, true>(A, Sb, (unsigned)s); break;
+        case 1: step_vjp_gather_link<unsigned, 1, true>(A, Sb, (unsigned)s); break;
+        default: step_vjp_gather_link<unsigned, 2, true>(A, Sb, (unsigned)s); break;
     }
 }
 
@@ -135,11 +118,15 @@ SlabArgs slab_args(const glasma_ctx* ctx, int zc, int half) {
     return A;
 }
 
-int slab_energy(glasma_ctx* ctx, int zc, int half, double* out) {
+// slab energy of the fields U, E (the context's current fields when null)
+int slab_energy(glasma_ctx* ctx, int zc, int half, double* out, const double* U = nullptr, const double* E = nullptr) {
     const long nb = (ctx->g.S + kAdjBlock - 1) / kAdjBlock;
     int rc = ensure_red(ctx, (size_t)nb + 1);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_slab_energy, dim3((unsigned)nb), dim3(kAdjBlock), 0, ctx->stream, slab_args(ctx, zc, half), ctx->red);
+    SlabArgs A = slab_args(ctx, zc, half);
+    if (U) A.U = U;
+    if (E) A.E = E;
+    hipLaunchKernelGGL(k_slab_energy, dim3((unsigned)nb), dim3(kAdjBlock), 0, ctx->stream, A, ctx->red);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kAdjBlock), 0, ctx->stream, ctx->red, nb, ctx->red + nb);
     GL_HIP(ctx, hipMemcpyAsync(out, ctx->red + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -160,13 +147,14 @@ StepAdjArgs adj_args(const glasma_ctx* ctx) {
     return A;
 }
 
-// passes A, B, C1, C2 of one backward step; Sb: 54-plane scratch for the staple cotangents
-int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A, double* Sb) {
+// one backward step in two launches (A with B and C1's Sigbar, then C2 with C1's own-link term); Sb: 54-plane
+// scratch for the staple cotangents
+int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A0, double* Sb) {
     const long S = ctx->g.S;
     const long nsb = (S + kAdjBlock - 1) / kAdjBlock;
+    StepAdjArgs A = A0;
+    A.Sb = Sb;
     hipLaunchKernelGGL(k_vjp_local, adj_grid(S, 1), dim3(kAdjBlock), 0, ctx->stream, A, nsb);
-    hipLaunchKernelGGL(k_vjp_down, adj_grid(S, 1), dim3(kAdjBlock), 0, ctx->stream, A, nsb);
-    hipLaunchKernelGGL(k_vjp_sigbar, adj_grid(S, 3), dim3(kAdjBlock), 0, ctx->stream, A, Sb, nsb);
     hipLaunchKernelGGL(k_vjp_gather, adj_grid(S, 3), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb, nsb);
     GL_HIP(ctx, hipGetLastError());
     return 0;
@@ -188,6 +176,24 @@ int load_state(glasma_ctx* ctx, const State& s) {
     GL_HIP(ctx, hipMemcpyAsync(ctx->U[ctx->cur], s.U, kUPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     GL_HIP(ctx, hipMemcpyAsync(ctx->E, s.E, kEPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     GL_HIP(ctx, hipMemcpyAsync(ctx->J[ctx->cur], s.J, kJPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
+}
+
+// steps of one checkpoint segment from seg[0], each state into the next slot (U, E and J out of place) and U_h
+// into Uh[i]
+int run_segment(glasma_ctx* ctx, const std::vector<State>& seg, const std::vector<double*>& Uh, int len) {
+    for (int i = 0; i < len; i++) {
+        StepArgs F = step_args(ctx, ctx->cur, true);
+        F.U = seg[i].U;
+        F.Un = seg[i + 1].U;
+        F.E = seg[i].E;
+        F.En = seg[i + 1].E;
+        F.J = seg[i].J;
+        F.Jn = seg[i + 1].J;
+        F.UOld = Uh[i];
+        const int rc = launch_step_args(ctx, F);
+        if (rc) return rc;
+    }
     return 0;
 }
 
@@ -340,13 +346,15 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     tape.A1 = rc ? nullptr : b.get(ctx, 8 * S, &rc);
     tape.V0 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
     tape.V1 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
-    std::vector<State> ck(nseg > 0 ? nseg : 0), seg(seglen + 1);
+    // checkpoints at the starts of all segments but the last, whose states the forward run leaves in the segment
+    // slots
+    std::vector<State> ck(nseg > 1 ? nseg - 1 : 0), seg(seglen + 1);
     for (auto& s : ck) {
         s.U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
         s.E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
         s.J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
     }
-    for (size_t i = 1; i < seg.size(); i++) {  // seg[0] is the segment's checkpoint itself
+    for (size_t i = 1; i < seg.size(); i++) {  // seg[0] is the segment's first state itself
         seg[i].U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
         seg[i].E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
         seg[i].J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
@@ -367,19 +375,29 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     std::vector<double> h;
     rho_to_dev_layout(ctx, rho, h);
     GL_HIP(ctx, hipMemcpy(rho_d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    // forward with checkpoints
+    // forward: the segments before the last on the context's fields, from a checkpoint each; the last segment from
+    // the context's fields (left as they are) into the segment slots, so that its backward sweep recomputes nothing
     if ((rc = ic_left_from_charge(ctx, mv, rho_d, tape))) return rc;
-    for (int n = 0; n < nsteps; n++) {
+    const int nlast = nseg > 0 ? (nseg - 1) * ckpt : 0;
+    for (int n = 0; n < nlast; n++) {
         if (n % ckpt == 0 && (rc = save_state(ctx, ck[n / ckpt]))) return rc;
         launch_step(ctx, false);
     }
+    State fin = {ctx->U[ctx->cur], ctx->E, ctx->J[ctx->cur]};
+    if (nseg > 0) {
+        seg[0] = fin;
+        if ((rc = run_segment(ctx, seg, Uh, nsteps - nlast))) return rc;
+        fin = seg[nsteps - nlast];
+    }
     GL_HIP(ctx, hipGetLastError());
-    if ((rc = slab_energy(ctx, zc, half, energy))) return rc;
+    if ((rc = slab_energy(ctx, zc, half, energy, fin.U, fin.E))) return rc;
     // cotangent of the final state
     hipLaunchKernelGGL(k_fill0, agrid(kUPlanes * Sp), dim3(kAdjBlock), 0, ctx->stream, cU[0], kUPlanes * Sp);
     hipLaunchKernelGGL(k_fill0, agrid(kJPlanes * Sp), dim3(kAdjBlock), 0, ctx->stream, cJ[0], kJPlanes * Sp);
     {
         SlabArgs A = slab_args(ctx, zc, half);
+        A.U = fin.U;
+        A.E = fin.E;
         A.Eb = cE;
         hipLaunchKernelGGL(k_slab_vjp, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, cU[0]);
     }
@@ -387,21 +405,9 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     // backward sweep, one segment at a time
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int n0 = sg * ckpt, n1 = (n0 + ckpt < nsteps) ? n0 + ckpt : nsteps;
-        // recompute the segment straight into its slots: U and J ping-pong from slot i to slot i+1, UOld
-        // lands in Uh[i], and E (updated in place by the step) is copied into slot i+1 first
-        seg[0] = ck[sg];
-        for (int n = n0; n < n1; n++) {
-            const int i = n - n0;
-            GL_HIP(ctx, hipMemcpyAsync(seg[i + 1].E, seg[i].E, kEPlanes * Sp * sizeof(double), hipMemcpyDeviceToDevice,
-                                       ctx->stream));
-            StepArgs F = step_args(ctx, ctx->cur, true);
-            F.U = seg[i].U;
-            F.Un = seg[i + 1].U;
-            F.E = seg[i + 1].E;
-            F.J = seg[i].J;
-            F.Jn = seg[i + 1].J;
-            F.UOld = Uh[i];
-            if ((rc = launch_step_args(ctx, F))) return rc;
+        if (sg < nseg - 1) {  // recompute the segment from its checkpoint
+            seg[0] = ck[sg];
+            if ((rc = run_segment(ctx, seg, Uh, n1 - n0))) return rc;
         }
         for (int n = n1 - 1; n >= n0; n--) {
             const int i = n - n0;
@@ -420,8 +426,10 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
             cb ^= 1;
         }
     }
-    // the initial state again (its fields are the linearisation point of the initial-condition map)
-    if ((rc = ic_left_from_charge(ctx, mv, rho_d, tape))) return rc;
+    // the initial state in the context's fields again (the linearisation point of the initial-condition map): with
+    // one segment the fields were never advanced; the context's UOld is the initial condition's throughout, as no
+    // step above wrote it
+    if (nseg > 1 && (rc = load_state(ctx, ck[0]))) return rc;
     if ((rc = ic_left_vjp(ctx, mv, tape, cU[cb], cE, cJ[cb], rho_bar, work))) return rc;
     GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     GL_HIP(ctx, hipMemcpy(h.data(), rho_bar, h.size() * sizeof(double), hipMemcpyDeviceToHost));

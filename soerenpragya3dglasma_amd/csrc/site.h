@@ -77,6 +77,13 @@ GHD double* site_ptr(double* base, int NP, int p0, I s) {
 #endif
     return q;
 }
+// keeps the compiler from moving instructions across this point (device only): bounds how much of the next
+// stage's loads it hoists, and so the live registers
+GHD void sched_fence() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_sched_barrier(0);
+#endif
+}
 // streamed outputs are written non-temporally so that they do not evict neighbour data from L2 / MALL
 GHD void st_nt(double* p, double v0, double v1) {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GLASMA_NO_NT_STORES)
@@ -275,7 +282,8 @@ GHD long tile_site(const Geom& g, const TileMap& m, long b) {
 struct StepArgs {
     const double* __restrict__ U;  // time t links
     double* __restrict__ Un;       // time t+dt links
-    double* __restrict__ E;        // in place
+    double* __restrict__ E;        // time t electric fields (updated in place when En is null)
+    double* __restrict__ En = nullptr;  // time t+dt electric fields, when not in place
     const double* __restrict__ J;  // time t currents (J+, J-, J+stat, J-stat)
     double* __restrict__ Jn;       // time t+dt currents
     double* __restrict__ UOld;     // written when non-null
@@ -369,7 +377,7 @@ GHD void link_finish_j(const StepArgs& A, I s, M3& Uo, const M3& Sig, M3& Uh, co
 #pragma unroll
         for (int a = 0; a < 8; a++) Em[a] += -A.dtz * ((jp[a] - jm[a]) / kSqrt2);
     }
-    stv<I>(A.E, kEPlanes, 8 * MU, s, Em);
+    stv<I>(A.En ? A.En : A.E, kEPlanes, 8 * MU, s, Em);
     M3 D;
     expi_m1(A.cU[MU], Em, D);
     add_mul(D, Uo, Uh);

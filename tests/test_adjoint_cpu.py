@@ -87,7 +87,8 @@ def test_slab_energy_vjp_against_finite_differences():
 
 @pytest.mark.parametrize("N,bc", [((4, 3, 5), 0), ((4, 4, 4), 1), ((3, 3, 1), 0), ((2, 2, 3), 0), ((2, 3, 1), 1)])
 def test_gather_form_equals_scatter_form(N, bc):
-    """The device's atomic-free staple adjoint (C1 + C2) equals the scatter form, boundary layers included."""
+    """The atomic-free staple adjoint (C1 + C2) equals the scatter form, boundary layers included, and the device's
+    two-pass form equals the separate passes bit for bit."""
     hc = _lib()
     hc.hc_step_vjp_gather.argtypes = [ctypes.POINTER(ctypes.c_int), dp, ctypes.c_int, ctypes.c_double] + [dp] * 11
     a, dt = (1.0, 0.9, 0.7), 0.1
@@ -109,3 +110,10 @@ def test_gather_form_equals_scatter_form(N, bc):
     m = LY.tile(np.ones((54, S))) > 0
     assert np.abs(Ub1 - Ub2)[m].max() < 1e-14 * np.abs(Ub1[m]).max()
     assert np.array_equal(Eb1, Eb2) and np.array_equal(Jb1, Jb2)
+    # the device's two passes (A with B folded in and the Sigbar store, then the gather with the own-link term)
+    # make the same sums as the separate passes, in the same order
+    hc.hc_step_vjp_fused.argtypes = hc.hc_step_vjp_gather.argtypes
+    Ub3, Eb3, Jb3, Sb3 = np.zeros_like(U), wE.copy(), np.zeros_like(J), np.zeros_like(U)
+    hc.hc_step_vjp_fused(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub3), p(Eb3), p(wJ), p(Jb3), p(Sb3))
+    assert np.array_equal(Ub3[m], Ub2[m]) and np.array_equal(Eb3, Eb2) and np.array_equal(Jb3, Jb2)
+    assert np.array_equal(Sb3[m], Sb[m])
