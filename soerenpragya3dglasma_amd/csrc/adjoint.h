@@ -59,36 +59,24 @@ GHD void acc_lvr(double s, const double* l, const M3& V, const double* r, M3& Vb
     m3_axpy(4.0 * s, P, Vbar);
 }
 
-// ebar += cU Re tr(i t^a Zbar^dag), Zbar = sum_k ad_A^k(X^dag Xbar)/(k+1)!, A = i cU e^a t^a.
-// Only the traceless anti-Hermitian part of Y = X^dag Xbar reaches ebar (Re tr(i t^a H) = 0 for Hermitian H, and
-// the trace commutes with A), and ad_A keeps that part: in the basis i t^b it is y^b = -2 Re tr(i t^b Y), on which
-// ad_A acts as y'^c = -cU f^{abc} e^a y^b with the structure constants of su(3), [t^a, t^b] = i f^{abc} t^c.  Then
-// Zbar's part is z = sum_k y_k / (k+1)! and ebar^a += (cU / 2) z^a.  Each term of the series is 54 multiply-adds on
-// eight numbers instead of two 3 x 3 complex products.
-GHD void dexp_adjoint(double cU, const double* e, const M3& X, const M3& Xbar, double* ebar) {
+// z = sum_k ad_A^k(y) / (k+1)! for A = i c e^a t^a, on coordinates in the basis i t^a of su(3) (y is overwritten).  In
+// that basis ad_A acts as y'^r = -c f^{pqr} e^p y^q with the structure constants of su(3), [t^a, t^b] = i f^{abc} t^c:
+// each term of the series is 54 multiply-adds on eight numbers instead of two 3 x 3 complex products.
+GHD void ad_series(double c, const double* e, double* y, double* z) {
     // the nonzero f^{pqr}, p < q < r (0-based, t^a = lambda^a / 2 in retr_igen's order), and their values
     constexpr int tri[9][3] = {{0, 1, 2}, {0, 3, 6}, {0, 4, 5}, {1, 3, 5}, {1, 4, 6},
                                {2, 3, 4}, {2, 5, 6}, {3, 4, 7}, {5, 6, 7}};
     constexpr double h3 = 0.86602540378443864676;  // sqrt(3) / 2
     constexpr double fv[9] = {1.0, 0.5, -0.5, 0.5, 0.5, 0.5, -0.5, h3, h3};
-    double y[8], z[8];
-    {
-        M3 Y;
-        mul_dn(X, Xbar, Y);
-        retr_igen(Y, y);
-    }
 #pragma unroll
-    for (int a = 0; a < 8; a++) {
-        y[a] *= -2.0;
-        z[a] = y[a];
-    }
-    // w[t] = -cU f^{pqr} (e_p, e_q, e_r) of triple t: y'_r += w_p y_q - w_q y_p, y'_p += w_q y_r - w_r y_q,
+    for (int a = 0; a < 8; a++) z[a] = y[a];
+    // w[t] = -c f^{pqr} (e_p, e_q, e_r) of triple t: y'_r += w_p y_q - w_q y_p, y'_p += w_q y_r - w_r y_q,
     // y'_q += w_r y_p - w_p y_r
     double w[9][3];
 #pragma unroll
     for (int t = 0; t < 9; t++)
 #pragma unroll
-        for (int k = 0; k < 3; k++) w[t][k] = -cU * fv[t] * e[tri[t][k]];
+        for (int k = 0; k < 3; k++) w[t][k] = -c * fv[t] * e[tri[t][k]];
     double fact = 1.0;
     for (int it = 1; it <= 40; it++) {
         double yn[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -110,6 +98,22 @@ GHD void dexp_adjoint(double cU, const double* e, const M3& X, const M3& Xbar, d
         for (int a = 0; a < 8; a++) z[a] = fma(fact, y[a], z[a]);
         if (mx * fact < 1e-19) break;
     }
+}
+
+// ebar += cU Re tr(i t^a Zbar^dag), Zbar = sum_k ad_A^k(X^dag Xbar)/(k+1)!, A = i cU e^a t^a.
+// Only the traceless anti-Hermitian part of Y = X^dag Xbar reaches ebar (Re tr(i t^a H) = 0 for Hermitian H, and
+// the trace commutes with A), and ad_A keeps that part: in the basis i t^b it is y^b = -2 Re tr(i t^b Y).  Then
+// Zbar's part is z = ad_series(y) and ebar^a += (cU / 2) z^a.
+GHD void dexp_adjoint(double cU, const double* e, const M3& X, const M3& Xbar, double* ebar) {
+    double y[8], z[8];
+    {
+        M3 Y;
+        mul_dn(X, Xbar, Y);
+        retr_igen(Y, y);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++) y[a] *= -2.0;
+    ad_series(cU, e, y, z);
 #pragma unroll
     for (int a = 0; a < 8; a++) ebar[a] = fma(0.5 * cU, z[a], ebar[a]);
 }

@@ -20,7 +20,7 @@ __device__ __forceinline__ void store_m(double* p, const M3& A) {
     }
 }
 
-__global__ void k_su3_batch(int op, long n, double c, const double* __restrict__ in, const double* __restrict__ in2,
+__global__ void __launch_bounds__(128) k_su3_batch(int op, long n, double c, const double* __restrict__ in, const double* __restrict__ in2,
                             double* __restrict__ out) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;

@@ -135,7 +135,7 @@ __device__ __forceinline__ double hash_uniform(unsigned long long seed, unsigned
     return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 // uniform in [-1, 1) per component: links exp(i s_U alpha t) with alpha_a uniform, E and J uniform
-__global__ void k_fill_random(double* __restrict__ U, double* __restrict__ E, double* __restrict__ J, long S,
+__global__ void __launch_bounds__(256) k_fill_random(double* __restrict__ U, double* __restrict__ E, double* __restrict__ J, long S,
                               unsigned long long seed, double su, double se, double sj) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;

@@ -115,7 +115,7 @@ __global__ void k_load_real(double* __restrict__ F, const double* __restrict__ r
 // SetNewWilsonLines for all z: one thread per transverse site walks z in the reference's order
 // (L: Nz-1 down to 0 with exp(-i A t / sqrt 2); R: 0 up to Nz-1 with exp(+i A t / sqrt 2)),
 // W <- exp(+-i A(z) t / sqrt 2) W, V(z) = W.
-__global__ void k_wilson_scan(const double* __restrict__ F, double* __restrict__ V, Geom g, int right) {
+__global__ void __launch_bounds__(256) k_wilson_scan(const double* __restrict__ F, double* __restrict__ V, Geom g, int right) {
     long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s2 >= g.P) return;
     M3 W, X, T;
@@ -175,7 +175,7 @@ __global__ void k_evolve_phase(double* __restrict__ F, Geom g, double az, double
 }
 
 // E_perp(t + dt/2) = MatrixILog(coff_mu, U_mu UOld_mu^dag), E_z = 0 for z in [zlo, zhi]  (:66-92)
-__global__ void k_sheet_efield(const double* __restrict__ U, const double* __restrict__ UO, double* __restrict__ E,
+__global__ void __launch_bounds__(256) k_sheet_efield(const double* __restrict__ U, const double* __restrict__ UO, double* __restrict__ E,
                                Geom g, int zlo, int zhi, double coff0, double coff1) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = (long)(zhi - zlo + 1) * g.P;
@@ -196,7 +196,7 @@ __global__ void k_sheet_efield(const double* __restrict__ U, const double* __res
 
 // J_stat = sqrt2 * sum_{i=x,y} (E_i(x) - U_i^dag(x-i) E_i(x-i) U_i(x-i)) inside [zlo, zhi], 0 outside
 // (:98-125); also F = (J_stat, 0) for the continuity inversion.
-__global__ void k_sheet_jstat(const double* __restrict__ U, const double* __restrict__ E, double* __restrict__ J,
+__global__ void __launch_bounds__(256) k_sheet_jstat(const double* __restrict__ U, const double* __restrict__ E, double* __restrict__ J,
                               double* __restrict__ F, Geom g, int zlo, int zhi, int wstat) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.S) return;
@@ -438,7 +438,7 @@ __global__ void k_zero(double* __restrict__ p, long n) {
 }
 
 // J+stat cotangent jsb (real, a*S + s) -> E_x, E_y and U_x, U_y cotangents (gathered at y from y + i)
-__global__ void k_ic_jstat_vjp(const double* __restrict__ U, const double* __restrict__ E,
+__global__ void __launch_bounds__(256) k_ic_jstat_vjp(const double* __restrict__ U, const double* __restrict__ E,
                                const double* __restrict__ jsb, double* __restrict__ Ub, double* __restrict__ Eb,
                                Geom g) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -493,47 +493,27 @@ __device__ void solve8(double (&A)[8][8], double* b) {
 // derivative G = sum_k (-ad_A)^k/(k+1)!, M^dag dM = G(i coff t dE); in components kappa = -2 Re tr(i t M^dag dM)
 // = Ghat coff dE, so kappabar = Ghat^-T Ebar / coff and Mbar = 2 i M H(kappabar).  Ubar += Mbar UOld,
 // UOldbar = Mbar^dag U.
-__global__ void k_ic_ilog_vjp(const double* __restrict__ U, const double* __restrict__ UO,
+__global__ void __launch_bounds__(256) k_ic_ilog_vjp(const double* __restrict__ U, const double* __restrict__ UO,
                               const double* __restrict__ E, const double* __restrict__ Eb, double* __restrict__ Ub,
                               double* __restrict__ UOb, Geom g, double coff0, double coff1) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.S) return;
     for (int mu = 0; mu < 2; mu++) {
         const double coff = mu == 0 ? coff0 : coff1;
-        M3 Um, Uo, M, Hm, A;
+        M3 Um, Uo, M;
         ldm<long>(U, kUPlanes, 18 * mu, s, Um);
         ldm<long>(UO, kUPlanes, 18 * mu, s, Uo);
         mul_nd(Um, Uo, M);
         double e[8], eb[8];
         ldv<long>(E, kEPlanes, 8 * mu, s, e);
         ldv<long>(Eb, kEPlanes, 8 * mu, s, eb);
-        herm_from_alg(coff, e, Hm);
-        for (int k = 0; k < 9; k++) { A.re[k] = -Hm.im[k]; A.im[k] = Hm.re[k]; }
+        // row b = column b of Ghat: the coordinates (basis i t^a) of sum_k (-ad_A)^k (i t^b) / (k+1)!, A = i coff e^a t^a,
+        // so -ad_A is ad_series' map with c = -coff
         double G[8][8];
         for (int b = 0; b < 8; b++) {
             double unit[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             unit[b] = 1.0;
-            M3 Y, Z, T1, T2, Hb;
-            herm_from_alg(1.0, unit, Hb);
-            for (int k = 0; k < 9; k++) { Y.re[k] = -Hb.im[k]; Y.im[k] = Hb.re[k]; }  // i t^b
-            Z = Y;
-            double fact = 1.0;
-            for (int kk = 1; kk <= 40; kk++) {
-                mul_nn(A, Y, T1);
-                mul_nn(Y, A, T2);
-                double mx = 0.0;
-                for (int q = 0; q < 9; q++) {
-                    Y.re[q] = T2.re[q] - T1.re[q];  // -ad_A
-                    Y.im[q] = T2.im[q] - T1.im[q];
-                    mx = fmax(mx, fmax(fabs(Y.re[q]), fabs(Y.im[q])));
-                }
-                fact /= (double)(kk + 1);
-                m3_axpy(fact, Y, Z);
-                if (mx * fact < 1e-19) break;
-            }
-            double col[8];
-            retr_igen(Z, col);
-            for (int a = 0; a < 8; a++) G[b][a] = -2.0 * col[a];  // transpose: row b = column b of Ghat
+            ad_series(-coff, e, unit, G[b]);
         }
         double kb[8];
         for (int a = 0; a < 8; a++) kb[a] = eb[a] / coff;
@@ -553,7 +533,7 @@ __global__ void k_ic_ilog_vjp(const double* __restrict__ U, const double* __rest
 }
 
 // V cotangent from the link cotangents of U_mu = V^dag(x) V(x + mu), mu = x, y (gathered at y)
-__global__ void k_links_vjp(const double* __restrict__ V, const double* __restrict__ Ub, double* __restrict__ Vb,
+__global__ void __launch_bounds__(256) k_links_vjp(const double* __restrict__ V, const double* __restrict__ Ub, double* __restrict__ Vb,
                             Geom g) {
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.S) return;
@@ -577,30 +557,43 @@ __global__ void k_links_vjp(const double* __restrict__ V, const double* __restri
     stm<long>(Vb, kGPlanes, 0, s, acc);
 }
 
-// reverse of the left-moving Wilson-line scan V(z) = X(z) V(z+1), X(z) = exp(-i A(z) t / sqrt2):
-// Abar (real, a*S + s) from Vbar, one thread per transverse site walking z upwards
-__global__ void k_scan_vjp_left(const double* __restrict__ A, const double* __restrict__ V,
-                                const double* __restrict__ Vb, double* __restrict__ Ab, Geom g) {
+// Reverse of the left-moving Wilson-line scan V(z) = X(z) V(z+1), X(z) = exp(-i A(z) t / sqrt2), in two kernels.
+// The total cotangent of V(z) is its own Vbar(z) plus, through V(z-1) = X(z-1) V(z), X(z-1)^dag times that of V(z-1);
+// with X(z') ... X(z-1) = V(z') V(z)^dag (V unitary) this is Wbar(z) = V(z) sum_{z' <= z} V(z')^dag Vbar(z'), a running
+// sum along z (k_scan_vjp_sum, one thread per transverse site, Wbar in place of Vbar), after which every layer's
+// exponential adjoint is independent (k_scan_vjp_site, one thread per site): Xbar = Wbar(z) V(z+1)^dag.
+__global__ void __launch_bounds__(256) k_scan_vjp_sum(const double* __restrict__ V, double* __restrict__ Vb, Geom g) {
     long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s2 >= g.P) return;
-    const double c = -1.0 / kSqrt2;
-    M3 carry, Wb, Wn, X, Xb, T;
-    m3_zero(carry);
+    M3 acc, Vz, B, T;
+    m3_zero(acc);
     for (int z = 0; z < g.nz; z++) {
         const long s = s2 + g.P * z;
-        ldm<long>(Vb, kGPlanes, 0, s, Wb);
-        m3_axpy(1.0, carry, Wb);
-        if (z + 1 < g.nz) ldm<long>(V, kGPlanes, 0, s + g.P, Wn);
-        else set_unit(Wn);
-        double a[8], ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int q = 0; q < 8; q++) a[q] = A[(long)q * g.S + s];
-        expi(c, a, X);
-        mul_nd(Wb, Wn, Xb);  // Xbar = Wbar W(z+1)^dag
-        dexp_adjoint(c, a, X, Xb, ab);
-        for (int q = 0; q < 8; q++) Ab[(long)q * g.S + s] = ab[q];
-        mul_dn(X, Wb, T);  // carried into W(z+1)
-        carry = T;
+        ldm<long>(V, kGPlanes, 0, s, Vz);
+        ldm<long>(Vb, kGPlanes, 0, s, B);
+        mul_dn(Vz, B, T);
+        m3_axpy(1.0, T, acc);
+        mul_nn(Vz, acc, T);
+        stm<long>(Vb, kGPlanes, 0, s, T);
     }
+}
+// Abar (real, a*S + s) from Wbar
+__global__ void __launch_bounds__(256) k_scan_vjp_site(const double* __restrict__ A, const double* __restrict__ V,
+                                                       const double* __restrict__ Wb, double* __restrict__ Ab, Geom g) {
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.S) return;
+    const double c = -1.0 / kSqrt2;
+    const int z = (int)(s / g.P);
+    M3 W, Wn, X, Xb;
+    ldm<long>(Wb, kGPlanes, 0, s, W);
+    if (z + 1 < g.nz) ldm<long>(V, kGPlanes, 0, s + g.P, Wn);
+    else set_unit(Wn);
+    double a[8], ab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 8; q++) a[q] = A[(long)q * g.S + s];
+    expi(c, a, X);
+    mul_nd(W, Wn, Xb);
+    dexp_adjoint(c, a, X, Xb, ab);
+    for (int q = 0; q < 8; q++) Ab[(long)q * g.S + s] = ab[q];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -820,8 +813,10 @@ int ic_left_vjp(glasma_ctx* ctx, const glasma_mv_params* mv, const IcTape& tape,
     // links -> Wilson lines -> Poisson solutions
     hipLaunchKernelGGL(k_links_vjp, grid1(g.S), dim3(256), 0, st, tape.V1, Ub, V1b, g);
     hipLaunchKernelGGL(k_links_vjp, grid1(g.S), dim3(256), 0, st, tape.V0, UOb, V0b, g);
-    hipLaunchKernelGGL(k_scan_vjp_left, grid1(g.P), dim3(256), 0, st, tape.A1, tape.V1, V1b, A1b, g);
-    hipLaunchKernelGGL(k_scan_vjp_left, grid1(g.P), dim3(256), 0, st, tape.A0, tape.V0, V0b, A0b, g);
+    hipLaunchKernelGGL(k_scan_vjp_sum, grid1(g.P), dim3(256), 0, st, tape.V1, V1b, g);
+    hipLaunchKernelGGL(k_scan_vjp_sum, grid1(g.P), dim3(256), 0, st, tape.V0, V0b, g);
+    hipLaunchKernelGGL(k_scan_vjp_site, grid1(g.S), dim3(256), 0, st, tape.A1, tape.V1, V1b, A1b, g);
+    hipLaunchKernelGGL(k_scan_vjp_site, grid1(g.S), dim3(256), 0, st, tape.A0, tape.V0, V0b, A0b, g);
     // rho_ev cotangent = poisson(A1bar); rho_s cotangent = evolve_R(that) + poisson(A0bar)
     hipLaunchKernelGGL(k_load_real, grid1(g.S), dim3(256), 0, st, F, A1b, g);
     GL_RC(poisson_all(ctx, F));
