@@ -494,13 +494,82 @@ GHD int z_sources(const Geom& g, int zy, int dz, int* zs) {
     return n;
 }
 
-// OWN (the device form, after the fused pass A): the own-link term of C1 (own_link_term) is added first, as C1
-// would have added it to Ub.
-template <class I, int LAM, bool OWN = false>
-GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
+// The gather with the own-link term on a layer where every staple term has its source at the plain neighbour position
+// (periodic z, or 1 <= z <= nz - 2 with open z; z_sources gives exactly one source there).  For each transverse
+// direction d the six links a = U_lam(x+d), b = U_lam(x-d), c = U_d(x+lam), e = U_d(x-d+lam), f = U_d(x-d),
+// h = U_d(x) are all that the six staple terms of d and d's part of the staple sum Sigma_lam(x) read, so each is read
+// once (25 matrix reads per link against 49), and the products h a and f^dag b serve two terms each:
+//   E_d staples (coefficient cE[lam]): Sbar_d(x)^dag a c^dag, Sbar_d(x-d) b e, (f^dag b) Sbar_d(x-d+lam)^dag,
+//     (h a) Sbar_d(x+lam);
+//   E_lam staples (cE[d]): f^dag Sbar_lam(x-d)^dag e, h Sbar_lam(x+d)^dag c^dag;
+//   Sigma_lam += cE[d] (c (h a)^dag + e^dag (f^dag b)^dag), the own-link term Mbar_lam Sigma_lam^dag added last.
+template <class I, int LAM>
+GHD void gather_link_interior(const StepAdjArgs& A, const double* __restrict__ Sb, I s, int x, int y, int z) {
     const Geom& g = A.g;
-    int x, y, z;
-    site_coords<I>(g, s, x, y, z);
+    constexpr int lam = LAM;
+    const double cl = A.cE[lam];
+    M3 acc, Sig;
+    ldm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
+    m3_zero(Sig);
+    // not unrolled: unrolled, the compiler interleaves the two directions and runs out of registers
+    for (int k = 1; k <= 2; k++) {
+        const int d = (lam + k) % 3;
+        const double cd = A.cE[d];
+        const I s_pd = nbr<I>(g, x, y, z, d, 1, lam, 0), s_md = nbr<I>(g, x, y, z, d, -1, lam, 0);
+        const I s_pl = nbr<I>(g, x, y, z, lam, 1, d, 0), s_mdpl = nbr<I>(g, x, y, z, lam, 1, d, -1);
+        {
+            M3 a, h, c, T1, Sbar;
+            ldm<I>(A.U, kUPlanes, 18 * lam, s_pd, a);
+            ldm<I>(A.U, kUPlanes, 18 * d, s, h);
+            ldm<I>(A.U, kUPlanes, 18 * d, s_pl, c);
+            ldm<I>(Sb, kUPlanes, 18 * d, s, Sbar);
+            mul_dn(Sbar, a, T1);
+            acc_mul_nd(cl, T1, c, acc);
+            mul_nn(h, a, T1);
+            ldm<I>(Sb, kUPlanes, 18 * d, s_pl, Sbar);
+            acc_mul_nn(cl, T1, Sbar, acc);
+            acc_mul_nd(cd, c, T1, Sig);
+            ldm<I>(Sb, kUPlanes, 18 * lam, s_pd, Sbar);
+            mul_nd(h, Sbar, T1);
+            acc_mul_nd(cd, T1, c, acc);
+        }
+        sched_fence();
+        {
+            M3 b, f, e, T1, T2, Sbar;
+            ldm<I>(A.U, kUPlanes, 18 * lam, s_md, b);
+            ldm<I>(A.U, kUPlanes, 18 * d, s_md, f);
+            ldm<I>(A.U, kUPlanes, 18 * d, s_mdpl, e);
+            ldm<I>(Sb, kUPlanes, 18 * d, s_md, Sbar);
+            mul_nn(Sbar, b, T1);
+            acc_mul_nn(cl, T1, e, acc);
+            mul_dn(f, b, T1);
+            ldm<I>(Sb, kUPlanes, 18 * d, s_mdpl, Sbar);
+            acc_mul_nd(cl, T1, Sbar, acc);
+            mul_dd(e, T1, T2);
+            m3_axpy(cd, T2, Sig);
+            ldm<I>(Sb, kUPlanes, 18 * lam, s_md, Sbar);
+            mul_dd(f, Sbar, T1);
+            acc_mul_nn(cd, T1, e, acc);
+        }
+        sched_fence();
+    }
+    double eb[8];
+    ldv<I>(A.Eb, kEPlanes, 8 * lam, s, eb);
+    M3 Mb, T;
+    m_from_retr_cot(A.gc[lam], eb, Mb);
+    mul_nd(Mb, Sig, T);
+    m3_axpy(1.0, T, acc);
+    stm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
+}
+
+// the layers where gather_link_interior applies
+GHD bool gather_interior(const Geom& g, int z) { return g.bc || (z >= 1 && z + 2 <= g.nz); }
+
+// The gather for any layer, each staple term from its sources (z_sources).  OWN (the device form, after the fused pass
+// A): the own-link term of C1 (own_link_term) is added first, as C1 would have added it to Ub.
+template <class I, int LAM, bool OWN>
+GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb, I s, int x, int y, int z) {
+    const Geom& g = A.g;
     constexpr int lam = LAM;
     {
         M3 acc, Sbar, P, Q, R, T1;
@@ -599,6 +668,16 @@ GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ S
         }
         stm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
     }
+}
+
+// OWN: the device form (interior layers by gather_link_interior, the others by gather_link_general with the own-link
+// term); else the C2 pass of the separate form
+template <class I, int LAM, bool OWN = false>
+GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
+    int x, y, z;
+    site_coords<I>(A.g, s, x, y, z);
+    if (OWN && gather_interior(A.g, z)) gather_link_interior<I, LAM>(A, Sb, s, x, y, z);
+    else gather_link_general<I, LAM, OWN>(A, Sb, s, x, y, z);
 }
 
 template <class I, bool OWN = false>

@@ -53,8 +53,10 @@ __global__ void __launch_bounds__(kAdjBlock) k_vjp_local(StepAdjArgs A, long nsb
     const long s = sb * kAdjBlock + threadIdx.x;
     if (s < A.g.S) step_vjp_local<unsigned, true>(A, (unsigned)s);
 }
-// the staple pass gives each thread one link direction (a third of the work per thread; the three directions run
-// in one launch): the own-link term, then the twelve staple terms gathered from the Sigbar of the neighbours
+// The staple pass gives each thread one link direction (a third of the work per thread; the three directions run
+// in one launch): the own-link term and the twelve staple terms gathered from the Sigbar of the neighbours.
+// Measured at 48 x 48 x 128: 283 us at one wave per SIMD; the interior layers at two waves (252 registers) in one
+// launch and the two open-z boundary layers in a second took 274 + 33 us.
 __global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb, long nsb) {
     long sb;
     int dir;
