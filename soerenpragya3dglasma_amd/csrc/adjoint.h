@@ -13,9 +13,9 @@
 //   C (staples): the staple terms of the E update, either scattered into the links of neighbouring sites
 //     (step_vjp_staples, with an accumulation functor) or, without atomics, as C1 (step_vjp_sigbar) +
 //     C2 (step_vjp_gather).
-// The device runs two passes: A with B folded in and C1's Sigbar store (step_vjp_local<I, true>), then C2 with C1's
-// own-link term (step_vjp_gather<I, true>); the host build also runs the separate passes and the scatter form, which
-// the tests compare with it.
+// The device runs two passes: A with B folded in (step_vjp_local<I, true>), then C2 with C1's own-link term
+// (step_vjp_gather<I, true>), which forms each Sigbar it needs from Ebar; the host build also runs the separate passes
+// and the scatter form, which the tests compare with it.
 // The exponential's adjoint: with A = i cU E' t, dexp_A[H] = e^A sum_k (-ad_A)^k H/(k+1)!, so
 // Zbar = sum_k ad_A^k (X^dag Xbar)/(k+1)! and Ebar^a += cU Re tr(i t^a Zbar^dag).
 #pragma once
@@ -130,7 +130,6 @@ struct StepAdjArgs {
     double* __restrict__ Eb;         // in: cotangent of E(t+dt); out: cotangent of E(t) (in place)
     const double* __restrict__ Jb1;  // cotangent of J(t+dt)
     double* __restrict__ Jb;         // cotangent of J(t)
-    double* __restrict__ Sb = nullptr;  // Sigbar_mu = U_mu^dag Mbar_mu (written by the fused pass A)
     Geom g;
     double gc[3], cE[3], cU[3], dtz;
 };
@@ -203,8 +202,8 @@ GHD void down_terms(const StepAdjArgs& A, const M3& Uz, const double* jp, const 
 }
 
 // Pass A at site s.  FUSED (the device form): pass B's terms at s are added here, with the J+-stat' cotangents of
-// the site above formed again from the inputs (jstat_cot), and Sigbar_mu(s) = U_mu^dag Mbar_mu is stored to A.Sb for
-// the staple gather.  The sums are those of the separate passes, in the same order.
+// the site above formed again from the inputs (jstat_cot).  The sums are those of the separate passes, in the same
+// order.
 template <class I, bool FUSED = false>
 GHD void step_vjp_local(const StepAdjArgs& A, I s) {
     const Geom& g = A.g;
@@ -251,13 +250,6 @@ GHD void step_vjp_local(const StepAdjArgs& A, I s) {
         else stm<I>(A.Ub, kUPlanes, 18 * mu, s, T);
         dexp_adjoint(A.cU[mu], e1, X, Xb, eb);
         stv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
-        if (FUSED) {  // U_mu(s) read again rather than kept live through the exponential's adjoint
-            M3 Mb;
-            m_from_retr_cot(A.gc[mu], eb, Mb);
-            ldm<I>(A.U, kUPlanes, 18 * mu, s, U0);
-            mul_dn(U0, Mb, T);
-            stm<I>(A.Sb, kUPlanes, 18 * mu, s, T);
-        }
         if (mu == 2)
             for (int a = 0; a < 8; a++) ebz[a] = eb[a];
         if (FUSED) sched_fence();
@@ -494,17 +486,47 @@ GHD int z_sources(const Geom& g, int zy, int dz, int* zs) {
     return n;
 }
 
+// Mbar_mu(s) = -i gc_mu H(Ebar_mu(s)), the cotangent of U_mu Sigma_mu; Sigbar_mu = U_mu^dag Mbar_mu
+template <class I>
+GHD void mbar_at(const StepAdjArgs& A, int mu, I s, M3& Mb) {
+    double eb[8];
+    ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
+    m_from_retr_cot(A.gc[mu], eb, Mb);
+}
+template <class I>
+GHD void sigbar_at(const StepAdjArgs& A, int mu, I s, M3& Sbar) {
+    M3 U, Mb;
+    ldm<I>(A.U, kUPlanes, 18 * mu, s, U);
+    mbar_at<I>(A, mu, s, Mb);
+    mul_dn(U, Mb, Sbar);
+}
+// C += c W^dag
+GHD void m3_axpy_dag(double c, const M3& W, M3& C) {
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            C.re[mi(i, j)] = fma(c, W.re[mi(j, i)], C.re[mi(i, j)]);
+            C.im[mi(i, j)] = fma(-c, W.im[mi(j, i)], C.im[mi(i, j)]);
+        }
+}
+
 // The gather with the own-link term on a layer where every staple term has its source at the plain neighbour position
 // (periodic z, or 1 <= z <= nz - 2 with open z; z_sources gives exactly one source there).  For each transverse
 // direction d the six links a = U_lam(x+d), b = U_lam(x-d), c = U_d(x+lam), e = U_d(x-d+lam), f = U_d(x-d),
-// h = U_d(x) are all that the six staple terms of d and d's part of the staple sum Sigma_lam(x) read, so each is read
-// once (25 matrix reads per link against 49), and the products h a and f^dag b serve two terms each:
-//   E_d staples (coefficient cE[lam]): Sbar_d(x)^dag a c^dag, Sbar_d(x-d) b e, (f^dag b) Sbar_d(x-d+lam)^dag,
-//     (h a) Sbar_d(x+lam);
-//   E_lam staples (cE[d]): f^dag Sbar_lam(x-d)^dag e, h Sbar_lam(x+d)^dag c^dag;
-//   Sigma_lam += cE[d] (c (h a)^dag + e^dag (f^dag b)^dag), the own-link term Mbar_lam Sigma_lam^dag added last.
+// h = U_d(x) are all that the six staple terms of d and d's part of the staple sum Sigma_lam(x) read, and each
+// Sigbar_mu(x') = U_mu(x')^dag Mbar_mu(x') a term reads has its link among them, so the terms need only Mbar, formed from
+// Ebar (8 numbers) at the source site:
+//   E_d staples (coefficient cE[lam]): Sbar_d(x)^dag a c^dag = Mbar_d(x)^dag W, (h a) Sbar_d(x+lam) = W Mbar_d(x+lam),
+//     Sbar_d(x-d) b e = f^dag Mbar_d(x-d) V, (f^dag b) Sbar_d(x-d+lam)^dag = (f^dag b) Mbar_d(x-d+lam)^dag e;
+//   E_lam staples (cE[d]): f^dag Sbar_lam(x-d)^dag e = f^dag Mbar_lam(x-d)^dag V, h Sbar_lam(x+d)^dag c^dag =
+//     h Mbar_lam(x+d)^dag a c^dag;
+//   Sigma_lam += cE[d] (c (h a)^dag + e^dag (f^dag b)^dag) = cE[d] (W^dag + e^dag (f^dag b)^dag);
+// with W = h a c^dag and V = b e, and the own-link term Mbar_lam(x) Sigma_lam^dag added last.  The two terms at x-d
+// share one product: Mbar is linear in Ebar, so cE[lam] Mbar_d + cE[d] Mbar_lam^dag = Mbar of the combined vector
+// (Mbar^dag = -Mbar).  Each link reads 12 links and 13 Ebar (against 25 links and 24 Sigbar in the plain form).
 template <class I, int LAM>
-GHD void gather_link_interior(const StepAdjArgs& A, const double* __restrict__ Sb, I s, int x, int y, int z) {
+GHD void gather_link_interior(const StepAdjArgs& A, I s, int x, int y, int z) {
     const Geom& g = A.g;
     constexpr int lam = LAM;
     const double cl = A.cE[lam];
@@ -518,45 +540,51 @@ GHD void gather_link_interior(const StepAdjArgs& A, const double* __restrict__ S
         const I s_pd = nbr<I>(g, x, y, z, d, 1, lam, 0), s_md = nbr<I>(g, x, y, z, d, -1, lam, 0);
         const I s_pl = nbr<I>(g, x, y, z, lam, 1, d, 0), s_mdpl = nbr<I>(g, x, y, z, lam, 1, d, -1);
         {
-            M3 a, h, c, T1, Sbar;
+            M3 a, h, c, W, T1, Mb;
             ldm<I>(A.U, kUPlanes, 18 * lam, s_pd, a);
             ldm<I>(A.U, kUPlanes, 18 * d, s, h);
             ldm<I>(A.U, kUPlanes, 18 * d, s_pl, c);
-            ldm<I>(Sb, kUPlanes, 18 * d, s, Sbar);
-            mul_dn(Sbar, a, T1);
-            acc_mul_nd(cl, T1, c, acc);
             mul_nn(h, a, T1);
-            ldm<I>(Sb, kUPlanes, 18 * d, s_pl, Sbar);
-            acc_mul_nn(cl, T1, Sbar, acc);
-            acc_mul_nd(cd, c, T1, Sig);
-            ldm<I>(Sb, kUPlanes, 18 * lam, s_pd, Sbar);
-            mul_nd(h, Sbar, T1);
-            acc_mul_nd(cd, T1, c, acc);
+            mul_nd(T1, c, W);
+            m3_axpy_dag(cd, W, Sig);
+            mbar_at<I>(A, d, s, Mb);
+            mul_dn(Mb, W, T1);
+            m3_axpy(cl, T1, acc);
+            mbar_at<I>(A, d, s_pl, Mb);
+            acc_mul_nn(cl, W, Mb, acc);
+            mul_nd(a, c, T1);
+            mbar_at<I>(A, lam, s_pd, Mb);
+            mul_dn(Mb, T1, W);
+            acc_mul_nn(cd, h, W, acc);
         }
         sched_fence();
         {
-            M3 b, f, e, T1, T2, Sbar;
+            M3 b, f, e, V, T1, Mb;
             ldm<I>(A.U, kUPlanes, 18 * lam, s_md, b);
             ldm<I>(A.U, kUPlanes, 18 * d, s_md, f);
             ldm<I>(A.U, kUPlanes, 18 * d, s_mdpl, e);
-            ldm<I>(Sb, kUPlanes, 18 * d, s_md, Sbar);
-            mul_nn(Sbar, b, T1);
-            acc_mul_nn(cl, T1, e, acc);
+            mul_nn(b, e, V);
+            {
+                double ed[8], el[8], w[8];
+                ldv<I>(A.Eb, kEPlanes, 8 * d, s_md, ed);
+                ldv<I>(A.Eb, kEPlanes, 8 * lam, s_md, el);
+                for (int q = 0; q < 8; q++) w[q] = cl * A.gc[d] * ed[q] - cd * A.gc[lam] * el[q];
+                m_from_retr_cot(1.0, w, Mb);
+            }
+            mul_nn(Mb, V, T1);
+            mul_dn(f, T1, V);
+            m3_axpy(1.0, V, acc);
             mul_dn(f, b, T1);
-            ldm<I>(Sb, kUPlanes, 18 * d, s_mdpl, Sbar);
-            acc_mul_nd(cl, T1, Sbar, acc);
-            mul_dd(e, T1, T2);
-            m3_axpy(cd, T2, Sig);
-            ldm<I>(Sb, kUPlanes, 18 * lam, s_md, Sbar);
-            mul_dd(f, Sbar, T1);
-            acc_mul_nn(cd, T1, e, acc);
+            mul_dd(e, T1, V);
+            m3_axpy(cd, V, Sig);
+            mbar_at<I>(A, d, s_mdpl, Mb);
+            mul_nd(T1, Mb, V);
+            acc_mul_nn(cl, V, e, acc);
         }
         sched_fence();
     }
-    double eb[8];
-    ldv<I>(A.Eb, kEPlanes, 8 * lam, s, eb);
     M3 Mb, T;
-    m_from_retr_cot(A.gc[lam], eb, Mb);
+    mbar_at<I>(A, lam, s, Mb);
     mul_nd(Mb, Sig, T);
     m3_axpy(1.0, T, acc);
     stm<I>(A.Ub, kUPlanes, 18 * lam, s, acc);
@@ -568,7 +596,7 @@ GHD bool gather_interior(const Geom& g, int z) { return g.bc || (z >= 1 && z + 2
 // The gather for any layer, each staple term from its sources (z_sources).  OWN (the device form, after the fused pass
 // A): the own-link term of C1 (own_link_term) is added first, as C1 would have added it to Ub.
 template <class I, int LAM, bool OWN>
-GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb, I s, int x, int y, int z) {
+GHD void gather_link_general(const StepAdjArgs& A, I s, int x, int y, int z) {
     const Geom& g = A.g;
     constexpr int lam = LAM;
     {
@@ -582,7 +610,7 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
             const int mu = (lam + k) % 3;  // staples of E_mu in which U_lam is a nu = lam link
             const double c = A.cE[lam];
             // forward R: x = y; Rbar = Sb^dag P Q^dag, P = U_lam(x+mu), Q = U_mu(x+lam)
-            ldm<I>(Sb, kUPlanes, 18 * mu, s, Sbar);
+            sigbar_at<I>(A, mu, s, Sbar);
             ldm<I>(A.U, kUPlanes, 18 * lam, nbr<I>(g, x, y, z, mu, 1, lam, 0), P);
             ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, x, y, z, lam, 1, mu, 0), Q);
             mul_dn(Sbar, P, T1);
@@ -595,7 +623,7 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
                 for (int iz = 0; iz < nzs; iz++) {
                     const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
                     const I sx = (I)sidx(g, xx, yy, zz);
-                    ldm<I>(Sb, kUPlanes, 18 * mu, sx, Sbar);
+                    sigbar_at<I>(A, mu, sx, Sbar);
                     ldm<I>(A.U, kUPlanes, 18 * lam, sx, R);
                     ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, xx, yy, zz, lam, 1, mu, 0), Q);
                     mul_nn(Sbar, R, T1);
@@ -611,7 +639,7 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
                     const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
                     const I sx = (I)sidx(g, xx, yy, zz);
                     const I sm = nbr<I>(g, xx, yy, zz, lam, -1, mu, 0);
-                    ldm<I>(Sb, kUPlanes, 18 * mu, sx, Sbar);
+                    sigbar_at<I>(A, mu, sx, Sbar);
                     ldm<I>(A.U, kUPlanes, 18 * mu, sm, Q);
                     ldm<I>(A.U, kUPlanes, 18 * lam, sm, R);
                     mul_dn(Q, R, T1);
@@ -625,7 +653,7 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
                 for (int iz = 0; iz < nzs; iz++) {
                     const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
                     const I sx = (I)sidx(g, xx, yy, zz);
-                    ldm<I>(Sb, kUPlanes, 18 * mu, sx, Sbar);
+                    sigbar_at<I>(A, mu, sx, Sbar);
                     ldm<I>(A.U, kUPlanes, 18 * mu, nbr<I>(g, xx, yy, zz, lam, -1, mu, 0), Q);
                     ldm<I>(A.U, kUPlanes, 18 * lam, nbr<I>(g, xx, yy, zz, mu, 1, lam, -1), P);
                     mul_nn(Q, P, T1);
@@ -644,7 +672,7 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
                 for (int iz = 0; iz < nzs; iz++) {
                     const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
                     const I sx = (I)sidx(g, xx, yy, zz);
-                    ldm<I>(Sb, kUPlanes, 18 * lam, sx, Sbar);
+                    sigbar_at<I>(A, lam, sx, Sbar);
                     ldm<I>(A.U, kUPlanes, 18 * nu, sx, R);
                     ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, lam, 1, nu, 0), P);
                     mul_dd(R, Sbar, T1);
@@ -658,7 +686,7 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
                 for (int iz = 0; iz < nzs; iz++) {
                     const int xx = xwrap(xs, g.nx), yy = xwrap(ys, g.ny), zz = zs[iz];
                     const I sx = (I)sidx(g, xx, yy, zz);
-                    ldm<I>(Sb, kUPlanes, 18 * lam, sx, Sbar);
+                    sigbar_at<I>(A, lam, sx, Sbar);
                     ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, nu, -1, lam, 0), R);
                     ldm<I>(A.U, kUPlanes, 18 * nu, nbr<I>(g, xx, yy, zz, lam, 1, nu, -1), P);
                     mul_nd(R, Sbar, T1);
@@ -673,18 +701,18 @@ GHD void gather_link_general(const StepAdjArgs& A, const double* __restrict__ Sb
 // OWN: the device form (interior layers by gather_link_interior, the others by gather_link_general with the own-link
 // term); else the C2 pass of the separate form
 template <class I, int LAM, bool OWN = false>
-GHD void step_vjp_gather_link(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
+GHD void step_vjp_gather_link(const StepAdjArgs& A, I s) {
     int x, y, z;
     site_coords<I>(A.g, s, x, y, z);
-    if (OWN && gather_interior(A.g, z)) gather_link_interior<I, LAM>(A, Sb, s, x, y, z);
-    else gather_link_general<I, LAM, OWN>(A, Sb, s, x, y, z);
+    if (OWN && gather_interior(A.g, z)) gather_link_interior<I, LAM>(A, s, x, y, z);
+    else gather_link_general<I, LAM, OWN>(A, s, x, y, z);
 }
 
 template <class I, bool OWN = false>
-GHD void step_vjp_gather(const StepAdjArgs& A, const double* __restrict__ Sb, I s) {
-    step_vjp_gather_link<I, 0, OWN>(A, Sb, s);
-    step_vjp_gather_link<I, 1, OWN>(A, Sb, s);
-    step_vjp_gather_link<I, 2, OWN>(A, Sb, s);
+GHD void step_vjp_gather(const StepAdjArgs& A, I s) {
+    step_vjp_gather_link<I, 0, OWN>(A, s);
+    step_vjp_gather_link<I, 1, OWN>(A, s);
+    step_vjp_gather_link<I, 2, OWN>(A, s);
 }
 
 // Slab energy 1/2 sum_{|z - zc| < half} sum_mu c_mu (E_mu^2 + 4 Re tr(1 - P_mu)), c_mu = (a_mu/a^3)^2,

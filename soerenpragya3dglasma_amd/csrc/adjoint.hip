@@ -54,19 +54,19 @@ __global__ void __launch_bounds__(kAdjBlock) k_vjp_local(StepAdjArgs A, long nsb
     if (s < A.g.S) step_vjp_local<unsigned, true>(A, (unsigned)s);
 }
 // The staple pass gives each thread one link direction (a third of the work per thread; the three directions run
-// in one launch): the own-link term and the twelve staple terms gathered from the Sigbar of the neighbours.
+// in one launch): the own-link term and the twelve staple terms of the neighbours' E updates.
 // Measured at 48 x 48 x 128: 283 us at one wave per SIMD; the interior layers at two waves (252 registers) in one
 // launch and the two open-z boundary layers in a second took 274 + 33 us.
-__global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, const double* Sb, long nsb) {
+__global__ void __launch_bounds__(kAdjBlock) k_vjp_gather(StepAdjArgs A, long nsb) {
     long sb;
     int dir;
     adj_block(nsb, 3, sb, dir);
     const long s = sb * kAdjBlock + threadIdx.x;
     if (s >= A.g.S) return;
     switch (dir) {
-        case 0: step_vjp_gather_link<unsigned, 0, true>(A, Sb, (unsigned)s); break;
-        case 1: step_vjp_gather_link<unsigned, 1, true>(A, Sb, (unsigned)s); break;
-        default: step_vjp_gather_link<unsigned, 2, true>(A, Sb, (unsigned)s); break;
+        case 0: step_vjp_gather_link<unsigned, 0, true>(A, (unsigned)s); break;
+        case 1: step_vjp_gather_link<unsigned, 1, true>(A, (unsigned)s); break;
+        default: step_vjp_gather_link<unsigned, 2, true>(A, (unsigned)s); break;
     }
 }
 
@@ -149,15 +149,12 @@ StepAdjArgs adj_args(const glasma_ctx* ctx) {
     return A;
 }
 
-// one backward step in two launches (A with B and C1's Sigbar, then C2 with C1's own-link term); Sb: 54-plane
-// scratch for the staple cotangents
-int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A0, double* Sb) {
+// one backward step in two launches (A with B folded in, then C2 with C1's own-link term)
+int run_vjp(glasma_ctx* ctx, const StepAdjArgs& A) {
     const long S = ctx->g.S;
     const long nsb = (S + kAdjBlock - 1) / kAdjBlock;
-    StepAdjArgs A = A0;
-    A.Sb = Sb;
     hipLaunchKernelGGL(k_vjp_local, adj_grid(S, 1), dim3(kAdjBlock), 0, ctx->stream, A, nsb);
-    hipLaunchKernelGGL(k_vjp_gather, adj_grid(S, 3), dim3(kAdjBlock), 0, ctx->stream, A, (const double*)Sb, nsb);
+    hipLaunchKernelGGL(k_vjp_gather, adj_grid(S, 3), dim3(kAdjBlock), 0, ctx->stream, A, nsb);
     GL_HIP(ctx, hipGetLastError());
     return 0;
 }
@@ -265,7 +262,6 @@ int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double
     double* cE = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
     double* cJ1 = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
     double* cJ0 = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
-    double* Sb = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
     if (rc) return rc;
     GL_HIP(ctx, hipMemsetAsync(cU1, 0, kUPlanes * Sp * sizeof(double), ctx->stream));
     GL_HIP(ctx, hipMemsetAsync(cE, 0, kEPlanes * Sp * sizeof(double), ctx->stream));
@@ -291,7 +287,7 @@ int glasma_step_vjp(glasma_ctx* ctx, double* Ub, double* Eb, double* Jpb, double
     A.Eb = cE;
     A.Jb1 = cJ1;
     A.Jb = cJ0;
-    if ((rc = run_vjp(ctx, A, Sb))) return rc;
+    if ((rc = run_vjp(ctx, A))) return rc;
     // back to the state at t
     ctx->cur ^= 1;
     if ((rc = load_state(ctx, s0))) return rc;
@@ -372,7 +368,6 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     double* cE = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
     double* rho_bar = rc ? nullptr : b.get(ctx, 8 * S, &rc);
     double* work = rc ? nullptr : b.get(ctx, 54 * Sp + 36 * Sp + 32 * S, &rc);
-    double* Sb = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
     if (rc) return rc;
     std::vector<double> h;
     rho_to_dev_layout(ctx, rho, h);
@@ -424,7 +419,7 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
             A.Eb = cE;
             A.Jb1 = cJ[cb];
             A.Jb = cJ[cb ^ 1];
-            if ((rc = run_vjp(ctx, A, Sb))) return rc;
+            if ((rc = run_vjp(ctx, A))) return rc;
             cb ^= 1;
         }
     }

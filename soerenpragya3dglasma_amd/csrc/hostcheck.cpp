@@ -109,26 +109,24 @@ void hc_step_vjp_gather(const int* n, const double* a, int bc, double dt, const 
     for (long s = 0; s < A.g.S; s++) step_vjp_local<unsigned long>(A, (unsigned long)s);
     for (long s = 0; s < A.g.S; s++) step_vjp_down<unsigned long>(A, (unsigned long)s);
     for (long s = 0; s < A.g.S; s++) step_vjp_sigbar<unsigned long>(A, (unsigned long)s, Sb);
-    for (long s = 0; s < A.g.S; s++) step_vjp_gather<unsigned long>(A, Sb, (unsigned long)s);
+    for (long s = 0; s < A.g.S; s++) step_vjp_gather<unsigned long>(A, (unsigned long)s);
 }
 
-// one backward step in the device's two passes: A with B folded in and the Sigbar store, then the gather with the
-// own-link term; Sb: 54-plane scratch
+// one backward step in the device's two passes: A with B folded in, then the gather with the own-link term
 void hc_step_vjp_fused(const int* n, const double* a, int bc, double dt, const double* U, const double* Uh,
                        const double* E1, const double* J, const double* J1, const double* Ub1, double* Ub, double* Eb,
-                       const double* Jb1, double* Jb, double* Sb) {
+                       const double* Jb1, double* Jb) {
     StepArgs F;
     fill_step_coefficients(F, a, dt);
     StepAdjArgs A;
     A.U = U; A.Uh = Uh; A.E1 = E1; A.J = J; A.J1 = J1; A.Ub1 = Ub1; A.Ub = Ub; A.Eb = Eb; A.Jb1 = Jb1; A.Jb = Jb;
-    A.Sb = Sb;
     A.g.nx = n[0]; A.g.ny = n[1]; A.g.nz = n[2]; A.g.bc = bc;
     A.g.P = (long)n[0] * n[1];
     A.g.S = A.g.P * n[2];
     for (int m = 0; m < 3; m++) { A.gc[m] = F.gc[m]; A.cE[m] = F.cE[m]; A.cU[m] = F.cU[m]; }
     A.dtz = F.dtz;
     for (long s = 0; s < A.g.S; s++) step_vjp_local<unsigned long, true>(A, (unsigned long)s);
-    for (long s = 0; s < A.g.S; s++) step_vjp_gather<unsigned long, true>(A, Sb, (unsigned long)s);
+    for (long s = 0; s < A.g.S; s++) step_vjp_gather<unsigned long, true>(A, (unsigned long)s);
 }
 
 }

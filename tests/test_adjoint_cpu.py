@@ -110,11 +110,10 @@ def test_gather_form_equals_scatter_form(N, bc):
     m = LY.tile(np.ones((54, S))) > 0
     assert np.abs(Ub1 - Ub2)[m].max() < 1e-14 * np.abs(Ub1[m]).max()
     assert np.array_equal(Eb1, Eb2) and np.array_equal(Jb1, Jb2)
-    # the device's two passes (A with B folded in and the Sigbar store, then the gather with the own-link term, whose
-    # interior layers read each link once and sum in another order): E, J and Sigbar cotangents bit for bit, U's to
-    # round-off
-    hc.hc_step_vjp_fused.argtypes = hc.hc_step_vjp_gather.argtypes
-    Ub3, Eb3, Jb3, Sb3 = np.zeros_like(U), wE.copy(), np.zeros_like(J), np.zeros_like(U)
-    hc.hc_step_vjp_fused(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub3), p(Eb3), p(wJ), p(Jb3), p(Sb3))
+    # the device's two passes (A with B folded in, then the gather with the own-link term, whose interior layers read
+    # each link once, form Sigbar from Ebar and sum in another order): E and J cotangents bit for bit, U's to round-off
+    hc.hc_step_vjp_fused.argtypes = hc.hc_step_vjp_gather.argtypes[:-1]
+    Ub3, Eb3, Jb3 = np.zeros_like(U), wE.copy(), np.zeros_like(J)
+    hc.hc_step_vjp_fused(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub3), p(Eb3), p(wJ), p(Jb3))
     assert np.abs(Ub3 - Ub2)[m].max() < 1e-14 * np.abs(Ub2[m]).max()
-    assert np.array_equal(Eb3, Eb2) and np.array_equal(Jb3, Jb2) and np.array_equal(Sb3[m], Sb[m])
+    assert np.array_equal(Eb3, Eb2) and np.array_equal(Jb3, Jb2)
