@@ -3,6 +3,7 @@
 // energy, and the whole map charge density -> left-moving MV sheet -> nsteps leapfrog steps -> slab energy, with
 // the forward run checkpointed every ckpt steps and each segment but the last recomputed once during the backward
 // sweep.
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -179,8 +180,9 @@ int load_state(glasma_ctx* ctx, const State& s) {
 }
 
 // steps of one checkpoint segment from seg[0], each state into the next slot (U, E and J out of place) and U_h
-// into Uh[i]
-int run_segment(glasma_ctx* ctx, const std::vector<State>& seg, const std::vector<double*>& Uh, int len) {
+// into Uh[i], on stream st (the context's stream when null)
+int run_segment(glasma_ctx* ctx, const std::vector<State>& seg, const std::vector<double*>& Uh, int len,
+                hipStream_t st = nullptr) {
     for (int i = 0; i < len; i++) {
         StepArgs F = step_args(ctx, ctx->cur, true);
         F.U = seg[i].U;
@@ -190,9 +192,17 @@ int run_segment(glasma_ctx* ctx, const std::vector<State>& seg, const std::vecto
         F.J = seg[i].J;
         F.Jn = seg[i + 1].J;
         F.UOld = Uh[i];
-        const int rc = launch_step_args(ctx, F);
+        const int rc = launch_step_args(ctx, F, st);
         if (rc) return rc;
     }
+    return 0;
+}
+
+// the gradient's second stream and its events (default priority, kept in the context)
+int ensure_seg_stream(glasma_ctx* ctx) {
+    if (!ctx->stream2) GL_HIP(ctx, hipStreamCreate(&ctx->stream2));
+    for (hipEvent_t& ev : ctx->seg_ev)
+        if (!ev) GL_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     return 0;
 }
 
@@ -212,6 +222,7 @@ struct DevBuf {
         }
         if (next < pool.size()) {
             (void)hipStreamSynchronize(ctx->stream);  // the old array may still be read by queued work
+            if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
             hipFree(pool[next].second);
             pool[next] = {doubles, q};
         } else {
@@ -345,20 +356,40 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     tape.V0 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
     tape.V1 = rc ? nullptr : b.get(ctx, 18 * Sp, &rc);
     // checkpoints at the starts of all segments but the last, whose states the forward run leaves in the segment
-    // slots
-    std::vector<State> ck(nseg > 1 ? nseg - 1 : 0), seg(seglen + 1);
-    for (auto& s : ck) {
+    // slots; the slots in two sets (segment sg uses set sg % 2), so that one segment's recompute, on the second
+    // stream, overlaps the backward sweep of the segment after it, when the second set takes at most a quarter of the
+    // device memory (else one set, each segment recomputed just before its sweep)
+    int nset = 1;
+    if (nseg > 1) {
+        size_t freeb = 0, totalb = 0;
+        const size_t set_bytes = (size_t)seglen * (2 * kUPlanes + kEPlanes + kJPlanes) * Sp * sizeof(double);
+        if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && set_bytes <= totalb / 4) nset = 2;
+        const char* env = std::getenv("GLASMA_GRAD_SETS");  // 1: one set always (read on every call)
+        if (env && std::atoi(env) == 1) nset = 1;
+    }
+    // ck[0] is the initial state in the context's fields, which no step writes; tmp: the forward run's other states
+    std::vector<State> ck(nseg > 1 ? nseg - 1 : 0), seg[2], tmp(nseg > 1 ? 2 : 0);
+    std::vector<double*> Uh[2];
+    for (size_t k = 1; k < ck.size(); k++) {
+        ck[k].U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+        ck[k].E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+        ck[k].J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    }
+    for (auto& s : tmp) {
         s.U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
         s.E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
         s.J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
     }
-    for (size_t i = 1; i < seg.size(); i++) {  // seg[0] is the segment's first state itself
-        seg[i].U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
-        seg[i].E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
-        seg[i].J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+    for (int q = 0; q < nset; q++) {
+        seg[q].resize(seglen + 1);
+        for (size_t i = 1; i < seg[q].size(); i++) {  // seg[q][0] is the segment's first state itself
+            seg[q][i].U = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
+            seg[q][i].E = rc ? nullptr : b.get(ctx, kEPlanes * Sp, &rc);
+            seg[q][i].J = rc ? nullptr : b.get(ctx, kJPlanes * Sp, &rc);
+        }
+        Uh[q].resize(seglen > 0 ? seglen : 0);
+        for (auto& p : Uh[q]) p = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
     }
-    std::vector<double*> Uh(seglen > 0 ? seglen : 0);
-    for (auto& p : Uh) p = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
     double* cU[2];
     double* cJ[2];
     cU[0] = rc ? nullptr : b.get(ctx, kUPlanes * Sp, &rc);
@@ -369,22 +400,34 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
     double* rho_bar = rc ? nullptr : b.get(ctx, 8 * S, &rc);
     double* work = rc ? nullptr : b.get(ctx, 54 * Sp + 36 * Sp + 32 * S, &rc);
     if (rc) return rc;
+    if (nset > 1 && (rc = ensure_seg_stream(ctx))) return rc;
     std::vector<double> h;
     rho_to_dev_layout(ctx, rho, h);
     GL_HIP(ctx, hipMemcpy(rho_d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-    // forward: the segments before the last on the context's fields, from a checkpoint each; the last segment from
-    // the context's fields (left as they are) into the segment slots, so that its backward sweep recomputes nothing
+    // forward, every step out of place: the segments before the last from the initial state in the context's fields
+    // (left as they are), each checkpoint written by the step that reaches it and the other states alternating between
+    // the two tmp states; then the last segment into the segment slots, so that its backward sweep recomputes nothing
     if ((rc = ic_left_from_charge(ctx, mv, rho_d, tape))) return rc;
     const int nlast = nseg > 0 ? (nseg - 1) * ckpt : 0;
-    for (int n = 0; n < nlast; n++) {
-        if (n % ckpt == 0 && (rc = save_state(ctx, ck[n / ckpt]))) return rc;
-        launch_step(ctx, false);
-    }
     State fin = {ctx->U[ctx->cur], ctx->E, ctx->J[ctx->cur]};
+    if (nseg > 1) ck[0] = fin;
+    for (int n = 0; n < nlast; n++) {
+        const State nx = ((n + 1) % ckpt == 0 && n + 1 < nlast) ? ck[(n + 1) / ckpt] : tmp[n & 1];
+        StepArgs F = step_args(ctx, ctx->cur, false);
+        F.U = fin.U;
+        F.Un = nx.U;
+        F.E = fin.E;
+        F.En = nx.E;
+        F.J = fin.J;
+        F.Jn = nx.J;
+        if ((rc = launch_step_args(ctx, F))) return rc;
+        fin = nx;
+    }
     if (nseg > 0) {
-        seg[0] = fin;
-        if ((rc = run_segment(ctx, seg, Uh, nsteps - nlast))) return rc;
-        fin = seg[nsteps - nlast];
+        const int q = (nseg - 1) % nset;
+        seg[q][0] = fin;
+        if ((rc = run_segment(ctx, seg[q], Uh[q], nsteps - nlast))) return rc;
+        fin = seg[q][nsteps - nlast];
     }
     GL_HIP(ctx, hipGetLastError());
     if ((rc = slab_energy(ctx, zc, half, energy, fin.U, fin.E))) return rc;
@@ -398,22 +441,39 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
         A.Eb = cE;
         hipLaunchKernelGGL(k_slab_vjp, agrid(S), dim3(kAdjBlock), 0, ctx->stream, A, cU[0]);
     }
+    // events: seg_ev[q] the recompute into set q done (second stream), seg_ev[2 + q] the sweep over set q done (the
+    // context's stream), seg_ev[4] the forward run done (the checkpoints written)
+    hipEvent_t* ev = ctx->seg_ev;
+    if (nset > 1) {
+        GL_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+        GL_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ev[4], 0));
+    }
     int cb = 0;
-    // backward sweep, one segment at a time
+    // backward sweep, one segment at a time; with two sets the recompute of segment sg - 1 is queued on the second
+    // stream first
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int n0 = sg * ckpt, n1 = (n0 + ckpt < nsteps) ? n0 + ckpt : nsteps;
-        if (sg < nseg - 1) {  // recompute the segment from its checkpoint
-            seg[0] = ck[sg];
-            if ((rc = run_segment(ctx, seg, Uh, n1 - n0))) return rc;
+        const int q = sg % nset;
+        if (nset == 1 && sg < nseg - 1) {
+            seg[0][0] = ck[sg];
+            if ((rc = run_segment(ctx, seg[0], Uh[0], ckpt))) return rc;
         }
+        if (nset > 1 && sg >= 1) {
+            const int q1 = (sg - 1) & 1;  // the set the sweep of segment sg + 1 read
+            if (sg + 1 <= nseg - 1) GL_HIP(ctx, hipStreamWaitEvent(ctx->stream2, ev[2 + q1], 0));
+            seg[q1][0] = ck[sg - 1];
+            if ((rc = run_segment(ctx, seg[q1], Uh[q1], ckpt, ctx->stream2))) return rc;
+            GL_HIP(ctx, hipEventRecord(ev[q1], ctx->stream2));
+        }
+        if (nset > 1 && sg < nseg - 1) GL_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev[q], 0));
         for (int n = n1 - 1; n >= n0; n--) {
             const int i = n - n0;
             StepAdjArgs A = adj_args(ctx);
-            A.U = seg[i].U;
-            A.Uh = Uh[i];
-            A.E1 = seg[i + 1].E;
-            A.J = seg[i].J;
-            A.J1 = seg[i + 1].J;
+            A.U = seg[q][i].U;
+            A.Uh = Uh[q][i];
+            A.E1 = seg[q][i + 1].E;
+            A.J = seg[q][i].J;
+            A.J1 = seg[q][i + 1].J;
             A.Ub1 = cU[cb];
             A.Ub = cU[cb ^ 1];
             A.Eb = cE;
@@ -422,11 +482,10 @@ int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, con
             if ((rc = run_vjp(ctx, A))) return rc;
             cb ^= 1;
         }
+        if (nset > 1 && sg >= 2) GL_HIP(ctx, hipEventRecord(ev[2 + q], ctx->stream));  // set q: next for sg - 2
     }
-    // the initial state in the context's fields again (the linearisation point of the initial-condition map): with
-    // one segment the fields were never advanced; the context's UOld is the initial condition's throughout, as no
-    // step above wrote it
-    if (nseg > 1 && (rc = load_state(ctx, ck[0]))) return rc;
+    // the context's fields and UOld still hold the initial state (the linearisation point of the initial-condition
+    // map): no step above wrote them
     if ((rc = ic_left_vjp(ctx, mv, tape, cU[cb], cE, cJ[cb], rho_bar, work))) return rc;
     GL_HIP(ctx, hipStreamSynchronize(ctx->stream));
     GL_HIP(ctx, hipMemcpy(h.data(), rho_bar, h.size() * sizeof(double), hipMemcpyDeviceToHost));

@@ -675,6 +675,7 @@ int glasma_create(const glasma_params* p, glasma_ctx** out) {
 int glasma_destroy(glasma_ctx* ctx) {
     if (!ctx) return 0;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
     double* bufs[] = {ctx->U[0], ctx->U[1], ctx->UOld, ctx->E, ctx->J[0], ctx->J[1], ctx->red, ctx->G, ctx->Gtop,
                       ctx->UG, ctx->EG, ctx->fftbuf};
     for (double* b : bufs)
@@ -683,7 +684,10 @@ int glasma_destroy(glasma_ctx* ctx) {
         if (w.second) hipFree(w.second);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    for (hipEvent_t ev : ctx->seg_ev)
+        if (ev) hipEventDestroy(ev);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) hipStreamDestroy(ctx->stream2);
     delete ctx;
     return 0;
 }

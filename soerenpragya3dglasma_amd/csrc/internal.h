@@ -66,6 +66,10 @@ struct glasma_ctx {
     // work arrays of the slab-energy and gradient calls (doubles, pointer), handed out in request order and kept for
     // the next call, so that repeated calls of one size allocate nothing; freed by glasma_destroy
     std::vector<std::pair<size_t, double*>> wpool;
+    // second stream and events of the gradient call (created on first use, default priority): a checkpoint segment's
+    // recompute there overlaps the previous segment's backward sweep
+    hipStream_t stream2 = nullptr;
+    hipEvent_t seg_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::string err;
 };
 
@@ -91,7 +95,8 @@ __global__ void k_set_identity(double* __restrict__ U, long S, int nmat);
 // starts at (l % b1) * s1 + (l / b1) * s2 (units of complex entries).  sign -1 forward, +1 backward.
 // leapfrog launches (leapfrog.hip)
 int launch_step(glasma_ctx* ctx, bool write_old);
-int launch_step_args(glasma_ctx* ctx, const StepArgs& A);  // explicit buffers; ctx->cur is left alone
+// explicit buffers; ctx->cur is left alone; on stream st (the context's stream when null)
+int launch_step_args(glasma_ctx* ctx, const StepArgs& A, hipStream_t st = nullptr);
 StepArgs step_args(const glasma_ctx* ctx, int cur, bool write_old);
 // ensemble step over nrow row-form blocks (whole rows or one segment), nseg two-segment row-form blocks and
 // nplain plain-form blocks, in that order
@@ -100,7 +105,7 @@ int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_ev
 TileMap default_tile_map(const Geom& g);
 TileMap tile_map(const Geom& g, int mode, int bw, int bh);  // the requested order, or the next simpler one
 TileMap ctx_tile_map(const glasma_ctx* ctx);                // the context's tuned order, or the default
-int launch_step_map(glasma_ctx* ctx, const StepArgs& A, const TileMap& tm);
+int launch_step_map(glasma_ctx* ctx, const StepArgs& A, const TileMap& tm, hipStream_t st = nullptr);
 
 // the left-moving MV sheet from a given charge density, and its reverse-mode derivative (mv.hip)
 struct IcTape {

@@ -414,16 +414,19 @@ TileMap default_tile_map(const Geom& g) {
 TileMap ctx_tile_map(const glasma_ctx* ctx) { return ctx->tm_tuned ? ctx->tm : default_tile_map(ctx->g); }
 
 // one step with explicit buffers and tile order (the context supplies the stream)
-int launch_step_map(glasma_ctx* ctx, const StepArgs& A, const TileMap& tm) {
+int launch_step_map(glasma_ctx* ctx, const StepArgs& A, const TileMap& tm, hipStream_t st) {
     static const int waves = env_int("GLASMA_STEP_WAVES", 0);  // 0: the measured best of each form
     static const int four = env_int("GLASMA_STEP_FOUR", 1);
-    if (A.g.S < (1L << 32)) launch<unsigned int>(waves, four, tm, A, ctx->stream);
-    else launch<unsigned long>(waves, four, tm, A, ctx->stream);
+    if (!st) st = ctx->stream;
+    if (A.g.S < (1L << 32)) launch<unsigned int>(waves, four, tm, A, st);
+    else launch<unsigned long>(waves, four, tm, A, st);
     return 0;
 }
 
 // one step with explicit buffers (the context supplies geometry, stream and tile order)
-int launch_step_args(glasma_ctx* ctx, const StepArgs& A) { return launch_step_map(ctx, A, ctx_tile_map(ctx)); }
+int launch_step_args(glasma_ctx* ctx, const StepArgs& A, hipStream_t st) {
+    return launch_step_map(ctx, A, ctx_tile_map(ctx), st);
+}
 
 int launch_step(glasma_ctx* ctx, bool write_old) {
     const int rc = launch_step_args(ctx, step_args(ctx, ctx->cur, write_old));

@@ -193,3 +193,20 @@ def test_gradient_work_arrays_reused_across_calls():
     for (e, gr), ref in [((e3, g3), fresh(12, 3)), ((e4, g4), fresh(4, 4))]:
         print(e, ref[0], np.abs(gr - ref[1]).max() / np.abs(ref[1]).max())
         assert e == ref[0] and np.abs(gr - ref[1]).max() <= 1e-12 * np.abs(ref[1]).max()
+
+
+@pytest.mark.parametrize("nsteps,ckpt", [(12, 3), (7, 3), (2, 1)])
+def test_gradient_one_or_two_segment_sets(monkeypatch, nsteps, ckpt):
+    """The gradient with the segment slots in two sets (a segment's recompute on a second stream, overlapping the
+    sweep of the segment after it) equals the gradient with one set (each segment recomputed just before its sweep):
+    energies bit for bit, gradients to 1e-12 of their largest entry (the final state's atomic scatter)."""
+    from soerenpragya3dglasma_amd import Glasma
+    N = (8, 8, 16)
+    rho, mv = _left_mover_case(N)
+    out = {}
+    for sets in ("1", "2"):
+        monkeypatch.setenv("GLASMA_GRAD_SETS", sets)
+        with Glasma(N, (1.0, 1.0, 1.0), 0.25) as g:
+            out[sets] = g.slab_energy_gradient(mv, rho, nsteps, ckpt, 9, 4)
+    (e1, g1), (e2, g2) = out["1"], out["2"]
+    assert e1 == e2 and np.abs(g1 - g2).max() <= 1e-12 * np.abs(g1).max()
