@@ -315,7 +315,8 @@ int glasma_slab_energy_from_charge(glasma_ctx* ctx, const glasma_mv_params* mv, 
  * reverse-mode differentiation of the steps (forward states checkpointed every ckpt steps, each segment but the
  * last recomputed once) and of the initial conditions.  The context holds the initial state afterwards.  Both calls keep
  * their device work arrays in the context (counted by glasma_memory_bytes, freed by glasma_destroy), so that
- * repeated calls of one size allocate nothing. */
+ * repeated calls of one size allocate nothing.  When memory allows, a segment's recompute runs on a second stream
+ * of the context (created on first use, default priority) beside the backward sweep; the call returns after both. */
 int glasma_slab_energy_gradient(glasma_ctx* ctx, const glasma_mv_params* mv, const double* rho, int nsteps, int ckpt,
                                 int zc, int half, double* energy, double* grad_rho);
 
