@@ -236,10 +236,13 @@ GHD void step_vjp_local(const StepAdjArgs& A, I s) {
         double e1[8], eb[8];
         ldm<I>(A.Ub1, kUPlanes, 18 * mu, s, Ub1);
         ldm<I>(A.Uh, kUPlanes, 18 * mu, s, Uh);
-        ldm<I>(A.U, kUPlanes, 18 * mu, s, U0);
+        if (!FUSED) ldm<I>(A.U, kUPlanes, 18 * mu, s, U0);
         ldv<I>(A.E1, kEPlanes, 8 * mu, s, e1);
         ldv<I>(A.Eb, kEPlanes, 8 * mu, s, eb);
         expi(A.cU[mu], e1, X);
+        // FUSED: U = X^dag U_h (X unitary to round-off) instead of reading the time-t links: this pass is bound by
+        // its HBM reads
+        if (FUSED) mul_dn(X, Uh, U0);
         mul_nd(Ub1, Uh, Xb);
         mul_dn(X, Ub1, Uhbar);
         if (mu == 2) m3_axpy(1.0, Uhbz, Uhbar);

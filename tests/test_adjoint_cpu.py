@@ -110,10 +110,11 @@ def test_gather_form_equals_scatter_form(N, bc):
     m = LY.tile(np.ones((54, S))) > 0
     assert np.abs(Ub1 - Ub2)[m].max() < 1e-14 * np.abs(Ub1[m]).max()
     assert np.array_equal(Eb1, Eb2) and np.array_equal(Jb1, Jb2)
-    # the device's two passes (A with B folded in, then the gather with the own-link term, whose interior layers read
-    # each link once, form Sigbar from Ebar and sum in another order): E and J cotangents bit for bit, U's to round-off
+    # the device's two passes (A with B folded in and the time-t links formed as X^dag U_h, then the gather with the
+    # own-link term, whose interior layers read each link once, form Sigbar from Ebar and sum in another order): all
+    # cotangents to round-off
     hc.hc_step_vjp_fused.argtypes = hc.hc_step_vjp_gather.argtypes[:-1]
     Ub3, Eb3, Jb3 = np.zeros_like(U), wE.copy(), np.zeros_like(J)
     hc.hc_step_vjp_fused(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub3), p(Eb3), p(wJ), p(Jb3))
-    assert np.abs(Ub3 - Ub2)[m].max() < 1e-14 * np.abs(Ub2[m]).max()
-    assert np.array_equal(Eb3, Eb2) and np.array_equal(Jb3, Jb2)
+    for a, b_, mask in ((Ub3, Ub2, m), (Eb3, Eb2, slice(None)), (Jb3, Jb2, slice(None))):
+        assert np.abs(a - b_)[mask].max() < 1e-14 * np.abs(b_[mask]).max()
