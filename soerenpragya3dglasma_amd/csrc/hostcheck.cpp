@@ -22,6 +22,12 @@ void hc_su3_ilog(double c, const double* U, double* out) { M3 m; to_m3(U, m); il
 void hc_su3_adj_mul(const double* V, const double* al, double* out) { M3 m; to_m3(V, m); adj_mul(m, al, out); }
 void hc_su3_inv_adj_mul(const double* V, const double* al, double* out) { M3 m; to_m3(V, m); inv_adj_mul(m, al, out); }
 void hc_su3_retrigenu(const double* V, double* out) { M3 m; to_m3(V, m); retr_igen(m, out); }
+// z = sum_k ad_A^k(y) / (k+1)! on su(3) coordinates, A = i c e^a t^a (adjoint.h ad_series)
+void hc_ad_series(double c, const double* e, const double* y, double* z) {
+    double yy[8];
+    std::memcpy(yy, y, sizeof yy);
+    ad_series(c, e, yy, z);
+}
 
 // One fused leapfrog step over tiled device-layout arrays, sequentially over sites.
 void hc_step(const int* n, const double* a, int bc, double dt, const double* U, double* Un, double* E,

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from soerenpragya3dglasma_amd import layout as LY
-from tests.helpers import hostcheck_lib, random_fields
+from tests.helpers import T, herm, hostcheck_lib, random_fields
 
 dp = ctypes.POINTER(ctypes.c_double)
 
@@ -118,3 +118,28 @@ def test_gather_form_equals_scatter_form(N, bc):
     hc.hc_step_vjp_fused(n, av, bc, dt, p(U), p(Uh), p(E1), p(J), p(Jn), p(wU), p(Ub3), p(Eb3), p(wJ), p(Jb3))
     for a, b_, mask in ((Ub3, Ub2, m), (Eb3, Eb2, slice(None)), (Jb3, Jb2, slice(None))):
         assert np.abs(a - b_)[mask].max() < 1e-14 * np.abs(b_[mask]).max()
+
+
+def test_ad_series_against_matrix_series():
+    """ad_series (the su(3) structure-constant form behind the exponential's adjoint and the logarithm's 8 x 8
+    Jacobian) against the commutator series sum_k ad_A^k(Y) / (k+1)! of 3 x 3 complex matrices, A = i c e^a t^a,
+    Y = i y^b t^b, read back as z^a = -2 Re tr(i t^a Z): 1e-14 of the largest coordinate for |c e| up to ~1 (the
+    step's exponents are ~0.1), 1e-11 for exponents of a few units, where both series lose digits to cancellation
+    (their terms grow to ~10-50 before they fall).  A wrong structure constant, sign or generator order changes the
+    first-order term, c e x y, and fails it at every scale."""
+    hc = hostcheck_lib()
+    hc.hc_ad_series.argtypes = [ctypes.c_double, dp, dp, dp]
+    rng = np.random.default_rng(7)
+    for scale, tol in ((0.01, 1e-14), (0.3, 1e-14), (1.5, 1e-11)):
+        for _ in range(5):
+            c = float(rng.normal())
+            e, y, z = rng.normal(size=8) * scale, rng.normal(size=8), np.zeros(8)
+            hc.hc_ad_series(c, p(e), p(y), p(z))
+            A, Y = 1j * c * herm(e), 1j * herm(y)
+            Z, term, fact = Y.copy(), Y.copy(), 1.0
+            for k in range(1, 80):
+                term = A @ term - term @ A
+                fact /= k + 1
+                Z = Z + fact * term
+            zr = np.array([-2.0 * np.real(np.trace(1j * T[a] @ Z)) for a in range(8)])
+            assert np.abs(z - zr).max() <= tol * max(1.0, np.abs(zr).max()), (scale, np.abs(z - zr).max())
