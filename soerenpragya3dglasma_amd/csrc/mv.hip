@@ -933,13 +933,30 @@ namespace {
 
 using namespace glasma;
 
+namespace glasma {
+namespace {
+// The part of reset_fields a sheet initial state needs.  The sheets write the current links, UOld and E at every
+// site (the collision's two sheets cover [0, (Nz-2)/2] and [Nz/2, Nz-1], a single nucleus all of z), and whole planes
+// of the current J (k_sheet_jstat, k_store_j); the planes of the nucleus a single-nucleus driver leaves out must be
+// zero.  The other buffers of the ping-pong pairs are written by the first step before anything reads them.
+int reset_for_sheets(glasma_ctx* ctx, int driver) {
+    ctx->cur = 0;
+    ctx->t = 0.0;
+    ctx->tsteps = 0;
+    if (driver != 0)
+        GL_HIP(ctx, hipMemsetAsync(ctx->J[0], 0, (size_t)kJPlanes * pad_sites(ctx->g.S) * sizeof(double), ctx->stream));
+    return 0;
+}
+}  // namespace
+}  // namespace glasma
+
 extern "C" int glasma_mv_initial_state(glasma_ctx* ctx, int driver, const glasma_mv_params* mv, const double* xi_l,
                                        const double* xi_r) {
     if (!ctx) return GLASMA_ERR_ARG;
     if (!mv_valid(mv)) return set_error(ctx, GLASMA_ERR_ARG, "invalid MV parameters");
     if (driver < 0 || driver > 2) return set_error(ctx, GLASMA_ERR_ARG, "driver must be 0, 1 or 2");
     if ((driver != 2 && !xi_l) || (driver != 1 && !xi_r)) return set_error(ctx, GLASMA_ERR_ARG, "missing charges");
-    int rc = reset_fields(ctx);
+    int rc = reset_for_sheets(ctx, driver);
     if (rc) return rc;
     const int nz = ctx->g.nz;
     Wk w;
@@ -978,7 +995,7 @@ extern "C" int glasma_tmd_initial_state(glasma_ctx* ctx, int driver, const glasm
     if (driver < 0 || driver > 2) return set_error(ctx, GLASMA_ERR_ARG, "driver must be 0, 1 or 2");
     if ((driver != 2 && (!noise_l || !nucleons_l)) || (driver != 1 && (!noise_r || !nucleons_r)))
         return set_error(ctx, GLASMA_ERR_ARG, "missing noise or nucleon positions");
-    int rc = reset_fields(ctx);
+    int rc = reset_for_sheets(ctx, driver);
     if (rc) return rc;
     const int nz = ctx->g.nz;
     Wk w;
