@@ -32,6 +32,7 @@ struct Wk {
     double* rho = nullptr;
     double* V = nullptr;
     double* xi = nullptr;
+    double* prof = nullptr;  // z profiles of a separable charge (mv_sheet): 4 nz doubles
 };
 
 __device__ __forceinline__ long fidx(const Geom& g, int a, long s) {
@@ -206,6 +207,80 @@ __global__ void k_sheet_filter(double* __restrict__ F, Geom g, double ax, double
     }
 }
 
+// one transverse slice of F (complex, component-major: entry (a, s2) at a P + s2) = (xi, 0)
+__global__ void k_load_xi(double* __restrict__ F, const double* __restrict__ xi, long P) {
+    const long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s2 >= P) return;
+    for (int a = 0; a < 8; a++) {
+        F[2 * (a * P + s2)] = xi[a + 8 * s2];
+        F[2 * (a * P + s2) + 1] = 0.0;
+    }
+}
+
+// z profiles of the separable MV charge rho(x, y, z) = c(z) xi(x, y): prof[z] = c(z), k_charge's factor; then
+// prof[2 nz + 2 pz (+1)] = sum_z c(z) e^{-2 pi i pz z / nz}, and prof[nz + z] = the real part of the inverse transform
+// of that times k_evolve_phase's factor (the profile evolved by dt).  Direct sums (nz^2 terms), one thread per output.
+__global__ void k_profile_c(double* __restrict__ prof, Geom g, double mult_norm, int center, double width) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= g.nz) return;
+    const double u = (z - center) / width;
+    prof[z] = mult_norm * (exp(-0.5 * u * u) / sqrt(2.0 * kPi * width * width));
+}
+__global__ void k_profile_dft(double* __restrict__ prof, Geom g) {
+    const int pz = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pz >= g.nz) return;
+    double re = 0.0, im = 0.0;
+    for (int z = 0; z < g.nz; z++) {
+        double sn, cs;
+        sincos(-2.0 * kPi * (double)(((long)pz * z) % g.nz) / (double)g.nz, &sn, &cs);
+        re = fma(prof[z], cs, re);
+        im = fma(prof[z], sn, im);
+    }
+    prof[2 * g.nz + 2 * pz] = re;
+    prof[2 * g.nz + 2 * pz + 1] = im;
+}
+__global__ void k_profile_evolve(double* __restrict__ prof, Geom g, double az, double dt, int right) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= g.nz) return;
+    double acc = 0.0;
+    for (int pz = 0; pz < g.nz; pz++) {
+        const double om = sqrt((2.0 - 2.0 * cos(2.0 * kPi * (double)pz / (double)g.nz)) / (az * az));
+        double sgn;
+        if (!right) sgn = (pz < g.nz / 2) ? -1.0 : 1.0;
+        else sgn = (pz < g.nz / 2) ? 1.0 : -1.0;
+        double sp, cp, sn, cs;
+        sincos(sgn * om * dt, &sp, &cp);
+        sincos(2.0 * kPi * (double)(((long)pz * z) % g.nz) / (double)g.nz, &sn, &cs);
+        const double re = prof[2 * g.nz + 2 * pz], im = prof[2 * g.nz + 2 * pz + 1];
+        const double er = cp * re - sp * im, ei = sp * re + cp * im;  // evolved coefficient
+        acc += er * cs - ei * sn;                                       // real part of it times e^{+i ...}
+    }
+    prof[g.nz + z] = acc / g.nz;
+}
+
+// The path-ordered Wilson lines of k_wilson_scan for a potential separable in z, A(x, y, z) = cz(z) a(x, y), with a
+// in the real parts of one transverse slice of F (entry (a, s2) at a P + s2)
+__global__ void __launch_bounds__(256) k_wilson_scan_sep(const double* __restrict__ F, const double* __restrict__ cz,
+                                                         double* __restrict__ V, Geom g, int right) {
+    const long s2 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s2 >= g.P) return;
+    double a0[8];
+    for (int a = 0; a < 8; a++) a0[a] = F[2 * (a * g.P + s2)];
+    M3 W, X, T;
+    set_unit(W);
+    const double c = (right ? 1.0 : -1.0) / kSqrt2;
+    for (int k = 0; k < g.nz; k++) {
+        const int z = right ? k : g.nz - 1 - k;
+        const double f = cz[z];
+        double col[8];
+        for (int a = 0; a < 8; a++) col[a] = f * a0[a];
+        expi(c, col, X);
+        mul_nn(X, W, T);
+        W = T;
+        stm<long>(V, kGPlanes, 0, s2 + g.P * z, W);
+    }
+}
+
 // E_perp(t + dt/2) = MatrixILog(coff_mu, U_mu UOld_mu^dag), E_z = 0 for z in [zlo, zhi]  (:66-92)
 __global__ void __launch_bounds__(256) k_sheet_efield(const double* __restrict__ U, const double* __restrict__ UO, double* __restrict__ E,
                                Geom g, int zlo, int zhi, double coff0, double coff1) {
@@ -340,12 +415,14 @@ int alloc_work(glasma_ctx* ctx, Wk& w) {
     GL_HIP(ctx, hipMalloc(&w.rho, 8 * (size_t)S * sizeof(double)));
     GL_HIP(ctx, hipMalloc(&w.V, (size_t)kGPlanes * Sp * sizeof(double)));
     GL_HIP(ctx, hipMalloc(&w.xi, 8 * (size_t)ctx->g.P * sizeof(double)));
+    GL_HIP(ctx, hipMalloc(&w.prof, 4 * (size_t)ctx->g.nz * sizeof(double)));
     return 0;
 }
 void free_work(Wk& w) {
     if (w.rho) hipFree(w.rho);
     if (w.V) hipFree(w.V);
     if (w.xi) hipFree(w.xi);
+    if (w.prof) hipFree(w.prof);
     w = Wk();
 }
 
@@ -433,14 +510,15 @@ int single_sheet(glasma_ctx* ctx, Wk& w, bool right, int zlo, int zhi, int wstat
     return 0;
 }
 
-// CollidingObjects::SetWilsonLines and InitialConditions::SingleSheet for one MV nucleus in fewer transforms.  The
-// reference smooths the charge (one transverse round trip), solves Poisson (another), and for U(t + dt) evolves the
-// smoothed charge along z (a z round trip) and solves Poisson again, taking real parts in between (the filters are
-// real and even in p, so those imaginary parts are round-off).  The filters are linear and act on separate axes,
-// so here U(t + dt)'s potential is one transverse transform of the charge, one z transform, the product of the three
-// factors (k_sheet_filter) and the two inverse transforms, and UOld's is one transverse round trip with smoothing x
-// Poisson: 12 transform passes instead of 14 plus 7 separate filter and copy passes.  m_eff = 0 keeps the separate
-// steps (their smoothing factor is 0/0 at p = 0, which the reference's order spreads over the slice).
+// CollidingObjects::SetWilsonLines and InitialConditions::SingleSheet for one MV nucleus, using that its charge is
+// separable, rho(x, y, z) = c(z) xi(x, y) (SetChargeDensities, k_charge).  The reference smooths the charge (a
+// transverse round trip), solves Poisson (another), and for U(t + dt) evolves the smoothed charge along z (a z round
+// trip) and solves Poisson again, taking real parts in between.  Those filters are linear and each acts on one set of
+// axes, so both potentials factor: A0 = c(z) a(x, y) and A1 = c_ev(z) a(x, y), with a = the smoothing x Poisson of xi
+// on one transverse slice (k_sheet_filter) and c_ev = the real part of c evolved by dt (k_profile_*; the filters are
+// real and even in p, so the real parts taken in between are exact up to round-off).  The Wilson lines are scanned
+// from these factors (k_wilson_scan_sep): no transform of the whole lattice.  m_eff = 0 keeps the separate steps
+// (their smoothing factor is 0/0 at p = 0, which the reference's order spreads over the slice).
 int mv_sheet(glasma_ctx* ctx, Wk& w, const glasma_mv_params* mv, const double* xi_host, bool right, int zlo, int zhi,
              int wstat, int wj) {
     const Geom& g = ctx->g;
@@ -454,23 +532,24 @@ int mv_sheet(glasma_ctx* ctx, Wk& w, const glasma_mv_params* mv, const double* x
     const long nsl = (long)(zhi - zlo + 1) * g.P;
     double* U = ctx->U[ctx->cur];
     double* J = ctx->J[ctx->cur];
-    // U(t + dt): the potential of the evolved smoothed charge (SetWilsonLinesForFirstStep, :33-60)
-    hipLaunchKernelGGL(k_charge, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.xi, g, mult * norm, center, sigma);
-    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, -1));
-    GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, -1));
-    hipLaunchKernelGGL(k_sheet_filter, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, a[0], a[1], a[2], mv->lambda,
-                       mv->m_eff, dt, right ? 1 : 0, 1);
-    GL_FFT(ctx, fft_z(ctx->stream, w.F, g, 8, +1));
-    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, +1));
-    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.V, g, right ? 1 : 0);
-    hipLaunchKernelGGL(k_sheet_links, grid1(nsl), dim3(256), 0, ctx->stream, w.V, U, g, zlo, zhi);
-    // UOld: the links of the sheet at t = 0 (:14-27)
-    hipLaunchKernelGGL(k_charge, grid1(g.S), dim3(256), 0, ctx->stream, w.F, w.xi, g, mult * norm, center, sigma);
-    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, -1));
-    hipLaunchKernelGGL(k_sheet_filter, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, a[0], a[1], a[2], mv->lambda,
+    // the transverse factor a(x, y) in the first slice of F
+    Geom g1 = g;
+    g1.nz = 1;
+    g1.S = g.P;
+    hipLaunchKernelGGL(k_load_xi, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.xi, g.P);
+    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, 1, -1));
+    hipLaunchKernelGGL(k_sheet_filter, grid1(g.P), dim3(256), 0, ctx->stream, w.F, g1, a[0], a[1], a[2], mv->lambda,
                        mv->m_eff, dt, right ? 1 : 0, 0);
-    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, g.nz, +1));
-    hipLaunchKernelGGL(k_wilson_scan, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.V, g, right ? 1 : 0);
+    GL_FFT(ctx, fft_xy(ctx->stream, w.F, g, 8, 1, +1));
+    // the z profiles c(z) and c_ev(z)
+    hipLaunchKernelGGL(k_profile_c, grid1(g.nz), dim3(256), 0, ctx->stream, w.prof, g, mult * norm, center, sigma);
+    hipLaunchKernelGGL(k_profile_dft, grid1(g.nz), dim3(256), 0, ctx->stream, w.prof, g);
+    hipLaunchKernelGGL(k_profile_evolve, grid1(g.nz), dim3(256), 0, ctx->stream, w.prof, g, a[2], dt, right ? 1 : 0);
+    // U(t + dt) from A1 (SetWilsonLinesForFirstStep, :33-60), UOld from A0 (:14-27)
+    hipLaunchKernelGGL(k_wilson_scan_sep, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.prof + g.nz, w.V, g,
+                       right ? 1 : 0);
+    hipLaunchKernelGGL(k_sheet_links, grid1(nsl), dim3(256), 0, ctx->stream, w.V, U, g, zlo, zhi);
+    hipLaunchKernelGGL(k_wilson_scan_sep, grid1(g.P), dim3(256), 0, ctx->stream, w.F, w.prof, w.V, g, right ? 1 : 0);
     hipLaunchKernelGGL(k_sheet_links, grid1(nsl), dim3(256), 0, ctx->stream, w.V, ctx->UOld, g, zlo, zhi);
     // E(t + dt/2), J-stat and J as single_sheet (:66-209)
     const double acube = a[0] * a[1] * a[2];
