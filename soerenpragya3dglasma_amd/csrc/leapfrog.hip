@@ -106,16 +106,20 @@ __device__ __forceinline__ void ld_row(const double (*r)[RL], int slot, M3& M) {
 // the tile's own rows -- U_nu(x), U_nu(x+x) for MU = x, U_mu(x+-x) and U_x(x-x) for NU = x -- read from
 // the LDS copy instead of global memory (slots: own 1 + lane, x+1 at sp, x-1 at sm).  Same products in the
 // same order as link_update.
-template <class I, int MU, int NU, int RL>
+// PRE: the forward staple's operands from global memory, U_nu(x+mu) (MU != 0) and U_mu(x+nu) (NU != 0), come in
+// pre[0] and pre[1] (read before the row copy's barrier, first_staple_operands).
+template <class I, int MU, int NU, int RL, bool PRE = false>
 __device__ __forceinline__ void staples_row(const StepArgs& A, I s, int x, int y, int z, int lane, int sp, int sm,
-                                            const double (*row)[18][RL], M3& Sig) {
+                                            const double (*row)[18][RL], M3& Sig, const M3* pre = nullptr) {
     const Geom& g = A.g;
     const double cn = A.cE[NU];
     M3 P1, P2, T;
     // forward staple U_nu(x+mu) U_mu^dag(x+nu) U_nu^dag(x)
     if (MU == 0) ld_row(row[NU], sp, P1);
+    else if (PRE) P1 = pre[0];
     else ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(g, x, y, z, MU, 1, NU, 0), P1);
     if (NU == 0) ld_row(row[MU], sp, P2);
+    else if (PRE) P2 = pre[1];
     else ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(g, x, y, z, NU, 1, MU, 0), P2);
     mul_nd(P1, P2, T);
     ld_row(row[NU], lane + 1, P1);
@@ -150,17 +154,27 @@ __device__ __forceinline__ bool wait_ready(SH& sh) {
     return ok;
 }
 
-// link_update with the row operands from LDS
-template <class I, int MU, int RL>
+// the global operands of link MU's first forward staple (NU = MU + 1), read before the row copy's barrier so that
+// their latency overlaps the barrier's wait: pre[0] = U_nu(x+mu) when MU != 0, pre[1] = U_mu(x+nu) when NU != 0
+template <class I, int MU>
+__device__ __forceinline__ void first_staple_operands(const StepArgs& A, int x, int y, int z, M3* pre) {
+    constexpr int NU = (MU + 1) % 3;
+    if (MU != 0) ldm<I>(A.U, kUPlanes, 18 * NU, nbr<I>(A.g, x, y, z, MU, 1, NU, 0), pre[0]);
+    if (NU != 0) ldm<I>(A.U, kUPlanes, 18 * MU, nbr<I>(A.g, x, y, z, NU, 1, MU, 0), pre[1]);
+}
+
+// link_update with the row operands from LDS (PRE: the first staple's global operands in pre, first_staple_operands)
+template <class I, int MU, int RL, bool PRE = false>
 __device__ __forceinline__ void link_update_row(const StepArgs& A, I s, int x, int y, int z, int lane, int sp,
-                                                int sm, const double (*row)[18][RL], M3& Uh) {
+                                                int sm, const double (*row)[18][RL], M3& Uh,
+                                                const M3* pre = nullptr) {
     M3 Sig;
 #pragma unroll
     for (int k = 0; k < 9; k++) { Sig.re[k] = 0.0; Sig.im[k] = 0.0; }
     // scheduling barriers keep the compiler from hoisting the next staple pair's loads into this one:
     // that fits the form into 168 registers without spills (three waves per SIMD, 31.3 -> 30.1 ms at
     // 256 x 256 x 1024), the waves hiding the latency instead
-    staples_row<I, MU, (MU + 1) % 3, RL>(A, s, x, y, z, lane, sp, sm, row, Sig);
+    staples_row<I, MU, (MU + 1) % 3, RL, PRE>(A, s, x, y, z, lane, sp, sm, row, Sig, pre);
     __builtin_amdgcn_sched_barrier(0);
     staples_row<I, MU, (MU + 2) % 3, RL>(A, s, x, y, z, lane, sp, sm, row, Sig);
     __builtin_amdgcn_sched_barrier(0);
@@ -306,6 +320,139 @@ __device__ __forceinline__ void step_tile4(const StepArgs& A, long first, StepSh
     }
 }
 
+// One link wavefront of step_tile4_pre (link direction MU): the row copy of U_MU (ROW) with the first staple's global
+// operands read in the same burst, the block barrier, the link update and, for MU = 2, the J advection after the J
+// wavefront's hand-over.  Each wavefront's branch holds its own barrier (the branches are uniform: every wavefront
+// runs exactly one), so that the operands read before it stay live in that branch only.
+template <class I, int MU, bool ROW, bool SEG, int RL>
+__device__ __forceinline__ void link_wave(const StepArgs& A, long first, I s, bool valid, int x, int y, int z, int lane,
+                                          int sp, int sm, int nhalo, int x0, StepShared<RL>& sh) {
+    M3 pre[2];
+    if (ROW) {
+        // lanes past the end of the lattice (last tile only, whole-row tiles only) copy site 0; no valid lane
+        // reads their slots
+        M3 M;
+        ldm<I>(A.U, kUPlanes, 18 * MU, s, M);
+        first_staple_operands<I, MU>(A, x, y, z, pre);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            sh.row[MU][2 * k][lane + 1] = M.re[k];
+            sh.row[MU][2 * k + 1][lane + 1] = M.im[k];
+        }
+        if (lane < nhalo) {
+            long hs;
+            if (SEG) {
+                hs = halo_site(A.g, first, x0, lane);
+            } else {
+                const int xt = x - lane;  // the tile's first x
+                hs = (long)s - x + (lane == 0 ? xwrap(xt - 1, A.g.nx) : xwrap(xt + kTile, A.g.nx));
+            }
+            ldm<I>(A.U, kUPlanes, 18 * MU, (I)hs, M);
+            const int slot = lane == 0 ? 0 : kTile + lane;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                sh.row[MU][2 * k][slot] = M.re[k];
+                sh.row[MU][2 * k + 1][slot] = M.im[k];
+            }
+        }
+    }
+    if (MU == 0 && lane == 0) sh.ready = 0;
+    __syncthreads();
+    M3 Uh;
+    if (ROW) {
+        if (valid) link_update_row<I, MU, RL, true>(A, s, x, y, z, lane, sp, sm, sh.row, Uh, pre);
+    } else if (valid) {
+        link_update<I, MU>(A, s, x, y, z, Uh);
+    }
+    if (MU == 2) {
+        const bool handed_over = wait_ready(sh);
+        if (valid) {
+            double jps[8], jms[8], jpsU[8], jmsU[8], jp[8], jm[8];
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                jps[a] = sh.j[0][a][lane];
+                jms[a] = sh.j[1][a][lane];
+                jpsU[a] = sh.j[2][a][lane];
+                jmsU[a] = sh.j[3][a][lane];
+                jp[a] = sh.j[4][a][lane];
+                jm[a] = sh.j[5][a][lane];
+            }
+            if (!handed_over) jp[0] = jm[0] = __builtin_nan("");
+            j_advect_with<I>(A, s, Uh, jps, jms, jpsU, jmsU, jp, jm);
+        }
+    }
+}
+
+// step_tile4 with each link wavefront's branch holding its own barrier and the first staple's global operands read
+// before it (link_wave).  Measured for the two-segment row form at 96 x 96 x 384: 1.69-1.72 against 1.71-1.75 ms; for
+// whole rows at 256 x 256 x 1024 it was slower (29.96 against 29.62 ms: 4 spilled registers), so k_step4 keeps
+// step_tile4 and the ensemble takes this form for its two-segment blocks.
+template <class I, bool ROW, bool SEG, int RL = SEG ? kRowSeg : kRow>
+__device__ __forceinline__ void step_tile4_pre(const StepArgs& A, long first, StepShared<RL>& sh) {
+    // the wave index through readfirstlane: uniform to the compiler, so the per-wave branches are scalar branches
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const long sl = first + lane;
+    const bool valid = sl < A.g.S;
+    const I s = (I)(valid ? sl : 0);
+    int x, y, z;
+    site_coords<I>(A.g, s, x, y, z);
+    if (wave == 3) {
+        __syncthreads();
+        if (valid) {
+            double jps[8], jms[8], jp[8], jm[8];
+            const int zm = zdn(A.g, z);
+            jstat_new_keep<I>(A, s, (I)(sl + A.g.P * (long)(zm - z)), jps, jms, jp, jm);
+            stv<I>(A.Jn, kJPlanes, JPS * 8, s, jps);
+            stv<I>(A.Jn, kJPlanes, JMS * 8, s, jms);
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                sh.j[0][a][lane] = jps[a];
+                sh.j[1][a][lane] = jms[a];
+                sh.j[4][a][lane] = jp[a];
+                sh.j[5][a][lane] = jm[a];
+            }
+            if (ROW) {
+                // U_z(x) of J+-stat'(x+z) from the row copy; on the open top layer J+-stat'(x+z) is
+                // J+-stat'(x) itself (jstat_new_up), already in jps, jms
+                const int zp = zup(A.g, z);
+                if (zp != z) {
+                    M3 Uz;
+                    ld_row(sh.row[2], lane + 1, Uz);
+                    jstat_core<I>(A, (I)(sl + A.g.P * (long)(zp - z)), s, Uz, jps, jms);
+                }
+            } else {
+                jstat_new_up<I>(A, s, z, jps, jms);
+            }
+#pragma unroll
+            for (int a = 0; a < 8; a++) {
+                sh.j[2][a][lane] = jps[a];
+                sh.j[3][a][lane] = jms[a];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&sh.ready, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+    }
+    // LDS slots of x+1 and x-1: the next / previous lane, or halo slots; in a tile of several short rows, the
+    // lane of the periodic neighbour in the same row; with two row segments (SEG), row_slots
+    int sp, sm, nhalo, x0 = 0;
+    const bool one_row = A.g.nx % kTile == 0;
+    if (SEG) {
+        x0 = __builtin_amdgcn_readfirstlane(x);  // lane 0's x: every lane of the wave is active here
+        const RowSlots rs = row_slots(A.g, x0, lane, x);
+        sp = rs.sp;
+        sm = rs.sm;
+        nhalo = rs.nhalo;
+    } else {
+        sp = one_row ? lane + 2 : lane + 1 + (x + 1 == A.g.nx ? 1 - A.g.nx : 1);
+        sm = one_row ? lane : lane + 1 + (x == 0 ? A.g.nx - 1 : -1);
+        nhalo = one_row ? 2 : 0;
+    }
+    if (wave == 0) link_wave<I, 0, ROW, SEG, RL>(A, first, s, valid, x, y, z, lane, sp, sm, nhalo, x0, sh);
+    else if (wave == 1) link_wave<I, 1, ROW, SEG, RL>(A, first, s, valid, x, y, z, lane, sp, sm, nhalo, x0, sh);
+    else link_wave<I, 2, ROW, SEG, RL>(A, first, s, valid, x, y, z, lane, sp, sm, nhalo, x0, sh);
+}
+
 // Four-wavefront variant (step_tile4); two such blocks fill the 8 wavefront slots of a CU at two waves
 // per SIMD.
 template <class I, int W, bool ROW, bool SEG>
@@ -326,7 +473,8 @@ k_step_batch(const StepArgs* __restrict__ table, const int* __restrict__ block_e
              const unsigned* __restrict__ block_site) {
     __shared__ StepShared<SEG ? kRowSeg : kRow> sh;
     const StepArgs A = table[block_event[blockIdx.x]];
-    step_tile4<unsigned int, ROW, SEG>(A, (long)block_site[blockIdx.x], sh);
+    if constexpr (SEG) step_tile4_pre<unsigned int, ROW, SEG>(A, (long)block_site[blockIdx.x], sh);
+    else step_tile4<unsigned int, ROW, SEG>(A, (long)block_site[blockIdx.x], sh);
 }
 
 int launch_step_batch(hipStream_t st, const StepArgs* table, const int* block_event, const unsigned* block_site,
