@@ -277,11 +277,12 @@ int glasma_load_state(glasma_ctx* ctx, const char* file);
 /* ---- tile-order autotuner (no counterpart in the reference, which has no GPU kernels) -------------------- */
 /* The order in which the step's blocks visit the lattice sets how much neighbour-link traffic the XCD-local
  * L2 caches absorb.  glasma_tune_step times the step on the context's current fields under every order
- * that fits the lattice (XCD column bands of several shapes walked in z, plane bands, linear), each with one
+ * that fits the lattice (XCD column bands of several shapes walked in z, plane bands, z slabs, linear), each with one
  * warm-up and `reps` timed launches into scratch buffers -- the fields, UOld and the clock are unchanged, and
  * no order changes any result -- and keeps the fastest for the context's later steps.  Optional outputs
  * (arrays of at least GLASMA_TUNE_MAX entries): the candidates' codes (mode*10000 + band width in tiles *100
- * + band height in rows), their ms per step, the index of the chosen one.  Needs 880 B per site of scratch
+ * + band height in rows; mode 0 linear, 1 plane bands, 2 column bands, 3 z slabs), their ms per step, the index of
+ * the chosen one.  Needs 880 B per site of scratch
  * device memory while it runs.  glasma_tile_order reports the code in use and whether it was tuned.
  * glasma_batch_tune tunes each distinct lattice of an ensemble once (events of the same lattice share the
  * order) and rebuilds the ensemble's work list. */

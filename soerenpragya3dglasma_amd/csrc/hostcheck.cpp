@@ -29,6 +29,16 @@ void hc_ad_series(double c, const double* e, const double* y, double* z) {
     ad_series(c, e, yy, z);
 }
 
+// first site of every block of a step under the tile order (mode, bpb, bw, bh, nbx, tpbp) (site.h tile_site)
+void hc_tile_sites(const int* n, int mode, int bpb, int bw, int bh, int nbx, int tpbp, long nblocks, long* out) {
+    Geom g;
+    g.nx = n[0]; g.ny = n[1]; g.nz = n[2]; g.bc = 0;
+    g.P = (long)n[0] * n[1];
+    g.S = g.P * n[2];
+    TileMap m = {mode, bpb, bw, bh, nbx, tpbp};
+    for (long b = 0; b < nblocks; b++) out[b] = tile_site(g, m, b);
+}
+
 // One fused leapfrog step over tiled device-layout arrays, sequentially over sites.
 void hc_step(const int* n, const double* a, int bc, double dt, const double* U, double* Un, double* E,
              const double* J, double* Jn, double* UOld) {

@@ -543,6 +543,14 @@ TileMap tile_map(const Geom& g, int mode, int bw, int bh) {
             }
         }
     }
+    if (mode == 3) {
+        const long nt = pad_sites(g.S) / kTile;
+        if (nt % kXcds == 0 && nt / kXcds <= 0x7fffffffL) {
+            m.mode = 3;
+            m.bpb = (int)(nt / kXcds);
+            return m;
+        }
+    }
     if (mode >= 1 && g.P % (kXcds * kTile) == 0) {
         m.mode = 1;
         m.bpb = (int)(g.P / (kXcds * kTile));

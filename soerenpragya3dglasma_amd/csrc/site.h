@@ -240,6 +240,8 @@ GHD void plaq_mat(const double* __restrict__ U, int kind, int m1, I s1, int m2, 
 //   mode 2: column bands of bw tiles (64 x-sites each) by bh rows; XCD k takes bands k, k+8, ... and
 //           walks each band through all z planes before the next one, so the few planes of one band that
 //           are in flight at once share their z neighbours in L2.
+//   mode 3: z slabs: XCD k takes the k-th eighth of the tiles in linear order (bpb tiles: whole planes and their
+//           y neighbours on one XCD, for planes whose tile count is not a multiple of 8).
 constexpr int kXcds = 8;
 
 // whether the step can take its row operands from an LDS copy of the tile: nx a multiple of 64 (a tile is part
@@ -266,6 +268,7 @@ struct TileMap {
 GHD long tile_site(const Geom& g, const TileMap& m, long b) {
     if (m.mode == 0) return b * kTile;
     const long k = b % kXcds, i = b / kXcds;
+    if (m.mode == 3) return (k * m.bpb + i) * kTile;
     if (m.mode == 1) {
         const long z = i / m.bpb, j = i % m.bpb;
         return z * g.P + k * (g.P / kXcds) + j * kTile;

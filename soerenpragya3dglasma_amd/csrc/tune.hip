@@ -22,7 +22,7 @@ struct Cand {
 // the orders worth timing for this lattice, each once
 std::vector<TileMap> candidates(const Geom& g) {
     static const Cand want[] = {{2, 4, 4}, {2, 2, 8}, {2, 1, 16}, {2, 1, 8}, {2, 4, 8}, {2, 2, 4}, {2, 4, 2},
-                                {2, 8, 4}, {2, 8, 2}, {2, 2, 16}, {1, 0, 0}, {0, 0, 0}};
+                                {2, 8, 4}, {2, 8, 2}, {2, 2, 16}, {1, 0, 0}, {3, 0, 0}, {0, 0, 0}};
     std::vector<TileMap> out;
     for (const Cand& c : want) {
         const TileMap m = tile_map(g, c.mode, c.bw, c.bh);

@@ -196,7 +196,9 @@ class Glasma:
     @staticmethod
     def _order(code: int) -> str:
         mode, bw, bh = code // 10000, (code // 100) % 100, code % 100
-        return f"bands {bw}x{bh}" if mode == 2 else ("plane bands" if mode == 1 else "linear")
+        if mode == 2:
+            return f"bands {bw}x{bh}"
+        return {1: "plane bands", 3: "z slabs"}.get(mode, "linear")
 
     def tune_step(self, reps: int = 3) -> dict:
         """Time the step under every block -> tile order that fits the lattice (on scratch outputs: the

@@ -143,3 +143,4 @@ def test_ad_series_against_matrix_series():
                 Z = Z + fact * term
             zr = np.array([-2.0 * np.real(np.trace(1j * T[a] @ Z)) for a in range(8)])
             assert np.abs(z - zr).max() <= tol * max(1.0, np.abs(zr).max()), (scale, np.abs(z - zr).max())
+
