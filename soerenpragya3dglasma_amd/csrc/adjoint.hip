@@ -223,7 +223,7 @@ struct DevBuf {
         if (next < pool.size()) {
             (void)hipStreamSynchronize(ctx->stream);  // the old array may still be read by queued work
             if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
-            hipFree(pool[next].second);
+            (void)hipFree(pool[next].second);
             pool[next] = {doubles, q};
         } else {
             pool.push_back({doubles, q});
