@@ -328,10 +328,23 @@ def main():
         t1 = time.perf_counter()
         st = g.coulomb_gauge(tol=0.0, maxit=args.gauge_iters, alpha=2.0)
         t2 = time.perf_counter()
-        gauge = {"iterations": int(st[0]), "ms_per_iteration": 1e3 * (t2 - t1) / max(1, int(st[0])),
-                 "iterations_per_s": max(1, int(st[0])) / (t2 - t1),
+        n1 = max(1, int(st[0]))
+        gauge = {"iterations": int(st[0]), "ms_per_iteration": 1e3 * (t2 - t1) / n1,
+                 "iterations_per_s": n1 / (t2 - t1),
                  "first_call_ms": 1e3 * (t1 - t0), "max_deviation": float(st[1])}
-        log(f"[bench] Coulomb gauge fixing: {gauge['ms_per_iteration']:.1f} ms per iteration")
+        # a call of four times as many iterations (continuing from the same G): the cost of one more iteration,
+        # and the fixed cost of a call (U^G and E^G formed once from the final G), which the short call's
+        # ms_per_iteration spreads over its few iterations
+        st2 = g.coulomb_gauge(tol=0.0, maxit=4 * args.gauge_iters, alpha=2.0)
+        t3 = time.perf_counter()
+        n2 = int(st2[0])
+        if n2 > n1:
+            per_it = ((t3 - t2) - (t2 - t1)) / (n2 - n1)
+            gauge["ms_per_added_iteration"] = 1e3 * per_it
+            gauge["ms_fixed_per_call"] = 1e3 * ((t2 - t1) - n1 * per_it)
+        log(f"[bench] Coulomb gauge fixing: {gauge['ms_per_iteration']:.1f} ms per iteration over {n1} iterations"
+            + (f", {gauge['ms_per_added_iteration']:.1f} ms per added iteration and "
+               f"{gauge['ms_fixed_per_call']:.1f} ms fixed per call" if n2 > n1 else ""))
         g.gauge_reset()
         gauge["consistent_to_1e-10"] = converging_gauge(g, 1e-10, 300)
         log(f"[bench] consistent gauge fixing to 1e-10: {gauge['consistent_to_1e-10']}")
