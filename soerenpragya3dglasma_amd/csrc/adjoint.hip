@@ -213,6 +213,9 @@ struct DevBuf {
     size_t next = 0;
     double* get(glasma_ctx* ctx, size_t doubles, int* rc) {
         auto& pool = ctx->wpool;
+        // a gradient call that returned early on an error can leave a recompute queued on the second stream, still
+        // writing into the pool's arrays: wait for it before this call hands them out again
+        if (next == 0 && ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
         if (next < pool.size() && pool[next].first >= doubles) return pool[next++].second;
         double* q = nullptr;
         const hipError_t e = hipMalloc(&q, doubles * sizeof(double));
