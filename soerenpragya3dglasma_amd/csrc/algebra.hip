@@ -77,9 +77,9 @@ extern "C" int glasma_su3_batch(int op, long n, double c, const double* in, cons
         if (hipGetLastError() != hipSuccess) rc = GLASMA_ERR_HIP;
     }
     if (rc == 0 && hipMemcpy(out, dout, sizeof(double) * wout * n, hipMemcpyDeviceToHost) != hipSuccess) rc = GLASMA_ERR_HIP;
-    if (din) hipFree(din);
-    if (din2) hipFree(din2);
-    if (dout) hipFree(dout);
+    if (din) (void)hipFree(din);
+    if (din2) (void)hipFree(din2);
+    if (dout) (void)hipFree(dout);
     if (rc) (void)hipGetLastError();  // leave no stale error for the next launch check
     return rc;
 }

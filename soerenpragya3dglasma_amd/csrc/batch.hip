@@ -71,13 +71,13 @@ int graph_for(glasma_batch* b, int nsteps, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return bfail(b, rc, "kernel launch failed during graph capture");
     }
     if (e != hipSuccess) return bfail(b, GLASMA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return bfail(b, GLASMA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     b->graphs[nsteps] = x;
     *out = x;
@@ -122,9 +122,9 @@ int glasma_batch_destroy(glasma_batch* b) {
     if (b->stream) hipStreamSynchronize(b->stream);
     for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
     for (glasma_ctx* c : b->ev) glasma_destroy(c);
-    if (b->table) hipFree(b->table);
-    if (b->block_event) hipFree(b->block_event);
-    if (b->block_site) hipFree(b->block_site);
+    if (b->table) (void)hipFree(b->table);
+    if (b->block_event) (void)hipFree(b->block_event);
+    if (b->block_site) (void)hipFree(b->block_site);
     if (b->e0) hipEventDestroy(b->e0);
     if (b->e1) hipEventDestroy(b->e1);
     if (b->stream) hipStreamDestroy(b->stream);

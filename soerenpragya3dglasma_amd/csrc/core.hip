@@ -27,7 +27,7 @@ int hip_fail(glasma_ctx* ctx, hipError_t e, const char* what) {
 }
 int ensure_red(glasma_ctx* ctx, size_t doubles) {
     if (ctx->red_cap >= doubles) return 0;
-    if (ctx->red) hipFree(ctx->red);
+    if (ctx->red) (void)hipFree(ctx->red);
     ctx->red = nullptr;
     ctx->red_cap = 0;
     GL_HIP(ctx, hipMalloc(&ctx->red, doubles * sizeof(double)));
@@ -36,7 +36,7 @@ int ensure_red(glasma_ctx* ctx, size_t doubles) {
 }
 int ensure_fftbuf(glasma_ctx* ctx, size_t doubles) {
     if (ctx->fftbuf_cap >= doubles) return 0;
-    if (ctx->fftbuf) hipFree(ctx->fftbuf);
+    if (ctx->fftbuf) (void)hipFree(ctx->fftbuf);
     ctx->fftbuf = nullptr;
     ctx->fftbuf_cap = 0;
     GL_HIP(ctx, hipMalloc(&ctx->fftbuf, doubles * sizeof(double)));
@@ -76,7 +76,7 @@ int upload_aos(glasma_ctx* ctx, const double* host_with_ghosts, double* dev_soa,
                            NP, p0);
         e = hipStreamSynchronize(ctx->stream);
     }
-    hipFree(tmp);
+    (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(ctx, e, "upload_aos");
     return 0;
 }
@@ -105,7 +105,7 @@ int download_aos(glasma_ctx* ctx, const double* dev_soa, double* host_with_ghost
     hipError_t e = hipMemcpyAsync(host_with_ghosts + (size_t)P * R, tmp, (size_t)S * R * sizeof(double),
                                   hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(tmp);
+    (void)hipFree(tmp);
     if (e != hipSuccess) return hip_fail(ctx, e, "download_aos");
     host_sync_ghosts(ctx, host_with_ghosts, R);
     return 0;
@@ -679,9 +679,9 @@ int glasma_destroy(glasma_ctx* ctx) {
     double* bufs[] = {ctx->U[0], ctx->U[1], ctx->UOld, ctx->E, ctx->J[0], ctx->J[1], ctx->red, ctx->G, ctx->Gtop,
                       ctx->UG, ctx->EG, ctx->fftbuf};
     for (double* b : bufs)
-        if (b) hipFree(b);
+        if (b) (void)hipFree(b);
     for (auto& w : ctx->wpool)
-        if (w.second) hipFree(w.second);
+        if (w.second) (void)hipFree(w.second);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (hipEvent_t ev : ctx->seg_ev)

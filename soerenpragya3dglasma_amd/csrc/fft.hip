@@ -322,7 +322,7 @@ hipError_t twiddles_split(int n, const dbl2** out) {
     if (e != hipSuccess) return e;
     e = hipMemcpy(d, h.data(), sizeof(dbl2) * (size_t)ns, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(d);
+        (void)hipFree(d);
         return e;
     }
     g_tws[key] = d;
@@ -357,7 +357,7 @@ hipError_t quarter_wave(int n, const dbl2** out) {
     if (e != hipSuccess) return e;
     e = hipMemcpy(d, h.data(), sizeof(dbl2) * (size_t)n, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(d);
+        (void)hipFree(d);
         return e;
     }
     g_wq[key] = d;
@@ -1116,7 +1116,7 @@ extern "C" int glasma_fft_host(double* data, int nd, const int* dims, long batch
         inner *= n;
     }
     if (rc == 0 && hipMemcpy(data, dev, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = GLASMA_ERR_HIP;
-    hipFree(dev);
+    (void)hipFree(dev);
     if (rc) (void)hipGetLastError();  // leave no stale error for the next launch check
     return rc;
 }

@@ -707,8 +707,8 @@ int ensure_gauge(glasma_ctx* ctx) {
         e = hipGetLastError();
     }
     if (e != hipSuccess) {
-        if (G) hipFree(G);
-        if (Gtop) hipFree(Gtop);
+        if (G) (void)hipFree(G);
+        if (Gtop) (void)hipFree(Gtop);
         return hip_fail(ctx, e, "ensure_gauge");
     }
     ctx->G = G;

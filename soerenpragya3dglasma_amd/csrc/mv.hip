@@ -419,10 +419,10 @@ int alloc_work(glasma_ctx* ctx, Wk& w) {
     return 0;
 }
 void free_work(Wk& w) {
-    if (w.rho) hipFree(w.rho);
-    if (w.V) hipFree(w.V);
-    if (w.xi) hipFree(w.xi);
-    if (w.prof) hipFree(w.prof);
+    if (w.rho) (void)hipFree(w.rho);
+    if (w.V) (void)hipFree(w.V);
+    if (w.xi) (void)hipFree(w.xi);
+    if (w.prof) (void)hipFree(w.prof);
     w = Wk();
 }
 
@@ -882,8 +882,8 @@ int tmd_wilson_lines(glasma_ctx* ctx, Wk& w, const glasma_tmd_params* p, const d
         hipLaunchKernelGGL(k_tmd_charge, grid1(g.S), dim3(256), 0, ctx->stream, w.F, g, C);
         rc = wilson_from_charge(ctx, w, std::numeric_limits<double>::infinity(), p->m_eff, right);
     }
-    hipStreamSynchronize(ctx->stream);  // the positions buffer is freed below
-    hipFree(pos);
+    (void)hipStreamSynchronize(ctx->stream);  // the positions buffer is freed below
+    (void)hipFree(pos);
     return rc;
 }
 
@@ -1115,7 +1115,7 @@ extern "C" int glasma_set_wilson_lines(glasma_ctx* ctx, char dir, const glasma_m
             e = hipMemcpyAsync(rho_out + 8 * g.P, tmp, 8 * (size_t)g.S * sizeof(double), hipMemcpyDeviceToHost,
                                ctx->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-            hipFree(tmp);
+            (void)hipFree(tmp);
         }
         if (e != hipSuccess) rc = hip_fail(ctx, e, "set_wilson_lines rho");
         else host_sync_ghosts(ctx, rho_out, 8);

@@ -110,9 +110,9 @@ int glasma_tune_step(glasma_ctx* ctx, int reps, int* ncand, int* codes, double* 
         t[i] = f / reps;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(sU);
-    hipFree(sE);
-    hipFree(sJ);
+    (void)hipFree(sU);
+    (void)hipFree(sE);
+    (void)hipFree(sJ);
     if (e != hipSuccess) {
         const int code = hip_code(e);
         return set_error(ctx, code, std::string("glasma_tune_step: ") + hipGetErrorString(e));
@@ -157,9 +157,9 @@ int glasma_copy_ceiling(glasma_ctx* ctx, int reps, double* ms) {
     if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
     if (e == hipSuccess) e = hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(sU);
-    hipFree(sE);
-    hipFree(sJ);
+    (void)hipFree(sU);
+    (void)hipFree(sE);
+    (void)hipFree(sJ);
     if (e != hipSuccess) return set_error(ctx, hip_code(e), std::string("glasma_copy_ceiling: ") + hipGetErrorString(e));
     *ms = f / reps;
     return 0;
